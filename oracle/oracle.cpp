@@ -1,0 +1,1024 @@
+// oracle/oracle.cpp -- CPU restatement of the rt-nurbs hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker: only tests/, the smoke
+// entry point and the benchmark's CPU-baseline leg may load it.  The product
+// (rt-nurbs_amd/) never links or calls it.
+//
+// Every function restates the reference algorithm in the reference's own
+// floating-point operation order (left-to-right sums, no FMA contraction:
+// compile with -ffp-contract=off), templated on T so the same code also gives
+// the FP64 oracle.  Citations are file:line in Frostbite16/rt-nurbs.
+//
+// Function                      reference                           pinned by
+// ---------------------------   ---------------------------------   -----------------------------
+// lookat / camera_perspective   transform.h:46-60, camera.h:63-141  golden fixtures + closed form
+// gen_rays                      camera.h:162-174 (host 1/sqrtf,     golden fixtures + pixel-centre
+//                               vector_ops.h:102-106),               closed form
+//                               generateRaysKernel.cu:17-34
+// segs / decompose              bezier_patch.h:46-53, :87-139,      golden fixtures, A4.3 identity,
+//                               nurbs.h:144-250                      patch-count closed form
+// knot refinement               nurbs.h:97-140, :255-486            golden fixtures (p<=4)
+// patch_bounds                  bezier_patch.h:154-169              golden fixtures, brute force
+// build_bvh (exhaustive SAH)    bvh.h:47-179, bounds_ops.h:72-77    golden fixtures, invariants
+// patch_delta                   main.cpp:18-43                      golden fixtures
+// ray planes                    math_utils.h:33-54                  invariants (tests)
+// slab test                     bounds_ops.h:83-114                 brute force (tests)
+// bilinear seed                 bezier_patch.h:273-327              golden fixtures (per-pixel)
+// de Casteljau / eval           bezier_patch.h:57-80, :173-191      closed-form Bernstein (tests)
+// Newton                        bezier_patch.h:198-266, matrix_ops  analytic sphere/torus, fixtures
+//                               h:14-32, math_utils.h:12-17,:70-74
+// per-pixel traversal           intersect.cu:33-193                 golden fixtures, brute force
+//
+// Extensions that the reference does not have (degree up to 7, per-ray work
+// counters, FP64 instantiation, jittered rays) are marked where they occur.
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+
+namespace orc {
+
+constexpr int MAXDEG = 7;          // reference caps at 4 (constants.h:31); extension
+constexpr int STACK_DEPTH = 64;    // reference: int[32] with no overflow check (intersect.cu:59)
+
+template <class T> struct v3 { T x, y, z; };
+template <class T> struct v4 { T x, y, z, w; };
+
+template <class T> static inline v3<T> sub3(v3<T> a, v3<T> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class T> static inline v3<T> add3(v3<T> a, v3<T> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class T> static inline v3<T> mul3(v3<T> a, T s) { return {a.x * s, a.y * s, a.z * s}; }
+template <class T> static inline T dot3(v3<T> a, v3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class T> static inline v3<T> cross3(v3<T> a, v3<T> b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+// host branch of reciprocal_length (vector_ops.h:102-106): 1/sqrt, never rsqrt
+template <class T> static inline T rlen(v3<T> a) { return T(1) / std::sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+template <class T> static inline v3<T> normalize3(v3<T> a) { return mul3(a, rlen(a)); }
+// length (vector_ops.h:70-74)
+template <class T> static inline T len3(v3<T> a) { return std::sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+
+template <class T> static inline v4<T> sub4(v4<T> a, v4<T> b) { return {a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w}; }
+template <class T> static inline v4<T> mul4(v4<T> a, T s) { return {a.x * s, a.y * s, a.z * s, a.w * s}; }
+template <class T> static inline v4<T> add4(v4<T> a, v4<T> b) { return {a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w}; }
+// (1-u)*a + u*b, each product formed first (bezier_patch.h:63)
+template <class T> static inline v4<T> lerp4(v4<T> a, v4<T> b, T u) { return add4(mul4(a, T(1) - u), mul4(b, u)); }
+
+// ----------------------------------------------------------------- matrices
+// column-major 4x4 (math_types.h:8-10); multiply (matrix_ops.h:101-114)
+static void mat_mul(const float* a, const float* b, float* r) {
+  float out[16];
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++) {
+      float sum = 0;
+      for (int k = 0; k < 4; k++) sum += a[k * 4 + i] * b[j * 4 + k];
+      out[j * 4 + i] = sum;
+    }
+  std::memcpy(r, out, sizeof(out));
+}
+// 3x3 Leibniz determinant (matrix_ops.h:37-45)
+static float det3(float m0, float m1, float m2, float m3, float m4, float m5, float m6, float m7, float m8) {
+  return m0 * m4 * m8 + m3 * m7 * m2 + m6 * m1 * m5 - m6 * m4 * m2 - m3 * m1 * m8 - m0 * m7 * m5;
+}
+// adjugate + Cramer inverse (matrix_ops.h:68-98, :148-171)
+static void mat_inverse(const float* m, float* r) {
+  float a[16];
+  a[0] = det3(m[5], m[6], m[7], m[9], m[10], m[11], m[13], m[14], m[15]);
+  a[4] = -det3(m[4], m[6], m[7], m[8], m[10], m[11], m[12], m[14], m[15]);
+  a[8] = det3(m[4], m[5], m[7], m[8], m[9], m[11], m[12], m[13], m[15]);
+  a[12] = -det3(m[4], m[5], m[6], m[8], m[9], m[10], m[12], m[13], m[14]);
+  a[1] = -det3(m[1], m[2], m[3], m[9], m[10], m[11], m[13], m[14], m[15]);
+  a[5] = det3(m[0], m[2], m[3], m[8], m[10], m[11], m[12], m[14], m[15]);
+  a[9] = -det3(m[0], m[1], m[3], m[8], m[9], m[11], m[12], m[13], m[15]);
+  a[13] = det3(m[0], m[1], m[2], m[8], m[9], m[10], m[12], m[13], m[14]);
+  a[2] = det3(m[1], m[2], m[3], m[5], m[6], m[7], m[13], m[14], m[15]);
+  a[6] = -det3(m[0], m[2], m[3], m[4], m[6], m[7], m[12], m[14], m[15]);
+  a[10] = det3(m[0], m[1], m[3], m[4], m[5], m[7], m[12], m[13], m[15]);
+  a[14] = -det3(m[0], m[1], m[2], m[4], m[5], m[6], m[12], m[13], m[14]);
+  a[3] = -det3(m[1], m[2], m[3], m[5], m[6], m[7], m[9], m[10], m[11]);
+  a[7] = det3(m[0], m[2], m[3], m[4], m[6], m[7], m[8], m[10], m[11]);
+  a[11] = -det3(m[0], m[1], m[3], m[4], m[5], m[7], m[8], m[9], m[11]);
+  a[15] = det3(m[0], m[1], m[2], m[4], m[5], m[6], m[8], m[9], m[10]);
+  float det = m[0] * a[0] + m[1] * a[4] + m[2] * a[8] + m[3] * a[12];
+  float ad = det < 0.0f ? -det : det;
+  if (ad < 1e-8f) {
+    for (int i = 0; i < 16; i++) r[i] = (i % 5 == 0) ? 1.0f : 0.0f;
+    return;
+  }
+  float di = 1.0f / det;
+  for (int i = 0; i < 16; i++) r[i] = a[i] * di;
+}
+static void mat_scale(float sx, float sy, float sz, float* m) {
+  std::memset(m, 0, 16 * sizeof(float));
+  m[0] = sx; m[5] = sy; m[10] = sz; m[15] = 1.0f;
+}
+static void mat_translate(float tx, float ty, float tz, float* m) {
+  std::memset(m, 0, 16 * sizeof(float));
+  m[0] = 1.0f; m[5] = 1.0f; m[10] = 1.0f; m[15] = 1.0f;
+  m[12] = tx; m[13] = ty; m[14] = tz;
+}
+
+// ------------------------------------------------------ curve/surface eval
+// deCasteljau1 (bezier_patch.h:57-65)
+template <class T> static inline v4<T> decas1(const v4<T>* P, int n, T u) {
+  v4<T> Q[MAXDEG + 1];
+  for (int i = 0; i <= n; i++) Q[i] = P[i];
+  for (int k = 1; k <= n; k++)
+    for (int i = 0; i <= n - k; i++) Q[i] = lerp4(Q[i], Q[i + 1], u);
+  return Q[0];
+}
+// deCasteljauDers (bezier_patch.h:72-80).  The reference also updates Q[n-k]
+// from the unset Q[n] at k=1; that entry never reaches Q[0], so it is omitted.
+template <class T> static inline v4<T> decas_der(const v4<T>* P, int n, T u) {
+  v4<T> Q[MAXDEG + 1];
+  for (int i = 0; i < n; i++) Q[i] = mul4(sub4(P[i + 1], P[i]), T(n));
+  for (int k = 1; k < n; k++)
+    for (int i = 0; i <= n - 1 - k; i++) Q[i] = lerp4(Q[i], Q[i + 1], u);
+  return Q[0];
+}
+// BezierPatchEval (bezier_patch.h:173-191): point, d/du, d/dv in homogeneous space
+template <class T>
+static inline void patch_eval(const v4<T>* P, int p, int q, T u, T v, v4<T>& du, v4<T>& dv, v4<T>& pt) {
+  v4<T> curve[MAXDEG + 1];
+  for (int i = 0; i <= p; i++) curve[i] = decas1(&P[i * (q + 1)], q, v);
+  du = decas_der(curve, p, u);
+  pt = decas1(curve, p, u);
+  for (int j = 0; j <= q; j++) {
+    v4<T> col[MAXDEG + 1];
+    for (int i = 0; i <= p; i++) col[i] = P[i * (q + 1) + j];
+    curve[j] = decas1(col, p, u);
+  }
+  dv = decas_der(curve, q, v);
+}
+
+// -------------------------------------------------------------- PCG / rand
+// pcg_hash (math_utils.h:12-17)
+static inline uint32_t pcg_hash(uint32_t x) {
+  x = x * 747796405u + 2891336453u;
+  uint32_t w = ((x >> ((x >> 28u) + 4u)) ^ x) * 277803737u;
+  return (w >> 22u) ^ w;
+}
+// randf (math_utils.h:70-74)
+template <class T> static inline T randf(uint32_t& seed) {
+  seed = pcg_hash(seed);
+  return T(seed) * T(2.3283064e-10f);
+}
+
+// ------------------------------------------------------------- ray helpers
+template <class T> struct RayPlane { v3<T> n1, n2; T d1, d2; };
+
+// getRayPlanes (math_utils.h:33-54), host reciprocal_length
+template <class T> static inline RayPlane<T> ray_planes(v3<T> o, v3<T> d) {
+  T ax = std::fabs(d.x), ay = std::fabs(d.y), az = std::fabs(d.z);
+  bool xd = (ax > ay) && (ax > az);
+  v3<T> base = xd ? v3<T>{d.x, d.y, T(0)} : v3<T>{d.y, d.z, T(0)};
+  T den = rlen(base);
+  v3<T> n1 = xd ? mul3(v3<T>{d.y, -d.x, T(0)}, den) : mul3(v3<T>{T(0), d.z, -d.y}, den);
+  v3<T> n2 = cross3(n1, mul3(d, T(1) / len3(d)));
+  T d1 = -dot3(n1, o);
+  T d2 = -dot3(n2, o);
+  return {n1, n2, d1, d2};
+}
+
+// intersectBound (bounds_ops.h:83-114)
+template <class T>
+static inline T slab(const float* bmin, const float* bmax, v3<T> o, v3<T> rd) {
+  T tmin = 0, tmax = INFINITY;
+  T tx0 = (T(bmin[0]) - o.x) * rd.x, tx1 = (T(bmax[0]) - o.x) * rd.x;
+  T ty0 = (T(bmin[1]) - o.y) * rd.y, ty1 = (T(bmax[1]) - o.y) * rd.y;
+  T tz0 = (T(bmin[2]) - o.z) * rd.z, tz1 = (T(bmax[2]) - o.z) * rd.z;
+  T nx = std::fmin(tx0, tx1), fx = std::fmax(tx0, tx1);
+  T ny = std::fmin(ty0, ty1), fy = std::fmax(ty0, ty1);
+  T nz = std::fmin(tz0, tz1), fz = std::fmax(tz0, tz1);
+  tmin = std::fmax(nz, std::fmax(ny, std::fmax(nx, tmin)));
+  tmax = std::fmin(fz, std::fmin(fy, std::fmin(fx, tmax)));
+  return (tmin <= tmax && tmax > 0) ? tmin : T(INFINITY);
+}
+
+// dehom (bezier_patch.h:32-35): multiply by 1/w
+template <class T> static inline v3<T> dehom(v4<T> c) {
+  T iw = T(1) / c.w;
+  return {c.x * iw, c.y * iw, c.z * iw};
+}
+
+// biliniearRay (bezier_patch.h:273-327), restated as written.  GARP's
+// n = cross(n, pa) is absent in the reference, so tx and vx are analytically
+// zero and the accept test runs on rounding noise; kept so seeds match.
+template <class T>
+static inline void bilinear_seed(v3<T> q00, v3<T> q10, v3<T> q11, v3<T> q01, v3<T> orig, v3<T> dir, T& su, T& sv) {
+  v3<T> e10 = sub3(q10, q00);
+  v3<T> e11 = sub3(q11, q10);
+  v3<T> e00 = sub3(q01, q00);
+  v3<T> qn = cross3(e10, sub3(q01, q11));
+  q00 = sub3(q00, orig);
+  q10 = sub3(q10, orig);
+  T a = dot3(cross3(q00, dir), e00);
+  T c = dot3(qn, dir);
+  T b = dot3(cross3(q10, dir), e11);
+  b -= a + c;
+  T det = b * b - T(4) * a * c;
+  su = T(0.5f); sv = T(0.5f);
+  if (det < 0) return;
+  det = std::sqrt(det);
+  T ux[2];
+  if (c == 0) {
+    ux[0] = -a / b; ux[1] = -1;
+  } else {
+    ux[0] = (-b - std::copysign(det, b)) / T(2);
+    ux[1] = a / ux[0];
+    ux[0] /= c;
+  }
+  T best_t = INFINITY;
+  for (int i = 0; i < 2; i++) {
+    if (ux[i] < 0 || ux[i] > 1) continue;
+    v3<T> pb = add3(mul3(e00, T(1) - ux[i]), mul3(e11, ux[i]));
+    v3<T> n = cross3(dir, pb);
+    T dd = dot3(n, n);
+    T tx = dot3(n, pb);
+    T vx = dot3(n, dir);
+    if (tx > 0 && 0 <= vx && vx <= dd && tx < best_t) {
+      su = ux[i];
+      sv = vx / dd;
+      best_t = tx;
+    }
+  }
+}
+
+template <class T> struct NewtonOut {
+  bool hit;
+  v3<T> S, Su, Sv;
+  T u, v;
+  int evals;  // extension: number of surface evaluations performed
+};
+
+// S, dS/du, dS/dv from a homogeneous evaluation (bezier_patch.h:211-222)
+template <class T>
+static inline void dehom_ders(v4<T> pt, v4<T> du, v4<T> dv, v3<T>& S, v3<T>& Su, v3<T>& Sv) {
+  T invW = T(1) / pt.w;
+  S = {pt.x * invW, pt.y * invW, pt.z * invW};
+  Su = {(du.x - S.x * du.w) * invW, (du.y - S.y * du.w) * invW, (du.z - S.z * du.w) * invW};
+  Sv = {(dv.x - S.x * dv.w) * invW, (dv.y - S.y * dv.w) * invW, (dv.z - S.z * dv.w) * invW};
+}
+
+// newtonRhapson (bezier_patch.h:198-266) with the 2x2 inverse of
+// matrix_ops.h:14-32; max_iter / thresholds are parameters (reference: 7, 1e-5, 1e-4)
+template <class T>
+static NewtonOut<T> newton(const RayPlane<T>& rp, const v4<T>* P, int p, int q, T u0, T v0, T thr_det, T thr_res,
+                           uint32_t seed, int max_iter) {
+  T u = u0, v = v0;
+  NewtonOut<T> out{};
+  out.hit = false;
+  for (int it = 1; it <= max_iter; it++) {
+    v4<T> du, dv, pt;
+    patch_eval(P, p, q, u, v, du, dv, pt);
+    out.evals = it;
+    v3<T> S, Su, Sv;
+    dehom_ders(pt, du, dv, S, Su, Sv);
+    T r0 = dot3(rp.n1, S) + rp.d1;
+    T r1 = dot3(rp.n2, S) + rp.d2;
+    T err = std::fabs(r0) + std::fabs(r1);
+    if (err < thr_res) {
+      out.hit = true; out.S = S; out.Su = Su; out.Sv = Sv; out.u = u; out.v = v;
+      return out;
+    }
+    T m0 = dot3(rp.n1, Su), m1 = dot3(rp.n2, Su), m2 = dot3(rp.n1, Sv), m3 = dot3(rp.n2, Sv);
+    T rand1 = randf<T>(seed);
+    T rand2 = randf<T>(seed);
+    T det = m0 * m3 - m2 * m1;
+    if (std::fabs(det) < thr_det) {
+      u = u + T(0.1f) * ((u0 - u) * rand1);
+      v = v + T(0.1f) * ((v0 - v) * rand2);
+    } else {
+      T i0 = m3 / det, i1 = -m1 / det, i2 = -m2 / det, i3 = m0 / det;
+      T ju = i0 * r0 + i2 * r1;
+      T jv = i1 * r0 + i3 * r1;
+      u = u - ju;
+      v = v - jv;
+    }
+    u = std::fmin(std::fmax(u, T(0)), T(1));
+    v = std::fmin(std::fmax(v, T(0)), T(1));
+  }
+  return out;
+}
+
+}  // namespace orc
+
+using namespace orc;
+
+// ========================================================================
+// exported C API (ctypes)
+// ========================================================================
+extern "C" {
+
+// Transform::lookAt (transform.h:46-60), host normalize
+void orc_lookat(const float* eye, const float* look, const float* up, float* out16) {
+  v3<float> pos{eye[0], eye[1], eye[2]};
+  v3<float> dir = normalize3(sub3(v3<float>{look[0], look[1], look[2]}, pos));
+  v3<float> right = normalize3(cross3(v3<float>{up[0], up[1], up[2]}, dir));
+  v3<float> nu = cross3(dir, right);
+  float m[16] = {right.x, right.y, right.z, 0.0f, nu.x, nu.y, nu.z, 0.0f,
+                 dir.x,   dir.y,   dir.z,   0.0f, pos.x, pos.y, pos.z, 1.0f};
+  std::memcpy(out16, m, sizeof(m));
+}
+
+// init_camera + initCameraTransforms + getPerspectiveCamera (camera.h:63-141);
+// returns camera_from_raster.  fov in degrees as the reference passes it.
+void orc_camera_perspective(int W, int H, float fov_deg, float znear, float zfar, float* camera_from_raster) {
+  float aspect = float(W) / float(H);
+  float wminx, wminy, wmaxx, wmaxy;
+  if (aspect > 1.0f) { wminx = -aspect; wminy = -1.0f; wmaxx = aspect; wmaxy = 1.0f; }
+  else { wminx = -1.0f; wminy = -1.0f / aspect; wmaxx = 1.0f; wmaxy = 1.0f / aspect; }
+  float s[16], t[16], ndc_from_screen[16], raster_from_ndc[16], raster_from_screen[16], screen_from_raster[16];
+  mat_scale(1 / (wmaxx - wminx), 1 / (wmaxy - wminy), 1, s);
+  mat_translate(-wminx, -wmaxy, 0, t);
+  mat_mul(s, t, ndc_from_screen);
+  mat_scale(float(W), float(-H), 1, raster_from_ndc);
+  mat_mul(raster_from_ndc, ndc_from_screen, raster_from_screen);
+  mat_inverse(raster_from_screen, screen_from_raster);
+  const float f = zfar, n = znear;
+  float pt[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, f / (f - n), 1, 0, 0, -f * n / (f - n), 0};
+  const float PI = 3.14159265f;
+  float inv_tan = 1.0f / tanf(((PI / 180) * fov_deg) / 2);
+  float sc[16], screen_from_camera[16], inv_sfc[16];
+  mat_scale(inv_tan, inv_tan, 1, sc);
+  mat_mul(sc, pt, screen_from_camera);
+  mat_inverse(screen_from_camera, inv_sfc);
+  mat_mul(inv_sfc, screen_from_raster, camera_from_raster);
+}
+
+// calculateRay<perspective_projection> (camera.h:162-174) over a W x H grid,
+// pixel-major.  jx/jy: optional per-ray raster offsets replacing the 0.5
+// pixel-centre offsets (extension: jittered spp); spp rays per pixel are stored
+// consecutively, ray index = pixel*spp + sample.
+void orc_gen_rays(const float* wfc, const float* cfr, int W, int H, int spp, const float* jx, const float* jy,
+                  float* ox, float* oy, float* oz, float* dx, float* dy, float* dz) {
+  for (int y = 0; y < H; y++)
+    for (int x = 0; x < W; x++)
+      for (int s = 0; s < spp; s++) {
+        long r = ((long)y * W + x) * spp + s;
+        float px = float(x) + (jx ? jx[r] : 0.5f);
+        float py = float(y) + (jy ? jy[r] : 0.5f);
+        float e[4] = {px, py, 0.0f, 1.0f};
+        float c[4];
+        for (int i = 0; i < 4; i++)
+          c[i] = cfr[0 + i] * e[0] + cfr[4 + i] * e[1] + cfr[8 + i] * e[2] + cfr[12 + i] * e[3];
+        v3<float> pc = normalize3(v3<float>{c[0], c[1], c[2]});
+        float d4[4];
+        float pv[4] = {pc.x, pc.y, pc.z, 0.0f};
+        for (int i = 0; i < 4; i++)
+          d4[i] = wfc[0 + i] * pv[0] + wfc[4 + i] * pv[1] + wfc[8 + i] * pv[2] + wfc[12 + i] * pv[3];
+        ox[r] = wfc[12]; oy[r] = wfc[13]; oz[r] = wfc[14];
+        dx[r] = d4[0]; dy[r] = d4[1]; dz[r] = d4[2];
+      }
+}
+
+// segs (bezier_patch.h:46-53)
+int orc_segs(const float* U, int last, int p, float* out) {
+  int nb = 0;
+  out[nb++] = U[p];
+  for (int i = p + 1; i <= last - p - 1; ++i)
+    if (U[i] != U[i - 1]) out[nb++] = U[i];
+  out[nb] = U[last - p];
+  return nb;
+}
+
+// A5.7 in one direction (nurbs.h:144-250).  dir 0: net (n+1) x (m+1) -> strips
+// (p+1) x (m+1); dir 1: net (n+1) x (m+1) -> patches (n+1) x (q+1).  Returns the
+// number of segments; span_start receives the knot index a of every segment.
+static int decompose_dir(int n, int p, const float* U, int m, int q, const float* V, int dir, const v4<float>* P,
+                         v4<float>* Q, int* span_start) {
+  int nb = 0;
+  float alphas[MAXDEG];
+  if (dir == 0) {
+    const int rs = m + 1, ns = (p + 1) * rs;
+    auto QW = [&](int b, int i, int r) -> v4<float>& { return Q[(long)b * ns + i * rs + r]; };
+    auto PW = [&](int i, int r) -> const v4<float>& { return P[(long)i * rs + r]; };
+    int a = p, b = p + 1;
+    for (int i = 0; i <= p; i++)
+      for (int r = 0; r <= m; r++) QW(nb, i, r) = PW(i, r);
+    int mu = n + p + 1;
+    if (span_start) span_start[0] = a;
+    while (b < mu) {
+      int i = b;
+      while (b < mu && U[b + 1] == U[b]) b++;
+      int mult = b - i + 1;
+      if (mult < p) {
+        float numer = U[b] - U[a];
+        for (int j = p; j > mult; j--) alphas[j - mult - 1] = numer / (U[a + j] - U[a]);
+        int r = p - mult;
+        for (int j = 1; j <= r; j++) {
+          int save = r - j, s = mult + j;
+          for (int k = p; k >= s; k--) {
+            float alpha = alphas[k - s];
+            for (int row = 0; row <= m; row++)
+              QW(nb, k, row) = add4(mul4(QW(nb, k, row), alpha), mul4(QW(nb, k - 1, row), 1.0f - alpha));
+          }
+          if (b < mu)
+            for (int row = 0; row <= m; row++) QW(nb + 1, save, row) = QW(nb, p, row);
+        }
+      }
+      nb = nb + 1;
+      if (b < mu) {
+        for (int i2 = p - mult; i2 <= p; i2++)
+          for (int row = 0; row <= m; row++) QW(nb, i2, row) = PW(b - p + i2, row);
+        a = b;
+        b = b + 1;
+        if (span_start) span_start[nb] = a;
+      }
+    }
+  } else {
+    const int irs = m + 1, ors = q + 1, ns = (n + 1) * ors;
+    auto QW = [&](int b, int r, int j) -> v4<float>& { return Q[(long)b * ns + r * ors + j]; };
+    auto PW = [&](int r, int j) -> const v4<float>& { return P[(long)r * irs + j]; };
+    int a = q, b = q + 1;
+    for (int i = 0; i <= q; i++)
+      for (int r = 0; r <= n; r++) QW(nb, r, i) = PW(r, i);
+    int mv = m + q + 1;
+    if (span_start) span_start[0] = a;
+    while (b < mv) {
+      int i = b;
+      while (b < mv && V[b + 1] == V[b]) b++;
+      int mult = b - i + 1;
+      if (mult < q) {
+        float numer = V[b] - V[a];
+        for (int j = q; j > mult; j--) alphas[j - mult - 1] = numer / (V[a + j] - V[a]);
+        int r = q - mult;
+        for (int j = 1; j <= r; j++) {
+          int save = r - j, s = mult + j;
+          for (int k = q; k >= s; k--) {
+            float alpha = alphas[k - s];
+            for (int row = 0; row <= n; row++)
+              QW(nb, row, k) = add4(mul4(QW(nb, row, k), alpha), mul4(QW(nb, row, k - 1), 1.0f - alpha));
+          }
+          if (b < mv)
+            for (int row = 0; row <= n; row++) QW(nb + 1, row, save) = QW(nb, row, q);
+        }
+      }
+      nb = nb + 1;
+      if (b < mv) {
+        for (int i2 = q - mult; i2 <= q; i2++)
+          for (int row = 0; row <= n; row++) QW(nb, row, i2) = PW(row, b - q + i2);
+        a = b;
+        b = b + 1;
+        if (span_start) span_start[nb] = a;
+      }
+    }
+  }
+  return nb;
+}
+
+// Number of Bezier segments of a clamped knot vector (= segs' return value).
+int orc_count_segments(const float* U, int n, int p) {
+  std::vector<float> tmp(n + 2);
+  return orc_segs(U, n + p + 1, p, tmp.data());
+}
+
+// getPatch (bezier_patch.h:87-139).  Pw: (n+1)*(m+1) homogeneous points (x,y,z,w),
+// u-major.  Outputs (caller-sized via orc_count_segments): cp[nbU*nbV*(p+1)*(q+1)*4],
+// uv_range[nbU*nbV*4] = (u0,u1,v0,v1), knot_u[nbU], knot_v[nbU*nbV] (span start
+// index a per segment; every strip's V pass yields the same list).  Returns 0 / -1.
+int orc_decompose(int n, int p, const float* U, int m, int q, const float* V, const float* Pw, float* cp,
+                  float* uv_range, int* knot_u, int* knot_v) {
+  if (p < 1 || q < 1 || p > MAXDEG || q > MAXDEG) return -1;
+  std::vector<float> ub(n - p + 2), vb(m - q + 2);
+  int nbU = orc_segs(U, n + p + 1, p, ub.data());
+  int nbV = orc_segs(V, m + q + 1, q, vb.data());
+  std::vector<v4<float>> strips((size_t)nbU * (p + 1) * (m + 1));
+  const v4<float>* P = reinterpret_cast<const v4<float>*>(Pw);
+  decompose_dir(n, p, U, m, q, V, 0, P, strips.data(), knot_u);
+  const int ss = (p + 1) * (m + 1), ps = (p + 1) * (q + 1);
+  v4<float>* out = reinterpret_cast<v4<float>*>(cp);
+  for (int s = 0; s < nbU; s++)
+    decompose_dir(p, p, U, m, q, V, 1, strips.data() + (size_t)s * ss, out + (size_t)s * nbV * ps,
+                  knot_v ? knot_v + (size_t)s * nbV : nullptr);
+  for (int i = 0; i < nbU; i++)
+    for (int j = 0; j < nbV; j++) {
+      float* r = uv_range + 4 * ((size_t)i * nbV + j);
+      r[0] = ub[i]; r[1] = ub[i + 1]; r[2] = vb[j]; r[3] = vb[j + 1];
+    }
+  return 0;
+}
+
+// findSpan A2.1 (nurbs.h:35-49)
+int orc_find_span(int n, int p, float u, const float* U) {
+  if (u == U[n + 1]) return n;
+  int low = p, high = n + 1, mid = (low + high) / 2;
+  while (u < U[mid] || u >= U[mid + 1]) {
+    if (u < U[mid]) high = mid; else low = mid;
+    mid = (low + high) / 2;
+  }
+  return mid;
+}
+
+// basisFuncEval A2.2 (nurbs.h:53-64) + surfacePoint A4.3 (nurbs.h:68-92), FP64
+// (used by tests as the decomposition identity; not on the render path)
+void orc_surface_point_f64(int n, int p, const float* Uf, int m, int q, const float* Vf, const float* Pw, double u,
+                           double v, double* out3) {
+  auto span = [](int nn, int pp, double x, const float* K) {
+    if (x >= K[nn + 1]) return nn;
+    int lo = pp, hi = nn + 1, mid = (lo + hi) / 2;
+    while (x < K[mid] || x >= K[mid + 1]) {
+      if (x < K[mid]) hi = mid; else lo = mid;
+      mid = (lo + hi) / 2;
+    }
+    return mid;
+  };
+  auto basis = [](int i, double x, int pp, const float* K, double* N) {
+    N[0] = 1.0;
+    for (int j = 1; j <= pp; j++) {
+      double saved = 0.0;
+      for (int r = 0; r < j; r++) {
+        double temp = N[r] / ((double)K[i + r + 1] - (double)K[i + 1 - j + r]);
+        N[r] = saved + ((double)K[i + r + 1] - x) * temp;
+        saved = (x - (double)K[i + 1 - j + r]) * temp;
+      }
+      N[j] = saved;
+    }
+  };
+  double Nu[MAXDEG + 1], Nv[MAXDEG + 1];
+  int us = span(n, p, u, Uf), vs = span(m, q, v, Vf);
+  basis(us, u, p, Uf, Nu);
+  basis(vs, v, q, Vf, Nv);
+  double S[4] = {0, 0, 0, 0};
+  for (int l = 0; l <= q; l++) {
+    double t[4] = {0, 0, 0, 0};
+    for (int k = 0; k <= p; k++)
+      for (int c = 0; c < 4; c++) t[c] += Nu[k] * (double)Pw[4 * ((size_t)(us - p + k) * (m + 1) + (vs - q + l)) + c];
+    for (int c = 0; c < 4; c++) S[c] += Nv[l] * t[c];
+  }
+  for (int c = 0; c < 3; c++) out3[c] = S[c] / S[3];
+}
+
+// ------------------------------------------------------ knot refinement
+// curveKnotInsert A5.1 (nurbs.h:97-140).  Returns 0, or -1 where the reference
+// prints an error and leaves the outputs unset.
+static int curve_knot_insert(int np, int p, const float* UP, float u, int k, int s, int r, int& nq, float* UQ,
+                             const v4<float>* Pw, v4<float>* Qw) {
+  if (r + s > p) return -1;
+  if (p > MAXDEG) return -1;
+  int mp = np + p + 1;
+  nq = np + r;
+  int L = 0;
+  if (r <= 0) return 0;
+  v4<float> Rw[MAXDEG + 1];
+  for (int i = 0; i <= k; i++) UQ[i] = UP[i];
+  for (int i = 1; i <= r; i++) UQ[k + i] = u;
+  for (int i = k + 1; i <= mp; i++) UQ[i + r] = UP[i];
+  for (int i = 0; i <= k - p; i++) Qw[i] = Pw[i];
+  for (int i = k - s; i <= np; i++) Qw[i + r] = Pw[i];
+  for (int i = 0; i <= p - s; i++) Rw[i] = Pw[k - p + i];
+  for (int j = 1; j <= r; j++) {
+    L = k - p + j;
+    for (int i = 0; i <= p - j - s; i++) {
+      float alpha = (u - UP[L + i]) / (UP[i + k + 1] - UP[L + i]);
+      Rw[i] = add4(mul4(Rw[i + 1], alpha), mul4(Rw[i], 1.0f - alpha));
+    }
+    Qw[L] = Rw[0];
+    Qw[k + r - j - s] = Rw[p - j - s];
+  }
+  for (int i = L + 1; i < k - s; i++) Qw[i] = Rw[i - L];
+  return 0;
+}
+
+static int multiplicity(const float* U, int k, float u) {
+  int s = 0;
+  for (int i = k; i >= 0 && U[i] == u; --i) ++s;
+  return s;
+}
+
+// knotsToAdd (nurbs.h:255-281)
+static int knots_to_add(const v3<float>* P, const float* U, int i, float C, int p) {
+  float vsum = 0;
+  v3<float> Vj[MAXDEG + 1];
+  const int base = i - p + 1;
+  for (int j = i - p + 1; j <= i; j++) {
+    float d = U[j + p] - U[j];
+    Vj[j - base] = (d > 0.0f) ? mul3(sub3(P[j], P[j - 1]), (float)p / d) : v3<float>{0, 0, 0};
+    vsum += len3(Vj[j - base]);
+  }
+  float amod = 0;
+  for (int j = i - p + 2; j <= i; j++) {
+    float d = U[j + p - 1] - U[j];
+    v3<float> A = (d > 0.0f) ? mul3(sub3(Vj[j - base], Vj[j - 1 - base]), (float)(p - 1) / d) : v3<float>{0, 0, 0};
+    amod = std::fmax(amod, len3(A));
+  }
+  float num = amod * powf(U[i + 1] - U[i], 1.5f);
+  float den = sqrtf(vsum / p);
+  if (!(den > 1e-9f)) return 0;
+  float nf = C * num / den;
+  if (!(nf > 1e-9f)) return 0;
+  return (int)nf;
+}
+
+// curvatureRefinementU/V (nurbs.h:290-486) on caller arrays.  dir 0 refines U
+// (rows of length m+1 are the curves' other index), dir 1 refines V.
+// Phase 1 (count): add_knots[n_spans] receives the per-span insert counts and
+// the function returns the total.  Phase 2 (apply, out_knots != null): writes
+// the refined knot vector and net (caller-sized: old size + total).
+int orc_refine(int dir, int n, int p, const float* U, int m, const float* Pw, float C, int* add_knots,
+               float* out_knots, float* out_Pw) {
+  const v4<float>* P = reinterpret_cast<const v4<float>*>(Pw);
+  int deg = p;
+  int nn = dir == 0 ? n : m;     // index of last cp along the refined direction
+  int other = dir == 0 ? m : n;  // index of last cp along the other direction
+  auto at = [&](int along, int c) -> const v4<float>& {
+    return dir == 0 ? P[(size_t)along * (m + 1) + c] : P[(size_t)c * (m + 1) + along];
+  };
+  std::vector<v3<float>> curve(nn + 1);
+  int total = 0;
+  for (int i = deg; i <= nn; i++) {
+    int ni = 0;
+    for (int c = 0; c <= other; c++) {
+      for (int a = 0; a <= nn; a++) {
+        const v4<float>& w = at(a, c);
+        curve[a] = v3<float>{w.x / w.w, w.y / w.w, w.z / w.w};
+      }
+      ni = std::max(ni, knots_to_add(curve.data(), U, i, C, deg));
+    }
+    add_knots[i - deg] = ni;
+    total += ni;
+  }
+  if (!out_knots) return total;
+  int nfin = nn + total;
+  std::vector<float> UA(nfin + deg + 2), UB(nfin + deg + 2);
+  std::memcpy(UA.data(), U, (nn + deg + 2) * sizeof(float));
+  // nets stored as [other][along] for convenience, stride nfin+1
+  std::vector<v4<float>> A((size_t)(other + 1) * (nfin + 1)), B((size_t)(other + 1) * (nfin + 1));
+  for (int c = 0; c <= other; c++)
+    for (int a = 0; a <= nn; a++) A[(size_t)c * (nfin + 1) + a] = at(a, c);
+  float *Usrc = UA.data(), *Udst = UB.data();
+  v4<float>*src = A.data(), *dst = B.data();
+  std::vector<v4<float>> cin(nfin + 1), cout_(nfin + 1);
+  int ncur = nn, nq = 0, rc = 0;
+  for (int i = deg; i <= nn; i++) {
+    int cnt = add_knots[i - deg];
+    if (cnt <= 0) continue;
+    float lo = U[i], hi = U[i + 1];
+    for (int j = 1; j <= cnt; j++) {
+      float u = lo + (hi - lo) * j / (float)(cnt + 1);
+      int k = orc_find_span(ncur, deg, u, Usrc);
+      int s = multiplicity(Usrc, k, u);
+      for (int c = 0; c <= other; c++) {
+        for (int a = 0; a <= ncur; a++) cin[a] = src[(size_t)c * (nfin + 1) + a];
+        if (curve_knot_insert(ncur, deg, Usrc, u, k, s, 1, nq, Udst, cin.data(), cout_.data()) != 0) rc = -1;
+        for (int a = 0; a <= ncur + 1; a++) dst[(size_t)c * (nfin + 1) + a] = cout_[a];
+      }
+      std::swap(src, dst);
+      std::swap(Usrc, Udst);
+      ncur++;
+    }
+  }
+  std::memcpy(out_knots, Usrc, (nfin + deg + 2) * sizeof(float));
+  v4<float>* O = reinterpret_cast<v4<float>*>(out_Pw);
+  for (int c = 0; c <= other; c++)
+    for (int a = 0; a <= nfin; a++) {
+      if (dir == 0) O[(size_t)a * (m + 1) + c] = src[(size_t)c * (nfin + 1) + a];
+      else O[(size_t)c * (nfin + 1) + a] = src[(size_t)c * (nfin + 1) + a];
+    }
+  return rc == 0 ? total : -1;
+}
+
+// bounds (bezier_patch.h:154-169): per-coordinate true division by w
+void orc_patch_bounds(const float* cp, int p, int q, float* out6) {
+  float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int i = 0; i < (p + 1) * (q + 1); i++)
+    for (int j = 0; j < 3; j++) {
+      float c = cp[4 * i + j] / cp[4 * i + 3];
+      mn[j] = fminf(mn[j], c);
+      mx[j] = fmaxf(mx[j], c);
+    }
+  out6[0] = mn[0]; out6[1] = mn[1]; out6[2] = mn[2];
+  out6[3] = mx[0]; out6[4] = mx[1]; out6[5] = mx[2];
+}
+
+// patchDelta (main.cpp:18-43)
+float orc_patch_delta(const float* cp, int p, int q) {
+  const v4<float>* P = reinterpret_cast<const v4<float>*>(cp);
+  auto deh = [&](int idx) { return dehom(P[idx]); };
+  int row = q + 1;
+  v3<float> c00 = deh(0), c10 = deh(p * row + 0), c01 = deh(q), c11 = deh(p * row + q);
+  float d = 0.0f;
+  for (int i = 0; i <= p; ++i)
+    for (int j = 0; j <= q; ++j) {
+      float u = (float)i / p, v = (float)j / q;
+      float B[3];
+      const float* a00 = &c00.x; const float* a10 = &c10.x; const float* a01 = &c01.x; const float* a11 = &c11.x;
+      for (int c = 0; c < 3; c++)
+        B[c] = (1 - u) * (1 - v) * a00[c] + u * (1 - v) * a10[c] + (1 - u) * v * a01[c] + u * v * a11[c];
+      v3<float> Pij = deh(i * row + j);
+      float dx = Pij.x - B[0], dy = Pij.y - B[1], dz = Pij.z - B[2];
+      d = fmaxf(d, sqrtf(dx * dx + dy * dy + dz * dz));
+    }
+  return d;
+}
+
+// -------------------------------------------------------- exhaustive SAH
+// BVHNode (bvh.h:13-17): 6 floats + left_first + prim_count (32 bytes)
+struct Node { float b[6]; uint32_t left_first, prim_count; };
+typedef float Box[6];
+
+static inline void box_union(float* r, const float* a, const float* b) {
+  r[0] = fminf(a[0], b[0]); r[1] = fminf(a[1], b[1]); r[2] = fminf(a[2], b[2]);
+  r[3] = fmaxf(a[3], b[3]); r[4] = fmaxf(a[4], b[4]); r[5] = fmaxf(a[5], b[5]);
+}
+static inline float centroid(const float* b, int axis) { return .5f * (b[axis] + b[axis + 3]); }
+// bound_area (bounds_ops.h:72-77)
+static inline float area(const float* b) {
+  float ex = b[3] - b[0], ey = b[4] - b[1], ez = b[5] - b[2];
+  return ex * ey + ey * ez + ez * ex;
+}
+
+struct Build { const float* bounds; uint32_t* idx; int used; };
+
+// evaluateSAH (bvh.h:47-69)
+static float eval_sah(const Build& B, const Node& nd, int axis, float pos) {
+  float lb[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  float rb[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  int lc = 0, rc = 0;
+  for (uint32_t i = 0; i < nd.prim_count; i++) {
+    const float* pb = B.bounds + 6 * (size_t)B.idx[nd.left_first + i];
+    if (centroid(pb, axis) < pos) { lc++; box_union(lb, lb, pb); }
+    else { rc++; box_union(rb, rb, pb); }
+  }
+  float cost = lc * area(lb) + rc * area(rb);
+  return cost > 0 ? cost : 1e30f;
+}
+
+static void update_bounds(const Build& B, Node* nodes, uint32_t ni) {
+  Node& nd = nodes[ni];
+  float bb[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  for (uint32_t i = nd.left_first; i < nd.left_first + nd.prim_count; i++) box_union(bb, bb, B.bounds + 6 * (size_t)B.idx[i]);
+  std::memcpy(nd.b, bb, sizeof(bb));
+}
+
+// subdivide (bvh.h:85-143)
+static void subdivide(Build& B, Node* nodes, uint32_t ni) {
+  Node& nd = nodes[ni];
+  int best_axis = 1;
+  float best_pos = 0, best_cost = 1e30f;
+  for (int axis = 0; axis < 3; axis++)
+    for (uint32_t i = 0; i < nd.prim_count; i++) {
+      float cand = centroid(B.bounds + 6 * (size_t)B.idx[nd.left_first + i], axis);
+      float cost = eval_sah(B, nd, axis, cand);
+      if (cost < best_cost) { best_pos = cand; best_axis = axis; best_cost = cost; }
+    }
+  int i = nd.left_first, j = i + nd.prim_count - 1;
+  while (i <= j) {
+    if (centroid(B.bounds + 6 * (size_t)B.idx[i], best_axis) < best_pos) i++;
+    else { std::swap(B.idx[i], B.idx[j]); j--; }
+  }
+  int lc = i - nd.left_first;
+  if (lc == 0 || lc == (int)nd.prim_count) return;
+  int l = B.used++, r = B.used++;
+  nodes[l].left_first = nd.left_first; nodes[l].prim_count = lc;
+  nodes[r].left_first = i; nodes[r].prim_count = nd.prim_count - lc;
+  nd.left_first = l; nd.prim_count = 0;
+  update_bounds(B, nodes, l);
+  update_bounds(B, nodes, r);
+  subdivide(B, nodes, l);
+  subdivide(B, nodes, r);
+}
+
+// buildBVH (bvh.h:148-179).  prim_bounds: 6 floats per primitive (from
+// orc_patch_bounds).  nodes: 2n-1 entries of 32 bytes (unused tail entries are
+// zeroed).  perm[i] = original primitive index stored at BVH position i.
+// Returns the number of nodes used.
+int orc_build_bvh(const float* prim_bounds, int n, void* nodes_out, uint32_t* perm) {
+  Node* nodes = reinterpret_cast<Node*>(nodes_out);
+  std::memset(nodes, 0, sizeof(Node) * (size_t)(2 * n - 1));
+  std::vector<uint32_t> idx(n);
+  Build B{prim_bounds, idx.data(), 1};
+  Node& root = nodes[0];
+  root.left_first = 0; root.prim_count = n;
+  float bb[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  for (int i = 0; i < n; i++) { idx[i] = i; box_union(bb, bb, prim_bounds + 6 * (size_t)i); }
+  std::memcpy(root.b, bb, sizeof(bb));
+  subdivide(B, nodes, 0);
+  for (int i = 0; i < n; i++) perm[i] = idx[i];
+  return B.used;
+}
+
+// ---------------------------------------------------- per-pixel traversal
+// Patches in BVH order: cp (float4 block), cp_off[i] (float4 index of patch i's
+// net), pdeg[i], qdeg[i].  nodes as orc_build_bvh.  Params: max_iter, thr_det,
+// thr_res (reference: 7, 1e-5, 1e-4), flat_threshold, cam_k, delta (may be null:
+// flat-skip test skipped).  RNG seed of ray r = r + seed_base (reference: pixel idx).
+// Outputs per ray (arrays of n_rays; any may be null):
+//   hit_prim (BVH-order index or -1), t, u, v, dpdu[3], dpdv[3], counters[8] =
+//   {newton_calls, newton_evals, skips, interior_visits, leaf_visits, patch_tests,
+//   stack_overflow, newton_hits}.
+struct TraceParams {
+  int max_iter;
+  double thr_det, thr_res, flat_threshold, cam_k;  // FP32 values are held exactly
+  uint32_t seed_base;
+};
+
+}  // extern "C"
+
+template <class T>
+static void trace_ray(long r, const Node* nodes, const v4<T>* cp, const int* cp_off, const int* pdeg, const int* qdeg,
+                      const float* delta, const TraceParams& prm, const float* ox, const float* oy, const float* oz,
+                      const float* dx, const float* dy, const float* dz, int* hit_prim, T* t_out, T* u_out, T* v_out,
+                      T* dpdu, T* dpdv, int64_t* counters) {
+  v3<T> o{T(ox[r]), T(oy[r]), T(oz[r])}, d{T(dx[r]), T(dy[r]), T(dz[r])};
+  int64_t nc = 0, ne = 0, sc = 0, niv = 0, nlv = 0, npt = 0, ovf = 0, nh = 0;
+  v3<T> rd{T(1) / d.x, T(1) / d.y, T(1) / d.z};
+  RayPlane<T> rp = ray_planes(o, d);
+  int stack[STACK_DEPTH];
+  int off = 0;
+  T t_max = INFINITY;
+  int best = -1;
+  T bt = 0, bu = 0, bv = 0;
+  v3<T> bsu{0, 0, 0}, bsv{0, 0, 0};
+  const Node* cur = &nodes[0];
+  T it = slab(cur->b, cur->b + 3, o, rd);
+  bool go = it < t_max;
+  const T thr_det = T(prm.thr_det), thr_res = T(prm.thr_res), flat = T(prm.flat_threshold), camk = T(prm.cam_k);
+  while (go) {
+    if (cur->prim_count > 0) {
+      nlv++;
+      it = slab(cur->b, cur->b + 3, o, rd);
+      for (uint32_t i = 0; i < cur->prim_count; i++) {
+        int pi = (int)(cur->left_first + i);
+        bool do_newton = true;
+        npt++;
+        const v4<T>* P = cp + cp_off[pi];
+        int p = pdeg[pi], q = qdeg[pi];
+        v3<T> q00 = dehom(P[0]), q10 = dehom(P[p * (q + 1)]), q11 = dehom(P[p * (q + 1) + q]), q01 = dehom(P[q]);
+        T gu, gv;
+        bilinear_seed(q00, q10, q11, q01, o, d, gu, gv);
+        if (delta && T(delta[pi]) * camk / it < flat) {
+          v4<T> du, dv, pt;
+          patch_eval(P, p, q, gu, gv, du, dv, pt);
+          v3<T> S, Su, Sv;
+          dehom_ders(pt, du, dv, S, Su, Sv);
+          T resid = std::fabs(dot3(rp.n1, S) + rp.d1) + std::fabs(dot3(rp.n2, S) + rp.d2);
+          if (resid * camk / it < flat) {
+            sc++;
+            T t = dot3(sub3(S, o), d);
+            if (0 < t && t < t_max) {
+              t_max = t; best = pi; bt = t; bu = gu; bv = gv; bsu = Su; bsv = Sv;
+              do_newton = false;
+            }
+          }
+        }
+        if (do_newton) {
+          nc++;
+          NewtonOut<T> nr = newton(rp, P, p, q, gu, gv, thr_det, thr_res, (uint32_t)r + prm.seed_base, prm.max_iter);
+          ne += nr.evals;
+          if (nr.hit) {
+            nh++;
+            T t = (nr.S.x - o.x) * d.x + (nr.S.y - o.y) * d.y + (nr.S.z - o.z) * d.z;
+            if (0 < t && t < t_max) {
+              best = pi; bt = t; bu = nr.u; bv = nr.v; bsu = nr.Su; bsv = nr.Sv;
+              t_max = t;
+            }
+          }
+        }
+      }
+      if (off == 0) break;
+      cur = &nodes[stack[--off]];
+    } else {
+      niv++;
+      const Node& L = nodes[cur->left_first];
+      const Node& R = nodes[cur->left_first + 1];
+      T lt = slab(L.b, L.b + 3, o, rd);
+      T rt = slab(R.b, R.b + 3, o, rd);
+      it = std::fmin(lt, rt);
+      if (it >= t_max) {
+        if (off == 0) break;
+        cur = &nodes[stack[--off]];
+      } else if (lt <= it) {
+        if (rt < t_max) {
+          if (off < STACK_DEPTH) stack[off++] = cur->left_first + 1; else ovf = 1;
+        }
+        cur = &nodes[cur->left_first];
+      } else {
+        if (lt < t_max) {
+          if (off < STACK_DEPTH) stack[off++] = cur->left_first; else ovf = 1;
+        }
+        cur = &nodes[cur->left_first + 1];
+      }
+    }
+  }
+  if (hit_prim) hit_prim[r] = best;
+  if (best >= 0) {
+    if (t_out) t_out[r] = bt;
+    if (u_out) u_out[r] = bu;
+    if (v_out) v_out[r] = bv;
+    if (dpdu) { dpdu[3 * r] = bsu.x; dpdu[3 * r + 1] = bsu.y; dpdu[3 * r + 2] = bsu.z; }
+    if (dpdv) { dpdv[3 * r] = bsv.x; dpdv[3 * r + 1] = bsv.y; dpdv[3 * r + 2] = bsv.z; }
+  }
+  if (counters) {
+    int64_t* c = counters + 8 * r;
+    c[0] = nc; c[1] = ne; c[2] = sc; c[3] = niv; c[4] = nlv; c[5] = npt; c[6] = ovf; c[7] = nh;
+  }
+}
+
+extern "C" {
+
+void orc_trace_f32(const void* nodes, const float* cp, const int* cp_off, const int* pdeg, const int* qdeg,
+                   const float* delta, int max_iter, float thr_det, float thr_res, float flat_threshold, float cam_k,
+                   uint32_t seed_base, long n_rays, const float* ox, const float* oy, const float* oz, const float* dx,
+                   const float* dy, const float* dz, int* hit_prim, float* t, float* u, float* v, float* dpdu,
+                   float* dpdv, int64_t* counters, int n_threads) {
+  TraceParams prm{max_iter, thr_det, thr_res, flat_threshold, cam_k, seed_base};
+  const Node* nd = reinterpret_cast<const Node*>(nodes);
+  const v4<float>* P = reinterpret_cast<const v4<float>*>(cp);
+#pragma omp parallel for schedule(dynamic, 256) num_threads(n_threads > 0 ? n_threads : 1)
+  for (long r = 0; r < n_rays; r++)
+    trace_ray<float>(r, nd, P, cp_off, pdeg, qdeg, delta, prm, ox, oy, oz, dx, dy, dz, hit_prim, t, u, v, dpdu, dpdv,
+                     counters);
+}
+
+// FP64 instantiation of the same traversal (the FP64 oracle): inputs are the
+// FP32 scene and rays widened exactly; thresholds as given.
+void orc_trace_f64(const void* nodes, const float* cp, long n_cp, const int* cp_off, const int* pdeg, const int* qdeg,
+                   int max_iter, double thr_det, double thr_res, uint32_t seed_base, long n_rays, const float* ox,
+                   const float* oy, const float* oz, const float* dx, const float* dy, const float* dz,
+                   int* hit_prim, double* t, double* u, double* v, double* dpdu, double* dpdv, int64_t* counters,
+                   int n_threads) {
+  std::vector<v4<double>> P(n_cp);
+  for (long i = 0; i < n_cp; i++) P[i] = {cp[4 * i], cp[4 * i + 1], cp[4 * i + 2], cp[4 * i + 3]};
+  const Node* nd = reinterpret_cast<const Node*>(nodes);
+  TraceParams prm{max_iter, thr_det, thr_res, 0, 0, seed_base};
+#pragma omp parallel for schedule(dynamic, 256) num_threads(n_threads > 0 ? n_threads : 1)
+  for (long r = 0; r < n_rays; r++)
+    trace_ray<double>(r, nd, P.data(), cp_off, pdeg, qdeg, nullptr, prm, ox, oy, oz, dx, dy, dz, hit_prim, t, u, v,
+                      dpdu, dpdv, counters);
+}
+
+// FP64 Newton polish of one (ray, patch) pair from (u,v): plain Newton with no
+// clamping to [0,1] and no RNG, run until the step is below 1e-15 or 50
+// iterations.  Gives the FP64 reference point for error reporting.  Returns the
+// number of iterations, or -1 on a singular Jacobian.  out: t, u, v, n[3] (unit
+// normal dpdu x dpdv), resid.
+int orc_polish_f64(const float* cp, int p, int q, const float* o3, const float* d3, double u, double v, double* out) {
+  std::vector<v4<double>> P((p + 1) * (q + 1));
+  for (int i = 0; i < (p + 1) * (q + 1); i++) P[i] = {cp[4 * i], cp[4 * i + 1], cp[4 * i + 2], cp[4 * i + 3]};
+  v3<double> o{o3[0], o3[1], o3[2]}, d{d3[0], d3[1], d3[2]};
+  RayPlane<double> rp = ray_planes(o, d);
+  int it = 0;
+  v3<double> S, Su, Sv;
+  double r0 = 0, r1 = 0;
+  for (it = 0; it < 50; it++) {
+    v4<double> du, dv, pt;
+    patch_eval(P.data(), p, q, u, v, du, dv, pt);
+    dehom_ders(pt, du, dv, S, Su, Sv);
+    r0 = dot3(rp.n1, S) + rp.d1;
+    r1 = dot3(rp.n2, S) + rp.d2;
+    double m0 = dot3(rp.n1, Su), m1 = dot3(rp.n2, Su), m2 = dot3(rp.n1, Sv), m3 = dot3(rp.n2, Sv);
+    double det = m0 * m3 - m2 * m1;
+    if (det == 0) return -1;
+    double su = (m3 * r0 - m2 * r1) / det, sv = (-m1 * r0 + m0 * r1) / det;
+    u -= su; v -= sv;
+    if (std::fabs(su) + std::fabs(sv) < 1e-15) { it++; break; }
+  }
+  v4<double> du, dv, pt;
+  patch_eval(P.data(), p, q, u, v, du, dv, pt);
+  dehom_ders(pt, du, dv, S, Su, Sv);
+  double dn = std::sqrt(dot3(d, d));
+  out[0] = dot3(sub3(S, o), d) / (dn * dn);
+  out[1] = u; out[2] = v;
+  v3<double> n = cross3(Su, Sv);
+  double nl = std::sqrt(dot3(n, n));
+  out[3] = n.x / nl; out[4] = n.y / nl; out[5] = n.z / nl;
+  out[6] = std::fabs(dot3(rp.n1, S) + rp.d1) + std::fabs(dot3(rp.n2, S) + rp.d2);
+  return it;
+}
+
+// Single-patch building blocks exported for unit tests.
+void orc_patch_eval_f32(const float* cp, int p, int q, float u, float v, float* out12) {
+  v4<float> du, dv, pt;
+  patch_eval(reinterpret_cast<const v4<float>*>(cp), p, q, u, v, du, dv, pt);
+  float r[12] = {pt.x, pt.y, pt.z, pt.w, du.x, du.y, du.z, du.w, dv.x, dv.y, dv.z, dv.w};
+  std::memcpy(out12, r, sizeof(r));
+}
+void orc_ray_planes_f32(const float* o3, const float* d3, float* out8) {
+  RayPlane<float> rp = ray_planes(v3<float>{o3[0], o3[1], o3[2]}, v3<float>{d3[0], d3[1], d3[2]});
+  float r[8] = {rp.n1.x, rp.n1.y, rp.n1.z, rp.n2.x, rp.n2.y, rp.n2.z, rp.d1, rp.d2};
+  std::memcpy(out8, r, sizeof(r));
+}
+float orc_slab_f32(const float* b6, const float* o3, const float* d3) {
+  v3<float> o{o3[0], o3[1], o3[2]};
+  v3<float> rd{1 / d3[0], 1 / d3[1], 1 / d3[2]};
+  return slab(b6, b6 + 3, o, rd);
+}
+void orc_bilinear_seed_f32(const float* cp, int p, int q, const float* o3, const float* d3, float* out2) {
+  const v4<float>* P = reinterpret_cast<const v4<float>*>(cp);
+  v3<float> q00 = dehom(P[0]), q10 = dehom(P[p * (q + 1)]), q11 = dehom(P[p * (q + 1) + q]), q01 = dehom(P[q]);
+  bilinear_seed(q00, q10, q11, q01, v3<float>{o3[0], o3[1], o3[2]}, v3<float>{d3[0], d3[1], d3[2]}, out2[0], out2[1]);
+}
+uint32_t orc_pcg_hash(uint32_t x) { return pcg_hash(x); }
+int orc_newton_f32(const float* cp, int p, int q, const float* o3, const float* d3, float u0, float v0, float thr_det,
+                   float thr_res, uint32_t seed, int max_iter, float* out11) {
+  RayPlane<float> rp = ray_planes(v3<float>{o3[0], o3[1], o3[2]}, v3<float>{d3[0], d3[1], d3[2]});
+  NewtonOut<float> r = newton(rp, reinterpret_cast<const v4<float>*>(cp), p, q, u0, v0, thr_det, thr_res, seed, max_iter);
+  float o[11] = {r.S.x, r.S.y, r.S.z, r.Su.x, r.Su.y, r.Su.z, r.Sv.x, r.Sv.y, r.Sv.z, r.u, r.v};
+  std::memcpy(out11, o, sizeof(o));
+  return r.hit ? r.evals : -r.evals;
+}
+
+}  // extern "C"
