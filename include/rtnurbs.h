@@ -1,0 +1,228 @@
+/*
+ * rtnurbs.h -- C ABI of the MI355X (gfx950) NURBS ray-tracing path.
+ *
+ * Plain C: POD structs, raw pointers, sizes and int status codes.  Every
+ * entry point names the reference interface (Frostbite16/rt-nurbs, file:line)
+ * whose behaviour it provides.  The C++ drop-in layer with the reference's own
+ * names and signatures is include/rtnurbs.hpp, built on these functions.
+ *
+ * Conventions
+ *   - Status: RTN_OK (0) or a negative RTN_E* code; rtn_last_error() gives the
+ *     message of the calling thread's last failure.
+ *   - "device" pointers are HIP device (or managed) allocations, e.g. from
+ *     rtn_malloc; "host" pointers are ordinary host memory.
+ *   - stream arguments are hipStream_t passed as void*; NULL is the default
+ *     stream.  Functions without a stream argument are synchronous, like the
+ *     reference wrappers (intersect.cu:218, generateRaysKernel.cu:61,66).
+ */
+#ifndef RTNURBS_H
+#define RTNURBS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  RTN_OK = 0,
+  RTN_EINVAL = -1,      /* invalid argument (sizes, degrees, null pointers)  */
+  RTN_EHIP = -2,        /* HIP runtime error                                 */
+  RTN_ENOMEM = -3,      /* allocation failure                                */
+  RTN_EUNSUPPORTED = -4,/* outside the supported envelope (e.g. degree > 5)  */
+  RTN_EOVERFLOW = -5    /* traversal stack overflowed on some ray            */
+};
+
+#define RTN_MAX_DEGREE 5   /* reference: NURBS_MAX_DEGREE = 4 (constants.h:31) */
+
+/* ------------------------------------------------------------------ layouts
+ * Bit-compatible with the reference's structs so reference-side buffers can
+ * be passed unchanged. */
+typedef struct { float v[4]; } rtn_vec4;                 /* MathTypes::vector4f, math_types.h:36 */
+typedef struct {                                         /* BVHNode, bvh.h:13-17 (32 bytes)       */
+  float p_min_x, p_min_y, p_min_z, p_max_x, p_max_y, p_max_z;
+  uint32_t left_first, prim_count;
+} rtn_bvh_node;
+typedef struct {                                         /* bezierPatch, bezier_patch.h:10-16     */
+  rtn_vec4* control_point;
+  float u0, u1, v0, v1;
+  int p, q;
+  rtn_vec4* heap_pointer;
+} rtn_bezier_patch;
+typedef struct {                                         /* surfaceInteraction, gpu/interact.h:6-21 */
+  float t_hit;
+  float wo[3];
+  float uv[2];
+  float dpdu[3], dpdv[3], dndu[3], dndv[3];
+} rtn_surface_interaction;
+typedef struct {                                         /* SoAQueue::ray_queue, soa.h:6-14       */
+  float *origin_x, *origin_y, *origin_z;
+  float *direction_x, *direction_y, *direction_z;
+  int* idx_interact;
+} rtn_ray_queue;
+
+const char* rtn_last_error(void);
+int rtn_version(void);
+
+/* ------------------------------------------------------------------ memory
+ * CudaDevAlloc / CudaAllocManaged / CudaFree (memory.h:32-57).  Unlike the
+ * reference these return a status instead of printing and continuing. */
+int rtn_malloc(void** ptr, size_t bytes);
+int rtn_malloc_managed(void** ptr, size_t bytes);
+int rtn_free(void* ptr);
+/* kind: 0 host->device, 1 device->host, 2 device->device, 3 inferred */
+int rtn_memcpy(void* dst, const void* src, size_t bytes, int kind);
+int rtn_memset(void* dst, int value, size_t bytes);
+int rtn_device_synchronize(void);
+int rtn_stream_create(void** stream);
+int rtn_stream_destroy(void* stream);
+int rtn_stream_synchronize(void* stream);
+
+/* ------------------------------------------------------------------ camera
+ * Transform::lookAt (transform.h:46-60) + CameraOps::init_camera and
+ * getPerspectiveCamera (camera.h:63-141), host-side as in the reference.
+ * fov_deg in degrees (camera.h:69 applies radians()). */
+int rtn_camera_perspective(const float eye[3], const float look[3], const float up[3], int W, int H,
+                           float fov_deg, float znear, float zfar, float world_from_camera[16],
+                           float camera_from_raster[16]);
+
+/* calculateRaysWrapper<perspective> (generateRaysKernel.cu:39-67; ray math
+ * camera.h:162-174).  Writes W*H*spp rays into the device SoA queue, ray
+ * index = (y*W + x)*spp + s, and idx_interact = -1.  jitter = 0: pixel
+ * centres (x+0.5, y+0.5), the reference's rays; jitter = 1: per-ray raster
+ * offsets from rtn_jitter (an extension for multi-sample configurations). */
+int rtn_rays_perspective(const float world_from_camera[16], const float camera_from_raster[16], int W, int H,
+                         int spp, int jitter, uint32_t jitter_seed, const rtn_ray_queue* rays, void* stream);
+
+/* ----------------------------------------------------------- scene (GPU)
+ * Scene assembly for the trace path: NURBS surfaces are decomposed on the GPU
+ * into rational Bezier patches (getPatch, bezier_patch.h:87-139, A5.7 of
+ * nurbs.h:144-250), bounded (bezier_patch.h:154-169) and indexed by the
+ * exhaustive-SAH BVH of buildBVH (bvh.h:148-179), built on the GPU with the
+ * reference's node numbering and primitive order. */
+typedef struct rtn_scene rtn_scene;
+
+int rtn_scene_create(rtn_scene** scene);
+int rtn_scene_destroy(rtn_scene* scene);
+/* Copies one nurbsSurf (nurbs.h:17-22): U[n+p+2], V[m+q+2], Pw[(n+1)(m+1)]
+ * homogeneous, u-major.  Degrees 1..RTN_MAX_DEGREE. */
+int rtn_scene_add_nurbs(rtn_scene* scene, int n, int p, const float* U, int m, int q, const float* V,
+                        const float* Pw);
+/* Adds already-decomposed patches of one degree: cp[np*(p+1)*(q+1)*4]. */
+int rtn_scene_add_patches(rtn_scene* scene, int np, int p, int q, const float* cp);
+/* Runs decomposition, bounds and the BVH build on the device. */
+int rtn_scene_build(rtn_scene* scene);
+
+typedef struct {
+  int64_t n_surfaces, n_patches, n_cp, n_nodes, n_wide_nodes, max_depth, n_leaves;
+  int64_t deg_hist[RTN_MAX_DEGREE + 1][RTN_MAX_DEGREE + 1]; /* patches per (p,q) */
+  double build_ms_decompose, build_ms_bvh, build_ms_wide;
+} rtn_scene_stats;
+int rtn_scene_stats_get(const rtn_scene* scene, rtn_scene_stats* out);
+
+/* Copies scene arrays to host memory (any pointer may be NULL):
+ *   patch_cp   float[n_cp*4]     nets in BVH order (patch-major, row-major)
+ *   patch_off  int[n_patches]    float4 offset of each net
+ *   patch_pq   int[2*n_patches]  (p,q)
+ *   patch_uv   float[4*n_patches] (u0,u1,v0,v1) of the patch on its surface
+ *   patch_id   int[n_patches]    original id: surface offset + i*nbV + j
+ *   knot_idx   int[2*n_patches]  span-start knot index (u,v) of the patch
+ *   nodes      rtn_bvh_node[n_nodes] reference layout (bvh.h:13-17)
+ *   delta      float[n_patches]  patchDelta flatness (main.cpp:18-43), BVH order */
+int rtn_scene_download(const rtn_scene* scene, float* patch_cp, int* patch_off, int* patch_pq, float* patch_uv,
+                       int* patch_id, int* knot_idx, rtn_bvh_node* nodes, float* delta);
+
+/* --------------------------------------------------------------- tracing
+ * Plain method (intersect.cu:33-193): per-ray ordered closest-hit traversal,
+ * bilinear seed (bezier_patch.h:273-327), FP32 Newton (bezier_patch.h:198-266).
+ * With RTN_MODE_EXACT the FP32 operation order is the reference's (no FMA),
+ * so results equal the host reference bit for bit. */
+enum {
+  RTN_MODE_EXACT = 0,     /* reference binary BVH, reference op order            */
+  RTN_MODE_WIDE = 1,      /* 4-wide SoA BVH, FMA arithmetic, degree-binned Newton */
+  RTN_MODE_PERSISTENT = 2 /* persistent work-queue kernel over the wide BVH       */
+};
+enum {
+  RTN_SEED_PLAIN = 0,     /* the reference's bilinear seed                      */
+  RTN_SEED_LEARNED = 1    /* MLP seed, plain seed as fallback                    */
+};
+enum {
+  RTN_PREC_FP32 = 0,      /* FP32 Newton                                         */
+  RTN_PREC_MIXED = 1      /* FP32 Newton + FP64 step when cond(J) > cond_thr     */
+};
+
+typedef struct {
+  int mode;               /* RTN_MODE_*                                          */
+  int max_iter;           /* reference NEWTON_MAX_ITERATIONS = 7 (constants.h:40)*/
+  float thr_det;          /* reference 1e-5 (intersect.cu:127)                   */
+  float thr_res;          /* reference 1e-4 (intersect.cu:127)                   */
+  float flat_threshold;   /* reference flat-skip threshold in pixels (0 = off)   */
+  float cam_k;            /* 0.5*H/tan(vfov/2) (intersect.cu:212)                */
+  uint32_t seed_base;     /* Newton RNG seed of ray r = r + seed_base            */
+  int seed_mode;          /* RTN_SEED_*                                          */
+  int precision;          /* RTN_PREC_*                                          */
+  float cond_thr;         /* condition-number threshold of the FP64 step         */
+  int hull_cull;          /* 1: skip Newton when the net lies off the ray planes */
+  int count;              /* 1: fill the per-ray counters                        */
+  int reserved[6];
+} rtn_trace_params;
+
+void rtn_trace_params_default(rtn_trace_params* p);
+
+/* Device outputs, n_rays each (any may be NULL except prim):
+ *   prim   int     hit patch position in BVH order, -1 on a miss
+ *   t,u,v  float   hit distance (S-o).d and local patch (u,v)
+ *   dpdu, dpdv float[3*n]  surface derivatives at the hit
+ *   counters uint32[8*n]: newton calls, surface evaluations, flat skips,
+ *            interior visits, leaf visits, patch tests, stack overflow,
+ *            FP64 corrections */
+typedef struct {
+  int* prim;
+  float *t, *u, *v;
+  float *dpdu, *dpdv;
+  uint32_t* counters;
+} rtn_hits;
+
+/* Totals accumulated over a trace (device atomics; host-readable after sync). */
+typedef struct {
+  unsigned long long newton_calls, newton_evals, skips, interior, leaves, patch_tests, overflow, fp64_steps;
+} rtn_trace_totals;
+
+int rtn_trace(const rtn_scene* scene, const float* const rays_soa[6], int64_t n_rays, const rtn_trace_params* prm,
+              const rtn_hits* hits, rtn_trace_totals* totals_host, void* stream);
+
+/* calculateIntersectWrapper (intersect.cu:199-219; declared intersect.h:9-13)
+ * on reference-layout buffers: nodes[2N-1], patches[N] whose control_point
+ * pointers are device-visible, delta[N] in BVH order, interact[W*H].  As in
+ * the reference the 8th argument is the vertical field of view in radians,
+ * the call is synchronous, record fields are written only on a hit, and
+ * idx_interact[idx] = idx marks hits.  Counters (may be NULL) are added to. */
+int rtn_calculate_intersect(const rtn_ray_queue* rays, const rtn_bvh_node* nodes, const rtn_bezier_patch* patches,
+                            rtn_surface_interaction* interact, int W, int H, float flat_threshold, float v_fov,
+                            const float* delta, unsigned long long* newton_count, unsigned long long* skip_count);
+
+/* getPatch (bezier_patch.h:87-139) with the A5.7 passes run on the GPU.
+ * Host in, host out.  First call with cp == NULL to get nbU, nbV. */
+int rtn_get_patch(int n, int p, const float* U, int m, int q, const float* V, const float* Pw, int* nbU, int* nbV,
+                  float* cp, float* uv_range);
+
+/* buildBVH (bvh.h:148-179) on the GPU.  bounds: host float[6*np] (min xyz,
+ * max xyz per primitive, as BezierPatchOps::bounds).  nodes: host
+ * rtn_bvh_node[2np-1]; perm[i] = primitive stored at BVH position i. */
+int rtn_build_bvh(const float* bounds, int np, rtn_bvh_node* nodes, uint32_t* perm, int* nodes_used);
+
+/* BezierPatchOps::bounds (bezier_patch.h:154-169) for patches of one degree,
+ * on the GPU: cp host float[np*(p+1)(q+1)*4] -> bounds host float[6*np]. */
+int rtn_patch_bounds(const float* cp, int np, int p, int q, float* bounds);
+
+/* NurbsOps::curvatureRefinementU/V (nurbs.h:290-486), host-side as in the
+ * reference.  dir 0 refines U, 1 refines V.  Call with knots_out == NULL to
+ * get the number of inserted knots; then with arrays sized for the result. */
+int rtn_refine(int dir, int n, int p, const float* U, int m, const float* Pw, float C, int* inserted,
+               float* knots_out, float* Pw_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RTNURBS_H */

@@ -1,0 +1,226 @@
+// Core C-ABI: error state, memory, streams, camera and primary-ray generation.
+#include <cmath>
+#include <cstring>
+
+#include "rtn_exact.hpp"
+#include "rtn_internal.hpp"
+
+namespace rtn {
+
+static thread_local std::string g_err;
+
+void set_error(const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+}
+
+int hip_fail(hipError_t e, const char* what, const char* file, int line) {
+  set_error("%s:%d %s: %s", file, line, what, hipGetErrorString(e));
+  (void)hipGetLastError();
+  return e == hipErrorOutOfMemory ? RTN_ENOMEM : RTN_EHIP;
+}
+
+}  // namespace rtn
+
+using namespace rtn;
+
+extern "C" const char* rtn_last_error(void) { return g_err.c_str(); }
+extern "C" int rtn_version(void) { return 1; }
+
+// ----------------------------------------------------------------- memory
+extern "C" int rtn_malloc(void** ptr, size_t bytes) {
+  RTN_CHECK(ptr, RTN_EINVAL, "rtn_malloc: null out pointer");
+  RTN_HIP(hipMalloc(ptr, bytes ? bytes : 1));
+  return RTN_OK;
+}
+extern "C" int rtn_malloc_managed(void** ptr, size_t bytes) {
+  RTN_CHECK(ptr, RTN_EINVAL, "rtn_malloc_managed: null out pointer");
+  RTN_HIP(hipMallocManaged(ptr, bytes ? bytes : 1, hipMemAttachGlobal));
+  return RTN_OK;
+}
+extern "C" int rtn_free(void* ptr) {
+  if (ptr) RTN_HIP(hipFree(ptr));
+  return RTN_OK;
+}
+extern "C" int rtn_memcpy(void* dst, const void* src, size_t bytes, int kind) {
+  static const hipMemcpyKind kinds[4] = {hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice,
+                                         hipMemcpyDefault};
+  RTN_CHECK(kind >= 0 && kind < 4, RTN_EINVAL, "rtn_memcpy: bad kind %d", kind);
+  if (bytes) RTN_HIP(hipMemcpy(dst, src, bytes, kinds[kind]));
+  return RTN_OK;
+}
+extern "C" int rtn_memset(void* dst, int value, size_t bytes) {
+  if (bytes) RTN_HIP(hipMemset(dst, value, bytes));
+  return RTN_OK;
+}
+extern "C" int rtn_device_synchronize(void) {
+  RTN_HIP(hipDeviceSynchronize());
+  return RTN_OK;
+}
+extern "C" int rtn_stream_create(void** stream) {
+  RTN_CHECK(stream, RTN_EINVAL, "rtn_stream_create: null out pointer");
+  hipStream_t s;
+  RTN_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = (void*)s;
+  return RTN_OK;
+}
+extern "C" int rtn_stream_destroy(void* stream) {
+  if (stream) RTN_HIP(hipStreamDestroy((hipStream_t)stream));
+  return RTN_OK;
+}
+extern "C" int rtn_stream_synchronize(void* stream) {
+  RTN_HIP(hipStreamSynchronize((hipStream_t)stream));
+  return RTN_OK;
+}
+
+// ----------------------------------------------------------------- camera
+// Column-major 4x4 helpers in the reference's order (matrix_ops.h:101-171,
+// transform.h:14-38).
+namespace {
+void mat_mul(const float* a, const float* b, float* r) {
+  float out[16];
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++) {
+      float sum = 0;
+      for (int k = 0; k < 4; k++) sum += a[k * 4 + i] * b[j * 4 + k];
+      out[j * 4 + i] = sum;
+    }
+  memcpy(r, out, sizeof(out));
+}
+float det3(float m0, float m1, float m2, float m3, float m4, float m5, float m6, float m7, float m8) {
+  return m0 * m4 * m8 + m3 * m7 * m2 + m6 * m1 * m5 - m6 * m4 * m2 - m3 * m1 * m8 - m0 * m7 * m5;
+}
+void mat_inverse(const float* m, float* r) {
+  float a[16];
+  a[0] = det3(m[5], m[6], m[7], m[9], m[10], m[11], m[13], m[14], m[15]);
+  a[4] = -det3(m[4], m[6], m[7], m[8], m[10], m[11], m[12], m[14], m[15]);
+  a[8] = det3(m[4], m[5], m[7], m[8], m[9], m[11], m[12], m[13], m[15]);
+  a[12] = -det3(m[4], m[5], m[6], m[8], m[9], m[10], m[12], m[13], m[14]);
+  a[1] = -det3(m[1], m[2], m[3], m[9], m[10], m[11], m[13], m[14], m[15]);
+  a[5] = det3(m[0], m[2], m[3], m[8], m[10], m[11], m[12], m[14], m[15]);
+  a[9] = -det3(m[0], m[1], m[3], m[8], m[9], m[11], m[12], m[13], m[15]);
+  a[13] = det3(m[0], m[1], m[2], m[8], m[9], m[10], m[12], m[13], m[14]);
+  a[2] = det3(m[1], m[2], m[3], m[5], m[6], m[7], m[13], m[14], m[15]);
+  a[6] = -det3(m[0], m[2], m[3], m[4], m[6], m[7], m[12], m[14], m[15]);
+  a[10] = det3(m[0], m[1], m[3], m[4], m[5], m[7], m[12], m[13], m[15]);
+  a[14] = -det3(m[0], m[1], m[2], m[4], m[5], m[6], m[12], m[13], m[14]);
+  a[3] = -det3(m[1], m[2], m[3], m[5], m[6], m[7], m[9], m[10], m[11]);
+  a[7] = det3(m[0], m[2], m[3], m[4], m[6], m[7], m[8], m[10], m[11]);
+  a[11] = -det3(m[0], m[1], m[3], m[4], m[5], m[7], m[8], m[9], m[11]);
+  a[15] = det3(m[0], m[1], m[2], m[4], m[5], m[6], m[8], m[9], m[10]);
+  float det = m[0] * a[0] + m[1] * a[4] + m[2] * a[8] + m[3] * a[12];
+  float ad = det < 0.0f ? -det : det;
+  if (ad < 1e-8f) {  // identity fallback (matrix_ops.h:159-161)
+    for (int i = 0; i < 16; i++) r[i] = (i % 5 == 0) ? 1.0f : 0.0f;
+    return;
+  }
+  float di = 1.0f / det;
+  for (int i = 0; i < 16; i++) r[i] = a[i] * di;
+}
+void mat_diag(float sx, float sy, float sz, float* m) {
+  memset(m, 0, 64);
+  m[0] = sx; m[5] = sy; m[10] = sz; m[15] = 1.0f;
+}
+void mat_translate(float tx, float ty, float tz, float* m) {
+  mat_diag(1.0f, 1.0f, 1.0f, m);
+  m[12] = tx; m[13] = ty; m[14] = tz;
+}
+}  // namespace
+
+extern "C" int rtn_camera_perspective(const float eye[3], const float look[3], const float up[3], int W, int H,
+                                      float fov_deg, float znear, float zfar, float wfc[16], float cfr[16]) {
+  RTN_CHECK(eye && look && up && wfc && cfr && W > 0 && H > 0, RTN_EINVAL, "rtn_camera_perspective: bad arguments");
+  using namespace rtn::ex;
+  // lookAt (transform.h:46-60)
+  v3 pos{eye[0], eye[1], eye[2]};
+  v3 dir = normalize(sub(v3{look[0], look[1], look[2]}, pos));
+  v3 right = normalize(cross(v3{up[0], up[1], up[2]}, dir));
+  v3 nu = cross(dir, right);
+  const float m[16] = {right.x, right.y, right.z, 0.0f, nu.x, nu.y, nu.z, 0.0f,
+                       dir.x,   dir.y,   dir.z,   0.0f, pos.x, pos.y, pos.z, 1.0f};
+  memcpy(wfc, m, sizeof(m));
+  // init_camera / initCameraTransforms (camera.h:89-141)
+  float aspect = float(W) / float(H);
+  float wminx, wminy, wmaxx, wmaxy;
+  if (aspect > 1.0f) { wminx = -aspect; wminy = -1.0f; wmaxx = aspect; wmaxy = 1.0f; }
+  else { wminx = -1.0f; wminy = -1.0f / aspect; wmaxx = 1.0f; wmaxy = 1.0f / aspect; }
+  float s[16], t[16], ndc_from_screen[16], raster_from_ndc[16], raster_from_screen[16], screen_from_raster[16];
+  mat_diag(1 / (wmaxx - wminx), 1 / (wmaxy - wminy), 1, s);
+  mat_translate(-wminx, -wmaxy, 0, t);
+  mat_mul(s, t, ndc_from_screen);
+  mat_diag(float(W), float(-H), 1, raster_from_ndc);
+  mat_mul(raster_from_ndc, ndc_from_screen, raster_from_screen);
+  mat_inverse(raster_from_screen, screen_from_raster);
+  // getPerspectiveCamera (camera.h:63-85)
+  const float f = zfar, n = znear;
+  float pt[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, f / (f - n), 1, 0, 0, -f * n / (f - n), 0};
+  const float PI = 3.14159265f;
+  float inv_tan = 1.0f / tanf(((PI / 180) * fov_deg) / 2);
+  float sc[16], screen_from_camera[16], inv_sfc[16];
+  mat_diag(inv_tan, inv_tan, 1, sc);
+  mat_mul(sc, pt, screen_from_camera);
+  mat_inverse(screen_from_camera, inv_sfc);
+  mat_mul(inv_sfc, screen_from_raster, cfr);
+  return RTN_OK;
+}
+
+// ------------------------------------------------------------------ rays
+struct CamPayload {  // camera_gpu_payload (camera.h:32-35)
+  float wfc[16];
+  float cfr[16];
+};
+
+// calculateRays<perspective_projection> (generateRaysKernel.cu:15-35,
+// camera.h:162-174).  One thread per ray, 1D over W*H*spp; rays of a pixel are
+// consecutive.  The camera travels as a kernel argument (SGPR-resident)
+// instead of a process-global __constant__ symbol, so calls are re-entrant.
+__global__ void __launch_bounds__(256) k_rays_perspective(CamPayload cam, int W, int H, int spp, int jit, uint32_t seed,
+                                                          rtn_ray_queue q) {
+  const int64_t n = (int64_t)W * H * spp;
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int64_t pix = r / spp;
+  const int x = (int)(pix % W), y = (int)(pix / W);
+  const float ox = jit ? ex::jitter((uint32_t)r, seed, 0) : 0.5f;
+  const float oy = jit ? ex::jitter((uint32_t)r, seed, 1) : 0.5f;
+  const float e0 = float(x) + ox, e1 = float(y) + oy, e2 = 0.0f, e3 = 1.0f;
+  const float* c = cam.cfr;
+  float px = c[0] * e0 + c[4] * e1 + c[8] * e2 + c[12] * e3;
+  float py = c[1] * e0 + c[5] * e1 + c[9] * e2 + c[13] * e3;
+  float pz = c[2] * e0 + c[6] * e1 + c[10] * e2 + c[14] * e3;
+  ex::v3 pc = ex::normalize(ex::v3{px, py, pz});
+  const float* w = cam.wfc;
+  float dx = w[0] * pc.x + w[4] * pc.y + w[8] * pc.z + w[12] * 0.0f;
+  float dy = w[1] * pc.x + w[5] * pc.y + w[9] * pc.z + w[13] * 0.0f;
+  float dz = w[2] * pc.x + w[6] * pc.y + w[10] * pc.z + w[14] * 0.0f;
+  q.origin_x[r] = w[12];
+  q.origin_y[r] = w[13];
+  q.origin_z[r] = w[14];
+  q.direction_x[r] = dx;
+  q.direction_y[r] = dy;
+  q.direction_z[r] = dz;
+  if (q.idx_interact) q.idx_interact[r] = -1;
+}
+
+extern "C" int rtn_rays_perspective(const float wfc[16], const float cfr[16], int W, int H, int spp, int jitter,
+                                    uint32_t jitter_seed, const rtn_ray_queue* rays, void* stream) {
+  RTN_CHECK(wfc && cfr && rays && W > 0 && H > 0 && spp > 0, RTN_EINVAL, "rtn_rays_perspective: bad arguments");
+  RTN_CHECK(rays->origin_x && rays->origin_y && rays->origin_z && rays->direction_x && rays->direction_y &&
+                rays->direction_z,
+            RTN_EINVAL, "rtn_rays_perspective: null ray buffer");
+  const int64_t n = (int64_t)W * H * spp;
+  RTN_CHECK(n < (int64_t)1 << 32, RTN_EINVAL, "rtn_rays_perspective: too many rays");
+  CamPayload cam;
+  memcpy(cam.wfc, wfc, 64);
+  memcpy(cam.cfr, cfr, 64);
+  const int64_t blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(k_rays_perspective, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, cam, W, H, spp,
+                     jitter, jitter_seed, *rays);
+  RTN_HIP(hipGetLastError());
+  if (!stream) RTN_HIP(hipStreamSynchronize(nullptr));
+  return RTN_OK;
+}

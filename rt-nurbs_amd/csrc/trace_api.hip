@@ -1,0 +1,76 @@
+// rtn_trace: dispatch over the trace forms (exact / wide / persistent).
+#include <cstring>
+
+#include "rtn_internal.hpp"
+
+namespace rtn {
+int trace_exact_scene(const rtn_scene* s, const float* const rays[6], int64_t n, const rtn_trace_params* prm,
+                      const rtn_hits* hits, unsigned long long* d_totals, hipStream_t stream);
+int trace_wide_scene(const rtn_scene* s, const float* const rays[6], int64_t n, const rtn_trace_params* prm,
+                     const rtn_hits* hits, unsigned long long* d_totals, hipStream_t stream);
+}  // namespace rtn
+
+using namespace rtn;
+
+extern "C" void rtn_trace_params_default(rtn_trace_params* p) {
+  if (!p) return;
+  memset(p, 0, sizeof(*p));
+  p->mode = RTN_MODE_EXACT;
+  p->max_iter = 7;     // NEWTON_MAX_ITERATIONS (constants.h:40)
+  p->thr_det = 1e-5f;  // intersect.cu:127
+  p->thr_res = 1e-4f;
+  p->flat_threshold = 0.f;
+  p->cam_k = 0.f;
+  p->seed_base = 0;
+  p->seed_mode = RTN_SEED_PLAIN;
+  p->precision = RTN_PREC_FP32;
+  p->cond_thr = 1e4f;
+  p->hull_cull = 0;
+  p->count = 0;
+}
+
+extern "C" int rtn_trace(const rtn_scene* s, const float* const rays[6], int64_t n_rays, const rtn_trace_params* prm,
+                         const rtn_hits* hits, rtn_trace_totals* totals, void* stream) {
+  RTN_CHECK(s && s->built, RTN_EINVAL, "rtn_trace: scene not built");
+  RTN_CHECK(rays && prm && hits && hits->prim, RTN_EINVAL, "rtn_trace: null argument");
+  for (int k = 0; k < 6; k++) RTN_CHECK(rays[k], RTN_EINVAL, "rtn_trace: null ray component %d", k);
+  RTN_CHECK(n_rays >= 0 && n_rays < ((int64_t)1 << 32), RTN_EINVAL, "rtn_trace: bad ray count");
+  RTN_CHECK(prm->max_iter >= 1 && prm->max_iter <= 64, RTN_EINVAL, "rtn_trace: max_iter outside 1..64");
+  RTN_CHECK(!prm->count || hits->counters, RTN_EINVAL, "rtn_trace: count requested without a counter buffer");
+  if (n_rays == 0) return RTN_OK;
+  hipStream_t st = (hipStream_t)stream;
+  unsigned long long* d_tot = nullptr;
+  if (totals) {
+    RTN_HIP(hipMallocAsync((void**)&d_tot, 8 * sizeof(unsigned long long), st));
+    RTN_HIP(hipMemsetAsync(d_tot, 0, 8 * sizeof(unsigned long long), st));
+  }
+  int rc;
+  switch (prm->mode) {
+    case RTN_MODE_EXACT:
+      RTN_CHECK(prm->seed_mode == RTN_SEED_PLAIN && prm->precision == RTN_PREC_FP32 && !prm->hull_cull,
+                RTN_EUNSUPPORTED, "rtn_trace: the exact form runs the plain method only");
+      rc = trace_exact_scene(s, rays, n_rays, prm, hits, d_tot, st);
+      break;
+    case RTN_MODE_WIDE:
+    case RTN_MODE_PERSISTENT:
+      rc = trace_wide_scene(s, rays, n_rays, prm, hits, d_tot, st);
+      break;
+    default:
+      set_error("rtn_trace: unknown mode %d", prm->mode);
+      rc = RTN_EINVAL;
+  }
+  if (totals) {
+    if (rc == RTN_OK) {
+      unsigned long long h[8];
+      RTN_HIP(hipMemcpyAsync(h, d_tot, sizeof(h), hipMemcpyDeviceToHost, st));
+      RTN_HIP(hipStreamSynchronize(st));
+      memcpy(totals, h, sizeof(h));
+      if (h[6]) {
+        set_error("rtn_trace: traversal stack overflowed on %llu rays", h[6]);
+        rc = RTN_EOVERFLOW;
+      }
+    }
+    (void)hipFreeAsync(d_tot, st);
+  }
+  return rc;
+}

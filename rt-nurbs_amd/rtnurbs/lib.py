@@ -1,0 +1,383 @@
+"""ctypes binding of librtnurbs.so (the C ABI declared in include/rtnurbs.h).
+
+This is the Python host over the HIP library: device buffers, scenes, ray
+generation and the trace entry points.  There is no CPU fallback: importing
+the binding without the built library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "librtnurbs.so")
+
+RTN_OK, RTN_EINVAL, RTN_EHIP, RTN_ENOMEM, RTN_EUNSUPPORTED, RTN_EOVERFLOW = 0, -1, -2, -3, -4, -5
+MAX_DEGREE = 5
+MODE_EXACT, MODE_WIDE, MODE_PERSISTENT = 0, 1, 2
+SEED_PLAIN, SEED_LEARNED = 0, 1
+PREC_FP32, PREC_MIXED = 0, 1
+
+NODE_DTYPE = np.dtype([("b", np.float32, 6), ("left_first", np.uint32), ("prim_count", np.uint32)])
+INTERACT_DTYPE = np.dtype([("t_hit", np.float32), ("wo", np.float32, 3), ("uv", np.float32, 2),
+                           ("dpdu", np.float32, 3), ("dpdv", np.float32, 3), ("dndu", np.float32, 3),
+                           ("dndv", np.float32, 3)])
+assert INTERACT_DTYPE.itemsize == 72 and NODE_DTYPE.itemsize == 32
+
+EXPORTS = [
+    "rtn_last_error", "rtn_version", "rtn_malloc", "rtn_malloc_managed", "rtn_free", "rtn_memcpy", "rtn_memset",
+    "rtn_device_synchronize", "rtn_stream_create", "rtn_stream_destroy", "rtn_stream_synchronize",
+    "rtn_camera_perspective", "rtn_rays_perspective", "rtn_scene_create", "rtn_scene_destroy",
+    "rtn_scene_add_nurbs", "rtn_scene_add_patches", "rtn_scene_build", "rtn_scene_stats_get",
+    "rtn_scene_download", "rtn_trace_params_default", "rtn_trace", "rtn_calculate_intersect", "rtn_get_patch",
+    "rtn_build_bvh", "rtn_patch_bounds", "rtn_refine",
+]
+
+
+class RtnError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"rtnurbs error {code}: {msg}")
+        self.code = code
+
+
+class RayQueue(C.Structure):
+    _fields_ = [("origin_x", C.c_void_p), ("origin_y", C.c_void_p), ("origin_z", C.c_void_p),
+                ("direction_x", C.c_void_p), ("direction_y", C.c_void_p), ("direction_z", C.c_void_p),
+                ("idx_interact", C.c_void_p)]
+
+
+class TraceParams(C.Structure):
+    _fields_ = [("mode", C.c_int), ("max_iter", C.c_int), ("thr_det", C.c_float), ("thr_res", C.c_float),
+                ("flat_threshold", C.c_float), ("cam_k", C.c_float), ("seed_base", C.c_uint32),
+                ("seed_mode", C.c_int), ("precision", C.c_int), ("cond_thr", C.c_float),
+                ("hull_cull", C.c_int), ("count", C.c_int), ("reserved", C.c_int * 6)]
+
+
+class Hits(C.Structure):
+    _fields_ = [("prim", C.c_void_p), ("t", C.c_void_p), ("u", C.c_void_p), ("v", C.c_void_p),
+                ("dpdu", C.c_void_p), ("dpdv", C.c_void_p), ("counters", C.c_void_p)]
+
+
+class TraceTotals(C.Structure):
+    _fields_ = [(k, C.c_ulonglong) for k in ("newton_calls", "newton_evals", "skips", "interior", "leaves",
+                                             "patch_tests", "overflow", "fp64_steps")]
+
+
+class SceneStats(C.Structure):
+    _fields_ = [("n_surfaces", C.c_int64), ("n_patches", C.c_int64), ("n_cp", C.c_int64), ("n_nodes", C.c_int64),
+                ("n_wide_nodes", C.c_int64), ("max_depth", C.c_int64), ("n_leaves", C.c_int64),
+                ("deg_hist", (C.c_int64 * (MAX_DEGREE + 1)) * (MAX_DEGREE + 1)),
+                ("build_ms_decompose", C.c_double), ("build_ms_bvh", C.c_double), ("build_ms_wide", C.c_double)]
+
+
+_lib = None
+
+
+def load():
+    """Load librtnurbs.so; raises if it was not built (no CPU fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RtnError(RTN_EUNSUPPORTED, f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    vp, i, f, sz = C.c_void_p, C.c_int, C.c_float, C.c_size_t
+    L.rtn_last_error.restype = C.c_char_p
+    L.rtn_malloc.argtypes = [C.POINTER(vp), sz]
+    L.rtn_malloc_managed.argtypes = [C.POINTER(vp), sz]
+    L.rtn_free.argtypes = [vp]
+    L.rtn_memcpy.argtypes = [vp, vp, sz, i]
+    L.rtn_memset.argtypes = [vp, i, sz]
+    L.rtn_stream_create.argtypes = [C.POINTER(vp)]
+    L.rtn_stream_destroy.argtypes = [vp]
+    L.rtn_stream_synchronize.argtypes = [vp]
+    L.rtn_camera_perspective.argtypes = [vp, vp, vp, i, i, f, f, f, vp, vp]
+    L.rtn_rays_perspective.argtypes = [vp, vp, i, i, i, i, C.c_uint32, C.POINTER(RayQueue), vp]
+    L.rtn_scene_create.argtypes = [C.POINTER(vp)]
+    L.rtn_scene_destroy.argtypes = [vp]
+    L.rtn_scene_add_nurbs.argtypes = [vp, i, i, vp, i, i, vp, vp]
+    L.rtn_scene_add_patches.argtypes = [vp, i, i, i, vp]
+    L.rtn_scene_build.argtypes = [vp]
+    L.rtn_scene_stats_get.argtypes = [vp, C.POINTER(SceneStats)]
+    L.rtn_scene_download.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.rtn_trace_params_default.argtypes = [C.POINTER(TraceParams)]
+    L.rtn_trace_params_default.restype = None
+    L.rtn_trace.argtypes = [vp, C.POINTER(vp * 6), C.c_int64, C.POINTER(TraceParams), C.POINTER(Hits),
+                            C.POINTER(TraceTotals), vp]
+    L.rtn_calculate_intersect.argtypes = [C.POINTER(RayQueue), vp, vp, vp, i, i, f, f, vp, vp, vp]
+    L.rtn_get_patch.argtypes = [i, i, vp, i, i, vp, vp, C.POINTER(i), C.POINTER(i), vp, vp]
+    L.rtn_build_bvh.argtypes = [vp, i, vp, vp, C.POINTER(i)]
+    L.rtn_patch_bounds.argtypes = [vp, i, i, i, vp]
+    L.rtn_refine.argtypes = [i, i, i, vp, i, vp, f, C.POINTER(i), vp, vp]
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != RTN_OK:
+        msg = load().rtn_last_error()
+        raise RtnError(rc, msg.decode() if msg else "")
+    return rc
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+# ---------------------------------------------------------------- buffers
+class DeviceArray:
+    """A hipMalloc'd buffer with numpy shape/dtype metadata."""
+
+    def __init__(self, shape, dtype, managed=False):
+        self.shape = tuple(np.atleast_1d(shape)) if not isinstance(shape, tuple) else shape
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        fn = load().rtn_malloc_managed if managed else load().rtn_malloc
+        check(fn(C.byref(p), max(self.nbytes, 1)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, a, managed=False):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype, managed)
+        if a.nbytes:
+            check(load().rtn_memcpy(d.ptr, a.ctypes.data, a.nbytes, 0))
+        return d
+
+    def get(self):
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes:
+            check(load().rtn_memcpy(out.ctypes.data, self.ptr, self.nbytes, 1))
+        return out
+
+    def set(self, a):
+        a = np.ascontiguousarray(a, self.dtype)
+        assert a.nbytes == self.nbytes
+        check(load().rtn_memcpy(self.ptr, a.ctypes.data, a.nbytes, 0))
+
+    def fill_bytes(self, value):
+        check(load().rtn_memset(self.ptr, value, self.nbytes))
+
+    def free(self):
+        if self.ptr:
+            load().rtn_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def synchronize():
+    check(load().rtn_device_synchronize())
+
+
+# ---------------------------------------------------------------- camera
+def camera_matrices(cam):
+    """(world_from_camera, camera_from_raster) for a scenes.Camera."""
+    wfc = np.zeros(16, np.float32)
+    cfr = np.zeros(16, np.float32)
+    eye, look, up = (np.asarray(x, np.float32) for x in (cam.eye, cam.look, cam.up))
+    check(load().rtn_camera_perspective(eye.ctypes.data, look.ctypes.data, up.ctypes.data, cam.W, cam.H,
+                                        cam.fov_deg, cam.znear, cam.zfar, wfc.ctypes.data, cfr.ctypes.data))
+    return wfc, cfr
+
+
+class Rays:
+    """Device SoA ray queue (soa.h:6-14) of n rays."""
+
+    def __init__(self, n, with_idx=True):
+        self.n = int(n)
+        self.comp = [DeviceArray((self.n,), np.float32) for _ in range(6)]
+        self.idx = DeviceArray((self.n,), np.int32) if with_idx else None
+
+    def queue(self):
+        q = RayQueue(*[c.ptr for c in self.comp], self.idx.ptr if self.idx else None)
+        return q
+
+    def ptr_array(self):
+        return (C.c_void_p * 6)(*[c.ptr for c in self.comp])
+
+    def get(self):
+        return [c.get() for c in self.comp]
+
+    @classmethod
+    def from_host(cls, arrs):
+        r = cls(len(arrs[0]))
+        for d, a in zip(r.comp, arrs):
+            d.set(np.asarray(a, np.float32))
+        r.idx.fill_bytes(0xFF)
+        return r
+
+
+def generate_rays(cam, spp=1, jitter=False, seed=0, stream=None):
+    wfc, cfr = camera_matrices(cam)
+    rays = Rays(cam.W * cam.H * spp)
+    q = rays.queue()
+    check(load().rtn_rays_perspective(wfc.ctypes.data, cfr.ctypes.data, cam.W, cam.H, spp, int(jitter), seed,
+                                      C.byref(q), stream))
+    return rays
+
+
+# ---------------------------------------------------------------- scene
+class Scene:
+    def __init__(self, surfaces=(), patch_groups=()):
+        L = load()
+        h = C.c_void_p()
+        check(L.rtn_scene_create(C.byref(h)))
+        self.h = h.value
+        for s in surfaces:
+            self.add_nurbs(s)
+        for (np_, p, q, cp) in patch_groups:
+            self.add_patches(cp, p, q)
+
+    def add_nurbs(self, s):
+        U, V, Pw = (np.ascontiguousarray(a, np.float32) for a in (s.U, s.V, s.Pw))
+        check(load().rtn_scene_add_nurbs(self.h, s.n, s.p, U.ctypes.data, s.m, s.q, V.ctypes.data, Pw.ctypes.data))
+
+    def add_patches(self, cp, p, q):
+        cp = np.ascontiguousarray(cp, np.float32)
+        n = cp.size // (4 * (p + 1) * (q + 1))
+        check(load().rtn_scene_add_patches(self.h, n, p, q, cp.ctypes.data))
+
+    def build(self):
+        check(load().rtn_scene_build(self.h))
+        return self
+
+    def stats(self):
+        st = SceneStats()
+        check(load().rtn_scene_stats_get(self.h, C.byref(st)))
+        d = {k: getattr(st, k) for k, _ in SceneStats._fields_ if k != "deg_hist"}
+        d["deg_hist"] = np.array([[st.deg_hist[p][q] for q in range(MAX_DEGREE + 1)]
+                                  for p in range(MAX_DEGREE + 1)])
+        return d
+
+    def download(self):
+        st = self.stats()
+        np_, ncp, nn = st["n_patches"], st["n_cp"], st["n_nodes"]
+        out = dict(cp=np.zeros((ncp, 4), np.float32), off=np.zeros(np_, np.int32), pq=np.zeros((np_, 2), np.int32),
+                   uv=np.zeros((np_, 4), np.float32), pid=np.zeros(np_, np.int32),
+                   knot=np.zeros((np_, 2), np.int32), nodes=np.zeros(nn, NODE_DTYPE),
+                   delta=np.zeros(np_, np.float32))
+        check(load().rtn_scene_download(self.h, *[_ptr(out[k]) for k in
+                                                  ("cp", "off", "pq", "uv", "pid", "knot", "nodes", "delta")]))
+        return out
+
+    def __del__(self):
+        try:
+            if self.h:
+                load().rtn_scene_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def params(**kw):
+    p = TraceParams()
+    load().rtn_trace_params_default(C.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+class HitBuffers:
+    def __init__(self, n, counters=False):
+        self.n = n
+        self.prim = DeviceArray((n,), np.int32)
+        self.t = DeviceArray((n,), np.float32)
+        self.u = DeviceArray((n,), np.float32)
+        self.v = DeviceArray((n,), np.float32)
+        self.dpdu = DeviceArray((n, 3), np.float32)
+        self.dpdv = DeviceArray((n, 3), np.float32)
+        self.counters = DeviceArray((n, 8), np.uint32) if counters else None
+
+    def struct(self):
+        return Hits(self.prim.ptr, self.t.ptr, self.u.ptr, self.v.ptr, self.dpdu.ptr, self.dpdv.ptr,
+                    self.counters.ptr if self.counters else None)
+
+    def get(self):
+        d = dict(hit=self.prim.get(), t=self.t.get(), u=self.u.get(), v=self.v.get(), dpdu=self.dpdu.get(),
+                 dpdv=self.dpdv.get())
+        if self.counters is not None:
+            d["counters"] = self.counters.get()
+        return d
+
+
+def trace(scene, rays, prm, hits=None, totals=True, stream=None):
+    """Runs rtn_trace; returns (HitBuffers, totals dict or None)."""
+    if hits is None:
+        hits = HitBuffers(rays.n, counters=bool(prm.count))
+    hs = hits.struct()
+    tot = TraceTotals() if totals else None
+    arr = rays.ptr_array()
+    check(load().rtn_trace(scene.h, C.byref(arr), rays.n, C.byref(prm), C.byref(hs),
+                           C.byref(tot) if tot is not None else None, stream))
+    t = {k: getattr(tot, k) for k, _ in TraceTotals._fields_} if tot is not None else None
+    return hits, t
+
+
+# ------------------------------------------------------- reference-shaped
+def get_patch(s):
+    """getPatch on the GPU: (cp [np,(p+1)(q+1),4], uv [np,4], nbU, nbV)."""
+    L = load()
+    a, b = C.c_int(), C.c_int()
+    U, V, Pw = (np.ascontiguousarray(x, np.float32) for x in (s.U, s.V, s.Pw))
+    check(L.rtn_get_patch(s.n, s.p, U.ctypes.data, s.m, s.q, V.ctypes.data, Pw.ctypes.data, C.byref(a), C.byref(b),
+                          None, None))
+    npch = a.value * b.value
+    cp = np.zeros((npch, (s.p + 1) * (s.q + 1), 4), np.float32)
+    uv = np.zeros((npch, 4), np.float32)
+    check(L.rtn_get_patch(s.n, s.p, U.ctypes.data, s.m, s.q, V.ctypes.data, Pw.ctypes.data, C.byref(a), C.byref(b),
+                          cp.ctypes.data, uv.ctypes.data))
+    return cp, uv, a.value, b.value
+
+
+def patch_bounds(cp, p, q):
+    cp = np.ascontiguousarray(cp, np.float32)
+    n = cp.size // (4 * (p + 1) * (q + 1))
+    out = np.zeros((n, 6), np.float32)
+    check(load().rtn_patch_bounds(cp.ctypes.data, n, p, q, out.ctypes.data))
+    return out
+
+
+def build_bvh(bounds):
+    bounds = np.ascontiguousarray(bounds, np.float32)
+    n = len(bounds)
+    nodes = np.zeros(2 * n - 1, NODE_DTYPE)
+    perm = np.zeros(n, np.uint32)
+    used = C.c_int()
+    check(load().rtn_build_bvh(bounds.ctypes.data, n, nodes.ctypes.data, perm.ctypes.data, C.byref(used)))
+    return nodes, perm, used.value
+
+
+def refine(s, C_u=None, C_v=None):
+    """curvatureRefinementU then V (nurbs.h:290-486) on a scenes.Nurbs."""
+    from dataclasses import replace
+    L = load()
+    cur = s
+    for d, Cc in ((0, C_u), (1, C_v)):
+        if Cc is None or Cc <= 0:
+            continue
+        deg = cur.p if d == 0 else cur.q
+        K = np.ascontiguousarray(cur.U if d == 0 else cur.V, np.float32)
+        Pw = np.ascontiguousarray(cur.Pw, np.float32)
+        ins = C.c_int()
+        check(L.rtn_refine(d, cur.n, deg, K.ctypes.data, cur.m, Pw.ctypes.data, Cc, C.byref(ins), None, None))
+        tot = ins.value
+        nn = cur.n if d == 0 else cur.m
+        nk = np.zeros(nn + tot + deg + 2, np.float32)
+        if d == 0:
+            P2 = np.zeros(((cur.n + tot + 1) * (cur.m + 1), 4), np.float32)
+        else:
+            P2 = np.zeros(((cur.n + 1) * (cur.m + tot + 1), 4), np.float32)
+        check(L.rtn_refine(d, cur.n, deg, K.ctypes.data, cur.m, Pw.ctypes.data, Cc, C.byref(ins), nk.ctypes.data,
+                           P2.ctypes.data))
+        cur = replace(cur, n=cur.n + tot, U=nk, Pw=P2) if d == 0 else replace(cur, m=cur.m + tot, V=nk, Pw=P2)
+    return cur
