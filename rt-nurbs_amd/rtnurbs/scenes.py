@@ -132,7 +132,8 @@ def _random_knots(rng, n_cp: int, deg: int) -> np.ndarray:
     mults = []
     while sum(mults) < slots:
         mults.append(int(rng.integers(1, deg + 1)))
-    mults[-1] -= sum(mults) - slots
+    if mults:
+        mults[-1] -= sum(mults) - slots
     k = len(mults)
     while True:
         vals = np.sort(rng.uniform(0.0, 1.0, size=k))
