@@ -78,6 +78,12 @@ int rtn_device_synchronize(void);
 int rtn_stream_create(void** stream);
 int rtn_stream_destroy(void* stream);
 int rtn_stream_synchronize(void* stream);
+/* HIP events for timing kernels on a stream (the reference times with host
+ * std::chrono around launch + sync, benchmark.cpp:238-244). */
+int rtn_event_create(void** event);
+int rtn_event_destroy(void* event);
+int rtn_event_record(void* event, void* stream);
+int rtn_event_elapsed_ms(void* start, void* stop, float* ms);
 
 /* ------------------------------------------------------------------ camera
  * Transform::lookAt (transform.h:46-60) + CameraOps::init_camera and
@@ -176,7 +182,7 @@ void rtn_trace_params_default(rtn_trace_params* p);
  *   dpdu, dpdv float[3*n]  surface derivatives at the hit
  *   counters uint32[8*n]: newton calls, surface evaluations, flat skips,
  *            interior visits, leaf visits, patch tests, stack overflow,
- *            FP64 corrections */
+ *            Newton FP32 operations (evaluations priced per degree) */
 typedef struct {
   int* prim;
   float *t, *u, *v;
@@ -186,11 +192,22 @@ typedef struct {
 
 /* Totals accumulated over a trace (device atomics; host-readable after sync). */
 typedef struct {
-  unsigned long long newton_calls, newton_evals, skips, interior, leaves, patch_tests, overflow, fp64_steps;
+  unsigned long long newton_calls, newton_evals, skips, interior, leaves, patch_tests, overflow, newton_flops;
+  unsigned long long fp64_steps, hull_culled, learned_fallbacks, reserved;
 } rtn_trace_totals;
 
 int rtn_trace(const rtn_scene* scene, const float* const rays_soa[6], int64_t n_rays, const rtn_trace_params* prm,
               const rtn_hits* hits, rtn_trace_totals* totals_host, void* stream);
+
+/* FP64 reference trace (no reference counterpart: the reference is FP32
+ * only).  The plain method's traversal, seed and Newton iteration evaluated
+ * in FP64 with its own cap and tolerances, used as the error reference of
+ * the FP32 forms.  Outputs are device FP64 arrays of n (dpdu/dpdv 3n);
+ * t_second = nearest converged hit on another patch (tie detection);
+ * counters as rtn_hits (may be NULL).  Synchronous when stream is NULL. */
+int rtn_trace_f64(const rtn_scene* scene, const float* const rays_soa[6], int64_t n_rays, int max_iter,
+                  double thr_det, double thr_res, uint32_t seed_base, int* prim, double* t, double* u, double* v,
+                  double* dpdu, double* dpdv, double* t_second, uint32_t* counters, void* stream);
 
 /* calculateIntersectWrapper (intersect.cu:199-219; declared intersect.h:9-13)
  * on reference-layout buffers: nodes[2N-1], patches[N] whose control_point

@@ -813,6 +813,7 @@ struct TraceParams {
   int max_iter;
   double thr_det, thr_res, flat_threshold, cam_k;  // FP32 values are held exactly
   uint32_t seed_base;
+  double prune_slack;  // FP64 reference only: boxes entered beyond t_max*slack are skipped (1 = reference)
 };
 
 }  // extern "C"
@@ -821,7 +822,7 @@ template <class T>
 static void trace_ray(long r, const Node* nodes, const v4<T>* cp, const int* cp_off, const int* pdeg, const int* qdeg,
                       const float* delta, const TraceParams& prm, const float* ox, const float* oy, const float* oz,
                       const float* dx, const float* dy, const float* dz, int* hit_prim, T* t_out, T* u_out, T* v_out,
-                      T* dpdu, T* dpdv, int64_t* counters) {
+                      T* dpdu, T* dpdv, int64_t* counters, T* t2_out = nullptr) {
   v3<T> o{T(ox[r]), T(oy[r]), T(oz[r])}, d{T(dx[r]), T(dy[r]), T(dz[r])};
   int64_t nc = 0, ne = 0, sc = 0, niv = 0, nlv = 0, npt = 0, ovf = 0, nh = 0;
   v3<T> rd{T(1) / d.x, T(1) / d.y, T(1) / d.z};
@@ -830,7 +831,7 @@ static void trace_ray(long r, const Node* nodes, const v4<T>* cp, const int* cp_
   int off = 0;
   T t_max = INFINITY;
   int best = -1;
-  T bt = 0, bu = 0, bv = 0;
+  T bt = 0, bu = 0, bv = 0, t_second = INFINITY;
   v3<T> bsu{0, 0, 0}, bsv{0, 0, 0};
   const Node* cur = &nodes[0];
   T it = slab(cur->b, cur->b + 3, o, rd);
@@ -859,6 +860,7 @@ static void trace_ray(long r, const Node* nodes, const v4<T>* cp, const int* cp_
             sc++;
             T t = dot3(sub3(S, o), d);
             if (0 < t && t < t_max) {
+              t_second = t_max;
               t_max = t; best = pi; bt = t; bu = gu; bv = gv; bsu = Su; bsv = Sv;
               do_newton = false;
             }
@@ -872,8 +874,11 @@ static void trace_ray(long r, const Node* nodes, const v4<T>* cp, const int* cp_
             nh++;
             T t = (nr.S.x - o.x) * d.x + (nr.S.y - o.y) * d.y + (nr.S.z - o.z) * d.z;
             if (0 < t && t < t_max) {
+              t_second = t_max;
               best = pi; bt = t; bu = nr.u; bv = nr.v; bsu = nr.Su; bsv = nr.Sv;
               t_max = t;
+            } else if (0 < t && t < t_second) {
+              t_second = t;
             }
           }
         }
@@ -887,16 +892,17 @@ static void trace_ray(long r, const Node* nodes, const v4<T>* cp, const int* cp_
       T lt = slab(L.b, L.b + 3, o, rd);
       T rt = slab(R.b, R.b + 3, o, rd);
       it = std::fmin(lt, rt);
-      if (it >= t_max) {
+      const T lim = t_max * T(prm.prune_slack);
+      if (it >= lim) {
         if (off == 0) break;
         cur = &nodes[stack[--off]];
       } else if (lt <= it) {
-        if (rt < t_max) {
+        if (rt < lim) {
           if (off < STACK_DEPTH) stack[off++] = cur->left_first + 1; else ovf = 1;
         }
         cur = &nodes[cur->left_first];
       } else {
-        if (lt < t_max) {
+        if (lt < lim) {
           if (off < STACK_DEPTH) stack[off++] = cur->left_first; else ovf = 1;
         }
         cur = &nodes[cur->left_first + 1];
@@ -904,6 +910,7 @@ static void trace_ray(long r, const Node* nodes, const v4<T>* cp, const int* cp_
     }
   }
   if (hit_prim) hit_prim[r] = best;
+  if (t2_out) t2_out[r] = t_second;
   if (best >= 0) {
     if (t_out) t_out[r] = bt;
     if (u_out) u_out[r] = bu;
@@ -924,7 +931,7 @@ void orc_trace_f32(const void* nodes, const float* cp, const int* cp_off, const 
                    uint32_t seed_base, long n_rays, const float* ox, const float* oy, const float* oz, const float* dx,
                    const float* dy, const float* dz, int* hit_prim, float* t, float* u, float* v, float* dpdu,
                    float* dpdv, int64_t* counters, int n_threads) {
-  TraceParams prm{max_iter, thr_det, thr_res, flat_threshold, cam_k, seed_base};
+  TraceParams prm{max_iter, thr_det, thr_res, flat_threshold, cam_k, seed_base, 1.0};
   const Node* nd = reinterpret_cast<const Node*>(nodes);
   const v4<float>* P = reinterpret_cast<const v4<float>*>(cp);
 #pragma omp parallel for schedule(dynamic, 256) num_threads(n_threads > 0 ? n_threads : 1)
@@ -936,18 +943,18 @@ void orc_trace_f32(const void* nodes, const float* cp, const int* cp_off, const 
 // FP64 instantiation of the same traversal (the FP64 oracle): inputs are the
 // FP32 scene and rays widened exactly; thresholds as given.
 void orc_trace_f64(const void* nodes, const float* cp, long n_cp, const int* cp_off, const int* pdeg, const int* qdeg,
-                   int max_iter, double thr_det, double thr_res, uint32_t seed_base, long n_rays, const float* ox,
+                   int max_iter, double thr_det, double thr_res, double prune_slack, uint32_t seed_base, long n_rays, const float* ox,
                    const float* oy, const float* oz, const float* dx, const float* dy, const float* dz,
-                   int* hit_prim, double* t, double* u, double* v, double* dpdu, double* dpdv, int64_t* counters,
-                   int n_threads) {
+                   int* hit_prim, double* t, double* u, double* v, double* dpdu, double* dpdv, double* t2,
+                   int64_t* counters, int n_threads) {
   std::vector<v4<double>> P(n_cp);
   for (long i = 0; i < n_cp; i++) P[i] = {cp[4 * i], cp[4 * i + 1], cp[4 * i + 2], cp[4 * i + 3]};
   const Node* nd = reinterpret_cast<const Node*>(nodes);
-  TraceParams prm{max_iter, thr_det, thr_res, 0, 0, seed_base};
+  TraceParams prm{max_iter, thr_det, thr_res, 0, 0, seed_base, prune_slack};
 #pragma omp parallel for schedule(dynamic, 256) num_threads(n_threads > 0 ? n_threads : 1)
   for (long r = 0; r < n_rays; r++)
     trace_ray<double>(r, nd, P.data(), cp_off, pdeg, qdeg, nullptr, prm, ox, oy, oz, dx, dy, dz, hit_prim, t, u, v,
-                      dpdu, dpdv, counters);
+                      dpdu, dpdv, counters, t2);
 }
 
 // FP64 Newton polish of one (ray, patch) pair from (u,v): plain Newton with no

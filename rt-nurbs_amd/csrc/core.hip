@@ -77,6 +77,29 @@ extern "C" int rtn_stream_synchronize(void* stream) {
   return RTN_OK;
 }
 
+// ----------------------------------------------------------------- events
+extern "C" int rtn_event_create(void** ev) {
+  RTN_CHECK(ev, RTN_EINVAL, "rtn_event_create: null out pointer");
+  hipEvent_t e;
+  RTN_HIP(hipEventCreate(&e));
+  *ev = (void*)e;
+  return RTN_OK;
+}
+extern "C" int rtn_event_destroy(void* ev) {
+  if (ev) RTN_HIP(hipEventDestroy((hipEvent_t)ev));
+  return RTN_OK;
+}
+extern "C" int rtn_event_record(void* ev, void* stream) {
+  RTN_HIP(hipEventRecord((hipEvent_t)ev, (hipStream_t)stream));
+  return RTN_OK;
+}
+extern "C" int rtn_event_elapsed_ms(void* start, void* stop, float* ms) {
+  RTN_CHECK(ms, RTN_EINVAL, "rtn_event_elapsed_ms: null out pointer");
+  RTN_HIP(hipEventSynchronize((hipEvent_t)stop));
+  RTN_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+  return RTN_OK;
+}
+
 // ----------------------------------------------------------------- camera
 // Column-major 4x4 helpers in the reference's order (matrix_ops.h:101-171,
 // transform.h:14-38).
@@ -136,10 +159,11 @@ extern "C" int rtn_camera_perspective(const float eye[3], const float look[3], c
   RTN_CHECK(eye && look && up && wfc && cfr && W > 0 && H > 0, RTN_EINVAL, "rtn_camera_perspective: bad arguments");
   using namespace rtn::ex;
   // lookAt (transform.h:46-60)
-  v3 pos{eye[0], eye[1], eye[2]};
-  v3 dir = normalize(sub(v3{look[0], look[1], look[2]}, pos));
-  v3 right = normalize(cross(v3{up[0], up[1], up[2]}, dir));
-  v3 nu = cross(dir, right);
+  using V = v3<float>;
+  V pos{eye[0], eye[1], eye[2]};
+  V dir = normalize(sub(V{look[0], look[1], look[2]}, pos));
+  V right = normalize(cross(V{up[0], up[1], up[2]}, dir));
+  V nu = cross(dir, right);
   const float m[16] = {right.x, right.y, right.z, 0.0f, nu.x, nu.y, nu.z, 0.0f,
                        dir.x,   dir.y,   dir.z,   0.0f, pos.x, pos.y, pos.z, 1.0f};
   memcpy(wfc, m, sizeof(m));
@@ -192,7 +216,7 @@ __global__ void __launch_bounds__(256) k_rays_perspective(CamPayload cam, int W,
   float px = c[0] * e0 + c[4] * e1 + c[8] * e2 + c[12] * e3;
   float py = c[1] * e0 + c[5] * e1 + c[9] * e2 + c[13] * e3;
   float pz = c[2] * e0 + c[6] * e1 + c[10] * e2 + c[14] * e3;
-  ex::v3 pc = ex::normalize(ex::v3{px, py, pz});
+  ex::v3<float> pc = ex::normalize(ex::v3<float>{px, py, pz});
   const float* w = cam.wfc;
   float dx = w[0] * pc.x + w[4] * pc.y + w[8] * pc.z + w[12] * 0.0f;
   float dy = w[1] * pc.x + w[5] * pc.y + w[9] * pc.z + w[13] * 0.0f;

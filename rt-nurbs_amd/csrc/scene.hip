@@ -165,7 +165,8 @@ __global__ void k_prep(const float4* cp, const int* cp_off, const int* pq, int n
   const int p = pq[g] & 0xff, q = pq[g] >> 8, row = q + 1;
   const float4* c = cp + cp_off[g];
   using namespace ex;
-  v3 c00 = dehom(ld4(c[0])), c10 = dehom(ld4(c[p * row])), c11 = dehom(ld4(c[p * row + q])), c01 = dehom(ld4(c[q]));
+  v3<float> c00 = dehom(ld4<float>(c[0])), c10 = dehom(ld4<float>(c[p * row]));
+  v3<float> c11 = dehom(ld4<float>(c[p * row + q])), c01 = dehom(ld4<float>(c[q]));
   corners[3 * g + 0] = make_float4(c00.x, c00.y, c00.z, c10.x);
   corners[3 * g + 1] = make_float4(c10.y, c10.z, c11.x, c11.y);
   corners[3 * g + 2] = make_float4(c11.z, c01.x, c01.y, c01.z);
@@ -177,7 +178,7 @@ __global__ void k_prep(const float4* cp, const int* cp_off, const int* pq, int n
       float Bx = w00 * c00.x + w10 * c10.x + w01 * c01.x + w11 * c11.x;
       float By = w00 * c00.y + w10 * c10.y + w01 * c01.y + w11 * c11.y;
       float Bz = w00 * c00.z + w10 * c10.z + w01 * c01.z + w11 * c11.z;
-      v3 P = dehom(ld4(c[i * row + j]));
+      v3<float> P = dehom(ld4<float>(c[i * row + j]));
       float dx = P.x - Bx, dy = P.y - By, dz = P.z - Bz;
       d = fmaxf(d, sqrtf(dx * dx + dy * dy + dz * dz));
     }

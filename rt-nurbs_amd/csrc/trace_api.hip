@@ -41,8 +41,8 @@ extern "C" int rtn_trace(const rtn_scene* s, const float* const rays[6], int64_t
   hipStream_t st = (hipStream_t)stream;
   unsigned long long* d_tot = nullptr;
   if (totals) {
-    RTN_HIP(hipMallocAsync((void**)&d_tot, 8 * sizeof(unsigned long long), st));
-    RTN_HIP(hipMemsetAsync(d_tot, 0, 8 * sizeof(unsigned long long), st));
+    RTN_HIP(hipMallocAsync((void**)&d_tot, 12 * sizeof(unsigned long long), st));
+    RTN_HIP(hipMemsetAsync(d_tot, 0, 12 * sizeof(unsigned long long), st));
   }
   int rc;
   switch (prm->mode) {
@@ -61,7 +61,7 @@ extern "C" int rtn_trace(const rtn_scene* s, const float* const rays[6], int64_t
   }
   if (totals) {
     if (rc == RTN_OK) {
-      unsigned long long h[8];
+      unsigned long long h[12];
       RTN_HIP(hipMemcpyAsync(h, d_tot, sizeof(h), hipMemcpyDeviceToHost, st));
       RTN_HIP(hipStreamSynchronize(st));
       memcpy(totals, h, sizeof(h));

@@ -2,12 +2,15 @@
 //
 // One lane per ray over the reference's binary BVH (32-byte nodes fetched as
 // two 16-byte loads), the traversal stack in LDS (one column per lane, so the
-// 64 lanes of a wave hit 64 distinct banks), the bilinear seed and the FP32
-// Newton solve in the reference's operation order.  Degree (2,2) and (3,3)
-// patches run compile-time-degree evaluators whose de Casteljau triangles
-// stay in VGPRs; other degrees use a guarded unrolled evaluator (degree <= 5).
-// This translation unit is compiled with -ffp-contract=off, so results equal
-// the host reference bit for bit.
+// 64 lanes of a wave hit 64 distinct banks), the bilinear seed and the Newton
+// solve in the reference's operation order.  Degree (2,2) and (3,3) patches
+// run compile-time-degree evaluators whose de Casteljau triangles stay in
+// VGPRs; other degrees use a guarded unrolled evaluator (degree <= 5).
+//
+// T = float is the plain FP32 method; this translation unit is compiled with
+// -ffp-contract=off, so its results equal the host reference bit for bit.
+// T = double is the same traversal in FP64, used as the FP64 reference for
+// error reports (with its own iteration cap and tolerance).
 #include <cmath>
 
 #include "rtn_exact.hpp"
@@ -20,6 +23,7 @@ namespace {
 
 constexpr int TB = 128;  // lanes per block (2 waves)
 
+template <class T>
 struct ExactArgs {
   const float4* nodes;  // rtn_bvh_node viewed as 2 x float4
   const float *ox, *oy, *oz, *dx, *dy, *dz;
@@ -28,11 +32,13 @@ struct ExactArgs {
   const float* delta;
   float flat, cam_k;
   int max_iter;
-  float thr_det, thr_res;
+  T thr_det, thr_res;
+  T prune_slack;        // boxes entered beyond t_max * slack are skipped (1 = reference)
   uint32_t seed_base;
   // scene-form outputs
   int* prim;
-  float *t, *u, *v, *dpdu, *dpdv;
+  T *t, *u, *v, *dpdu, *dpdv;
+  T* t2;                // second-nearest converged hit on another patch (FP64 reference)
   uint32_t* counters;
   // drop-in outputs (surfaceInteraction AoS + idx_interact)
   rtn_surface_interaction* inter;
@@ -67,33 +73,51 @@ __device__ __forceinline__ void load_node(const float4* nodes, int i, float b[6]
   pc = __float_as_uint(c.w);
 }
 
-__device__ __forceinline__ float slab6(const float b[6], v3 o, v3 rd) {
-  return slab(b[0], b[3], b[1], b[4], b[2], b[5], o, rd);
+template <class T>
+__device__ __forceinline__ T slab6(const float b[6], v3<T> o, v3<T> rd) {
+  return slab<T>(T(b[0]), T(b[3]), T(b[1]), T(b[4]), T(b[2]), T(b[5]), o, rd);
 }
 
-template <int STACK, class Patches>
-__device__ __forceinline__ void trace_one(const ExactArgs& A, const Patches& PT, int64_t idx, int tid,
-                                          int (*stk)[TB], uint32_t cnt[7]) {
-  const v3 o{A.ox[idx], A.oy[idx], A.oz[idx]};
-  const v3 d{A.dx[idx], A.dy[idx], A.dz[idx]};
-  uint32_t nc = 0, ne = 0, sc = 0, niv = 0, nlv = 0, npt = 0, ovf = 0;
-  const v3 rd{1 / d.x, 1 / d.y, 1 / d.z};
-  const RayPlane rp = ray_planes(o, d);
+template <class T>
+__device__ __forceinline__ NewtonRes<T> newton_dispatch(const RayPlane<T>& rp, const float4* cp, int P, int Q, T gu,
+                                                        T gv, const ExactArgs<T>& A, uint32_t seed) {
+  if (P == 3 && Q == 3)
+    return newton<T>(rp, [&](T uu, T vv, v4<T>& du, v4<T>& dv, v4<T>& pt) { patch_eval<3, 3>(cp, uu, vv, du, dv, pt); },
+                     gu, gv, A.thr_det, A.thr_res, seed, A.max_iter);
+  if (P == 2 && Q == 2)
+    return newton<T>(rp, [&](T uu, T vv, v4<T>& du, v4<T>& dv, v4<T>& pt) { patch_eval<2, 2>(cp, uu, vv, du, dv, pt); },
+                     gu, gv, A.thr_det, A.thr_res, seed, A.max_iter);
+  return newton<T>(rp,
+                   [&](T uu, T vv, v4<T>& du, v4<T>& dv, v4<T>& pt) {
+                     patch_eval_rt<RTN_MAX_DEGREE>(cp, P, Q, uu, vv, du, dv, pt);
+                   },
+                   gu, gv, A.thr_det, A.thr_res, seed, A.max_iter);
+}
+
+template <int STACK, class T, class Patches>
+__device__ __forceinline__ void trace_one(const ExactArgs<T>& A, const Patches& PT, int64_t idx, int tid,
+                                          int (*stk)[TB], uint32_t cnt[8]) {
+  const v3<T> o{T(A.ox[idx]), T(A.oy[idx]), T(A.oz[idx])};
+  const v3<T> d{T(A.dx[idx]), T(A.dy[idx]), T(A.dz[idx])};
+  uint32_t nc = 0, ne = 0, sc = 0, niv = 0, nlv = 0, npt = 0, ovf = 0, nfl = 0;
+  const v3<T> rd{T(1) / d.x, T(1) / d.y, T(1) / d.z};
+  const RayPlane<T> rp = ray_planes(o, d);
   int off = 0;
-  float t_max = INFINITY;
+  T t_max = T(INFINITY), t_second = T(INFINITY);
   int best = -1;
-  float bt = 0.f, bu = 0.f, bv = 0.f;
-  v3 bsu{0.f, 0.f, 0.f}, bsv{0.f, 0.f, 0.f};
+  T bt = 0, bu = 0, bv = 0;
+  v3<T> bsu{0, 0, 0}, bsv{0, 0, 0};
   float cb[6];
   uint32_t lf, pc;
   load_node(A.nodes, 0, cb, lf, pc);
-  float it = slab6(cb, o, rd);
+  T it = slab6<T>(cb, o, rd);
   bool go = it < t_max;
   const uint32_t seed = (uint32_t)idx + A.seed_base;
+  const T slack = A.prune_slack;
   while (go) {
     if (pc > 0) {
       nlv++;
-      it = slab6(cb, o, rd);
+      it = slab6<T>(cb, o, rd);
       for (uint32_t i = 0; i < pc; i++) {
         const int pi = (int)(lf + i);
         bool do_newton = true;
@@ -101,20 +125,21 @@ __device__ __forceinline__ void trace_one(const ExactArgs& A, const Patches& PT,
         const float4* cp;
         int P, Q;
         PT.get(pi, cp, P, Q);
-        const v3 q00 = dehom(ld4(cp[0])), q10 = dehom(ld4(cp[P * (Q + 1)]));
-        const v3 q11 = dehom(ld4(cp[P * (Q + 1) + Q])), q01 = dehom(ld4(cp[Q]));
-        float gu, gv;
+        const v3<T> q00 = dehom(ld4<T>(cp[0])), q10 = dehom(ld4<T>(cp[P * (Q + 1)]));
+        const v3<T> q11 = dehom(ld4<T>(cp[P * (Q + 1) + Q])), q01 = dehom(ld4<T>(cp[Q]));
+        T gu, gv;
         bilinear_seed(q00, q10, q11, q01, o, d, gu, gv);
-        if (A.delta && A.delta[pi] * A.cam_k / it < A.flat) {  // flat-skip (intersect.cu:104-123)
-          v4 du, dv, pt;
+        if (A.delta && T(A.delta[pi]) * T(A.cam_k) / it < T(A.flat)) {  // flat-skip (intersect.cu:104-123)
+          v4<T> du, dv, pt;
           patch_eval_rt<RTN_MAX_DEGREE>(cp, P, Q, gu, gv, du, dv, pt);
-          v3 S, Su, Sv;
+          v3<T> S, Su, Sv;
           dehom_ders(pt, du, dv, S, Su, Sv);
-          const float resid = fabsf(dot(rp.n1, S) + rp.d1) + fabsf(dot(rp.n2, S) + rp.d2);
-          if (resid * A.cam_k / it < A.flat) {
+          const T resid = xabs(dot(rp.n1, S) + rp.d1) + xabs(dot(rp.n2, S) + rp.d2);
+          if (resid * T(A.cam_k) / it < T(A.flat)) {
             sc++;
-            const float t = dot(sub(S, o), d);
+            const T t = dot(sub(S, o), d);
             if (0 < t && t < t_max) {
+              t_second = t_max;
               t_max = t; best = pi; bt = t; bu = gu; bv = gv; bsu = Su; bsv = Sv;
               do_newton = false;
             }
@@ -122,26 +147,17 @@ __device__ __forceinline__ void trace_one(const ExactArgs& A, const Patches& PT,
         }
         if (do_newton) {
           nc++;
-          NewtonRes nr;
-          if (P == 3 && Q == 3) {
-            nr = newton(rp, [&](float uu, float vv, v4& du, v4& dv, v4& pt) { patch_eval<3, 3>(cp, uu, vv, du, dv, pt); },
-                        gu, gv, A.thr_det, A.thr_res, seed, A.max_iter);
-          } else if (P == 2 && Q == 2) {
-            nr = newton(rp, [&](float uu, float vv, v4& du, v4& dv, v4& pt) { patch_eval<2, 2>(cp, uu, vv, du, dv, pt); },
-                        gu, gv, A.thr_det, A.thr_res, seed, A.max_iter);
-          } else {
-            nr = newton(rp,
-                        [&](float uu, float vv, v4& du, v4& dv, v4& pt) {
-                          patch_eval_rt<RTN_MAX_DEGREE>(cp, P, Q, uu, vv, du, dv, pt);
-                        },
-                        gu, gv, A.thr_det, A.thr_res, seed, A.max_iter);
-          }
+          const NewtonRes<T> nr = newton_dispatch<T>(rp, cp, P, Q, gu, gv, A, seed);
           ne += nr.evals;
+          nfl += nr.evals * eval_flops(P, Q);
           if (nr.hit) {
-            const float t = (nr.S.x - o.x) * d.x + (nr.S.y - o.y) * d.y + (nr.S.z - o.z) * d.z;
+            const T t = (nr.S.x - o.x) * d.x + (nr.S.y - o.y) * d.y + (nr.S.z - o.z) * d.z;
             if (0 < t && t < t_max) {
+              t_second = t_max;
               best = pi; bt = t; bu = nr.u; bv = nr.v; bsu = nr.Su; bsv = nr.Sv;
               t_max = t;
+            } else if (0 < t && t < t_second) {
+              t_second = t;
             }
           }
         }
@@ -154,21 +170,22 @@ __device__ __forceinline__ void trace_one(const ExactArgs& A, const Patches& PT,
       uint32_t llf, lpc, rlf, rpc;
       load_node(A.nodes, (int)lf, lb, llf, lpc);
       load_node(A.nodes, (int)lf + 1, rb, rlf, rpc);
-      const float lt = slab6(lb, o, rd);
-      const float rt = slab6(rb, o, rd);
-      it = fminf(lt, rt);
-      if (it >= t_max) {
+      const T lt = slab6<T>(lb, o, rd);
+      const T rt = slab6<T>(rb, o, rd);
+      it = xmin(lt, rt);
+      const T lim = t_max * slack;
+      if (it >= lim) {
         if (off == 0) break;
         load_node(A.nodes, stk[--off][tid], cb, lf, pc);
       } else if (lt <= it) {
-        if (rt < t_max) {
+        if (rt < lim) {
           if (off < STACK) stk[off++][tid] = (int)lf + 1; else ovf = 1;
         }
 #pragma unroll
         for (int k = 0; k < 6; k++) cb[k] = lb[k];
         lf = llf; pc = lpc;
       } else {
-        if (lt < t_max) {
+        if (lt < lim) {
           if (off < STACK) stk[off++][tid] = (int)lf; else ovf = 1;
         }
 #pragma unroll
@@ -181,19 +198,20 @@ __device__ __forceinline__ void trace_one(const ExactArgs& A, const Patches& PT,
     rtn_surface_interaction* si = A.inter + idx;
     int ii = A.idx_interact[idx];
     if (best >= 0) {
-      si->t_hit = bt;
-      si->uv[0] = bu; si->uv[1] = bv;
-      si->dpdu[0] = bsu.x; si->dpdu[1] = bsu.y; si->dpdu[2] = bsu.z;
-      si->dpdv[0] = bsv.x; si->dpdv[1] = bsv.y; si->dpdv[2] = bsv.z;
+      si->t_hit = (float)bt;
+      si->uv[0] = (float)bu; si->uv[1] = (float)bv;
+      si->dpdu[0] = (float)bsu.x; si->dpdu[1] = (float)bsu.y; si->dpdu[2] = (float)bsu.z;
+      si->dpdv[0] = (float)bsv.x; si->dpdv[1] = (float)bsv.y; si->dpdv[2] = (float)bsv.z;
       ii = (int)idx;
       A.idx_interact[idx] = ii;
     }
     if (ii != -1) {
-      si->wo[0] = d.x * -1.0f; si->wo[1] = d.y * -1.0f; si->wo[2] = d.z * -1.0f;
+      si->wo[0] = (float)d.x * -1.0f; si->wo[1] = (float)d.y * -1.0f; si->wo[2] = (float)d.z * -1.0f;
     }
   }
   if (A.prim) {
     A.prim[idx] = best;
+    if (A.t2) A.t2[idx] = t_second;
     if (best >= 0) {
       if (A.t) A.t[idx] = bt;
       if (A.u) A.u[idx] = bu;
@@ -202,15 +220,16 @@ __device__ __forceinline__ void trace_one(const ExactArgs& A, const Patches& PT,
       if (A.dpdv) { A.dpdv[3 * idx] = bsv.x; A.dpdv[3 * idx + 1] = bsv.y; A.dpdv[3 * idx + 2] = bsv.z; }
     }
   }
+  cnt[0] = nc; cnt[1] = ne; cnt[2] = sc; cnt[3] = niv; cnt[4] = nlv; cnt[5] = npt; cnt[6] = ovf; cnt[7] = nfl;
   if (A.counters) {
     uint32_t* c = A.counters + 8 * idx;
-    c[0] = nc; c[1] = ne; c[2] = sc; c[3] = niv; c[4] = nlv; c[5] = npt; c[6] = ovf; c[7] = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) c[k] = cnt[k];
   }
-  cnt[0] = nc; cnt[1] = ne; cnt[2] = sc; cnt[3] = niv; cnt[4] = nlv; cnt[5] = npt; cnt[6] = ovf;
 }
 
-template <int STACK, bool GRID2D, class Patches>
-__global__ void __launch_bounds__(TB) k_trace_exact(ExactArgs A, Patches PT) {
+template <int STACK, bool GRID2D, class T, class Patches>
+__global__ void __launch_bounds__(TB) k_trace_exact(ExactArgs<T> A, Patches PT) {
   __shared__ int stk[STACK][TB];
   const int tid = threadIdx.x + threadIdx.y * blockDim.x;
   int64_t idx;
@@ -223,11 +242,11 @@ __global__ void __launch_bounds__(TB) k_trace_exact(ExactArgs A, Patches PT) {
     idx = (int64_t)blockIdx.x * TB + tid;
     valid = idx < A.n_rays;
   }
-  uint32_t cnt[7] = {0, 0, 0, 0, 0, 0, 0};
-  if (valid) trace_one<STACK>(A, PT, idx, tid, stk, cnt);
+  uint32_t cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (valid) trace_one<STACK, T>(A, PT, idx, tid, stk, cnt);
   if (A.totals) {  // every lane of the wave takes part in the reduction
 #pragma unroll
-    for (int k = 0; k < 7; k++) {
+    for (int k = 0; k < 8; k++) {
       unsigned long long x = cnt[k];
       for (int s = 32; s > 0; s >>= 1) x += __shfl_xor(x, s);
       if ((tid & 63) == 0 && x) atomicAdd(A.totals + k, x);
@@ -235,14 +254,26 @@ __global__ void __launch_bounds__(TB) k_trace_exact(ExactArgs A, Patches PT) {
   }
 }
 
+template <class T>
+int launch_scene(const rtn_scene* s, const ExactArgs<T>& A, int64_t n, hipStream_t stream) {
+  ScenePatches P{s->cp.ptr, s->cp_off.ptr, s->pq.ptr};
+  const unsigned blocks = (unsigned)((n + TB - 1) / TB);
+  if (s->max_depth < 32)
+    hipLaunchKernelGGL((k_trace_exact<32, false, T, ScenePatches>), dim3(blocks), dim3(TB), 0, stream, A, P);
+  else
+    hipLaunchKernelGGL((k_trace_exact<64, false, T, ScenePatches>), dim3(blocks), dim3(TB), 0, stream, A, P);
+  RTN_HIP(hipGetLastError());
+  return RTN_OK;
+}
+
 }  // namespace
 
 namespace rtn {
 
-// Launches the exact kernel over a 1-D ray array on scene buffers.
+// Launches the exact FP32 kernel over a 1-D ray array on scene buffers.
 int trace_exact_scene(const rtn_scene* s, const float* const rays[6], int64_t n, const rtn_trace_params* prm,
                       const rtn_hits* hits, unsigned long long* d_totals, hipStream_t stream) {
-  ExactArgs A{};
+  ExactArgs<float> A{};
   A.nodes = reinterpret_cast<const float4*>(s->nodes.ptr);
   A.ox = rays[0]; A.oy = rays[1]; A.oz = rays[2]; A.dx = rays[3]; A.dy = rays[4]; A.dz = rays[5];
   A.n_rays = n;
@@ -252,21 +283,39 @@ int trace_exact_scene(const rtn_scene* s, const float* const rays[6], int64_t n,
   A.max_iter = prm->max_iter;
   A.thr_det = prm->thr_det;
   A.thr_res = prm->thr_res;
+  A.prune_slack = 1.0f;
   A.seed_base = prm->seed_base;
   A.prim = hits->prim; A.t = hits->t; A.u = hits->u; A.v = hits->v; A.dpdu = hits->dpdu; A.dpdv = hits->dpdv;
   A.counters = prm->count ? hits->counters : nullptr;
   A.totals = d_totals;
-  ScenePatches P{s->cp.ptr, s->cp_off.ptr, s->pq.ptr};
-  const unsigned blocks = (unsigned)((n + TB - 1) / TB);
-  if (s->max_depth < 32)
-    hipLaunchKernelGGL((k_trace_exact<32, false, ScenePatches>), dim3(blocks), dim3(TB), 0, stream, A, P);
-  else
-    hipLaunchKernelGGL((k_trace_exact<64, false, ScenePatches>), dim3(blocks), dim3(TB), 0, stream, A, P);
-  RTN_HIP(hipGetLastError());
-  return RTN_OK;
+  return launch_scene<float>(s, A, n, stream);
 }
 
 }  // namespace rtn
+
+// FP64 reference trace: the plain method's traversal and seed in FP64 with
+// its own Newton cap and tolerances; boxes are pruned only beyond
+// t_max * (1 + 4e-5) so the second-nearest hit of near-tie pixels is found.
+extern "C" int rtn_trace_f64(const rtn_scene* s, const float* const rays[6], int64_t n, int max_iter, double thr_det,
+                             double thr_res, uint32_t seed_base, int* prim, double* t, double* u, double* v,
+                             double* dpdu, double* dpdv, double* t_second, uint32_t* counters, void* stream) {
+  RTN_CHECK(s && s->built && rays && prim, RTN_EINVAL, "rtn_trace_f64: bad arguments");
+  RTN_CHECK(max_iter >= 1 && max_iter <= 64, RTN_EINVAL, "rtn_trace_f64: max_iter outside 1..64");
+  ExactArgs<double> A{};
+  A.nodes = reinterpret_cast<const float4*>(s->nodes.ptr);
+  A.ox = rays[0]; A.oy = rays[1]; A.oz = rays[2]; A.dx = rays[3]; A.dy = rays[4]; A.dz = rays[5];
+  A.n_rays = n;
+  A.max_iter = max_iter;
+  A.thr_det = thr_det;
+  A.thr_res = thr_res;
+  A.prune_slack = 1.0 + 4e-5;
+  A.seed_base = seed_base;
+  A.prim = prim; A.t = t; A.u = u; A.v = v; A.dpdu = dpdu; A.dpdv = dpdv; A.t2 = t_second;
+  A.counters = counters;
+  RTN_TRY(launch_scene<double>(s, A, n, (hipStream_t)stream));
+  if (!stream) RTN_HIP(hipStreamSynchronize(nullptr));
+  return RTN_OK;
+}
 
 extern "C" int rtn_calculate_intersect(const rtn_ray_queue* rays, const rtn_bvh_node* nodes,
                                        const rtn_bezier_patch* patches, rtn_surface_interaction* interact, int W,
@@ -275,7 +324,7 @@ extern "C" int rtn_calculate_intersect(const rtn_ray_queue* rays, const rtn_bvh_
   RTN_CHECK(rays && nodes && patches && interact && W > 0 && H > 0, RTN_EINVAL,
             "rtn_calculate_intersect: bad arguments");
   RTN_CHECK(rays->idx_interact, RTN_EINVAL, "rtn_calculate_intersect: idx_interact is required");
-  ExactArgs A{};
+  ExactArgs<float> A{};
   A.nodes = reinterpret_cast<const float4*>(nodes);
   A.ox = rays->origin_x; A.oy = rays->origin_y; A.oz = rays->origin_z;
   A.dx = rays->direction_x; A.dy = rays->direction_y; A.dz = rays->direction_z;
@@ -288,6 +337,7 @@ extern "C" int rtn_calculate_intersect(const rtn_ray_queue* rays, const rtn_bvh_
   A.max_iter = 7;        // NEWTON_MAX_ITERATIONS (constants.h:40)
   A.thr_det = 1e-5f;     // intersect.cu:127
   A.thr_res = 1e-4f;
+  A.prune_slack = 1.0f;
   A.seed_base = 0;
   A.inter = interact;
   A.idx_interact = rays->idx_interact;
@@ -297,35 +347,29 @@ extern "C" int rtn_calculate_intersect(const rtn_ray_queue* rays, const rtn_bvh_
   A.totals = tot;
   DropInPatches P{patches};
   dim3 block(16, 8), grid((W + 15) / 16, (H + 7) / 8);
-  hipLaunchKernelGGL((k_trace_exact<32, true, DropInPatches>), grid, block, 0, nullptr, A, P);
+  hipLaunchKernelGGL((k_trace_exact<32, true, float, DropInPatches>), grid, block, 0, nullptr, A, P);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipDeviceSynchronize();
   int rc = RTN_OK;
   if (e != hipSuccess) rc = hip_fail(e, "calculateIntersect launch", __FILE__, __LINE__);
-  if (tot) {
-    unsigned long long h[8];
-    if (rc == RTN_OK && hipMemcpy(h, tot, sizeof(h), hipMemcpyDeviceToHost) == hipSuccess) {
-      // counters are added to, as the reference's atomicAdd does (intersect.cu:189-190)
-      if (newton_count) {
-        unsigned long long cur;
-        if (hipMemcpy(&cur, newton_count, 8, hipMemcpyDefault) == hipSuccess) {
-          cur += h[0];
-          (void)hipMemcpy(newton_count, &cur, 8, hipMemcpyDefault);
-        }
-      }
-      if (skip_count) {
-        unsigned long long cur;
-        if (hipMemcpy(&cur, skip_count, 8, hipMemcpyDefault) == hipSuccess) {
-          cur += h[2];
-          (void)hipMemcpy(skip_count, &cur, 8, hipMemcpyDefault);
-        }
-      }
-      if (h[6]) {
-        set_error("calculateIntersect: traversal stack (32 entries) overflowed on %llu rays", h[6]);
-        rc = RTN_EOVERFLOW;
+  unsigned long long h[8];
+  if (rc == RTN_OK && hipMemcpy(h, tot, sizeof(h), hipMemcpyDeviceToHost) == hipSuccess) {
+    // counters are added to, as the reference's atomicAdd does (intersect.cu:189-190)
+    unsigned long long* outs[2] = {newton_count, skip_count};
+    const unsigned long long adds[2] = {h[0], h[2]};
+    for (int k = 0; k < 2; k++) {
+      if (!outs[k]) continue;
+      unsigned long long cur;
+      if (hipMemcpy(&cur, outs[k], 8, hipMemcpyDefault) == hipSuccess) {
+        cur += adds[k];
+        (void)hipMemcpy(outs[k], &cur, 8, hipMemcpyDefault);
       }
     }
-    (void)hipFree(tot);
+    if (h[6]) {
+      set_error("calculateIntersect: traversal stack (32 entries) overflowed on %llu rays", h[6]);
+      rc = RTN_EOVERFLOW;
+    }
   }
+  (void)hipFree(tot);
   return rc;
 }

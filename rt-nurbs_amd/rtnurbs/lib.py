@@ -32,7 +32,8 @@ EXPORTS = [
     "rtn_camera_perspective", "rtn_rays_perspective", "rtn_scene_create", "rtn_scene_destroy",
     "rtn_scene_add_nurbs", "rtn_scene_add_patches", "rtn_scene_build", "rtn_scene_stats_get",
     "rtn_scene_download", "rtn_trace_params_default", "rtn_trace", "rtn_calculate_intersect", "rtn_get_patch",
-    "rtn_build_bvh", "rtn_patch_bounds", "rtn_refine",
+    "rtn_build_bvh", "rtn_patch_bounds", "rtn_refine", "rtn_trace_f64", "rtn_event_create", "rtn_event_destroy",
+    "rtn_event_record", "rtn_event_elapsed_ms",
 ]
 
 
@@ -62,7 +63,8 @@ class Hits(C.Structure):
 
 class TraceTotals(C.Structure):
     _fields_ = [(k, C.c_ulonglong) for k in ("newton_calls", "newton_evals", "skips", "interior", "leaves",
-                                             "patch_tests", "overflow", "fp64_steps")]
+                                             "patch_tests", "overflow", "newton_flops", "fp64_steps",
+                                             "hull_culled", "learned_fallbacks", "reserved")]
 
 
 class SceneStats(C.Structure):
@@ -111,6 +113,12 @@ def load():
     L.rtn_build_bvh.argtypes = [vp, i, vp, vp, C.POINTER(i)]
     L.rtn_patch_bounds.argtypes = [vp, i, i, i, vp]
     L.rtn_refine.argtypes = [i, i, i, vp, i, vp, f, C.POINTER(i), vp, vp]
+    L.rtn_trace_f64.argtypes = [vp, C.POINTER(vp * 6), C.c_int64, i, C.c_double, C.c_double, C.c_uint32,
+                                vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.rtn_event_create.argtypes = [C.POINTER(vp)]
+    L.rtn_event_destroy.argtypes = [vp]
+    L.rtn_event_record.argtypes = [vp, vp]
+    L.rtn_event_elapsed_ms.argtypes = [vp, vp, C.POINTER(C.c_float)]
     _lib = L
     return L
 
@@ -175,6 +183,33 @@ class DeviceArray:
 
 def synchronize():
     check(load().rtn_device_synchronize())
+
+
+class Event:
+    def __init__(self):
+        e = C.c_void_p()
+        check(load().rtn_event_create(C.byref(e)))
+        self.h = e.value
+
+    def record(self, stream=None):
+        check(load().rtn_event_record(self.h, stream))
+
+    def elapsed_ms(self, later):
+        ms = C.c_float()
+        check(load().rtn_event_elapsed_ms(self.h, later.h, C.byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        try:
+            load().rtn_event_destroy(self.h)
+        except Exception:
+            pass
+
+
+def stream_create():
+    s = C.c_void_p()
+    check(load().rtn_stream_create(C.byref(s)))
+    return s.value
 
 
 # ---------------------------------------------------------------- camera
@@ -381,3 +416,33 @@ def refine(s, C_u=None, C_v=None):
                            P2.ctypes.data))
         cur = replace(cur, n=cur.n + tot, U=nk, Pw=P2) if d == 0 else replace(cur, m=cur.m + tot, V=nk, Pw=P2)
     return cur
+
+
+class HitBuffersF64:
+    def __init__(self, n, counters=False):
+        self.n = n
+        self.prim = DeviceArray((n,), np.int32)
+        self.t = DeviceArray((n,), np.float64)
+        self.u = DeviceArray((n,), np.float64)
+        self.v = DeviceArray((n,), np.float64)
+        self.dpdu = DeviceArray((n, 3), np.float64)
+        self.dpdv = DeviceArray((n, 3), np.float64)
+        self.t2 = DeviceArray((n,), np.float64)
+        self.counters = DeviceArray((n, 8), np.uint32) if counters else None
+
+    def get(self):
+        d = dict(hit=self.prim.get(), t=self.t.get(), u=self.u.get(), v=self.v.get(), dpdu=self.dpdu.get(),
+                 dpdv=self.dpdv.get(), t2=self.t2.get())
+        if self.counters is not None:
+            d["counters"] = self.counters.get()
+        return d
+
+
+def trace_f64(scene, rays, max_iter=32, thr_det=1e-12, thr_res=1e-10, seed_base=0, counters=False, stream=None):
+    """FP64 reference trace on the device (plain method evaluated in FP64)."""
+    h = HitBuffersF64(rays.n, counters)
+    arr = rays.ptr_array()
+    check(load().rtn_trace_f64(scene.h, C.byref(arr), rays.n, max_iter, thr_det, thr_res, seed_base, h.prim.ptr,
+                               h.t.ptr, h.u.ptr, h.v.ptr, h.dpdu.ptr, h.dpdv.ptr, h.t2.ptr,
+                               h.counters.ptr if h.counters else None, stream))
+    return h

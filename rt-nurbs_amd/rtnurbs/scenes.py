@@ -196,3 +196,18 @@ def patch_count(s: Nurbs) -> int:
         inner = K[p + 1:n + 1]
         return len(np.unique(inner)) + 1 if len(inner) else 1
     return nseg(s.U, s.n, s.p) * nseg(s.V, s.m, s.q)
+
+
+C2_REFINE = 0.264   # curvature-refinement constant freezing config 2 at 10,018 patches
+C2_PATCHES = 10018
+
+
+def config2(seed: int = 0, W=1920, H=1080, n_surf: int = 64) -> Scene:
+    """64 seeded random NURBS (degree 2..5 per direction, nets 6-16 per side,
+    random interior multiplicities, random rigid placement in the 10^3 box),
+    refined by the reference's curvatureRefinementU/V (nurbs.h:290-486) at
+    C = 0.264 so the scene decomposes into 10,018 rational Bezier patches."""
+    from . import lib
+    base = config2_base(seed, n_surf)
+    surfs = [lib.refine(s, C2_REFINE, C2_REFINE) for s in base]
+    return Scene(surfs, c2_camera(W, H), f"c2-seed{seed}")
