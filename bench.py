@@ -29,14 +29,16 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def ray_flops(tot, n_rays):
+def ray_flops(tot, n_rays, exact):
     """Algorithmic FP32 operations of a trace, priced by a fixed table on the
     work the run performed: 40 per ray (reciprocal direction and ray planes),
-    66 per interior visit (two slab tests), 32 per leaf visit, 180 per patch
-    test (corners and bilinear seed), plus each Newton evaluation priced at
-    its degree (counted by the kernel)."""
-    return (40.0 * n_rays + 66.0 * tot["interior"] + 32.0 * tot["leaves"] + 180.0 * tot["patch_tests"]
-            + float(tot["newton_flops"]))
+    33 per box test (66 per binary interior visit, whose two child boxes are
+    tested), 32 per binary leaf visit, 180 per patch test (corners and
+    bilinear seed), plus each Newton evaluation priced at its degree by the
+    reference algorithm's operation count (counted by the kernel).  Work a
+    form avoids (culled patches, fewer iterations) is not priced."""
+    boxes = 66.0 * tot["interior"] + 32.0 * tot["leaves"] if exact else 33.0 * tot["interior"]
+    return 40.0 * n_rays + boxes + 180.0 * tot["patch_tests"] + float(tot["newton_flops"])
 
 
 def error_stats(g, ref, tie_rel=1e-5):
@@ -148,8 +150,14 @@ def main():
     n = rays.n
     stream = L.stream_create()
 
+    base = dict(max_iter=8, thr_res=1e-5, thr_det=1e-5)
     forms = {
-        "plain": dict(mode=L.MODE_EXACT, max_iter=8, thr_res=1e-5, thr_det=1e-5),
+        "plain": dict(mode=L.MODE_EXACT, **base),
+        "wide": dict(mode=L.MODE_WIDE, **base),
+        "wide_cull": dict(mode=L.MODE_WIDE, hull_cull=1, **base),
+        "persistent_cull": dict(mode=L.MODE_PERSISTENT, hull_cull=1, **base),
+        "persistent_cull_mixed": dict(mode=L.MODE_PERSISTENT, hull_cull=1, precision=L.PREC_MIXED, cond_thr=1e4,
+                                      **base),
     }
     if args.forms != "all":
         keep = set(args.forms.split(","))
@@ -169,10 +177,11 @@ def main():
         hits_c, tot = L.trace(scene, rays, prm_c)
         g = hits_c.get()
         hits, kern_ms, wall_ms = time_form(L, scene, rays, prm, args.steps, args.warmup, stream)
-        flops = ray_flops(tot, n)
+        flops = ray_flops(tot, n, kw["mode"] == L.MODE_EXACT)
         r = {"mrays_s": n / (kern_ms * 1e-3) / 1e6, "kernel_ms": kern_ms, "wall_ms": wall_ms,
              "newton_iters_per_ray": tot["newton_evals"] / n, "newton_calls_per_ray": tot["newton_calls"] / n,
              "patch_tests_per_ray": tot["patch_tests"] / n, "flops_per_ray": flops / n,
+             "hull_culled_per_ray": tot["hull_culled"] / n, "fp64_steps": tot["fp64_steps"],
              "tflops": flops / (kern_ms * 1e-3) / 1e12}
         if ref is not None:
             r["fp64"] = error_stats(g, ref)

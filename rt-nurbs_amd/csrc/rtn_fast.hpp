@@ -1,0 +1,493 @@
+// FP32 arithmetic of the fast trace forms: explicit FMA, tensor-product
+// Bernstein evaluation with the control net held in VGPRs, and the Newton
+// iteration of the plain method (same seed handling, PCG perturbation on a
+// singular Jacobian, clamping) in that arithmetic.  Results agree with the
+// reference-order path to rounding; they are checked against the FP64
+// reference, not bit for bit.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "rtn_exact.hpp"
+
+namespace rtn {
+namespace fast {
+
+using ex::pcg_hash;
+
+struct f3 { float x, y, z; };
+
+__device__ __forceinline__ float dot3(f3 a, f3 b) { return fmaf(a.x, b.x, fmaf(a.y, b.y, a.z * b.z)); }
+
+// Bernstein basis of degree N at t and its derivative, by the scalar
+// de Casteljau triangle: B^{N-1} first, then dB^N_i = N (B^{N-1}_{i-1} - B^{N-1}_i)
+// and B^N from one more step.
+template <int N>
+__device__ __forceinline__ void bernstein(float t, float (&b)[N + 1], float (&db)[N + 1]) {
+  const float s = 1.0f - t;
+  float c[N + 1];
+  c[0] = 1.0f;
+#pragma unroll
+  for (int k = 1; k < N; k++) {
+    c[k] = t * c[k - 1];
+#pragma unroll
+    for (int i = k - 1; i >= 1; i--) c[i] = fmaf(s, c[i], t * c[i - 1]);
+    c[0] = s * c[0];
+  }
+  // c = B^{N-1}
+#pragma unroll
+  for (int i = 0; i <= N; i++) {
+    const float lo = i > 0 ? c[i - 1] : 0.0f;
+    const float hi = i < N ? c[i] : 0.0f;
+    db[i] = (float)N * (lo - hi);
+    b[i] = fmaf(s, hi, t * lo);
+  }
+}
+
+// Homogeneous point and partials of a degree (P,Q) patch, net row-major
+// net[i*(Q+1)+j], streamed row by row from memory (L1/L2 resident; rays of a
+// wave that test the same patch share the cache lines).
+template <int P, int Q>
+__device__ __forceinline__ void eval_bern(const float4* __restrict__ net, float u, float v, float4& pt, float4& du,
+                                          float4& dv) {
+  float bu[P + 1], dbu[P + 1], bv[Q + 1], dbv[Q + 1];
+  bernstein<P>(u, bu, dbu);
+  bernstein<Q>(v, bv, dbv);
+  pt = make_float4(0.f, 0.f, 0.f, 0.f);
+  du = pt;
+  dv = pt;
+#pragma unroll
+  for (int i = 0; i <= P; i++) {
+    float4 r = make_float4(0.f, 0.f, 0.f, 0.f), rv = r;
+    const float4* row = net + i * (Q + 1);
+    asm volatile("" : "+v"(row));  // the row's loads are issued here, not hoisted with the whole net
+#pragma unroll
+    for (int j = 0; j <= Q; j++) {
+      const float4 c = row[j];
+      r.x = fmaf(bv[j], c.x, r.x); r.y = fmaf(bv[j], c.y, r.y); r.z = fmaf(bv[j], c.z, r.z); r.w = fmaf(bv[j], c.w, r.w);
+      rv.x = fmaf(dbv[j], c.x, rv.x); rv.y = fmaf(dbv[j], c.y, rv.y);
+      rv.z = fmaf(dbv[j], c.z, rv.z); rv.w = fmaf(dbv[j], c.w, rv.w);
+    }
+    pt.x = fmaf(bu[i], r.x, pt.x); pt.y = fmaf(bu[i], r.y, pt.y); pt.z = fmaf(bu[i], r.z, pt.z); pt.w = fmaf(bu[i], r.w, pt.w);
+    du.x = fmaf(dbu[i], r.x, du.x); du.y = fmaf(dbu[i], r.y, du.y);
+    du.z = fmaf(dbu[i], r.z, du.z); du.w = fmaf(dbu[i], r.w, du.w);
+    dv.x = fmaf(bu[i], rv.x, dv.x); dv.y = fmaf(bu[i], rv.y, dv.y);
+    dv.z = fmaf(bu[i], rv.z, dv.z); dv.w = fmaf(bu[i], rv.w, dv.w);
+  }
+}
+
+struct Plane2 { f3 n1, n2; float d1, d2; };
+
+// Ray planes (math_utils.h:33-54) in FMA arithmetic.
+__device__ __forceinline__ Plane2 planes(f3 o, f3 d) {
+  const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+  const bool xd = (ax > ay) && (ax > az);
+  const float b0 = xd ? d.x : d.y, b1 = xd ? d.y : d.z;
+  const float den = 1.0f / sqrtf(fmaf(b0, b0, b1 * b1));
+  f3 n1 = xd ? f3{d.y * den, -d.x * den, 0.f} : f3{0.f, d.z * den, -d.y * den};
+  const float il = 1.0f / sqrtf(dot3(d, d));
+  const f3 dn{d.x * il, d.y * il, d.z * il};
+  f3 n2{fmaf(n1.y, dn.z, -n1.z * dn.y), fmaf(n1.z, dn.x, -n1.x * dn.z), fmaf(n1.x, dn.y, -n1.y * dn.x)};
+  return {n1, n2, -dot3(n1, o), -dot3(n2, o)};
+}
+
+struct Hit {
+  bool hit;
+  float u, v;
+  f3 S, Su, Sv;
+  float m0, m1, m2, m3;  // Jacobian at the returned iterate
+  int evals;
+};
+
+__device__ __forceinline__ void dehom(float4 pt, float4 du, float4 dv, f3& S, f3& Su, f3& Sv) {
+  const float iw = 1.0f / pt.w;
+  S = {pt.x * iw, pt.y * iw, pt.z * iw};
+  Su = {fmaf(-S.x, du.w, du.x) * iw, fmaf(-S.y, du.w, du.y) * iw, fmaf(-S.z, du.w, du.z) * iw};
+  Sv = {fmaf(-S.x, dv.w, dv.x) * iw, fmaf(-S.y, dv.w, dv.y) * iw, fmaf(-S.z, dv.w, dv.z) * iw};
+}
+
+// Newton iteration of the plain method (bezier_patch.h:198-266) on a net in
+// registers.  Converged when |r0|+|r1| < thr_res, returning the pre-update
+// iterate; on |det J| < thr_det the iterate is pulled towards the seed by the
+// PCG draws; iterates are clamped to [0,1]^2.
+template <int P, int Q>
+__device__ __forceinline__ Hit newton(const float4* __restrict__ net, const Plane2& rp, float u0, float v0,
+                                      float thr_det, float thr_res, uint32_t seed, int max_iter) {
+  float u = u0, v = v0;
+  Hit h;
+  h.hit = false;
+  h.evals = 0;
+#pragma unroll 1
+  for (int it = 1; it <= max_iter; it++) {
+    float4 pt, du, dv;
+    eval_bern<P, Q>(net, u, v, pt, du, dv);
+    h.evals = it;
+    f3 S, Su, Sv;
+    dehom(pt, du, dv, S, Su, Sv);
+    const float r0 = dot3(rp.n1, S) + rp.d1;
+    const float r1 = dot3(rp.n2, S) + rp.d2;
+    const float m0 = dot3(rp.n1, Su), m1 = dot3(rp.n2, Su), m2 = dot3(rp.n1, Sv), m3 = dot3(rp.n2, Sv);
+    if (fabsf(r0) + fabsf(r1) < thr_res) {
+      h.hit = true;
+      h.u = u; h.v = v; h.S = S; h.Su = Su; h.Sv = Sv;
+      h.m0 = m0; h.m1 = m1; h.m2 = m2; h.m3 = m3;
+      return h;
+    }
+    seed = pcg_hash(seed);
+    const float rand1 = (float)seed * 2.3283064e-10f;
+    seed = pcg_hash(seed);
+    const float rand2 = (float)seed * 2.3283064e-10f;
+    const float det = fmaf(m0, m3, -m2 * m1);
+    if (fabsf(det) < thr_det) {
+      u = fmaf(0.1f, (u0 - u) * rand1, u);
+      v = fmaf(0.1f, (v0 - v) * rand2, v);
+    } else {
+      const float id = 1.0f / det;
+      u = u - fmaf(m3, r0, -m2 * r1) * id;
+      v = v - fmaf(-m1, r0, m0 * r1) * id;
+    }
+    u = fminf(fmaxf(u, 0.0f), 1.0f);
+    v = fminf(fmaxf(v, 0.0f), 1.0f);
+  }
+  return h;
+}
+
+// Exact convex-hull rejection: with positive weights every surface point is a
+// convex combination of the dehomogenised control points, so if all of them
+// lie farther than `margin` on one side of either ray plane, no iterate can
+// reach a residual |r0|+|r1| below margin.  Tested as n.P.xyz + d*P.w against
+// margin*P.w to avoid the divisions.
+template <int P, int Q>
+__device__ __forceinline__ bool hull_culled(const float4* __restrict__ net, const Plane2& rp, float margin) {
+  bool pos1 = true, neg1 = true, pos2 = true, neg2 = true;
+#pragma unroll
+  for (int k = 0; k < (P + 1) * (Q + 1); k++) {
+    const float4 c = net[k];
+    const float s1 = fmaf(rp.n1.x, c.x, fmaf(rp.n1.y, c.y, fmaf(rp.n1.z, c.z, rp.d1 * c.w)));
+    const float s2 = fmaf(rp.n2.x, c.x, fmaf(rp.n2.y, c.y, fmaf(rp.n2.z, c.z, rp.d2 * c.w)));
+    const float mw = margin * c.w;
+    pos1 = pos1 && (s1 > mw);
+    neg1 = neg1 && (s1 < -mw);
+    pos2 = pos2 && (s2 > mw);
+    neg2 = neg2 && (s2 < -mw);
+  }
+  return pos1 || neg1 || pos2 || neg2;
+}
+
+// 2-norm condition number of [m0 m2; m1 m3] from its Frobenius norm and
+// determinant: cond = (F^2 + sqrt(F^4 - 4 det^2)) / (2 |det|).
+__device__ __forceinline__ float cond2(float m0, float m1, float m2, float m3) {
+  const float f2 = fmaf(m0, m0, fmaf(m1, m1, fmaf(m2, m2, m3 * m3)));
+  const float det = fmaf(m0, m3, -m2 * m1);
+  const float disc = fmaxf(fmaf(f2, f2, -4.0f * det * det), 0.0f);
+  return (f2 + sqrtf(disc)) / (2.0f * fabsf(det));
+}
+
+}  // namespace fast
+}  // namespace rtn
+
+namespace rtn {
+namespace fast {
+
+// FP64 correction of an ill-conditioned FP32 hit: one Newton step in FP64 from
+// the FP32 (u,v) (evaluation, residual, Jacobian solve, no clamping), then the
+// corrected point and partials.  Returns false if the FP64 Jacobian is singular.
+template <int N>
+__device__ __forceinline__ void bernstein64(double t, double (&b)[N + 1], double (&db)[N + 1]) {
+  const double s = 1.0 - t;
+  double c[N + 1];
+  c[0] = 1.0;
+#pragma unroll
+  for (int k = 1; k < N; k++) {
+    c[k] = t * c[k - 1];
+#pragma unroll
+    for (int i = k - 1; i >= 1; i--) c[i] = fma(s, c[i], t * c[i - 1]);
+    c[0] = s * c[0];
+  }
+#pragma unroll
+  for (int i = 0; i <= N; i++) {
+    const double lo = i > 0 ? c[i - 1] : 0.0;
+    const double hi = i < N ? c[i] : 0.0;
+    db[i] = (double)N * (lo - hi);
+    b[i] = fma(s, hi, t * lo);
+  }
+}
+
+struct D3 { double x, y, z; };
+
+template <int P, int Q>
+__device__ __forceinline__ void eval64(const float4* __restrict__ net, double u, double v, D3& S, D3& Su, D3& Sv) {
+  double bu[P + 1], dbu[P + 1], bv[Q + 1], dbv[Q + 1];
+  bernstein64<P>(u, bu, dbu);
+  bernstein64<Q>(v, bv, dbv);
+  double pt[4] = {0, 0, 0, 0}, du[4] = {0, 0, 0, 0}, dv[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int i = 0; i <= P; i++) {
+    double r[4] = {0, 0, 0, 0}, rv[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int j = 0; j <= Q; j++) {
+      const float4 c = net[i * (Q + 1) + j];
+      const double cc[4] = {c.x, c.y, c.z, c.w};
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        r[k] = fma(bv[j], cc[k], r[k]);
+        rv[k] = fma(dbv[j], cc[k], rv[k]);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      pt[k] = fma(bu[i], r[k], pt[k]);
+      du[k] = fma(dbu[i], r[k], du[k]);
+      dv[k] = fma(bu[i], rv[k], dv[k]);
+    }
+  }
+  const double iw = 1.0 / pt[3];
+  S = {pt[0] * iw, pt[1] * iw, pt[2] * iw};
+  Su = {(du[0] - S.x * du[3]) * iw, (du[1] - S.y * du[3]) * iw, (du[2] - S.z * du[3]) * iw};
+  Sv = {(dv[0] - S.x * dv[3]) * iw, (dv[1] - S.y * dv[3]) * iw, (dv[2] - S.z * dv[3]) * iw};
+}
+
+// One FP64 Newton step on the FP64 ray planes from (u,v), then the corrected
+// point: the mixed-precision form's correction for cond(J) above threshold.
+template <int P, int Q>
+__device__ __noinline__ bool correct64(const float4* __restrict__ net, f3 o, f3 d, float& uo, float& vo,
+                                          f3& So, f3& Suo, f3& Svo, double& t_out) {
+  // FP64 ray planes from the FP32 ray
+  const double ox = o.x, oy = o.y, oz = o.z, dx = d.x, dy = d.y, dz = d.z;
+  const double ax = fabs(dx), ay = fabs(dy), az = fabs(dz);
+  const bool xd = (ax > ay) && (ax > az);
+  const double b0 = xd ? dx : dy, b1 = xd ? dy : dz;
+  const double den = 1.0 / sqrt(b0 * b0 + b1 * b1);
+  const D3 n1 = xd ? D3{dy * den, -dx * den, 0.0} : D3{0.0, dz * den, -dy * den};
+  const double il = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+  const D3 dn{dx * il, dy * il, dz * il};
+  const D3 n2{n1.y * dn.z - n1.z * dn.y, n1.z * dn.x - n1.x * dn.z, n1.x * dn.y - n1.y * dn.x};
+  const double d1 = -(n1.x * ox + n1.y * oy + n1.z * oz), d2 = -(n2.x * ox + n2.y * oy + n2.z * oz);
+  double u = uo, v = vo;
+  D3 S, Su, Sv;
+  eval64<P, Q>(net, u, v, S, Su, Sv);
+  const double r0 = n1.x * S.x + n1.y * S.y + n1.z * S.z + d1;
+  const double r1 = n2.x * S.x + n2.y * S.y + n2.z * S.z + d2;
+  const double m0 = n1.x * Su.x + n1.y * Su.y + n1.z * Su.z, m1 = n2.x * Su.x + n2.y * Su.y + n2.z * Su.z;
+  const double m2 = n1.x * Sv.x + n1.y * Sv.y + n1.z * Sv.z, m3 = n2.x * Sv.x + n2.y * Sv.y + n2.z * Sv.z;
+  const double det = m0 * m3 - m2 * m1;
+  if (det == 0.0) return false;
+  u -= (m3 * r0 - m2 * r1) / det;
+  v -= (-m1 * r0 + m0 * r1) / det;
+  eval64<P, Q>(net, u, v, S, Su, Sv);
+  uo = (float)u;
+  vo = (float)v;
+  So = {(float)S.x, (float)S.y, (float)S.z};
+  Suo = {(float)Su.x, (float)Su.y, (float)Su.z};
+  Svo = {(float)Sv.x, (float)Sv.y, (float)Sv.z};
+  t_out = (S.x - ox) * dx + (S.y - oy) * dy + (S.z - oz) * dz;
+  return true;
+}
+
+}  // namespace fast
+}  // namespace rtn
+
+namespace rtn {
+namespace fast {
+
+constexpr int MAXD = 5;
+
+// Bernstein basis of runtime degree n <= MAXD at t (and its derivative) into
+// b[0..MAXD], db[0..MAXD]; entries above n are zero.  Guarded unrolled loops:
+// for a wave-uniform n the guards are scalar branches.
+__device__ __forceinline__ void bernstein_rt(int n, float t, float (&b)[MAXD + 1], float (&db)[MAXD + 1]) {
+  const float s = 1.0f - t;
+  float c[MAXD + 1];
+#pragma unroll
+  for (int i = 0; i <= MAXD; i++) c[i] = 0.0f;
+  c[0] = 1.0f;
+#pragma unroll
+  for (int k = 1; k < MAXD; k++) {
+    if (k < n) {
+      c[k] = t * c[k - 1];
+#pragma unroll
+      for (int i = k - 1; i >= 1; i--) c[i] = fmaf(s, c[i], t * c[i - 1]);
+      c[0] = s * c[0];
+    }
+  }
+  const float fn = (float)n;
+#pragma unroll
+  for (int i = 0; i <= MAXD; i++) {
+    const float lo = i > 0 ? c[i - 1] : 0.0f;
+    const float hi = i < n ? c[i] : 0.0f;
+    db[i] = i <= n ? fn * (lo - hi) : 0.0f;
+    b[i] = i <= n ? fmaf(s, hi, t * lo) : 0.0f;
+  }
+}
+
+// Point and partials of a patch with runtime row count P+1 and compile-time
+// row length Q+1: rows stream from memory; the u-direction weights rotate
+// through registers (weight of row i is always in bu[0]).
+template <int Q>
+__device__ __forceinline__ void eval_q(const float4* __restrict__ net, int P, float u, float v, float4& pt,
+                                       float4& du, float4& dv) {
+  float bu[MAXD + 1], dbu[MAXD + 1], bv[Q + 1], dbv[Q + 1];
+  bernstein_rt(P, u, bu, dbu);
+  bernstein<Q>(v, bv, dbv);
+  pt = make_float4(0.f, 0.f, 0.f, 0.f);
+  du = pt;
+  dv = pt;
+#pragma unroll 1
+  for (int i = 0; i <= P; i++) {
+    const float4* row = net + i * (Q + 1);
+    float4 r = make_float4(0.f, 0.f, 0.f, 0.f), rv = r;
+#pragma unroll
+    for (int j = 0; j <= Q; j++) {
+      const float4 c = row[j];
+      r.x = fmaf(bv[j], c.x, r.x); r.y = fmaf(bv[j], c.y, r.y); r.z = fmaf(bv[j], c.z, r.z); r.w = fmaf(bv[j], c.w, r.w);
+      rv.x = fmaf(dbv[j], c.x, rv.x); rv.y = fmaf(dbv[j], c.y, rv.y);
+      rv.z = fmaf(dbv[j], c.z, rv.z); rv.w = fmaf(dbv[j], c.w, rv.w);
+    }
+    const float b = bu[0], db = dbu[0];
+    pt.x = fmaf(b, r.x, pt.x); pt.y = fmaf(b, r.y, pt.y); pt.z = fmaf(b, r.z, pt.z); pt.w = fmaf(b, r.w, pt.w);
+    du.x = fmaf(db, r.x, du.x); du.y = fmaf(db, r.y, du.y); du.z = fmaf(db, r.z, du.z); du.w = fmaf(db, r.w, du.w);
+    dv.x = fmaf(b, rv.x, dv.x); dv.y = fmaf(b, rv.y, dv.y); dv.z = fmaf(b, rv.z, dv.z); dv.w = fmaf(b, rv.w, dv.w);
+#pragma unroll
+    for (int k = 0; k < MAXD; k++) {
+      bu[k] = bu[k + 1];
+      dbu[k] = dbu[k + 1];
+    }
+  }
+}
+
+template <int Q>
+__device__ __forceinline__ Hit newton_q(const float4* __restrict__ net, int P, const Plane2& rp, float u0, float v0,
+                                        float thr_det, float thr_res, uint32_t seed, int max_iter) {
+  float u = u0, v = v0;
+  Hit h;
+  h.hit = false;
+  h.evals = 0;
+#pragma unroll 1
+  for (int it = 1; it <= max_iter; it++) {
+    float4 pt, du, dv;
+    eval_q<Q>(net, P, u, v, pt, du, dv);
+    h.evals = it;
+    f3 S, Su, Sv;
+    dehom(pt, du, dv, S, Su, Sv);
+    const float r0 = dot3(rp.n1, S) + rp.d1;
+    const float r1 = dot3(rp.n2, S) + rp.d2;
+    const float m0 = dot3(rp.n1, Su), m1 = dot3(rp.n2, Su), m2 = dot3(rp.n1, Sv), m3 = dot3(rp.n2, Sv);
+    if (fabsf(r0) + fabsf(r1) < thr_res) {
+      h.hit = true;
+      h.u = u; h.v = v; h.S = S; h.Su = Su; h.Sv = Sv;
+      h.m0 = m0; h.m1 = m1; h.m2 = m2; h.m3 = m3;
+      return h;
+    }
+    seed = pcg_hash(seed);
+    const float rand1 = (float)seed * 2.3283064e-10f;
+    seed = pcg_hash(seed);
+    const float rand2 = (float)seed * 2.3283064e-10f;
+    const float det = fmaf(m0, m3, -m2 * m1);
+    if (fabsf(det) < thr_det) {
+      u = fmaf(0.1f, (u0 - u) * rand1, u);
+      v = fmaf(0.1f, (v0 - v) * rand2, v);
+    } else {
+      const float id = 1.0f / det;
+      u = u - fmaf(m3, r0, -m2 * r1) * id;
+      v = v - fmaf(-m1, r0, m0 * r1) * id;
+    }
+    u = fminf(fmaxf(u, 0.0f), 1.0f);
+    v = fminf(fmaxf(v, 0.0f), 1.0f);
+  }
+  return h;
+}
+
+template <int Q>
+__device__ __forceinline__ bool hull_culled_q(const float4* __restrict__ net, int P, const Plane2& rp, float margin) {
+  bool pos1 = true, neg1 = true, pos2 = true, neg2 = true;
+#pragma unroll 1
+  for (int i = 0; i <= P; i++) {
+#pragma unroll
+    for (int j = 0; j <= Q; j++) {
+      const float4 c = net[i * (Q + 1) + j];
+      const float s1 = fmaf(rp.n1.x, c.x, fmaf(rp.n1.y, c.y, fmaf(rp.n1.z, c.z, rp.d1 * c.w)));
+      const float s2 = fmaf(rp.n2.x, c.x, fmaf(rp.n2.y, c.y, fmaf(rp.n2.z, c.z, rp.d2 * c.w)));
+      const float mw = margin * c.w;
+      pos1 = pos1 && (s1 > mw);
+      neg1 = neg1 && (s1 < -mw);
+      pos2 = pos2 && (s2 > mw);
+      neg2 = neg2 && (s2 < -mw);
+    }
+  }
+  return pos1 || neg1 || pos2 || neg2;
+}
+
+// Bernstein polynomial B_i^n(t) in FP64 from its closed form C(n,i) t^i (1-t)^(n-i),
+// evaluated with scalar loops (no arrays, so the rare FP64 step adds no
+// register arrays to the kernel); zero outside 0 <= i <= n.
+__device__ __forceinline__ double bern64(int n, int i, double t) {
+  if (i < 0 || i > n) return 0.0;
+  double c = 1.0, r = 1.0;
+  for (int k = 1; k <= i; k++) c = c * (double)(n - i + k) / (double)k;
+  for (int k = 0; k < i; k++) r *= t;
+  for (int k = 0; k < n - i; k++) r *= (1.0 - t);
+  return c * r;
+}
+
+// FP64 point and partials, runtime degrees (rows and columns streamed).
+__device__ __forceinline__ void eval64_rt(const float4* __restrict__ net, int P, int Q, double u, double v, D3& S,
+                                          D3& Su, D3& Sv) {
+  double pt[4] = {0, 0, 0, 0}, du[4] = {0, 0, 0, 0}, dv[4] = {0, 0, 0, 0};
+  for (int i = 0; i <= P; i++) {
+    const double bu = bern64(P, i, u), dbu = P * (bern64(P - 1, i - 1, u) - bern64(P - 1, i, u));
+    for (int j = 0; j <= Q; j++) {
+      const double bv = bern64(Q, j, v), dbv = Q * (bern64(Q - 1, j - 1, v) - bern64(Q - 1, j, v));
+      const float4 c = net[i * (Q + 1) + j];
+      const double cc[4] = {c.x, c.y, c.z, c.w};
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        pt[k] = fma(bu * bv, cc[k], pt[k]);
+        du[k] = fma(dbu * bv, cc[k], du[k]);
+        dv[k] = fma(bu * dbv, cc[k], dv[k]);
+      }
+    }
+  }
+  const double iw = 1.0 / pt[3];
+  S = {pt[0] * iw, pt[1] * iw, pt[2] * iw};
+  Su = {(du[0] - S.x * du[3]) * iw, (du[1] - S.y * du[3]) * iw, (du[2] - S.z * du[3]) * iw};
+  Sv = {(dv[0] - S.x * dv[3]) * iw, (dv[1] - S.y * dv[3]) * iw, (dv[2] - S.z * dv[3]) * iw};
+}
+
+// One FP64 Newton step from the FP32 (u,v) on the FP64 ray planes, then the
+// corrected point: the mixed-precision form's step for cond(J) > threshold.
+__device__ __forceinline__ bool correct64_rt(const float4* __restrict__ net, int P, int Q, f3 o, f3 d, float& uo,
+                                          float& vo, f3& So, f3& Suo, f3& Svo, float& t_out) {
+  const double ox = o.x, oy = o.y, oz = o.z, dx = d.x, dy = d.y, dz = d.z;
+  const double ax = fabs(dx), ay = fabs(dy), az = fabs(dz);
+  const bool xd = (ax > ay) && (ax > az);
+  const double b0 = xd ? dx : dy, b1 = xd ? dy : dz;
+  const double den = 1.0 / sqrt(b0 * b0 + b1 * b1);
+  const D3 n1 = xd ? D3{dy * den, -dx * den, 0.0} : D3{0.0, dz * den, -dy * den};
+  const double il = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+  const D3 dn{dx * il, dy * il, dz * il};
+  const D3 n2{n1.y * dn.z - n1.z * dn.y, n1.z * dn.x - n1.x * dn.z, n1.x * dn.y - n1.y * dn.x};
+  const double d1 = -(n1.x * ox + n1.y * oy + n1.z * oz), d2 = -(n2.x * ox + n2.y * oy + n2.z * oz);
+  double u = uo, v = vo;
+  D3 S, Su, Sv;
+  eval64_rt(net, P, Q, u, v, S, Su, Sv);
+  const double r0 = n1.x * S.x + n1.y * S.y + n1.z * S.z + d1;
+  const double r1 = n2.x * S.x + n2.y * S.y + n2.z * S.z + d2;
+  const double m0 = n1.x * Su.x + n1.y * Su.y + n1.z * Su.z, m1 = n2.x * Su.x + n2.y * Su.y + n2.z * Su.z;
+  const double m2 = n1.x * Sv.x + n1.y * Sv.y + n1.z * Sv.z, m3 = n2.x * Sv.x + n2.y * Sv.y + n2.z * Sv.z;
+  const double det = m0 * m3 - m2 * m1;
+  if (det == 0.0) return false;
+  u -= (m3 * r0 - m2 * r1) / det;
+  v -= (-m1 * r0 + m0 * r1) / det;
+  eval64_rt(net, P, Q, u, v, S, Su, Sv);
+  uo = (float)u;
+  vo = (float)v;
+  So = {(float)S.x, (float)S.y, (float)S.z};
+  Suo = {(float)Su.x, (float)Su.y, (float)Su.z};
+  Svo = {(float)Sv.x, (float)Sv.y, (float)Sv.z};
+  t_out = (float)((S.x - ox) * dx + (S.y - oy) * dy + (S.z - oz) * dz);
+  return true;
+}
+
+}  // namespace fast
+}  // namespace rtn

@@ -1,10 +1,444 @@
-// Fast trace forms over the 4-wide SoA BVH (placeholder until implemented).
+// Fast trace forms over the 4-wide SoA BVH: the wide kernel (one ray per
+// lane) and the persistent kernel (lanes refill from a global ray counter).
+//
+// Per lane: a short LDS stack of (node, entry t) entries ordered near-first;
+// popped entries are re-tested against the current closest hit.  The wave runs
+// in two phases (while-while): lanes without a pending leaf advance through
+// the tree until each has one, then the leaf phase processes one patch per
+// lane.  Newton is dispatched per degree class: the wave takes the class of
+// its first pending lane (readfirstlane), runs a compile-time Newton<P,Q> for
+// the lanes of that class, and repeats until no lane is pending, so no lane
+// ever runs a runtime-degree evaluator.  Control nets stream row by row from
+// L1/L2 (the rays of a wave mostly test the same patches).
+//
+// Arithmetic: explicit FMA and tensor-product Bernstein evaluation (rtn_fast.hpp);
+// the seed is the reference's bilinear seed in its own operation order; the
+// iteration keeps the plain method's convergence test, PCG perturbation on a
+// singular Jacobian and clamping.  Options: exact convex-hull rejection before
+// Newton, and an FP64 correction step for hits whose Jacobian condition number
+// exceeds a threshold (mixed precision).  Improving hits are written through
+// to the output arrays, as the reference writes its record (intersect.cu:133-138).
+#include <algorithm>
+
+#include "rtn_fast.hpp"
 #include "rtn_internal.hpp"
 
-namespace rtn {
-int trace_wide_scene(const rtn_scene*, const float* const[6], int64_t, const rtn_trace_params*, const rtn_hits*,
-                     unsigned long long*, hipStream_t) {
-  set_error("rtn_trace: wide/persistent forms not built yet");
-  return RTN_EUNSUPPORTED;
+using namespace rtn;
+using namespace rtn::fast;
+
+namespace {
+
+constexpr int WB = 128;   // lanes per block (2 waves)
+constexpr int WSTK = 20;  // stack entries per lane: 20 KB of LDS per block -> 4 waves/SIMD
+
+struct WArgs {
+  const WideNode* nodes;
+  const float4* cp;
+  const int* cp_off;
+  const int* pq;
+  const float4* corners;  // dehomogenised corners q00,q10,q11,q01 (12 floats per patch)
+  const float *ox, *oy, *oz, *dx, *dy, *dz;
+  int64_t n_rays;
+  int max_iter;
+  float thr_det, thr_res;
+  uint32_t seed_base;
+  int hull_cull;
+  float cull_margin;
+  int mixed;
+  float cond_thr;
+  const float2* learned_seed;  // learned form: seed per ray for its first patch test, or null
+  int* prim;
+  float *t, *u, *v, *dpdu, *dpdv;
+  uint32_t* counters;
+  unsigned long long* totals;
+  unsigned int* work;  // persistent: next ray to hand out
+};
+
+struct Cnt {  // per-ray work counters (counting builds only)
+  uint32_t nc, ne, ncull, nbox, nleaf, npt, ovf, nfl, n64, nfb;
+};
+
+struct Lane {
+  f3 o, d, rd;
+  Plane2 rp;
+  uint32_t r;
+  float t_max;
+  int best;
+};
+
+__device__ __forceinline__ void lane_start(Lane& L, const WArgs& A, uint32_t r) {
+  L.r = r;
+  L.o = {A.ox[r], A.oy[r], A.oz[r]};
+  L.d = {A.dx[r], A.dy[r], A.dz[r]};
+  // finite reciprocals keep o*rd free of 0*inf (a zero component only widens
+  // the slab interval, so the box test stays conservative)
+  L.rd = {1.0f / (fabsf(L.d.x) > 1e-30f ? L.d.x : copysignf(1e-30f, L.d.x)),
+          1.0f / (fabsf(L.d.y) > 1e-30f ? L.d.y : copysignf(1e-30f, L.d.y)),
+          1.0f / (fabsf(L.d.z) > 1e-30f ? L.d.z : copysignf(1e-30f, L.d.z))};
+  L.rp = planes(L.o, L.d);
+  L.t_max = INFINITY;
+  L.best = -1;
 }
+
+template <bool COUNT>
+__device__ __forceinline__ void lane_finish(const Lane& L, const WArgs& A, const Cnt& c) {
+  A.prim[L.r] = L.best;
+  if (COUNT && A.counters) {
+    uint32_t* o = A.counters + 8 * (int64_t)L.r;
+    o[0] = c.nc; o[1] = c.ne; o[2] = c.ncull; o[3] = c.nbox; o[4] = c.nleaf; o[5] = c.npt; o[6] = c.ovf; o[7] = c.nfl;
+  }
+}
+
+// Expand one wide node: test the four child boxes (slab test with FMA,
+// t = b*rd - o*rd) and push the hits far-to-near so the nearest is on top.
+template <bool COUNT>
+__device__ __forceinline__ void expand(Lane& L, Cnt& cn, const WideNode* __restrict__ nodes, int ni, int (*sc)[WB],
+                                       float (*st)[WB], int tid, int& off) {
+  const float4* base = reinterpret_cast<const float4*>(nodes + ni);
+  const float4 lx = base[0], hx = base[1], ly = base[2], hy = base[3], lz = base[4], hz = base[5];
+  const int4 ch = *reinterpret_cast<const int4*>(base + 6);
+  const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
+  const float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
+  const float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
+  const int cha[4] = {ch.x, ch.y, ch.z, ch.w};
+  const float ox = L.o.x * L.rd.x, oy = L.o.y * L.rd.y, oz = L.o.z * L.rd.z;
+  float te[4];
+  int code[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const float tx0 = fmaf(lxa[k], L.rd.x, -ox), tx1 = fmaf(hxa[k], L.rd.x, -ox);
+    const float ty0 = fmaf(lya[k], L.rd.y, -oy), ty1 = fmaf(hya[k], L.rd.y, -oy);
+    const float tz0 = fmaf(lza[k], L.rd.z, -oz), tz1 = fmaf(hza[k], L.rd.z, -oz);
+    const float tmin = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+    // a few ulps of slack on the exit distance keep the test conservative
+    const float tmax = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1)) * 1.0000005f;
+    // empty slots carry the sentinel child 0x7fffffff and are never entered
+    const bool hit = tmin <= tmax && tmin < L.t_max && cha[k] != 0x7fffffff;
+    te[k] = hit ? tmin : INFINITY;
+    code[k] = cha[k];
+  }
+  if (COUNT) cn.nbox += 4;
+#define RTN_CSWAP(a, b)                                \
+  if (te[a] < te[b]) {                                 \
+    float tt = te[a]; te[a] = te[b]; te[b] = tt;       \
+    int cc = code[a]; code[a] = code[b]; code[b] = cc; \
+  }
+  RTN_CSWAP(0, 1) RTN_CSWAP(2, 3) RTN_CSWAP(0, 2) RTN_CSWAP(1, 3) RTN_CSWAP(1, 2)
+#undef RTN_CSWAP
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    if (te[k] < INFINITY) {
+      if (off < WSTK) {
+        sc[off][tid] = code[k];
+        st[off][tid] = te[k];
+        off++;
+      } else {
+        cn.ovf = 1;
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ void record(Lane& L, const WArgs& A, int pi, float t, float u, float v, f3 Su, f3 Sv) {
+  L.t_max = t;
+  L.best = pi;
+  const uint32_t r = L.r;
+  if (A.t) A.t[r] = t;
+  if (A.u) A.u[r] = u;
+  if (A.v) A.v[r] = v;
+  if (A.dpdu) { A.dpdu[3 * r] = Su.x; A.dpdu[3 * r + 1] = Su.y; A.dpdu[3 * r + 2] = Su.z; }
+  if (A.dpdv) { A.dpdv[3 * r] = Sv.x; A.dpdv[3 * r + 1] = Sv.y; A.dpdv[3 * r + 2] = Sv.z; }
+}
+
+// Result of one patch test.
+struct LeafOut {
+  float t, u, v;
+  f3 S, Su, Sv;
+  int evals;     // surface evaluations (both attempts when a learned seed falls back)
+  int flags;     // 1 hit, 2 culled, 4 FP64 step done, 8 learned fallback, 16 FP64 step needed
+};
+
+struct LeafIn {
+  Plane2 rp;
+  f3 o, d;
+  float gu, gv, pu, pv;
+  uint32_t seed;
+  int max_iter, hull_cull, mixed, learned;
+  float thr_det, thr_res, cull_margin, cond_thr;
+};
+
+// Leaf work for one patch with compile-time row length Q+1 and runtime row
+// count P+1: optional hull rejection, Newton from the seed (plain-seed
+// fallback for the learned form), optional FP64 correction.
+template <int Q>
+__device__ __forceinline__ LeafOut leaf_q(const float4* __restrict__ net, int P, const LeafIn& in) {
+  LeafOut o;
+  o.flags = 0;
+  o.evals = 0;
+  if (in.hull_cull && hull_culled_q<Q>(net, P, in.rp, in.cull_margin)) {
+    o.flags = 2;
+    return o;
+  }
+  // attempt 0 from the given seed; for the learned form a failed attempt is
+  // repeated from the plain seed
+  Hit h;
+  float su = in.gu, sv = in.gv;
+#pragma unroll 1
+  for (int attempt = 0;; attempt++) {
+    h = newton_q<Q>(net, P, in.rp, su, sv, in.thr_det, in.thr_res, in.seed, in.max_iter);
+    o.evals += h.evals;
+    if (h.hit || !in.learned || attempt == 1) break;
+    o.flags |= 8;
+    su = in.pu;
+    sv = in.pv;
+  }
+  if (!h.hit) return o;
+  const float t = fmaf(h.S.x - in.o.x, in.d.x, fmaf(h.S.y - in.o.y, in.d.y, (h.S.z - in.o.z) * in.d.z));
+  // ill-conditioned hit: the FP64 step runs after the class loop (flag 16),
+  // in one runtime-degree copy shared by all classes
+  if (in.mixed && cond2(h.m0, h.m1, h.m2, h.m3) > in.cond_thr) o.flags |= 16;
+  o.flags |= 1;
+  o.t = t;
+  o.u = h.u;
+  o.v = h.v;
+  o.S = h.S;
+  o.Su = h.Su;
+  o.Sv = h.Sv;
+  return o;
+}
+
+// Degree classes: the row length Q+1 is compiled (q = 1..5 -> class 0..4);
+// the row count P+1 is a runtime loop.
+__device__ __forceinline__ int degree_class(int p, int q) { return q - 1; }
+
+__device__ __forceinline__ LeafOut leaf_dispatch(int cls, const float4* net, int p, const LeafIn& in) {
+  switch (cls) {  // wave-uniform: a scalar branch
+    case 0: return leaf_q<1>(net, p, in);
+    case 1: return leaf_q<2>(net, p, in);
+    case 2: return leaf_q<3>(net, p, in);
+    case 3: return leaf_q<4>(net, p, in);
+    default: return leaf_q<5>(net, p, in);
+  }
+}
+
+template <bool PERSISTENT, bool COUNT>
+__global__ void __launch_bounds__(WB)
+k_trace_wide(const WArgs* __restrict__ args) {
+  __shared__ int s_code[WSTK][WB];
+  __shared__ float s_te[WSTK][WB];
+  const WArgs& A = *args;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  Lane L;
+  Cnt cn{};
+  bool has_ray = false;   // a ray is in flight on this lane
+  int off = 0;            // stack depth
+  int cur = -1;           // wide node to expand, or -1
+  int cand_first = 0, cand_left = 0;  // primitives of the current leaf still to test
+  bool exhausted = true;  // no more rays to hand out
+  unsigned long long acc[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (PERSISTENT) {
+    exhausted = false;
+  } else {
+    const int64_t r = (int64_t)blockIdx.x * WB + tid;
+    if (r < A.n_rays) {
+      lane_start(L, A, (uint32_t)r);
+      has_ray = true;
+      cur = 0;
+    }
+  }
+  while (true) {
+    if (PERSISTENT && !exhausted) {
+      // lanes without a ray take consecutive rays from the global counter,
+      // one atomic per wave
+      const unsigned long long need = __ballot(!has_ray);
+      if (need) {
+        const int cnt = __popcll(need);
+        const int leader = __ffsll((long long)need) - 1;
+        unsigned int base = 0;
+        if (lane == leader) base = atomicAdd(A.work, (unsigned)cnt);
+        base = __shfl(base, leader);
+        if ((int64_t)base + cnt >= A.n_rays) exhausted = true;
+        if (!has_ray) {
+          const unsigned rank = __popcll(need & ((1ull << lane) - 1));
+          const int64_t r = (int64_t)base + rank;
+          if (r < A.n_rays) {
+            lane_start(L, A, (uint32_t)r);
+            if (COUNT) cn = Cnt{};
+            has_ray = true;
+            cur = 0;
+            off = 0;
+            cand_left = 0;
+          }
+        }
+      }
+    }
+    // ---- traversal: advance every lane that has no pending leaf
+    while (has_ray && cand_left == 0) {
+      if (cur >= 0) {
+        expand<COUNT>(L, cn, A.nodes, cur, s_code, s_te, tid, off);
+        cur = -1;
+        continue;
+      }
+      if (off == 0) {  // traversal finished: the ray is done
+        lane_finish<COUNT>(L, A, cn);
+        if (COUNT) {
+          acc[0] += cn.nc; acc[1] += cn.ne; acc[3] += cn.nbox; acc[4] += cn.nleaf; acc[5] += cn.npt;
+          acc[6] += cn.ovf; acc[7] += cn.nfl; acc[8] += cn.n64; acc[9] += cn.ncull; acc[10] += cn.nfb;
+        } else if (cn.ovf) {
+          acc[6] += 1;
+          cn.ovf = 0;
+        }
+        has_ray = false;
+        break;
+      }
+      off--;
+      const int code = s_code[off][tid];
+      const float te = s_te[off][tid];
+      if (te >= L.t_max) continue;
+      if (code < 0) {
+        cand_first = code & 0xffffff;
+        cand_left = ((code >> 24) & 0x7f) + 1;
+        if (COUNT) cn.nleaf++;
+      } else {
+        cur = code;
+      }
+    }
+    const bool pending = cand_left > 0;
+    if (!__ballot(pending)) {
+      if (exhausted && !__ballot(has_ray)) break;
+      continue;
+    }
+    // ---- leaf phase: one primitive per pending lane, Newton by degree class
+    int pi = 0, p = 0, q = 0, cls = -1;
+    float gu = 0.5f, gv = 0.5f;
+    if (pending) {
+      pi = cand_first;
+      cand_first++;
+      cand_left--;
+      if (COUNT) cn.npt++;
+      const int pqv = A.pq[pi];
+      p = pqv & 0xff;
+      q = pqv >> 8;
+      cls = degree_class(p, q);
+      // the reference's bilinear seed on the dehomogenised corners (bezier_patch.h:38-44, :273-327)
+      const float4 c0 = A.corners[3 * pi], c1 = A.corners[3 * pi + 1], c2 = A.corners[3 * pi + 2];
+      ex::bilinear_seed<float>({c0.x, c0.y, c0.z}, {c0.w, c1.x, c1.y}, {c1.z, c1.w, c2.x}, {c2.y, c2.z, c2.w},
+                               {L.o.x, L.o.y, L.o.z}, {L.d.x, L.d.y, L.d.z}, gu, gv);
+    }
+    LeafIn in;
+    in.rp = L.rp;
+    in.o = L.o;
+    in.d = L.d;
+    in.gu = gu;
+    in.gv = gv;
+    in.pu = gu;
+    in.pv = gv;
+    in.seed = L.r + A.seed_base;
+    in.max_iter = A.max_iter;
+    in.hull_cull = A.hull_cull;
+    in.mixed = A.mixed;
+    in.learned = 0;
+    in.thr_det = A.thr_det;
+    in.thr_res = A.thr_res;
+    in.cull_margin = A.cull_margin;
+    in.cond_thr = A.cond_thr;
+    const float4* net = A.cp + A.cp_off[pi];
+    LeafOut res;
+    res.flags = 0;
+    unsigned long long todo = __ballot(pending);
+    while (todo) {
+      const int leader = __ffsll((long long)todo) - 1;
+      const int c = __shfl(cls, leader);
+      const bool mine = pending && cls == c;
+      if (mine) res = leaf_dispatch(c, net, p, in);
+      todo &= ~__ballot(mine);
+    }
+    if (pending) {
+      if (res.flags & 16) {  // FP64 correction of an ill-conditioned hit
+        if (correct64_rt(net, p, q, L.o, L.d, res.u, res.v, res.S, res.Su, res.Sv, res.t)) res.flags |= 4;
+      }
+      if (COUNT) {
+        if (res.flags & 2) {
+          cn.ncull++;
+        } else {
+          cn.nc++;
+          cn.ne += res.evals;
+          cn.nfl += res.evals * ex::eval_flops(p, q);
+          if (res.flags & 4) cn.n64++;
+          if (res.flags & 8) cn.nfb++;
+        }
+      }
+      if ((res.flags & 1) && 0.0f < res.t && res.t < L.t_max) record(L, A, pi, res.t, res.u, res.v, res.Su, res.Sv);
+    }
+  }
+  if (A.totals) {  // every lane of the wave arrives here together
+#pragma unroll
+    for (int k = 0; k < 11; k++) {
+      unsigned long long x = acc[k];
+      for (int sft = 32; sft > 0; sft >>= 1) x += __shfl_xor(x, sft);
+      if (lane == 0 && x) atomicAdd(A.totals + k, x);
+    }
+  }
+}
+
+}  // namespace
+
+namespace rtn {
+
+int trace_wide_scene(const rtn_scene* s, const float* const rays[6], int64_t n, const rtn_trace_params* prm,
+                     const rtn_hits* hits, unsigned long long* d_totals, hipStream_t stream) {
+  RTN_CHECK(prm->seed_mode == RTN_SEED_PLAIN, RTN_EUNSUPPORTED, "rtn_trace: learned seeds need a loaded network");
+  RTN_CHECK(prm->flat_threshold <= 0.f, RTN_EUNSUPPORTED, "rtn_trace: flat-skip runs in the exact form only");
+  WArgs A{};
+  A.nodes = s->wide.ptr;
+  A.cp = s->cp.ptr;
+  A.cp_off = s->cp_off.ptr;
+  A.pq = s->pq.ptr;
+  A.corners = s->corners.ptr;
+  A.ox = rays[0]; A.oy = rays[1]; A.oz = rays[2]; A.dx = rays[3]; A.dy = rays[4]; A.dz = rays[5];
+  A.n_rays = n;
+  A.max_iter = prm->max_iter;
+  A.thr_det = prm->thr_det;
+  A.thr_res = prm->thr_res;
+  A.seed_base = prm->seed_base;
+  A.hull_cull = prm->hull_cull;
+  A.cull_margin = 2.0f * prm->thr_res + 1e-5f;
+  A.mixed = prm->precision == RTN_PREC_MIXED;
+  A.cond_thr = prm->cond_thr;
+  A.prim = hits->prim; A.t = hits->t; A.u = hits->u; A.v = hits->v; A.dpdu = hits->dpdu; A.dpdv = hits->dpdv;
+  A.counters = prm->count ? hits->counters : nullptr;
+  A.totals = d_totals;
+  const bool persistent = prm->mode == RTN_MODE_PERSISTENT;
+  // arguments and the ray counter live in one small device block (stream-ordered)
+  struct Block { WArgs a; unsigned int work; unsigned int pad[3]; };
+  Block* d_blk = nullptr;
+  RTN_HIP(hipMallocAsync((void**)&d_blk, sizeof(Block), stream));
+  A.work = persistent ? &d_blk->work : nullptr;
+  Block h{};
+  h.a = A;
+  h.work = 0;
+  RTN_HIP(hipMemcpyAsync(d_blk, &h, sizeof(Block), hipMemcpyHostToDevice, stream));
+  const bool count = prm->count != 0;
+  if (persistent) {
+    int dev = 0, cus = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + WB - 1) / WB, (int64_t)cus * 8));
+    if (count)
+      hipLaunchKernelGGL((k_trace_wide<true, true>), dim3(blocks), dim3(WB), 0, stream, &d_blk->a);
+    else
+      hipLaunchKernelGGL((k_trace_wide<true, false>), dim3(blocks), dim3(WB), 0, stream, &d_blk->a);
+  } else {
+    const unsigned blocks = (unsigned)((n + WB - 1) / WB);
+    if (count)
+      hipLaunchKernelGGL((k_trace_wide<false, true>), dim3(blocks), dim3(WB), 0, stream, &d_blk->a);
+    else
+      hipLaunchKernelGGL((k_trace_wide<false, false>), dim3(blocks), dim3(WB), 0, stream, &d_blk->a);
+  }
+  RTN_HIP(hipGetLastError());
+  RTN_HIP(hipFreeAsync(d_blk, stream));
+  return RTN_OK;
+}
+
 }  // namespace rtn

@@ -1,0 +1,76 @@
+"""GPU fast forms (wide 4-ary BVH, persistent work queue, hull rejection,
+mixed precision) against the exact plain method and the FP64 reference.
+
+The fast forms change the arithmetic (FMA, tensor-Bernstein evaluation) and
+the traversal order, not the method: per ray the same patches are tested
+from the same seeds, so hit patches agree with the exact form except where
+two hits are within 1e-4 relative in t (ties) or a Newton solve sits on its
+convergence boundary; t agrees to the FP32 Newton tolerance scale.  The
+persistent kernel performs the same per-ray arithmetic as the wide kernel,
+so the two are compared bit for bit."""
+import numpy as np
+import pytest
+
+from helpers import c2_small
+from rtnurbs import scenes
+
+pytestmark = pytest.mark.gpu
+
+CFG = dict(max_iter=8, thr_res=1e-5, thr_det=1e-5)
+
+CASES = [
+    ("c1", lambda: scenes.config1(0)),
+    ("mixed", lambda: scenes.Scene(c2_small(8, 2), scenes.c2_camera(240, 135))),
+    ("torus", lambda: scenes.Scene([scenes.torus()], scenes.reference_camera(320, 180))),
+]
+
+
+def _run(rtl, scene, rays, **kw):
+    h, tot = rtl.trace(scene, rays, rtl.params(**CFG, **kw))
+    return h.get(), tot
+
+
+@pytest.mark.parametrize("name,mk", CASES, ids=[c[0] for c in CASES])
+def test_wide_forms_agree(rtl, name, mk):
+    sc = mk()
+    print(f"case {name}", flush=True)
+    scene = rtl.Scene(sc.surfaces).build()
+    rays = rtl.generate_rays(sc.camera)
+    ex, _ = _run(rtl, scene, rays, mode=rtl.MODE_EXACT)
+    wd, tw = _run(rtl, scene, rays, mode=rtl.MODE_WIDE, count=1)
+    assert tw["overflow"] == 0
+    ps, tp = _run(rtl, scene, rays, mode=rtl.MODE_PERSISTENT, count=1)
+    # persistent == wide, bit for bit (same per-ray arithmetic)
+    assert np.array_equal(ps["hit"], wd["hit"])
+    m = wd["hit"] >= 0
+    for k in ("t", "u", "v"):
+        assert np.array_equal(ps[k][m].view(np.uint32), wd[k][m].view(np.uint32)), k
+    assert tp["newton_evals"] == tw["newton_evals"] and tp["patch_tests"] == tw["patch_tests"]
+    # hull rejection never removes a hit
+    cu, tc = _run(rtl, scene, rays, mode=rtl.MODE_WIDE, hull_cull=1, count=1)
+    assert tc["hull_culled"] > 0
+    agree = np.mean(cu["hit"] == wd["hit"])
+    assert agree > 0.9995, agree
+    # wide vs exact: hit patches agree except near ties / boundary solves
+    both = (ex["hit"] >= 0) & (wd["hit"] >= 0)
+    assert np.mean(ex["hit"] == wd["hit"]) > 0.995
+    same = both & (ex["hit"] == wd["hit"])
+    rel = np.abs(wd["t"][same] - ex["t"][same]) / np.abs(ex["t"][same])
+    assert np.quantile(rel, 0.99) < 1e-5 and rel.max() < 1e-3
+
+
+def test_mixed_precision_improves_t(rtl):
+    """On config 1 the FP64 step lowers the hit-t error of the corrected rays
+    against the FP64 reference."""
+    sc = scenes.config1(0)
+    scene = rtl.Scene(sc.surfaces).build()
+    rays = rtl.generate_rays(sc.camera)
+    ref = rtl.trace_f64(scene, rays).get()
+    a, _ = _run(rtl, scene, rays, mode=rtl.MODE_WIDE, hull_cull=1)
+    b, tb = _run(rtl, scene, rays, mode=rtl.MODE_WIDE, hull_cull=1, precision=rtl.PREC_MIXED, cond_thr=1e2,
+                 count=1)
+    assert tb["fp64_steps"] > 0
+    same = (a["hit"] >= 0) & (a["hit"] == ref["hit"]) & (b["hit"] == ref["hit"])
+    ea = np.abs(a["t"][same] - ref["t"][same]) / ref["t"][same]
+    eb = np.abs(b["t"][same] - ref["t"][same]) / ref["t"][same]
+    assert eb.mean() <= ea.mean()
