@@ -155,9 +155,9 @@ def main():
         "plain": dict(mode=L.MODE_EXACT, **base),
         "wide": dict(mode=L.MODE_WIDE, **base),
         "wide_cull": dict(mode=L.MODE_WIDE, hull_cull=1, **base),
-        "persistent_cull": dict(mode=L.MODE_PERSISTENT, hull_cull=1, **base),
-        "persistent_cull_mixed": dict(mode=L.MODE_PERSISTENT, hull_cull=1, precision=L.PREC_MIXED, cond_thr=1e4,
-                                      **base),
+        "wide_hull2": dict(mode=L.MODE_WIDE, hull_cull=2, **base),
+        "persistent_hull2": dict(mode=L.MODE_PERSISTENT, hull_cull=2, **base),
+        "wide_hull2_mixed": dict(mode=L.MODE_WIDE, hull_cull=2, precision=L.PREC_MIXED, cond_thr=1e4, **base),
     }
     if args.forms != "all":
         keep = set(args.forms.split(","))

@@ -169,7 +169,9 @@ typedef struct {
   int seed_mode;          /* RTN_SEED_*                                          */
   int precision;          /* RTN_PREC_*                                          */
   float cond_thr;         /* condition-number threshold of the FP64 step         */
-  int hull_cull;          /* 1: skip Newton when the net lies off the ray planes */
+  int hull_cull;          /* exact rejection before Newton (fast forms): 0 off; 1 per-plane sign test of
+                             the net; 2 projected-hull test at leaf entry (needs positive weights,
+                             nets with a weight <= 0 are never rejected) */
   int count;              /* 1: fill the per-ray counters                        */
   int reserved[6];
 } rtn_trace_params;

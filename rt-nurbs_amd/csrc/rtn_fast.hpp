@@ -491,3 +491,37 @@ __device__ __forceinline__ bool correct64_rt(const float4* __restrict__ net, int
 
 }  // namespace fast
 }  // namespace rtn
+
+namespace rtn {
+namespace fast {
+
+// Exact rejection of a (ray, patch) pair from its control points projected on
+// the ray planes.  With positive weights, the residual vector (r0, r1) of any
+// surface point is a convex combination of the projected control points
+// (c1/w, c2/w), c = n.P.xyz + d*P.w; Newton can only converge where
+// |r0| + |r1| < thr, i.e. inside the L1 ball of that radius.  The pair is
+// rejected when a line separates the projected points from the ball: tested
+// along the eight directions (+-1,0), (0,+-1), (+-1,+-1), whose L1-ball support
+// is the margin itself, as min(x - m*w) > 0 or max(x + m*w) < 0 with
+// x in {c1, c2, c1+c2, c1-c2}.  Runtime degrees; no divisions.
+__device__ __forceinline__ bool hull2d_culled(const float4* __restrict__ net, int n_cp, const Plane2& rp,
+                                              float margin) {
+  float lo0 = INFINITY, lo1 = INFINITY, lo2 = INFINITY, lo3 = INFINITY;
+  float hi0 = -INFINITY, hi1 = -INFINITY, hi2 = -INFINITY, hi3 = -INFINITY;
+#pragma unroll 4
+  for (int k = 0; k < n_cp; k++) {
+    const float4 c = net[k];
+    const float c1 = fmaf(rp.n1.x, c.x, fmaf(rp.n1.y, c.y, fmaf(rp.n1.z, c.z, rp.d1 * c.w)));
+    const float c2 = fmaf(rp.n2.x, c.x, fmaf(rp.n2.y, c.y, fmaf(rp.n2.z, c.z, rp.d2 * c.w)));
+    const float mw = margin * c.w;
+    const float s = c1 + c2, d = c1 - c2;
+    lo0 = fminf(lo0, c1 - mw); hi0 = fmaxf(hi0, c1 + mw);
+    lo1 = fminf(lo1, c2 - mw); hi1 = fmaxf(hi1, c2 + mw);
+    lo2 = fminf(lo2, s - mw); hi2 = fmaxf(hi2, s + mw);
+    lo3 = fminf(lo3, d - mw); hi3 = fmaxf(hi3, d + mw);
+  }
+  return fmaxf(fmaxf(lo0, lo1), fmaxf(lo2, lo3)) > 0.0f || fminf(fminf(hi0, hi1), fminf(hi2, hi3)) < 0.0f;
+}
+
+}  // namespace fast
+}  // namespace rtn

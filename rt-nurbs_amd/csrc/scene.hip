@@ -144,7 +144,7 @@ __global__ void k_headers(const SurfDesc* sd, int ns, int np, const float* brk, 
 __global__ void k_bounds(const float4* cp, const int* cp_off, const int* pq, int np, float* bounds) {
   int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= np) return;
-  const int p = pq[g] & 0xff, q = pq[g] >> 8;
+  const int p = pq[g] & 0xff, q = (pq[g] >> 8) & 0xff;
   const float4* c = cp + cp_off[g];
   float mn0 = INFINITY, mn1 = INFINITY, mn2 = INFINITY, mx0 = -INFINITY, mx1 = -INFINITY, mx2 = -INFINITY;
   for (int i = 0; i < (p + 1) * (q + 1); i++) {
@@ -158,11 +158,13 @@ __global__ void k_bounds(const float4* cp, const int* cp_off, const int* pq, int
   b[0] = mn0; b[1] = mn1; b[2] = mn2; b[3] = mx0; b[4] = mx1; b[5] = mx2;
 }
 
-// Seed corners (bezier_patch.h:38-44) and patchDelta (main.cpp:18-43).
-__global__ void k_prep(const float4* cp, const int* cp_off, const int* pq, int np, float4* corners, float* delta) {
+// Seed corners (bezier_patch.h:38-44) and patchDelta (main.cpp:18-43); sets
+// bit 16 of pq when every weight of the net is positive (the condition under
+// which the projected-hull rejection of the fast forms is exact).
+__global__ void k_prep(const float4* cp, const int* cp_off, int* pq, int np, float4* corners, float* delta) {
   int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= np) return;
-  const int p = pq[g] & 0xff, q = pq[g] >> 8, row = q + 1;
+  const int p = pq[g] & 0xff, q = (pq[g] >> 8) & 0xff, row = q + 1;
   const float4* c = cp + cp_off[g];
   using namespace ex;
   v3<float> c00 = dehom(ld4<float>(c[0])), c10 = dehom(ld4<float>(c[p * row]));
@@ -183,6 +185,9 @@ __global__ void k_prep(const float4* cp, const int* cp_off, const int* pq, int n
       d = fmaxf(d, sqrtf(dx * dx + dy * dy + dz * dz));
     }
   delta[g] = d;
+  bool pos = true;
+  for (int i = 0; i < (p + 1) * (q + 1); i++) pos = pos && c[i].w > 0.0f;
+  pq[g] = (pq[g] & 0xffff) | (pos ? 0x10000 : 0);
 }
 
 // Gather an array of T by permutation: out[i] = in[perm[i]] (n items of `w` elements).
@@ -706,7 +711,7 @@ extern "C" int rtn_scene_build(rtn_scene* s) {
   int64_t acc = 0;
   for (int i = 0; i < np; i++) {
     const int src = (int)perm_h[i];
-    const int p = pq_h[src] & 0xff, q = pq_h[src] >> 8;
+    const int p = pq_h[src] & 0xff, q = (pq_h[src] >> 8) & 0xff;
     new_off[i] = (int)acc;
     acc += (p + 1) * (q + 1);
     hist[p * (RTN_MAX_DEGREE + 1) + q]++;
@@ -715,10 +720,10 @@ extern "C" int rtn_scene_build(rtn_scene* s) {
   for (int i = 0; i < np;) {  // copy runs of nets that stay contiguous
     const int src = (int)perm_h[i];
     int j = i + 1;
-    int64_t len = (int64_t)((pq_h[src] & 0xff) + 1) * ((pq_h[src] >> 8) + 1);
+    int64_t len = (int64_t)((pq_h[src] & 0xff) + 1) * (((pq_h[src] >> 8) & 0xff) + 1);
     while (j < np && (int)perm_h[j] == (int)perm_h[j - 1] + 1) {
       const int sj = (int)perm_h[j];
-      len += (int64_t)((pq_h[sj] & 0xff) + 1) * ((pq_h[sj] >> 8) + 1);
+      len += (int64_t)((pq_h[sj] & 0xff) + 1) * (((pq_h[sj] >> 8) & 0xff) + 1);
       j++;
     }
     RTN_HIP(hipMemcpy(s->cp.ptr + new_off[i], d_cp.ptr + cp_off_h[src], len * sizeof(float4),
@@ -793,7 +798,7 @@ extern "C" int rtn_scene_download(const rtn_scene* s, float* patch_cp, int* patc
   if (patch_pq) {
     std::vector<int> pq(np);
     RTN_HIP(hipMemcpy(pq.data(), s->pq.ptr, np * 4, hipMemcpyDeviceToHost));
-    for (size_t i = 0; i < np; i++) { patch_pq[2 * i] = pq[i] & 0xff; patch_pq[2 * i + 1] = pq[i] >> 8; }
+    for (size_t i = 0; i < np; i++) { patch_pq[2 * i] = pq[i] & 0xff; patch_pq[2 * i + 1] = (pq[i] >> 8) & 0xff; }
   }
   if (patch_uv) RTN_HIP(hipMemcpy(patch_uv, s->uv.ptr, np * 16, hipMemcpyDeviceToHost));
   if (patch_id) RTN_HIP(hipMemcpy(patch_id, s->patch_id.ptr, np * 4, hipMemcpyDeviceToHost));

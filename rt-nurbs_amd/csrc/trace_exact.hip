@@ -62,7 +62,7 @@ struct ScenePatches {
     c = cp + off[i];
     const int v = pq[i];
     P = v & 0xff;
-    Q = v >> 8;
+    Q = (v >> 8) & 0xff;
   }
 };
 

@@ -175,7 +175,7 @@ __device__ __forceinline__ LeafOut leaf_q(const float4* __restrict__ net, int P,
   LeafOut o;
   o.flags = 0;
   o.evals = 0;
-  if (in.hull_cull && hull_culled_q<Q>(net, P, in.rp, in.cull_margin)) {
+  if (in.hull_cull == 1 && hull_culled_q<Q>(net, P, in.rp, in.cull_margin)) {
     o.flags = 2;
     return o;
   }
@@ -300,6 +300,19 @@ k_trace_wide(const WArgs* __restrict__ args) {
         cand_first = code & 0xffffff;
         cand_left = ((code >> 24) & 0x7f) + 1;
         if (COUNT) cn.nleaf++;
+        if (A.hull_cull == 2) {  // reject from the projected control points before any Newton pass
+          while (cand_left > 0) {
+            const int pqv = A.pq[cand_first];
+            const int np = ((pqv & 0xff) + 1) * (((pqv >> 8) & 0xff) + 1);
+            if (!(pqv & 0x10000) || !hull2d_culled(A.cp + A.cp_off[cand_first], np, L.rp, A.cull_margin)) break;
+            if (COUNT) {
+              cn.ncull++;
+              cn.npt++;
+            }
+            cand_first++;
+            cand_left--;
+          }
+        }
       } else {
         cur = code;
       }
@@ -319,7 +332,7 @@ k_trace_wide(const WArgs* __restrict__ args) {
       if (COUNT) cn.npt++;
       const int pqv = A.pq[pi];
       p = pqv & 0xff;
-      q = pqv >> 8;
+      q = (pqv >> 8) & 0xff;
       cls = degree_class(p, q);
       // the reference's bilinear seed on the dehomogenised corners (bezier_patch.h:38-44, :273-327)
       const float4 c0 = A.corners[3 * pi], c1 = A.corners[3 * pi + 1], c2 = A.corners[3 * pi + 2];

@@ -181,7 +181,7 @@ def random_surface(rng, deg_range=(2, 5), cp_range=(6, 16), spacing=0.36, zsig=0
 
 
 def c2_camera(W=1920, H=1080) -> Camera:
-    return Camera((5.0, 5.0, -12.0), (5.0, 5.0, 5.0), (0.0, 1.0, 0.0), 50.0, W, H)
+    return Camera((5.0, 5.0, -6.0), (5.0, 5.0, 5.0), (0.0, 1.0, 0.0), 60.0, W, H)
 
 
 def config2_base(seed: int = 0, n_surf: int = 64) -> list:
