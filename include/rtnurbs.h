@@ -173,7 +173,10 @@ typedef struct {
                              the net; 2 projected-hull test at leaf entry (needs positive weights,
                              nets with a weight <= 0 are never rejected) */
   int count;              /* 1: fill the per-ray counters                        */
-  int reserved[6];
+  int learned_iters;      /* learned seed: Newton iterations from the network's seed before the
+                             plain-seed fallback (0 = max_iter)                  */
+  int reserved[3];
+  const struct rtn_mlp* seed_net; /* RTN_SEED_LEARNED: the seed network          */
 } rtn_trace_params;
 
 void rtn_trace_params_default(rtn_trace_params* p);
@@ -200,6 +203,57 @@ typedef struct {
 
 int rtn_trace(const rtn_scene* scene, const float* const rays_soa[6], int64_t n_rays, const rtn_trace_params* prm,
               const rtn_hits* hits, rtn_trace_totals* totals_host, void* stream);
+
+/* Converged (ray, patch, u, v) solutions of the plain method (training pairs
+ * of the learned seed; no reference counterpart).  Runs the wide or
+ * persistent form with plain seeds and appends every converged Newton solve
+ * with t > 0 (not only the closest hit) to device arrays of capacity cap;
+ * *count_host = number of solves found (may exceed cap; only cap are stored). */
+int rtn_trace_pairs(const rtn_scene* scene, const float* const rays_soa[6], int64_t n_rays,
+                    const rtn_trace_params* prm, int* pair_ray, int* pair_patch, float* pair_uv, int64_t cap,
+                    int64_t* count_host, void* stream);
+
+/* ------------------------------------------------------------ learned seed
+ * No reference counterpart.  A 20 -> 64 -> 64 -> 64 -> 2 perceptron (GELU,
+ * tanh form; sigmoid outputs) predicting Newton's starting (u,v) for a
+ * (ray, patch) pair from the pair's features (ray and corner points in the
+ * patch bounding-box frame, degrees).  Inference and training run BF16
+ * MFMA with FP32 accumulation; FP32 master parameters, Adam.
+ * Parameter layout (RTN_MLP_NPARAM floats): W1[64][20] b1[64] W2[64][64]
+ * b2[64] W3[64][64] b3[64] W4[2][64] b4[2], weights row-major (out x in). */
+#define RTN_MLP_NIN 20
+#define RTN_MLP_HIDDEN 64
+#define RTN_MLP_NPARAM 9794
+typedef struct rtn_mlp rtn_mlp;
+typedef struct {
+  int epochs;             /* passes over the pairs, reshuffled each pass          */
+  int batch;              /* pairs per Adam step                                  */
+  float lr, lr_final;     /* cosine decay from lr to lr_final over all steps      */
+  float beta1, beta2, eps;
+  int reserved;
+  uint64_t seed;          /* shuffle seed                                         */
+} rtn_mlp_train_params;
+
+int rtn_mlp_create(rtn_mlp** net, uint64_t seed); /* Glorot-uniform weights, zero biases */
+void rtn_mlp_destroy(rtn_mlp* net);
+int rtn_mlp_set_params(rtn_mlp* net, const float* params_host); /* resets Adam state */
+int rtn_mlp_get_params(const rtn_mlp* net, float* params_host);
+/* Features (device float[20*n]) of device pairs (ray index, patch position in BVH order). */
+int rtn_mlp_pair_features(const rtn_scene* scene, const float* const rays_soa[6], int64_t n_rays,
+                          const int* pair_ray, const int* pair_patch, int64_t n, float* feats, void* stream);
+/* Inference: device feats[20*n] -> device uv[2*n].  valid (device int[n], may be
+ * NULL): pairs with valid[i] == 0 are skipped and their uv left untouched; the
+ * remaining pairs of each 64-pair group are packed before the matrix products,
+ * as in the trace kernels. */
+int rtn_mlp_forward(const rtn_mlp* net, const float* feats, const int* valid, int64_t n, float* uv, void* stream);
+/* Gradient (host float[RTN_MLP_NPARAM]) of mean_n |uv_n - target_n|^2 over device
+ * feats[20*n], targets[2*n]; *loss_host = that mean.  Synchronous. */
+int rtn_mlp_gradient(rtn_mlp* net, const float* feats, const float* targets, int64_t n, float* grad_host,
+                     float* loss_host);
+void rtn_mlp_train_params_default(rtn_mlp_train_params* tp);
+/* Adam training on device pairs; epoch_loss_host[epochs] = mean loss per epoch (may be NULL). */
+int rtn_mlp_train(rtn_mlp* net, const float* feats, const float* targets, int64_t n,
+                  const rtn_mlp_train_params* tp, float* epoch_loss_host, void* stream);
 
 /* FP64 reference trace (no reference counterpart: the reference is FP32
  * only).  The plain method's traversal, seed and Newton iteration evaluated

@@ -1,0 +1,130 @@
+"""CPU oracle of the learned Newton seed (test infrastructure only).
+
+The seed network has no counterpart in the reference: it is this project's
+design, and this module restates it in numpy FP64 so the GPU kernels can be
+checked against an independent implementation:
+
+  features(...)       the 20 pair features (ray and corner points in the
+                      patch bounding-box frame, degrees)
+  forward(...)        20 -> 64 -> 64 -> 64 -> 2, GELU (tanh form), sigmoid out
+  loss_and_grad(...)  mean over pairs of |uv - target|^2 and its gradient
+
+With quantize=True the functions round to BF16 (round to nearest even) at
+exactly the points where the GPU kernels feed the matrix cores: features,
+weights, hidden activations, and the backward-pass error signals of hidden
+layers 1..3 that enter products; the arithmetic between those points is
+FP64.  With quantize=False everything is FP64.
+
+Pinning (tests/test_mlp_oracle.py): the FP64 gradient is checked against
+torch.autograd in float64 and against central finite differences; GELU
+against torch's tanh-approximate GELU; features against closed forms.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+NIN, NH, NOUT = 20, 64, 2
+OFF_W1 = 0
+OFF_B1 = OFF_W1 + NH * NIN
+OFF_W2 = OFF_B1 + NH
+OFF_B2 = OFF_W2 + NH * NH
+OFF_W3 = OFF_B2 + NH
+OFF_B3 = OFF_W3 + NH * NH
+OFF_W4 = OFF_B3 + NH
+OFF_B4 = OFF_W4 + NOUT * NH
+NPARAM = OFF_B4 + NOUT
+
+GK0 = np.sqrt(2.0 / np.pi)
+GK1 = 0.044715
+
+
+def bf16(x):
+    """Round to the nearest BF16 value (ties to even), returned as float64."""
+    f = np.asarray(x, np.float32)
+    b = f.view(np.uint32).astype(np.uint64)
+    rnd = ((b >> 16) & 1) + 0x7FFF
+    r = ((b + rnd) >> 16) << 16
+    out = r.astype(np.uint32).view(np.float32).astype(np.float64)
+    return np.where(np.isnan(f), np.nan, out)
+
+
+def unpack(w):
+    w = np.asarray(w, np.float64)
+    assert w.size == NPARAM
+    return dict(W1=w[OFF_W1:OFF_B1].reshape(NH, NIN), b1=w[OFF_B1:OFF_W2],
+                W2=w[OFF_W2:OFF_B2].reshape(NH, NH), b2=w[OFF_B2:OFF_W3],
+                W3=w[OFF_W3:OFF_B3].reshape(NH, NH), b3=w[OFF_B3:OFF_W4],
+                W4=w[OFF_W4:OFF_B4].reshape(NOUT, NH), b4=w[OFF_B4:NPARAM])
+
+
+def pack(P):
+    return np.concatenate([P["W1"].ravel(), P["b1"], P["W2"].ravel(), P["b2"], P["W3"].ravel(), P["b3"],
+                           P["W4"].ravel(), P["b4"]])
+
+
+def gelu(x):
+    return 0.5 * x * (1.0 + np.tanh(GK0 * (x + GK1 * x ** 3)))
+
+
+def gelu_grad(x):
+    t = np.tanh(GK0 * (x + GK1 * x ** 3))
+    return 0.5 * (1.0 + t) + 0.5 * x * (1.0 - t * t) * GK0 * (1.0 + 3.0 * GK1 * x * x)
+
+
+def sigmoid(z):
+    return 1.0 / (1.0 + np.exp(-z))
+
+
+def features(o, d, bounds, corners, p, q):
+    """o, d: (n,3) ray origin/direction; bounds: (n,6) min xyz, max xyz of the
+    patch; corners: (n,4,3) dehomogenised q00, q10, q11, q01; p, q: (n,)."""
+    o, d, bounds, corners = (np.asarray(a, np.float64) for a in (o, d, bounds, corners))
+    dn = d / np.linalg.norm(d, axis=1, keepdims=True)
+    c = 0.5 * (bounds[:, :3] + bounds[:, 3:])
+    hs = 0.5 * np.maximum((bounds[:, 3:] - bounds[:, :3]).max(axis=1), 2e-20)
+    tc = np.maximum(0.0, ((c - o) * dn).sum(axis=1))
+    e = (o + tc[:, None] * dn - c) / hs[:, None]
+    cr = (corners - c[:, None, :]) / hs[:, None, None]
+    deg = np.stack([(np.asarray(p) - 3.5) / 1.5, (np.asarray(q) - 3.5) / 1.5], axis=1)
+    return np.concatenate([e, dn, deg, cr.reshape(-1, 12)], axis=1)
+
+
+def forward(w, X, quantize=False):
+    """X: (n, 20) -> (uv (n,2), cache)."""
+    P = unpack(w)
+    q = bf16 if quantize else (lambda a: np.asarray(a, np.float64))
+    x = q(X)
+    W = {k: q(P[k]) for k in ("W1", "W2", "W3", "W4")}
+    z1 = x @ W["W1"].T + P["b1"]
+    h1 = q(gelu(z1))
+    z2 = h1 @ W["W2"].T + P["b2"]
+    h2 = q(gelu(z2))
+    z3 = h2 @ W["W3"].T + P["b3"]
+    h3 = q(gelu(z3))
+    z4 = h3 @ W["W4"].T + P["b4"]
+    y = sigmoid(z4)
+    return y, dict(x=x, W=W, z1=z1, h1=h1, z2=z2, h2=h2, z3=z3, h3=h3, y=y)
+
+
+def loss_and_grad(w, X, T, quantize=False):
+    """Mean over pairs of |y - t|^2 and its gradient (flat, NPARAM)."""
+    n = X.shape[0]
+    y, c = forward(w, X, quantize)
+    q = bf16 if quantize else (lambda a: a)
+    T = np.asarray(T, np.float64)
+    e = y - T
+    loss = (e * e).sum() / n
+    dz4 = 2.0 * e * y * (1.0 - y) / n
+    g = {}
+    g["W4"] = dz4.T @ c["h3"]
+    g["b4"] = dz4.sum(0)
+    dz3 = (dz4 @ c["W"]["W4"]) * gelu_grad(c["z3"])
+    g["b3"] = dz3.sum(0)
+    g["W3"] = q(dz3).T @ c["h2"]
+    dz2 = (q(dz3) @ c["W"]["W3"]) * gelu_grad(c["z2"])
+    g["b2"] = dz2.sum(0)
+    g["W2"] = q(dz2).T @ c["h1"]
+    dz1 = (q(dz2) @ c["W"]["W2"]) * gelu_grad(c["z1"])
+    g["b1"] = dz1.sum(0)
+    g["W1"] = q(dz1).T @ c["x"]
+    return loss, pack(g)

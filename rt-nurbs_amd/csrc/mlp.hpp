@@ -1,0 +1,199 @@
+// Learned Newton seed: a 20 -> 64 -> 64 -> 64 -> 2 perceptron (GELU hidden
+// units, sigmoid outputs) mapping a (ray, patch) pair to a starting (u,v).
+//
+// Inference runs per wave on the matrix cores: v_mfma_f32_16x16x32_bf16 with
+// BF16 weights and activations and FP32 accumulation.  The pairs of a wave are
+// the N dimension (16 per block, up to 4 blocks), hidden units the M
+// dimension, so one layer is D^T = W X^T + b.  The output fragment of a layer
+// (lane l holds units 4*(l/16)+i, i<4, of pair l%16) becomes the next layer's
+// B fragment (lane l holds k = 8*(l/16)+i of pair l%16) without any data
+// movement: slot i of lane group g carries unit perm_unit(g,i) of each 32-unit
+// block, and the weight tiles store their k index in that same order.
+//
+// Operand layouts of v_mfma_f32_16x16x32_bf16 (wave64):
+//   A (16 x 32): lane l holds A[l%16][8*(l/16) + i], i = 0..7
+//   B (32 x 16): lane l holds B[8*(l/16) + i][l%16]
+//   D (16 x 16): lane l holds D[4*(l/16) + i][l%16], i = 0..3
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace rtn {
+namespace mlp {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int NIN = 20, NH = 64, NOUT = 2;
+
+// FP32 master parameters, row-major (out x in) weights then bias per layer.
+constexpr int OFF_W1 = 0;
+constexpr int OFF_B1 = OFF_W1 + NH * NIN;
+constexpr int OFF_W2 = OFF_B1 + NH;
+constexpr int OFF_B2 = OFF_W2 + NH * NH;
+constexpr int OFF_W3 = OFF_B2 + NH;
+constexpr int OFF_B3 = OFF_W3 + NH * NH;
+constexpr int OFF_W4 = OFF_B3 + NH;
+constexpr int OFF_B4 = OFF_W4 + NOUT * NH;
+constexpr int NPARAM = OFF_B4 + NOUT;  // 9794
+
+// BF16 A-operand tiles, 64 lanes x 8 values each.  Forward: layer 1 (4 row
+// blocks), layers 2 and 3 (4 row blocks x 2 k blocks, tile ob*2+kb), output
+// layer (rows 0,1 real, 2 k blocks).  Backward: W2^T and W3^T (natural k order).
+constexpr int T_L1 = 0, T_L2 = 4, T_L3 = 12, T_L4 = 20, T_T2 = 22, T_T3 = 30, NTILE = 38;
+constexpr int NTILE_FWD = 22;
+// FP32 biases as accumulator seeds: b1 | b2 | b3 | b4 padded to 16 rows.
+constexpr int BB1 = 0, BB2 = 64, BB3 = 128, BB4 = 192, NBIAS = 208;
+
+// Unit carried by slot i (0..7) of lane group g within a 32-unit k block.
+__host__ __device__ constexpr int perm_unit(int g, int i) { return i < 4 ? 4 * g + i : 16 + 4 * g + (i - 4); }
+
+// GELU, tanh form: 0.5 x (1 + tanh(k0 (x + k1 x^3))) = x * sigmoid(2 k0 (x + k1 x^3)).
+constexpr float GK0 = 0.7978845608028654f, GK1 = 0.044715f;
+constexpr float LOG2E = 1.4426950408889634f;
+
+__device__ __forceinline__ float fast_sigmoid(float z) {
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-z * LOG2E));
+}
+__device__ __forceinline__ float gelu(float x) {
+  const float z = 2.0f * GK0 * x * fmaf(GK1, x * x, 1.0f);
+  return x * fast_sigmoid(z);
+}
+// d gelu / dx = s + 2 x s (1 - s) k0 (1 + 3 k1 x^2), s = sigmoid(2 k0 (x + k1 x^3))
+__device__ __forceinline__ float gelu_grad(float x) {
+  const float x2 = x * x;
+  const float s = fast_sigmoid(2.0f * GK0 * x * fmaf(GK1, x2, 1.0f));
+  return fmaf(2.0f * x * s * (1.0f - s), GK0 * fmaf(3.0f * GK1, x2, 1.0f), s);
+}
+__device__ __forceinline__ float bf16r(float x) { return (float)(__bf16)x; }
+
+// Input features of a (ray, patch) pair in the frame of the patch's bounding
+// box: centre c = box centre, scale s = largest half extent.
+//   f[0..2]  ray point closest to c (ray parameter clamped at 0), (p - c) / s
+//   f[3..5]  unit ray direction
+//   f[6..7]  degrees, (p - 3.5) / 1.5 and (q - 3.5) / 1.5
+//   f[8..19] dehomogenised corner points q00, q10, q11, q01, (q - c) / s
+__device__ __forceinline__ void pair_features(float ox, float oy, float oz, float dx, float dy, float dz,
+                                              const float* __restrict__ bnd, float4 c0, float4 c1, float4 c2, int P,
+                                              int Q, float f[NIN]) {
+  const float il = 1.0f / sqrtf(fmaf(dx, dx, fmaf(dy, dy, dz * dz)));
+  const float nx = dx * il, ny = dy * il, nz = dz * il;
+  const float cx = 0.5f * (bnd[0] + bnd[3]), cy = 0.5f * (bnd[1] + bnd[4]), cz = 0.5f * (bnd[2] + bnd[5]);
+  const float hs = fmaxf(fmaxf(bnd[3] - bnd[0], bnd[4] - bnd[1]), fmaxf(bnd[5] - bnd[2], 2e-20f)) * 0.5f;
+  const float is = 1.0f / hs;
+  const float tc = fmaxf(0.0f, fmaf(cx - ox, nx, fmaf(cy - oy, ny, (cz - oz) * nz)));
+  f[0] = (fmaf(tc, nx, ox) - cx) * is;
+  f[1] = (fmaf(tc, ny, oy) - cy) * is;
+  f[2] = (fmaf(tc, nz, oz) - cz) * is;
+  f[3] = nx;
+  f[4] = ny;
+  f[5] = nz;
+  f[6] = ((float)P - 3.5f) * (1.0f / 1.5f);
+  f[7] = ((float)Q - 3.5f) * (1.0f / 1.5f);
+  const float q[12] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w, c2.x, c2.y, c2.z, c2.w};
+  const float cc[3] = {cx, cy, cz};
+#pragma unroll
+  for (int k = 0; k < 12; k++) f[8 + k] = (q[k] - cc[k % 3]) * is;
+}
+
+__device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ f32x4 bias4(const float* __restrict__ bias, int at) {
+  const float4 b = *reinterpret_cast<const float4*>(bias + at);
+  return f32x4{b.x, b.y, b.z, b.w};
+}
+
+// Orders this wave's LDS writes before its later LDS reads by other lanes.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Forward pass for nblk blocks of 16 pairs.  s_x[r][c] holds the features of
+// pair row r as BF16 (c = 8-value column group, features 20..31 zero).  The
+// output logits of pair r go to s_logit[r].  KEEP (training) also stores the
+// FP32 pre-activations of the three hidden layers, s_z[layer][unit][pair].
+// Must be called by all 64 lanes with nblk wave-uniform.
+template <bool KEEP>
+__device__ __forceinline__ void forward_wave(const bf16x8* __restrict__ tiles, const float* __restrict__ bias,
+                                             const bf16x8 (*s_x)[4], int nblk, int lane, float2* s_logit,
+                                             float* s_z) {
+  const int m = lane & 15, g = lane >> 4;
+#pragma unroll 1
+  for (int nb = 0; nb < nblk; nb++) {
+    f32x4 d[4];
+    bf16x8 h[2];
+    const bf16x8 x = s_x[16 * nb + m][g];
+#pragma unroll
+    for (int ob = 0; ob < 4; ob++) d[ob] = mfma(tiles[(T_L1 + ob) * 64 + lane], x, bias4(bias, BB1 + 16 * ob + 4 * g));
+#pragma unroll
+    for (int layer = 0; layer < 3; layer++) {
+      if (KEEP) {
+#pragma unroll
+        for (int ob = 0; ob < 4; ob++)
+#pragma unroll
+          for (int i = 0; i < 4; i++) s_z[(layer * NH + 16 * ob + 4 * g + i) * 64 + 16 * nb + m] = d[ob][i];
+      }
+#pragma unroll
+      for (int kb = 0; kb < 2; kb++)
+#pragma unroll
+        for (int i = 0; i < 8; i++) h[kb][i] = (__bf16)gelu(d[2 * kb + (i >> 2)][i & 3]);
+      if (layer == 2) break;
+      const int tl = layer == 0 ? T_L2 : T_L3;
+      const int bb = layer == 0 ? BB2 : BB3;
+#pragma unroll
+      for (int ob = 0; ob < 4; ob++) {
+        f32x4 acc = bias4(bias, bb + 16 * ob + 4 * g);
+        acc = mfma(tiles[(tl + ob * 2) * 64 + lane], h[0], acc);
+        d[ob] = mfma(tiles[(tl + ob * 2 + 1) * 64 + lane], h[1], acc);
+      }
+    }
+    f32x4 o = bias4(bias, BB4 + 4 * g);
+    o = mfma(tiles[(T_L4 + 0) * 64 + lane], h[0], o);
+    o = mfma(tiles[(T_L4 + 1) * 64 + lane], h[1], o);
+    if (g == 0) s_logit[16 * nb + m] = make_float2(o[0], o[1]);
+  }
+}
+
+// Packs a pair's features as one BF16 row of s_x (32 values, 20..31 zero).
+__device__ __forceinline__ void store_row(bf16x8 (*s_x)[4], int row, const float f[NIN], bool zero) {
+  bf16x8 r[4];
+#pragma unroll
+  for (int c = 0; c < 4; c++)
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const int k = 8 * c + i;
+      r[c][i] = (__bf16)((k < NIN && !zero) ? f[k < NIN ? k : 0] : 0.0f);
+    }
+#pragma unroll
+  for (int c = 0; c < 4; c++) s_x[row][c] = r[c];
+}
+
+// Seeds for the active lanes of a wave: active pairs are compacted to the
+// first rows, so a wave with k active lanes runs ceil(k/16) pair blocks.
+// Called by all 64 lanes; u, v are written for active lanes only.
+__device__ __forceinline__ void seed_wave(const bf16x8* __restrict__ tiles, const float* __restrict__ bias,
+                                          bool active, const float f[NIN], float& u, float& v, bf16x8 (*s_x)[4],
+                                          float2* s_logit, int lane) {
+  const unsigned long long bal = __ballot(active);
+  const int n_act = __popcll(bal);
+  const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  const int row = active ? __popcll(bal & below) : n_act + __popcll(~bal & below);
+  store_row(s_x, row, f, !active);
+  wave_lds_sync();
+  forward_wave<false>(tiles, bias, s_x, (n_act + 15) >> 4, lane, s_logit, nullptr);
+  wave_lds_sync();
+  if (active) {
+    const float2 lg = s_logit[row];
+    u = fast_sigmoid(lg.x);
+    v = fast_sigmoid(lg.y);
+  }
+  wave_lds_sync();  // the next call's writes come after these reads
+}
+
+}  // namespace mlp
+}  // namespace rtn

@@ -90,6 +90,15 @@ struct alignas(16) WideNode {
 };
 static_assert(sizeof(WideNode) == 128, "wide node is 128 bytes");
 
+// Training-pair output of the wide kernels (converged plain-method solves).
+struct PairOut {
+  int* ray;
+  int* patch;
+  float2* uv;
+  unsigned long long* count;
+  int64_t cap;
+};
+
 }  // namespace rtn
 
 // Scene object behind the opaque C handle.

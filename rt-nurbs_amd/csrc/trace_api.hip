@@ -7,7 +7,7 @@ namespace rtn {
 int trace_exact_scene(const rtn_scene* s, const float* const rays[6], int64_t n, const rtn_trace_params* prm,
                       const rtn_hits* hits, unsigned long long* d_totals, hipStream_t stream);
 int trace_wide_scene(const rtn_scene* s, const float* const rays[6], int64_t n, const rtn_trace_params* prm,
-                     const rtn_hits* hits, unsigned long long* d_totals, hipStream_t stream);
+                     const rtn_hits* hits, unsigned long long* d_totals, hipStream_t stream, const PairOut* pairs);
 }  // namespace rtn
 
 using namespace rtn;
@@ -36,6 +36,7 @@ extern "C" int rtn_trace(const rtn_scene* s, const float* const rays[6], int64_t
   for (int k = 0; k < 6; k++) RTN_CHECK(rays[k], RTN_EINVAL, "rtn_trace: null ray component %d", k);
   RTN_CHECK(n_rays >= 0 && n_rays < ((int64_t)1 << 32), RTN_EINVAL, "rtn_trace: bad ray count");
   RTN_CHECK(prm->max_iter >= 1 && prm->max_iter <= 64, RTN_EINVAL, "rtn_trace: max_iter outside 1..64");
+  RTN_CHECK(prm->hull_cull >= 0 && prm->hull_cull <= 2, RTN_EINVAL, "rtn_trace: hull_cull outside 0..2");
   RTN_CHECK(!prm->count || hits->counters, RTN_EINVAL, "rtn_trace: count requested without a counter buffer");
   if (n_rays == 0) return RTN_OK;
   hipStream_t st = (hipStream_t)stream;
@@ -53,7 +54,7 @@ extern "C" int rtn_trace(const rtn_scene* s, const float* const rays[6], int64_t
       break;
     case RTN_MODE_WIDE:
     case RTN_MODE_PERSISTENT:
-      rc = trace_wide_scene(s, rays, n_rays, prm, hits, d_tot, st);
+      rc = trace_wide_scene(s, rays, n_rays, prm, hits, d_tot, st, nullptr);
       break;
     default:
       set_error("rtn_trace: unknown mode %d", prm->mode);
@@ -73,4 +74,36 @@ extern "C" int rtn_trace(const rtn_scene* s, const float* const rays[6], int64_t
     (void)hipFreeAsync(d_tot, st);
   }
   return rc;
+}
+
+extern "C" int rtn_trace_pairs(const rtn_scene* s, const float* const rays[6], int64_t n_rays,
+                               const rtn_trace_params* prm, int* pair_ray, int* pair_patch, float* pair_uv,
+                               int64_t cap, int64_t* count_host, void* stream) {
+  RTN_CHECK(s && s->built, RTN_EINVAL, "rtn_trace_pairs: scene not built");
+  RTN_CHECK(rays && prm && pair_ray && pair_patch && pair_uv && count_host && cap >= 0, RTN_EINVAL,
+            "rtn_trace_pairs: bad arguments");
+  for (int k = 0; k < 6; k++) RTN_CHECK(rays[k], RTN_EINVAL, "rtn_trace_pairs: null ray component %d", k);
+  RTN_CHECK(n_rays >= 0 && n_rays < ((int64_t)1 << 31), RTN_EINVAL, "rtn_trace_pairs: bad ray count");
+  RTN_CHECK(prm->mode == RTN_MODE_WIDE || prm->mode == RTN_MODE_PERSISTENT, RTN_EUNSUPPORTED,
+            "rtn_trace_pairs: runs the wide or persistent form");
+  RTN_CHECK(prm->seed_mode == RTN_SEED_PLAIN && prm->precision == RTN_PREC_FP32 && !prm->count, RTN_EINVAL,
+            "rtn_trace_pairs: plain seeds, FP32, no counters");
+  RTN_CHECK(prm->max_iter >= 1 && prm->max_iter <= 64, RTN_EINVAL, "rtn_trace_pairs: max_iter outside 1..64");
+  *count_host = 0;
+  if (n_rays == 0) return RTN_OK;
+  hipStream_t st = (hipStream_t)stream;
+  DevBuf<int> prim;
+  DevBuf<unsigned long long> cnt;
+  RTN_TRY(prim.alloc((size_t)n_rays));
+  RTN_TRY(cnt.alloc(1));
+  RTN_HIP(hipMemsetAsync(cnt.ptr, 0, sizeof(unsigned long long), st));
+  rtn_hits h{};
+  h.prim = prim.ptr;
+  PairOut po{pair_ray, pair_patch, reinterpret_cast<float2*>(pair_uv), cnt.ptr, cap};
+  RTN_TRY(trace_wide_scene(s, rays, n_rays, prm, &h, nullptr, st, &po));
+  unsigned long long c = 0;
+  RTN_HIP(hipMemcpyAsync(&c, cnt.ptr, sizeof(c), hipMemcpyDeviceToHost, st));
+  RTN_HIP(hipStreamSynchronize(st));
+  *count_host = (int64_t)c;
+  return RTN_OK;
 }

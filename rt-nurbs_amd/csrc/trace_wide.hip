@@ -14,12 +14,16 @@
 // Arithmetic: explicit FMA and tensor-product Bernstein evaluation (rtn_fast.hpp);
 // the seed is the reference's bilinear seed in its own operation order; the
 // iteration keeps the plain method's convergence test, PCG perturbation on a
-// singular Jacobian and clamping.  Options: exact convex-hull rejection before
-// Newton, and an FP64 correction step for hits whose Jacobian condition number
-// exceeds a threshold (mixed precision).  Improving hits are written through
-// to the output arrays, as the reference writes its record (intersect.cu:133-138).
+// singular Jacobian and clamping.  Options: exact rejection of primitives from
+// their projected control points (at leaf entry, or per class before Newton),
+// the learned seed (network on the matrix cores for all pending pairs of the
+// wave, plain seed as fallback), and an FP64 correction step for hits whose
+// Jacobian condition number exceeds a threshold (mixed precision).  Improving
+// hits are written through to the output arrays, as the reference writes its
+// record (intersect.cu:133-138).
 #include <algorithm>
 
+#include "mlp.hpp"
 #include "rtn_fast.hpp"
 #include "rtn_internal.hpp"
 
@@ -46,7 +50,17 @@ struct WArgs {
   float cull_margin;
   int mixed;
   float cond_thr;
-  const float2* learned_seed;  // learned form: seed per ray for its first patch test, or null
+  // learned form: BF16 MFMA tiles and biases of the seed network, patch boxes
+  const mlp::bf16x8* mlp_tiles;
+  const float* mlp_bias;
+  const float* bounds;
+  int learned_iters;
+  // training pairs: every converged solve with t > 0 is appended
+  int* pair_ray;
+  int* pair_patch;
+  float2* pair_uv;
+  unsigned long long* pair_count;
+  int64_t pair_cap;
   int* prim;
   float *t, *u, *v, *dpdu, *dpdv;
   uint32_t* counters;
@@ -163,7 +177,7 @@ struct LeafIn {
   f3 o, d;
   float gu, gv, pu, pv;
   uint32_t seed;
-  int max_iter, hull_cull, mixed, learned;
+  int max_iter, learned_iters, hull_cull, mixed, learned;
   float thr_det, thr_res, cull_margin, cond_thr;
 };
 
@@ -185,7 +199,8 @@ __device__ __forceinline__ LeafOut leaf_q(const float4* __restrict__ net, int P,
   float su = in.gu, sv = in.gv;
 #pragma unroll 1
   for (int attempt = 0;; attempt++) {
-    h = newton_q<Q>(net, P, in.rp, su, sv, in.thr_det, in.thr_res, in.seed, in.max_iter);
+    h = newton_q<Q>(net, P, in.rp, su, sv, in.thr_det, in.thr_res, in.seed,
+                    attempt == 0 && in.learned ? in.learned_iters : in.max_iter);
     o.evals += h.evals;
     if (h.hit || !in.learned || attempt == 1) break;
     o.flags |= 8;
@@ -221,11 +236,14 @@ __device__ __forceinline__ LeafOut leaf_dispatch(int cls, const float4* net, int
   }
 }
 
-template <bool PERSISTENT, bool COUNT>
+template <bool PERSISTENT, bool COUNT, bool LEARNED>
 __global__ void __launch_bounds__(WB)
 k_trace_wide(const WArgs* __restrict__ args) {
   __shared__ int s_code[WSTK][WB];
   __shared__ float s_te[WSTK][WB];
+  // learned form: per-wave feature rows and output logits of the seed network
+  __shared__ mlp::bf16x8 s_x[LEARNED ? WB : 1][4];
+  __shared__ float2 s_logit[LEARNED ? WB : 1];
   const WArgs& A = *args;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -325,6 +343,7 @@ k_trace_wide(const WArgs* __restrict__ args) {
     // ---- leaf phase: one primitive per pending lane, Newton by degree class
     int pi = 0, p = 0, q = 0, cls = -1;
     float gu = 0.5f, gv = 0.5f;
+    float4 c0 = make_float4(0.f, 0.f, 0.f, 0.f), c1 = c0, c2 = c0;
     if (pending) {
       pi = cand_first;
       cand_first++;
@@ -335,7 +354,9 @@ k_trace_wide(const WArgs* __restrict__ args) {
       q = (pqv >> 8) & 0xff;
       cls = degree_class(p, q);
       // the reference's bilinear seed on the dehomogenised corners (bezier_patch.h:38-44, :273-327)
-      const float4 c0 = A.corners[3 * pi], c1 = A.corners[3 * pi + 1], c2 = A.corners[3 * pi + 2];
+      c0 = A.corners[3 * pi];
+      c1 = A.corners[3 * pi + 1];
+      c2 = A.corners[3 * pi + 2];
       ex::bilinear_seed<float>({c0.x, c0.y, c0.z}, {c0.w, c1.x, c1.y}, {c1.z, c1.w, c2.x}, {c2.y, c2.z, c2.w},
                                {L.o.x, L.o.y, L.o.z}, {L.d.x, L.d.y, L.d.z}, gu, gv);
     }
@@ -347,11 +368,24 @@ k_trace_wide(const WArgs* __restrict__ args) {
     in.gv = gv;
     in.pu = gu;
     in.pv = gv;
+    if (LEARNED) {  // network seed for every pending pair of the wave (matrix cores), plain seed kept as fallback
+      float f[mlp::NIN];
+      if (pending) {
+        mlp::pair_features(L.o.x, L.o.y, L.o.z, L.d.x, L.d.y, L.d.z, A.bounds + 6 * (int64_t)pi, c0, c1, c2, p, q, f);
+      } else {
+#pragma unroll
+        for (int k = 0; k < mlp::NIN; k++) f[k] = 0.0f;
+      }
+      const int wbase = tid & ~63;
+      mlp::seed_wave(A.mlp_tiles, A.mlp_bias, pending, f, in.gu, in.gv,
+                     reinterpret_cast<mlp::bf16x8(*)[4]>(&s_x[wbase][0]), &s_logit[wbase], lane);
+    }
     in.seed = L.r + A.seed_base;
     in.max_iter = A.max_iter;
+    in.learned_iters = A.learned_iters;
     in.hull_cull = A.hull_cull;
     in.mixed = A.mixed;
-    in.learned = 0;
+    in.learned = LEARNED;
     in.thr_det = A.thr_det;
     in.thr_res = A.thr_res;
     in.cull_margin = A.cull_margin;
@@ -382,6 +416,14 @@ k_trace_wide(const WArgs* __restrict__ args) {
           if (res.flags & 8) cn.nfb++;
         }
       }
+      if (A.pair_count && (res.flags & 1) && 0.0f < res.t) {
+        const unsigned long long k = atomicAdd(A.pair_count, 1ull);
+        if ((int64_t)k < A.pair_cap) {
+          A.pair_ray[k] = (int)L.r;
+          A.pair_patch[k] = pi;
+          A.pair_uv[k] = make_float2(res.u, res.v);
+        }
+      }
       if ((res.flags & 1) && 0.0f < res.t && res.t < L.t_max) record(L, A, pi, res.t, res.u, res.v, res.Su, res.Sv);
     }
   }
@@ -399,9 +441,25 @@ k_trace_wide(const WArgs* __restrict__ args) {
 
 namespace rtn {
 
+const void* mlp_tiles(const rtn_mlp* net);
+const float* mlp_bias(const rtn_mlp* net);
+
+template <bool P, bool C>
+static void launch_wide(bool learned, unsigned blocks, hipStream_t stream, const WArgs* a) {
+  if (learned)
+    hipLaunchKernelGGL((k_trace_wide<P, C, true>), dim3(blocks), dim3(WB), 0, stream, a);
+  else
+    hipLaunchKernelGGL((k_trace_wide<P, C, false>), dim3(blocks), dim3(WB), 0, stream, a);
+}
+
 int trace_wide_scene(const rtn_scene* s, const float* const rays[6], int64_t n, const rtn_trace_params* prm,
-                     const rtn_hits* hits, unsigned long long* d_totals, hipStream_t stream) {
-  RTN_CHECK(prm->seed_mode == RTN_SEED_PLAIN, RTN_EUNSUPPORTED, "rtn_trace: learned seeds need a loaded network");
+                     const rtn_hits* hits, unsigned long long* d_totals, hipStream_t stream,
+                     const PairOut* pairs) {
+  const bool learned = prm->seed_mode == RTN_SEED_LEARNED;
+  RTN_CHECK(prm->seed_mode == RTN_SEED_PLAIN || learned, RTN_EINVAL, "rtn_trace: unknown seed mode %d",
+            prm->seed_mode);
+  RTN_CHECK(!learned || prm->seed_net, RTN_EINVAL, "rtn_trace: the learned seed needs seed_net");
+  RTN_CHECK(prm->learned_iters >= 0 && prm->learned_iters <= 64, RTN_EINVAL, "rtn_trace: learned_iters outside 0..64");
   RTN_CHECK(prm->flat_threshold <= 0.f, RTN_EUNSUPPORTED, "rtn_trace: flat-skip runs in the exact form only");
   WArgs A{};
   A.nodes = s->wide.ptr;
@@ -419,6 +477,19 @@ int trace_wide_scene(const rtn_scene* s, const float* const rays[6], int64_t n, 
   A.cull_margin = 2.0f * prm->thr_res + 1e-5f;
   A.mixed = prm->precision == RTN_PREC_MIXED;
   A.cond_thr = prm->cond_thr;
+  if (learned) {
+    A.mlp_tiles = reinterpret_cast<const mlp::bf16x8*>(mlp_tiles(prm->seed_net));
+    A.mlp_bias = mlp_bias(prm->seed_net);
+    A.bounds = s->bounds.ptr;
+    A.learned_iters = prm->learned_iters > 0 ? prm->learned_iters : prm->max_iter;
+  }
+  if (pairs) {
+    A.pair_ray = pairs->ray;
+    A.pair_patch = pairs->patch;
+    A.pair_uv = pairs->uv;
+    A.pair_count = pairs->count;
+    A.pair_cap = pairs->cap;
+  }
   A.prim = hits->prim; A.t = hits->t; A.u = hits->u; A.v = hits->v; A.dpdu = hits->dpdu; A.dpdv = hits->dpdv;
   A.counters = prm->count ? hits->counters : nullptr;
   A.totals = d_totals;
@@ -439,15 +510,15 @@ int trace_wide_scene(const rtn_scene* s, const float* const rays[6], int64_t n, 
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + WB - 1) / WB, (int64_t)cus * 8));
     if (count)
-      hipLaunchKernelGGL((k_trace_wide<true, true>), dim3(blocks), dim3(WB), 0, stream, &d_blk->a);
+      launch_wide<true, true>(learned, blocks, stream, &d_blk->a);
     else
-      hipLaunchKernelGGL((k_trace_wide<true, false>), dim3(blocks), dim3(WB), 0, stream, &d_blk->a);
+      launch_wide<true, false>(learned, blocks, stream, &d_blk->a);
   } else {
     const unsigned blocks = (unsigned)((n + WB - 1) / WB);
     if (count)
-      hipLaunchKernelGGL((k_trace_wide<false, true>), dim3(blocks), dim3(WB), 0, stream, &d_blk->a);
+      launch_wide<false, true>(learned, blocks, stream, &d_blk->a);
     else
-      hipLaunchKernelGGL((k_trace_wide<false, false>), dim3(blocks), dim3(WB), 0, stream, &d_blk->a);
+      launch_wide<false, false>(learned, blocks, stream, &d_blk->a);
   }
   RTN_HIP(hipGetLastError());
   RTN_HIP(hipFreeAsync(d_blk, stream));

@@ -33,8 +33,11 @@ EXPORTS = [
     "rtn_scene_add_nurbs", "rtn_scene_add_patches", "rtn_scene_build", "rtn_scene_stats_get",
     "rtn_scene_download", "rtn_trace_params_default", "rtn_trace", "rtn_calculate_intersect", "rtn_get_patch",
     "rtn_build_bvh", "rtn_patch_bounds", "rtn_refine", "rtn_trace_f64", "rtn_event_create", "rtn_event_destroy",
-    "rtn_event_record", "rtn_event_elapsed_ms",
+    "rtn_event_record", "rtn_event_elapsed_ms", "rtn_trace_pairs", "rtn_mlp_create", "rtn_mlp_destroy",
+    "rtn_mlp_set_params", "rtn_mlp_get_params", "rtn_mlp_pair_features", "rtn_mlp_forward", "rtn_mlp_gradient",
+    "rtn_mlp_train_params_default", "rtn_mlp_train",
 ]
+MLP_NIN, MLP_HIDDEN, MLP_NPARAM = 20, 64, 9794
 
 
 class RtnError(RuntimeError):
@@ -53,7 +56,14 @@ class TraceParams(C.Structure):
     _fields_ = [("mode", C.c_int), ("max_iter", C.c_int), ("thr_det", C.c_float), ("thr_res", C.c_float),
                 ("flat_threshold", C.c_float), ("cam_k", C.c_float), ("seed_base", C.c_uint32),
                 ("seed_mode", C.c_int), ("precision", C.c_int), ("cond_thr", C.c_float),
-                ("hull_cull", C.c_int), ("count", C.c_int), ("reserved", C.c_int * 6)]
+                ("hull_cull", C.c_int), ("count", C.c_int), ("learned_iters", C.c_int),
+                ("reserved", C.c_int * 3), ("seed_net", C.c_void_p)]
+
+
+class MlpTrainParams(C.Structure):
+    _fields_ = [("epochs", C.c_int), ("batch", C.c_int), ("lr", C.c_float), ("lr_final", C.c_float),
+                ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float), ("reserved", C.c_int),
+                ("seed", C.c_uint64)]
 
 
 class Hits(C.Structure):
@@ -119,6 +129,20 @@ def load():
     L.rtn_event_destroy.argtypes = [vp]
     L.rtn_event_record.argtypes = [vp, vp]
     L.rtn_event_elapsed_ms.argtypes = [vp, vp, C.POINTER(C.c_float)]
+    i64 = C.c_int64
+    L.rtn_trace_pairs.argtypes = [vp, C.POINTER(vp * 6), i64, C.POINTER(TraceParams), vp, vp, vp, i64,
+                                  C.POINTER(i64), vp]
+    L.rtn_mlp_create.argtypes = [C.POINTER(vp), C.c_uint64]
+    L.rtn_mlp_destroy.argtypes = [vp]
+    L.rtn_mlp_destroy.restype = None
+    L.rtn_mlp_set_params.argtypes = [vp, vp]
+    L.rtn_mlp_get_params.argtypes = [vp, vp]
+    L.rtn_mlp_pair_features.argtypes = [vp, C.POINTER(vp * 6), i64, vp, vp, i64, vp, vp]
+    L.rtn_mlp_forward.argtypes = [vp, vp, vp, i64, vp, vp]
+    L.rtn_mlp_gradient.argtypes = [vp, vp, vp, i64, vp, C.POINTER(C.c_float)]
+    L.rtn_mlp_train_params_default.argtypes = [C.POINTER(MlpTrainParams)]
+    L.rtn_mlp_train_params_default.restype = None
+    L.rtn_mlp_train.argtypes = [vp, vp, vp, i64, C.POINTER(MlpTrainParams), vp, vp]
     _lib = L
     return L
 
@@ -313,11 +337,15 @@ class Scene:
 
 
 def params(**kw):
+    """rtn_trace_params with defaults; seed_net takes an Mlp (kept alive by the struct)."""
     p = TraceParams()
     load().rtn_trace_params_default(C.byref(p))
     for k, v in kw.items():
         if not hasattr(p, k):
             raise KeyError(k)
+        if k == "seed_net":
+            p._net = v
+            v = v.h if v is not None else None
         setattr(p, k, v)
     return p
 
@@ -446,3 +474,86 @@ def trace_f64(scene, rays, max_iter=32, thr_det=1e-12, thr_res=1e-10, seed_base=
                                h.t.ptr, h.u.ptr, h.v.ptr, h.dpdu.ptr, h.dpdv.ptr, h.t2.ptr,
                                h.counters.ptr if h.counters else None, stream))
     return h
+
+
+# ---------------------------------------------------------- learned seed
+class Mlp:
+    """The seed network (rtn_mlp): BF16 MFMA inference and Adam training on the GPU."""
+
+    def __init__(self, seed=0, params=None):
+        h = C.c_void_p()
+        check(load().rtn_mlp_create(C.byref(h), seed))
+        self.h = h.value
+        if params is not None:
+            self.set_params(params)
+
+    def set_params(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        assert w.size == MLP_NPARAM
+        check(load().rtn_mlp_set_params(self.h, w.ctypes.data))
+
+    def get_params(self):
+        w = np.zeros(MLP_NPARAM, np.float32)
+        check(load().rtn_mlp_get_params(self.h, w.ctypes.data))
+        return w
+
+    def forward(self, feats, valid=None):
+        """feats: DeviceArray (n, 20) float32 -> DeviceArray (n, 2); rows with
+        valid == 0 (device int32 array, optional) are skipped (left zero)."""
+        n = feats.shape[0]
+        uv = DeviceArray((n, 2), np.float32)
+        uv.fill_bytes(0)
+        check(load().rtn_mlp_forward(self.h, feats.ptr, valid.ptr if valid is not None else None, n, uv.ptr, None))
+        synchronize()
+        return uv
+
+    def gradient(self, feats, targets):
+        n = feats.shape[0]
+        g = np.zeros(MLP_NPARAM, np.float32)
+        loss = C.c_float()
+        check(load().rtn_mlp_gradient(self.h, feats.ptr, targets.ptr, n, g.ctypes.data, C.byref(loss)))
+        return g, loss.value
+
+    def train(self, feats, targets, **kw):
+        tp = MlpTrainParams()
+        load().rtn_mlp_train_params_default(C.byref(tp))
+        for k, v in kw.items():
+            if not hasattr(tp, k):
+                raise KeyError(k)
+            setattr(tp, k, v)
+        losses = np.zeros(tp.epochs, np.float32)
+        check(load().rtn_mlp_train(self.h, feats.ptr, targets.ptr, feats.shape[0], C.byref(tp),
+                                   losses.ctypes.data, None))
+        return losses
+
+    def __del__(self):
+        try:
+            if self.h:
+                load().rtn_mlp_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def trace_pairs(scene, rays, prm, cap):
+    """Converged plain-method solves: (ray idx, patch, uv) device arrays and the count found."""
+    pr = DeviceArray((cap,), np.int32)
+    pp = DeviceArray((cap,), np.int32)
+    puv = DeviceArray((cap, 2), np.float32)
+    cnt = C.c_int64()
+    arr = rays.ptr_array()
+    check(load().rtn_trace_pairs(scene.h, C.byref(arr), rays.n, C.byref(prm), pr.ptr, pp.ptr, puv.ptr, cap,
+                                 C.byref(cnt), None))
+    return pr, pp, puv, cnt.value
+
+
+def pair_features(scene, rays, pair_ray, pair_patch, n, out=None, out_offset=0):
+    """Features (n, 20) of device pairs; written into out[out_offset:] when given."""
+    if out is None:
+        out = DeviceArray((n, MLP_NIN), np.float32)
+        out_offset = 0
+    assert out_offset + n <= out.shape[0]
+    arr = rays.ptr_array()
+    check(load().rtn_mlp_pair_features(scene.h, C.byref(arr), rays.n, pair_ray.ptr, pair_patch.ptr, n,
+                                       out.ptr + out_offset * MLP_NIN * 4, None))
+    return out

@@ -1,0 +1,118 @@
+"""Learned seed on the GPU against the CPU oracle (oracle/mlp_oracle.py):
+BF16 MFMA inference, the training kernels' gradient, pair features and the
+training pairs produced by the plain method.
+
+Tolerances: the oracle's quantized mode rounds to BF16 at the same points as
+the kernels, leaving FP32 accumulation order, the hardware exp/reciprocal in
+GELU and sigmoid, and BF16 rounding-boundary flips: 2e-3 absolute on (u,v),
+2e-2 relative L2 per gradient tensor.  Against plain FP64 the BF16 rounding
+itself adds about 1e-2: 2e-2 on (u,v), 6e-2 per tensor."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from rtnurbs import scenes
+
+sys.path.insert(0, os.path.join(os.path.dirname(__file__), "..", "oracle"))
+import mlp_oracle as M  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+TENSORS = [("W1", M.OFF_W1, M.OFF_B1), ("b1", M.OFF_B1, M.OFF_W2), ("W2", M.OFF_W2, M.OFF_B2),
+           ("b2", M.OFF_B2, M.OFF_W3), ("W3", M.OFF_W3, M.OFF_B3), ("b3", M.OFF_B3, M.OFF_W4),
+           ("W4", M.OFF_W4, M.OFF_B4), ("b4", M.OFF_B4, M.NPARAM)]
+
+
+def _net(rtl, seed):
+    net = rtl.Mlp(seed=seed)
+    w = net.get_params()
+    rng = np.random.default_rng(seed)
+    for _, a, b in TENSORS[1::2]:
+        w[a:b] = rng.normal(0, 0.2, b - a)
+    net.set_params(w)
+    return net, net.get_params().astype(np.float64)
+
+
+def test_forward_matches_oracle(rtl):
+    net, w = _net(rtl, 3)
+    rng = np.random.default_rng(0)
+    n = 1000  # 15 full waves and a ragged one
+    X = rng.normal(0, 1, (n, M.NIN)).astype(np.float32)
+    valid = (rng.uniform(size=n) < 0.7).astype(np.int32)  # holes: rows are packed before the products
+    valid[:64] = 1
+    print("forward n=1000", flush=True)
+    Xd = rtl.DeviceArray.from_host(X)
+    uv_all = net.forward(Xd).get()
+    uv_m = net.forward(Xd, rtl.DeviceArray.from_host(valid)).get()
+    yq, _ = M.forward(w, X.astype(np.float64), quantize=True)
+    y, _ = M.forward(w, X.astype(np.float64))
+    assert np.abs(uv_all - yq).max() < 2e-3
+    assert np.abs(uv_all - y).max() < 2e-2
+    m = valid == 1
+    assert np.array_equal(uv_m[m], uv_all[m])  # packing does not change a pair's result
+    assert np.all(uv_m[~m] == 0)
+
+
+def test_gradient_matches_oracle(rtl):
+    net, w = _net(rtl, 5)
+    rng = np.random.default_rng(1)
+    n = 300
+    X = rng.normal(0, 1, (n, M.NIN)).astype(np.float32)
+    T = rng.uniform(0, 1, (n, 2)).astype(np.float32)
+    print("gradient n=300", flush=True)
+    g, loss = net.gradient(rtl.DeviceArray.from_host(X), rtl.DeviceArray.from_host(T))
+    lq, gq = M.loss_and_grad(w, X.astype(np.float64), T.astype(np.float64), quantize=True)
+    lf, gf = M.loss_and_grad(w, X.astype(np.float64), T.astype(np.float64))
+    assert abs(loss - lq) < 1e-3 * lq and abs(loss - lf) < 2e-2 * lf
+    for name, a, b in TENSORS:
+        rq = np.linalg.norm(g[a:b] - gq[a:b]) / np.linalg.norm(gq[a:b])
+        rf = np.linalg.norm(g[a:b] - gf[a:b]) / np.linalg.norm(gf[a:b])
+        cos = g[a:b] @ gf[a:b] / (np.linalg.norm(g[a:b]) * np.linalg.norm(gf[a:b]))
+        assert rq < 2e-2 and rf < 6e-2 and cos > 0.998, (name, rq, rf, cos)
+
+
+def _patch_geometry(dl):
+    """Bounds (true division by w) and dehomogenised corners per patch, from a scene download."""
+    cp, off, pq = dl["cp"].astype(np.float32), dl["off"], dl["pq"]
+    n = len(off)
+    bounds = np.zeros((n, 6), np.float32)
+    corners = np.zeros((n, 4, 3), np.float64)
+    for i in range(n):
+        p, q = pq[i]
+        net = cp[off[i]:off[i] + (p + 1) * (q + 1)]
+        xyz = net[:, :3] / net[:, 3:4]
+        bounds[i, :3], bounds[i, 3:] = xyz.min(0), xyz.max(0)
+        for k, j in enumerate((0, p * (q + 1), p * (q + 1) + q, q)):
+            corners[i, k] = net[j, :3] / net[j, 3]
+    return bounds, corners
+
+
+@pytest.mark.parametrize("name,mk", [("c1", lambda: scenes.config1(0, 96, 96)),
+                                     ("torus", lambda: scenes.Scene([scenes.torus()],
+                                                                     scenes.reference_camera(160, 90)))])
+def test_pairs_and_features(rtl, name, mk):
+    sc = mk()
+    print(f"pairs {name}", flush=True)
+    scene = rtl.Scene(sc.surfaces).build()
+    rays = rtl.generate_rays(sc.camera)
+    prm = rtl.params(mode=rtl.MODE_WIDE, max_iter=8, thr_res=1e-5, thr_det=1e-5, hull_cull=2)
+    pr, pp, puv, cnt = rtl.trace_pairs(scene, rays, prm, 4 * rays.n)
+    assert 0 < cnt <= 4 * rays.n
+    pr, pp, puv = pr.get()[:cnt], pp.get()[:cnt], puv.get()[:cnt]
+    assert np.all((puv >= 0) & (puv <= 1))
+    # every closest hit of the same form is among the converged solves
+    hits, _ = rtl.trace(scene, rays, prm)
+    h = hits.get()
+    have = set(zip(pr.tolist(), pp.tolist()))
+    for r in np.flatnonzero(h["hit"] >= 0)[::7]:
+        assert (int(r), int(h["hit"][r])) in have
+    # features against the oracle
+    f = rtl.pair_features(scene, rays, rtl.DeviceArray.from_host(pr), rtl.DeviceArray.from_host(pp), cnt).get()
+    dl = scene.download()
+    bounds, corners = _patch_geometry(dl)
+    o = np.stack([a for a in rays.get()[:3]], 1)[pr]
+    d = np.stack([a for a in rays.get()[3:]], 1)[pr]
+    ref = M.features(o, d, bounds[pp], corners[pp], dl["pq"][pp, 0], dl["pq"][pp, 1])
+    np.testing.assert_allclose(f, ref, rtol=1e-5, atol=2e-5)

@@ -1,0 +1,144 @@
+"""Pins the CPU oracle of the learned seed (oracle/mlp_oracle.py) against
+independent references: torch's tanh-approximate GELU and autograd in
+float64, central finite differences, torch's BF16 rounding, and closed forms
+of the pair features."""
+import os
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+sys.path.insert(0, os.path.join(os.path.dirname(__file__), "..", "oracle"))
+import mlp_oracle as M  # noqa: E402
+
+
+def _params(seed, scale=1.0):
+    rng = np.random.default_rng(seed)
+    w = np.zeros(M.NPARAM)
+    for off, fo, fi in ((M.OFF_W1, M.NH, M.NIN), (M.OFF_W2, M.NH, M.NH), (M.OFF_W3, M.NH, M.NH),
+                        (M.OFF_W4, M.NOUT, M.NH)):
+        a = np.sqrt(6.0 / (fo + fi)) * scale
+        w[off:off + fo * fi] = rng.uniform(-a, a, fo * fi)
+    for off, k in ((M.OFF_B1, M.NH), (M.OFF_B2, M.NH), (M.OFF_B3, M.NH), (M.OFF_B4, M.NOUT)):
+        w[off:off + k] = rng.normal(0, 0.1, k)
+    return w
+
+
+def _torch_loss(w, X, T):
+    P = M.unpack(w)
+    t = {k: torch.tensor(v, dtype=torch.float64, requires_grad=True) for k, v in P.items()}
+    x = torch.tensor(X, dtype=torch.float64)
+    g = lambda z: torch.nn.functional.gelu(z, approximate="tanh")  # noqa: E731
+    h = g(x @ t["W1"].T + t["b1"])
+    h = g(h @ t["W2"].T + t["b2"])
+    h = g(h @ t["W3"].T + t["b3"])
+    y = torch.sigmoid(h @ t["W4"].T + t["b4"])
+    loss = ((y - torch.tensor(T)) ** 2).sum() / X.shape[0]
+    loss.backward()
+    return loss.item(), M.pack({k: t[k].grad.numpy() for k in t})
+
+
+def test_gelu_matches_torch_tanh_gelu():
+    x = np.linspace(-8, 8, 2001)
+    xt = torch.tensor(x, dtype=torch.float64, requires_grad=True)
+    y = torch.nn.functional.gelu(xt, approximate="tanh")
+    y.sum().backward()
+    np.testing.assert_allclose(M.gelu(x), y.detach().numpy(), rtol=1e-14, atol=1e-15)
+    np.testing.assert_allclose(M.gelu_grad(x), xt.grad.numpy(), rtol=1e-12, atol=1e-14)
+
+
+def test_gradient_matches_autograd():
+    rng = np.random.default_rng(3)
+    X = rng.normal(0, 1, (37, M.NIN))
+    T = rng.uniform(0, 1, (37, 2))
+    w = _params(5)
+    l0, g0 = M.loss_and_grad(w, X, T)
+    l1, g1 = _torch_loss(w, X, T)
+    assert abs(l0 - l1) < 1e-14
+    np.testing.assert_allclose(g0, g1, rtol=1e-10, atol=1e-15)
+
+
+def test_gradient_matches_finite_differences():
+    rng = np.random.default_rng(4)
+    X = rng.normal(0, 1, (11, M.NIN))
+    T = rng.uniform(0, 1, (11, 2))
+    w = _params(6, scale=2.0)
+    _, g = M.loss_and_grad(w, X, T)
+    # a few entries of every tensor
+    idx = []
+    for a, b in ((M.OFF_W1, M.OFF_B1), (M.OFF_B1, M.OFF_W2), (M.OFF_W2, M.OFF_B2), (M.OFF_B2, M.OFF_W3),
+                 (M.OFF_W3, M.OFF_B3), (M.OFF_B3, M.OFF_W4), (M.OFF_W4, M.OFF_B4), (M.OFF_B4, M.NPARAM)):
+        idx += list(rng.choice(np.arange(a, b), size=min(6, b - a), replace=False))
+    h = 1e-6
+    for i in idx:
+        wp, wm = w.copy(), w.copy()
+        wp[i] += h
+        wm[i] -= h
+        fd = (M.loss_and_grad(wp, X, T)[0] - M.loss_and_grad(wm, X, T)[0]) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-7 * max(1.0, abs(g[i])) + 1e-9, (i, fd, g[i])
+
+
+def test_quantized_gradient_close_to_fp64():
+    """BF16 rounding at the matrix-core inputs perturbs the gradient by a few
+    parts in a thousand, not more."""
+    rng = np.random.default_rng(8)
+    X = rng.normal(0, 1, (200, M.NIN))
+    T = rng.uniform(0, 1, (200, 2))
+    w = _params(9)
+    _, g = M.loss_and_grad(w, X, T)
+    _, gq = M.loss_and_grad(w, X, T, quantize=True)
+    assert np.linalg.norm(gq - g) / np.linalg.norm(g) < 3e-2
+
+
+def test_bf16_rounding_matches_torch():
+    rng = np.random.default_rng(1)
+    x = np.concatenate([rng.normal(0, 10, 5000), [1.0, 1 + 2 ** -8, 1 + 3 * 2 ** -8, -1 - 2 ** -8, 0.0, 65504.0]])
+    ref = torch.tensor(x, dtype=torch.float32).to(torch.bfloat16).to(torch.float64).numpy()
+    np.testing.assert_array_equal(M.bf16(x), ref)
+    assert M.bf16(1 + 2 ** -8) == 1.0 and M.bf16(1 + 3 * 2 ** -8) == 1 + 2 ** -6
+
+
+def test_features_closed_forms():
+    # unit box centred at (1,2,3), half extent 0.5; a ray through the centre
+    bounds = np.array([[0.5, 1.5, 2.5, 1.5, 2.5, 3.5]])
+    corners = np.array([[[0.5, 1.5, 3.0], [1.5, 1.5, 3.0], [1.5, 2.5, 3.0], [0.5, 2.5, 3.0]]])
+    o = np.array([[1.0, 2.0, -7.0]])
+    d = np.array([[0.0, 0.0, 2.0]])
+    f = M.features(o, d, bounds, corners, np.array([3]), np.array([2]))[0]
+    np.testing.assert_allclose(f[:3], 0.0, atol=1e-15)           # closest point = centre
+    np.testing.assert_allclose(f[3:6], [0, 0, 1])                # unit direction
+    np.testing.assert_allclose(f[6:8], [-1 / 3, -1.0])           # degrees (p - 3.5) / 1.5
+    np.testing.assert_allclose(f[8:].reshape(4, 3), [[-1, -1, 0], [1, -1, 0], [1, 1, 0], [-1, 1, 0]])
+    # ray pointing away from the box: the closest point is the origin itself
+    f2 = M.features(np.array([[1.0, 2.0, 5.0]]), np.array([[0.0, 0.0, 1.0]]), bounds, corners, [3], [3])[0]
+    np.testing.assert_allclose(f2[:3], [0, 0, 4.0])
+    # similarity invariance: translate and scale the whole configuration
+    rng = np.random.default_rng(2)
+    o, d = rng.normal(size=(5, 3)), rng.normal(size=(5, 3))
+    lo = rng.normal(size=(5, 3))
+    b = np.concatenate([lo, lo + rng.uniform(0.1, 2, (5, 3))], axis=1)
+    cr = rng.normal(size=(5, 4, 3))
+    p = np.array([2, 3, 4, 5, 3])
+    a = M.features(o, d, b, cr, p, p)
+    s, t = 3.7, np.array([1.0, -2.0, 0.5])
+    a2 = M.features(o * s + t, d, b * s + np.tile(t, 2), cr * s + t, p, p)
+    np.testing.assert_allclose(a, a2, rtol=1e-12, atol=1e-12)
+
+
+def test_forward_known_answers():
+    w = np.zeros(M.NPARAM)
+    w[M.OFF_B4:] = [0.0, np.log(3.0)]   # sigmoid(0) = 0.5, sigmoid(log 3) = 0.75
+    y, _ = M.forward(w, np.ones((3, M.NIN)))
+    np.testing.assert_allclose(y, [[0.5, 0.75]] * 3)
+    # one path: x0 -> unit 0 of each layer -> u logit, weight 1 everywhere
+    w = np.zeros(M.NPARAM)
+    w[M.OFF_W1 + 0] = 1.0
+    w[M.OFF_W2 + 0] = 1.0
+    w[M.OFF_W3 + 0] = 1.0
+    w[M.OFF_W4 + 0] = 1.0
+    x = np.zeros((1, M.NIN))
+    x[0, 0] = 1.5
+    y, _ = M.forward(w, x)
+    assert y[0, 0] == pytest.approx(M.sigmoid(M.gelu(M.gelu(M.gelu(1.5)))))
+    assert y[0, 1] == 0.5
