@@ -139,6 +139,16 @@ int rtn_scene_stats_get(const rtn_scene* scene, rtn_scene_stats* out);
 int rtn_scene_download(const rtn_scene* scene, float* patch_cp, int* patch_off, int* patch_pq, float* patch_uv,
                        int* patch_id, int* knot_idx, rtn_bvh_node* nodes, float* delta);
 
+/* Silhouette rays (no reference counterpart; the mixed-precision workload).
+ * Ray i aims at a random point of a random patch of the built scene: patch k
+ * and (u,v) uniform, S its point, N = unit(S_u x S_v), T a random unit
+ * tangent, theta ~ U(0, max_deg) degrees; d = unit(T cos theta - N sin theta),
+ * origin S - t0 d with t0 ~ U(t0_lo, t0_hi).  Deterministic in (i, seed).
+ * aim_patch[n] (BVH position) and aim[4n] = (u, v, theta radians, t0) are
+ * optional device outputs. */
+int rtn_rays_silhouette(const rtn_scene* scene, int64_t n, uint32_t seed, float max_deg, float t0_lo, float t0_hi,
+                        const rtn_ray_queue* rays, int* aim_patch, float* aim, void* stream);
+
 /* --------------------------------------------------------------- tracing
  * Plain method (intersect.cu:33-193): per-ray ordered closest-hit traversal,
  * bilinear seed (bezier_patch.h:273-327), FP32 Newton (bezier_patch.h:198-266).
@@ -175,7 +185,8 @@ typedef struct {
   int count;              /* 1: fill the per-ray counters                        */
   int learned_iters;      /* learned seed: Newton iterations from the network's seed before the
                              plain-seed fallback (0 = max_iter)                  */
-  int reserved[3];
+  int any_hit;            /* fast forms: a ray ends at its first converged hit (shadow rays) */
+  int reserved[2];
   const struct rtn_mlp* seed_net; /* RTN_SEED_LEARNED: the seed network          */
 } rtn_trace_params;
 
@@ -254,6 +265,23 @@ void rtn_mlp_train_params_default(rtn_mlp_train_params* tp);
 /* Adam training on device pairs; epoch_loss_host[epochs] = mean loss per epoch (may be NULL). */
 int rtn_mlp_train(rtn_mlp* net, const float* feats, const float* targets, int64_t n,
                   const rtn_mlp_train_params* tp, float* epoch_loss_host, void* stream);
+
+/* Secondary rays of primary hits (no reference counterpart): per hit, in
+ * this order, one shadow ray towards the directional light (direction
+ * `light`, normalised) and one mirror ray, both from the hit point offset by
+ * eps * (1 + max|S|) along the normal facing the incoming ray.  Output rays
+ * are compacted; parent[i] = primary ray index, kind[i] = 0 shadow, 1 mirror.
+ * cap >= (want_shadow + want_mirror) * n.  *count_host = rays written. */
+int rtn_rays_secondary(const float* const rays_soa[6], const rtn_hits* hits, int64_t n, const float light[3],
+                       float eps, int want_shadow, int want_mirror, const rtn_ray_queue* out, int* parent, int* kind,
+                       int64_t cap, int64_t* count_host, void* stream);
+
+/* Lambert shading with one directional light (replaces the reference's debug
+ * shaders, testShading.cu:9-55): pixel = mean over its spp consecutive rays of
+ * albedo * max(0, N.L), N = unit(dpdu x dpdv) facing the ray (benchmark.cpp:91-92),
+ * 0 on a miss or where shadow_hit[r] != 0 (optional).  image: device float[W*H]. */
+int rtn_shade_lambert(const float* const rays_soa[6], const rtn_hits* hits, int W, int H, int spp,
+                      const float light[3], float albedo, const int* shadow_hit, float* image, void* stream);
 
 /* FP64 reference trace (no reference counterpart: the reference is FP32
  * only).  The plain method's traversal, seed and Newton iteration evaluated

@@ -49,6 +49,7 @@ struct WArgs {
   int hull_cull;
   float cull_margin;
   int mixed;
+  int any_hit;
   float cond_thr;
   // learned form: BF16 MFMA tiles and biases of the seed network, patch boxes
   const mlp::bf16x8* mlp_tiles;
@@ -424,7 +425,14 @@ k_trace_wide(const WArgs* __restrict__ args) {
           A.pair_uv[k] = make_float2(res.u, res.v);
         }
       }
-      if ((res.flags & 1) && 0.0f < res.t && res.t < L.t_max) record(L, A, pi, res.t, res.u, res.v, res.Su, res.Sv);
+      if ((res.flags & 1) && 0.0f < res.t && res.t < L.t_max) {
+        record(L, A, pi, res.t, res.u, res.v, res.Su, res.Sv);
+        if (A.any_hit) {  // occlusion query: the first hit ends the ray
+          off = 0;
+          cand_left = 0;
+          cur = -1;
+        }
+      }
     }
   }
   if (A.totals) {  // every lane of the wave arrives here together
@@ -477,6 +485,7 @@ int trace_wide_scene(const rtn_scene* s, const float* const rays[6], int64_t n, 
   A.cull_margin = 2.0f * prm->thr_res + 1e-5f;
   A.mixed = prm->precision == RTN_PREC_MIXED;
   A.cond_thr = prm->cond_thr;
+  A.any_hit = prm->any_hit;
   if (learned) {
     A.mlp_tiles = reinterpret_cast<const mlp::bf16x8*>(mlp_tiles(prm->seed_net));
     A.mlp_bias = mlp_bias(prm->seed_net);

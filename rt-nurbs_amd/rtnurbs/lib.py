@@ -35,7 +35,8 @@ EXPORTS = [
     "rtn_build_bvh", "rtn_patch_bounds", "rtn_refine", "rtn_trace_f64", "rtn_event_create", "rtn_event_destroy",
     "rtn_event_record", "rtn_event_elapsed_ms", "rtn_trace_pairs", "rtn_mlp_create", "rtn_mlp_destroy",
     "rtn_mlp_set_params", "rtn_mlp_get_params", "rtn_mlp_pair_features", "rtn_mlp_forward", "rtn_mlp_gradient",
-    "rtn_mlp_train_params_default", "rtn_mlp_train",
+    "rtn_mlp_train_params_default", "rtn_mlp_train", "rtn_rays_silhouette", "rtn_rays_secondary",
+    "rtn_shade_lambert",
 ]
 MLP_NIN, MLP_HIDDEN, MLP_NPARAM = 20, 64, 9794
 
@@ -57,7 +58,7 @@ class TraceParams(C.Structure):
                 ("flat_threshold", C.c_float), ("cam_k", C.c_float), ("seed_base", C.c_uint32),
                 ("seed_mode", C.c_int), ("precision", C.c_int), ("cond_thr", C.c_float),
                 ("hull_cull", C.c_int), ("count", C.c_int), ("learned_iters", C.c_int),
-                ("reserved", C.c_int * 3), ("seed_net", C.c_void_p)]
+                ("any_hit", C.c_int), ("reserved", C.c_int * 2), ("seed_net", C.c_void_p)]
 
 
 class MlpTrainParams(C.Structure):
@@ -143,6 +144,10 @@ def load():
     L.rtn_mlp_train_params_default.argtypes = [C.POINTER(MlpTrainParams)]
     L.rtn_mlp_train_params_default.restype = None
     L.rtn_mlp_train.argtypes = [vp, vp, vp, i64, C.POINTER(MlpTrainParams), vp, vp]
+    L.rtn_rays_silhouette.argtypes = [vp, i64, C.c_uint32, f, f, f, C.POINTER(RayQueue), vp, vp, vp]
+    L.rtn_rays_secondary.argtypes = [C.POINTER(vp * 6), C.POINTER(Hits), i64, vp, f, i, i, C.POINTER(RayQueue), vp,
+                                     vp, i64, C.POINTER(i64), vp]
+    L.rtn_shade_lambert.argtypes = [C.POINTER(vp * 6), C.POINTER(Hits), i, i, i, vp, f, vp, vp, vp]
     _lib = L
     return L
 
@@ -557,3 +562,54 @@ def pair_features(scene, rays, pair_ray, pair_patch, n, out=None, out_offset=0):
     check(load().rtn_mlp_pair_features(scene.h, C.byref(arr), rays.n, pair_ray.ptr, pair_patch.ptr, n,
                                        out.ptr + out_offset * MLP_NIN * 4, None))
     return out
+
+
+# ---------------------------------------------------------- ray producers
+def silhouette_rays(scene, n, seed=0, max_deg=5.0, t0=(1.0, 3.0), rays=None, offset=0, aim=False):
+    """n grazing rays aimed at random surface points (written at rays[offset:]
+    when rays is given).  With aim=True also returns (patch, [u, v, theta, t0])."""
+    if rays is None:
+        rays = Rays(n)
+        offset = 0
+    assert offset + n <= rays.n
+    q = RayQueue(*[c.ptr + 4 * offset for c in rays.comp], None)
+    ap = DeviceArray((n,), np.int32) if aim else None
+    au = DeviceArray((n, 4), np.float32) if aim else None
+    check(load().rtn_rays_silhouette(scene.h, n, seed, max_deg, t0[0], t0[1], C.byref(q),
+                                     ap.ptr if aim else None, au.ptr if aim else None, None))
+    return (rays, ap, au) if aim else rays
+
+
+def secondary_rays(rays, hits, light, eps=1e-4, shadow=True, mirror=True):
+    """Shadow and mirror rays of the primary hits: (Rays, parent, kind, count)."""
+    per = int(shadow) + int(mirror)
+    cap = max(1, per * rays.n)
+    out = Rays(cap, with_idx=False)
+    parent = DeviceArray((cap,), np.int32)
+    kind = DeviceArray((cap,), np.int32)
+    cnt = C.c_int64()
+    lt = np.asarray(light, np.float32)
+    hs = hits.struct()
+    arr = rays.ptr_array()
+    q = out.queue()
+    check(load().rtn_rays_secondary(C.byref(arr), C.byref(hs), rays.n, lt.ctypes.data, eps, int(shadow), int(mirror),
+                                    C.byref(q), parent.ptr, kind.ptr, cap, C.byref(cnt), None))
+    return out, parent, kind, cnt.value
+
+
+def shade_lambert(rays, hits, W, H, spp, light, albedo=0.8, shadow_hit=None):
+    img = DeviceArray((H, W), np.float32)
+    lt = np.asarray(light, np.float32)
+    hs = hits.struct()
+    arr = rays.ptr_array()
+    check(load().rtn_shade_lambert(C.byref(arr), C.byref(hs), W, H, spp, lt.ctypes.data, albedo,
+                                   shadow_hit.ptr if shadow_hit is not None else None, img.ptr, None))
+    return img
+
+
+def write_pgm(path, img):
+    """8-bit PGM of a [0,1] float image (row 0 at the top)."""
+    a = np.clip(np.asarray(img) * 255.0 + 0.5, 0, 255).astype(np.uint8)
+    with open(path, "wb") as f:
+        f.write(b"P5\n%d %d\n255\n" % (a.shape[1], a.shape[0]))
+        f.write(a.tobytes())

@@ -211,3 +211,26 @@ def config2(seed: int = 0, W=1920, H=1080, n_surf: int = 64) -> Scene:
     base = config2_base(seed, n_surf)
     surfs = [lib.refine(s, C2_REFINE, C2_REFINE) for s in base]
     return Scene(surfs, c2_camera(W, H), f"c2-seed{seed}")
+
+
+# ----------------------------------------------------------------- config 3
+C3_SILHOUETTE_MOD = 5   # pixels p with p % 5 == 4 (13,107 of 65,536) become silhouette rays
+
+
+def config3_rays(scene, seed=0, W=256, H=256, max_deg=5.0):
+    """Config 1's 256x256 primary rays with 20% of them (pixels p = 4 mod 5,
+    13,107 rays) replaced by silhouette rays: grazing angle U(0, 5) degrees
+    at random surface points, origin 1..3 units back along the ray.
+    Returns (Rays, boolean mask of the silhouette rays)."""
+    from . import lib
+    cam = c1_camera(W, H)
+    prim = lib.generate_rays(cam)
+    n = prim.n
+    mask = (np.arange(n) % C3_SILHOUETTE_MOD) == C3_SILHOUETTE_MOD - 1
+    k = int(mask.sum())
+    sil = lib.silhouette_rays(scene, k, seed=seed, max_deg=max_deg)
+    host = [a.copy() for a in prim.get()]
+    sh = sil.get()
+    for a, b in zip(host, sh):
+        a[mask] = b
+    return lib.Rays.from_host(host), mask

@@ -1,0 +1,127 @@
+"""Ray producers and shading on the GPU: silhouette rays (geometry against an
+FP64 Bernstein evaluation), secondary rays (shadow towards the light, mirror
+by the reflection law), any-hit occlusion against closest-hit, Lambert
+shading against numpy, and the mixed-precision form on config 3 (20%
+silhouette rays) against the FP64 reference trace."""
+import math
+
+import numpy as np
+import pytest
+
+from rtnurbs import scenes
+
+pytestmark = pytest.mark.gpu
+
+CFG = dict(max_iter=8, thr_res=1e-5, thr_det=1e-5)
+
+
+def _bern(n, t):
+    return np.array([math.comb(n, i) * t ** i * (1 - t) ** (n - i) for i in range(n + 1)])
+
+
+def _dbern(n, t):
+    if n == 0:
+        return np.zeros(1)
+    b = _bern(n - 1, t)
+    return n * (np.concatenate([[0.0], b]) - np.concatenate([b, [0.0]]))
+
+
+def _eval64(net, p, q, u, v):
+    """FP64 point and partials of a rational Bezier patch (closed-form Bernstein)."""
+    P = net.astype(np.float64).reshape(p + 1, q + 1, 4)
+    bu, bv, dbu, dbv = _bern(p, u), _bern(q, v), _dbern(p, u), _dbern(q, v)
+    pt = np.einsum("i,j,ijk->k", bu, bv, P)
+    pu = np.einsum("i,j,ijk->k", dbu, bv, P)
+    pv = np.einsum("i,j,ijk->k", bu, dbv, P)
+    S = pt[:3] / pt[3]
+    return S, (pu[:3] - S * pu[3]) / pt[3], (pv[:3] - S * pv[3]) / pt[3]
+
+
+def test_silhouette_rays_geometry(rtl):
+    sc = scenes.config1(0)
+    scene = rtl.Scene(sc.surfaces).build()
+    print("silhouette rays n=2000", flush=True)
+    rays, ap, au = rtl.silhouette_rays(scene, 2000, seed=3, aim=True)
+    o = np.stack(rays.get()[:3], 1).astype(np.float64)
+    d = np.stack(rays.get()[3:], 1).astype(np.float64)
+    ap, au = ap.get(), au.get()
+    dl = scene.download()
+    assert np.all(np.abs(np.linalg.norm(d, axis=1) - 1) < 1e-6)
+    assert np.all((au[:, 2] >= 0) & (au[:, 2] <= np.radians(5.0) + 1e-6))
+    assert np.all((au[:, 3] >= 1.0) & (au[:, 3] <= 3.0))
+    for i in range(0, 2000, 7):
+        k = ap[i]
+        p, q = dl["pq"][k]
+        net = dl["cp"][dl["off"][k]:dl["off"][k] + (p + 1) * (q + 1)]
+        S, Su, Sv = _eval64(net, p, q, au[i, 0], au[i, 1])
+        N = np.cross(Su, Sv)
+        N /= np.linalg.norm(N)
+        assert np.linalg.norm(o[i] + au[i, 3] * d[i] - S) < 2e-5 * (1 + np.abs(S).max())
+        assert abs(math.asin(min(1.0, abs(d[i] @ N))) - au[i, 2]) < 2e-3
+
+
+def test_secondaries_any_hit_and_shading(rtl):
+    sc = scenes.config1(0, 128, 128)
+    scene = rtl.Scene(sc.surfaces).build()
+    rays = rtl.generate_rays(sc.camera)
+    h, _ = rtl.trace(scene, rays, rtl.params(mode=rtl.MODE_WIDE, hull_cull=2, **CFG))
+    g = h.get()
+    light = np.array([0.3, -0.4, 1.0])
+    L = light / np.linalg.norm(light)
+    print("secondaries", flush=True)
+    sec, parent, kind, cnt = rtl.secondary_rays(rays, h, light, eps=1e-4)
+    nh = int((g["hit"] >= 0).sum())
+    assert cnt == 2 * nh
+    par, knd = parent.get()[:cnt], kind.get()[:cnt]
+    assert np.array_equal(np.sort(par[knd == 0]), np.flatnonzero(g["hit"] >= 0))
+    so = np.stack(sec.get()[:3], 1)[:cnt].astype(np.float64)
+    sd = np.stack(sec.get()[3:], 1)[:cnt].astype(np.float64)
+    ro = np.stack(rays.get()[:3], 1).astype(np.float64)
+    rd = np.stack(rays.get()[3:], 1).astype(np.float64)
+    S = ro[par] + g["t"][par, None] * rd[par]
+    N = np.cross(g["dpdu"][par], g["dpdv"][par]).astype(np.float64)
+    N /= np.linalg.norm(N, axis=1, keepdims=True)
+    N *= -np.sign((N * rd[par]).sum(1))[:, None]
+    assert np.abs(so - S).max() < 1e-3
+    assert np.all(((so - S) * N).sum(1) > 0)  # offset to the incoming side
+    np.testing.assert_allclose(sd[knd == 0], np.tile(L, ((knd == 0).sum(), 1)), atol=1e-6)
+    m = knd == 1
+    refl = rd[par[m]] - 2 * (rd[par[m]] * N[m]).sum(1)[:, None] * N[m]
+    np.testing.assert_allclose(sd[m], refl, atol=1e-5)
+    # any-hit occlusion equals closest-hit hit/miss on the same rays
+    shadow = rtl.Rays.from_host([a[:cnt][knd == 0] for a in sec.get()])
+    a, ta = rtl.trace(scene, shadow, rtl.params(mode=rtl.MODE_WIDE, hull_cull=2, any_hit=1, count=1, **CFG))
+    c, tc = rtl.trace(scene, shadow, rtl.params(mode=rtl.MODE_WIDE, hull_cull=2, count=1, **CFG))
+    assert np.array_equal(a.get()["hit"] >= 0, c.get()["hit"] >= 0)
+    assert ta["newton_calls"] <= tc["newton_calls"]
+    # Lambert shading (no shadows) against numpy
+    img = rtl.shade_lambert(rays, h, 128, 128, 1, light, albedo=0.8).get().ravel()
+    hit = g["hit"] >= 0
+    Np = np.cross(np.where(hit[:, None], g["dpdu"], 1.0), np.where(hit[:, None], g["dpdv"], 0.5)).astype(np.float64)
+    Np /= np.linalg.norm(Np, axis=1, keepdims=True) + 1e-300
+    Np *= np.where((Np * rd).sum(1) > 0, -1.0, 1.0)[:, None]
+    ref = np.where(hit, 0.8 * np.maximum(0, Np @ L), 0.0)
+    np.testing.assert_allclose(img, ref, atol=1e-5)
+
+
+def test_mixed_precision_on_silhouettes(rtl):
+    """Config 3: the FP64 correction step (cond(J) above the threshold) lowers
+    the hit-t error of the corrected rays against the FP64 reference trace."""
+    sc = scenes.config1(0)
+    scene = rtl.Scene(sc.surfaces).build()
+    rays, mask = scenes.config3_rays(scene, seed=0)
+    assert rays.n == 65536 and mask.sum() == 13107
+    print("config 3 mixed precision", flush=True)
+    ref = rtl.trace_f64(scene, rays).get()
+    base = dict(mode=rtl.MODE_WIDE, hull_cull=2, **CFG)
+    a, _ = rtl.trace(scene, rays, rtl.params(**base))
+    thr = 50.0  # enough corrected rays on 65k rays for a stable mean
+    b, tb = rtl.trace(scene, rays, rtl.params(precision=rtl.PREC_MIXED, cond_thr=thr, count=1, **base))
+    a, b = a.get(), b.get()
+    assert tb["fp64_steps"] > 0
+    same = (a["hit"] >= 0) & (a["hit"] == ref["hit"]) & (b["hit"] == ref["hit"])
+    changed = same & (a["t"] != b["t"])
+    assert changed.sum() > 0
+    ea = np.abs(a["t"][changed] - ref["t"][changed]) / ref["t"][changed]
+    eb = np.abs(b["t"][changed] - ref["t"][changed]) / ref["t"][changed]
+    assert eb.mean() < ea.mean() and np.median(eb) <= np.median(ea)
