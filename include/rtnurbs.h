@@ -120,6 +120,21 @@ int rtn_scene_add_patches(rtn_scene* scene, int np, int p, int q, const float* c
 /* Runs decomposition, bounds and the BVH build on the device. */
 int rtn_scene_build(rtn_scene* scene);
 
+/* Build options.  RTN_BVH_EXACT_SAH is buildBVH's exhaustive SAH (bvh.h:85-143,
+ * reference numbering; quadratic in the primitives of a node); RTN_BVH_BINNED_SAH
+ * is a binned-SAH build for large scenes (no reference counterpart): `bins`
+ * planes per axis, leaves of up to max_leaf primitives where splitting does not
+ * pay.  Both feed the same node layout and the same trace forms. */
+enum { RTN_BVH_EXACT_SAH = 0, RTN_BVH_BINNED_SAH = 1 };
+typedef struct {
+  int bvh;                /* RTN_BVH_*                                          */
+  int bins;               /* binned builder: planes per axis (default 32)       */
+  int max_leaf;           /* binned builder: largest leaf (default 4, <= 128)   */
+  int reserved[5];
+} rtn_build_params;
+void rtn_build_params_default(rtn_build_params* bp);
+int rtn_scene_build_ex(rtn_scene* scene, const rtn_build_params* bp);
+
 typedef struct {
   int64_t n_surfaces, n_patches, n_cp, n_nodes, n_wide_nodes, max_depth, n_leaves;
   int64_t deg_hist[RTN_MAX_DEGREE + 1][RTN_MAX_DEGREE + 1]; /* patches per (p,q) */
@@ -269,9 +284,12 @@ int rtn_mlp_train(rtn_mlp* net, const float* feats, const float* targets, int64_
 /* Secondary rays of primary hits (no reference counterpart): per hit, in
  * this order, one shadow ray towards the directional light (direction
  * `light`, normalised) and one mirror ray, both from the hit point offset by
- * eps * (1 + max|S|) along the normal facing the incoming ray.  Output rays
- * are compacted; parent[i] = primary ray index, kind[i] = 0 shadow, 1 mirror.
- * cap >= (want_shadow + want_mirror) * n.  *count_host = rays written. */
+ * eps * (1 + max|S|) along the normal facing the incoming ray.  Primary ray r
+ * owns slots per*r .. per*r+per-1 (per = want_shadow + want_mirror), so the
+ * layout is deterministic; parent[i] = r, kind[i] = 0 shadow, 1 mirror, -1
+ * empty (a miss).  The trace entry points skip rays whose kind is negative
+ * when given the kind array (rtn_frame does).  cap >= per * n.
+ * *count_host = valid (non-empty) rays. */
 int rtn_rays_secondary(const float* const rays_soa[6], const rtn_hits* hits, int64_t n, const float light[3],
                        float eps, int want_shadow, int want_mirror, const rtn_ray_queue* out, int* parent, int* kind,
                        int64_t cap, int64_t* count_host, void* stream);
@@ -282,6 +300,64 @@ int rtn_rays_secondary(const float* const rays_soa[6], const rtn_hits* hits, int
  * 0 on a miss or where shadow_hit[r] != 0 (optional).  image: device float[W*H]. */
 int rtn_shade_lambert(const float* const rays_soa[6], const rtn_hits* hits, int W, int H, int spp,
                       const float light[3], float albedo, const int* shadow_hit, float* image, void* stream);
+
+/* ---------------------------------------------- multi-scene batches, frames
+ * No reference counterpart (the reference traces one scene).  A batch merges
+ * built scenes into one set of device arrays with one BVH root per scene;
+ * rays carry their scene index (ray_scene[i] in 0..k-1), so rays of all
+ * scenes share one launch.  Hit prim values are batch-wide patch positions:
+ * scene j owns [base[j], base[j+1]) of rtn_batch_patch_base. */
+typedef struct rtn_batch rtn_batch;
+int rtn_batch_create(rtn_batch** batch, rtn_scene* const* scenes, int k);
+void rtn_batch_destroy(rtn_batch* batch);
+int rtn_batch_patch_base(const rtn_batch* batch, int64_t* base /* k+1 */);
+/* rtn_trace over a batch (wide or persistent form). */
+int rtn_batch_trace(const rtn_batch* batch, const float* const rays_soa[6], const int* ray_scene, int64_t n_rays,
+                    const rtn_trace_params* prm, const rtn_hits* hits, rtn_trace_totals* totals_host, void* stream);
+
+/* A frame: primary rays of a batch traced by the persistent kernel, then one
+ * shadow ray (any-hit, towards the light) and one mirror ray per hit, in the
+ * fixed slots of rtn_rays_secondary, traced by a second persistent launch
+ * (empty slots are skipped).  All buffers are allocated by rtn_frame_create;
+ * rtn_frame_run enqueues the whole sequence without allocating or copying
+ * from the host, so rtn_frame_capture can record it into a hipGraph that
+ * later rtn_frame_run calls replay.  Calls on one frame must not overlap. */
+typedef struct rtn_frame rtn_frame;
+typedef struct {
+  rtn_trace_params trace; /* mode RTN_MODE_PERSISTENT; count 0; any_hit 0      */
+  float light[3];         /* direction towards the light                        */
+  float eps;              /* secondary origin offset, eps * (1 + max|S|)        */
+  int shadow, mirror;     /* 0/1: generate shadow / mirror rays                 */
+  int reserved[4];
+} rtn_frame_params;
+typedef struct {
+  int64_t n_primary, n_secondary;  /* rays; n_secondary counts non-empty slots   */
+  int64_t n_secondary_slots;       /* (shadow + mirror) x n_primary              */
+  unsigned long long overflow;  /* rays whose traversal stack overflowed        */
+  int shape_lanes, shape_stack, shape_blocks_per_cu, captured;
+} rtn_frame_status_t;
+typedef struct { int lanes, stack, blocks_per_cu; float ms; } rtn_tune_entry;
+
+void rtn_frame_params_default(rtn_frame_params* fp);
+int rtn_frame_create(rtn_frame** frame, const rtn_batch* batch, const float* const rays_soa[6], const int* ray_scene,
+                     int64_t n_rays, const rtn_frame_params* fp);
+void rtn_frame_destroy(rtn_frame* frame);
+int rtn_frame_run(rtn_frame* frame, void* stream);
+int rtn_frame_capture(rtn_frame* frame, void* stream /* non-default */);
+int rtn_frame_status(const rtn_frame* frame, rtn_frame_status_t* out); /* synchronises the device */
+/* Device pointers of the frame's outputs (any argument may be NULL). */
+int rtn_frame_results(const rtn_frame* frame, rtn_hits* primary, rtn_hits* secondary, const int** parent,
+                      const int** kind, const float** secondary_rays_soa6);
+/* Launch shape: lanes per block (64/128/256 = 1/2/4 waves), traversal stack
+ * entries per lane (16/20/24/32; LDS = 8 B x entries x lanes), resident blocks
+ * per compute unit.  Drops a captured graph. */
+int rtn_frame_set_shape(rtn_frame* frame, int lanes, int stack, int blocks_per_cu);
+/* Autotuner: times the frame over every built launch shape at 2, 4 and 8
+ * blocks per CU (reps timed runs each, stack overflow rejects a shape) and
+ * keeps the fastest.  table (optional) receives each candidate, ms < 0 for
+ * rejected ones.  Drops a captured graph. */
+int rtn_frame_autotune(rtn_frame* frame, int reps, rtn_tune_entry* table, int table_cap, int* n_tried,
+                       void* stream);
 
 /* FP64 reference trace (no reference counterpart: the reference is FP32
  * only).  The plain method's traversal, seed and Newton iteration evaluated

@@ -2,8 +2,9 @@
 //   silhouette rays  grazing rays aimed at random surface points (mixed-
 //                    precision workload; no reference counterpart)
 //   secondaries      one shadow ray (towards a directional light) and one
-//                    mirror ray per primary hit, compacted (no reference
-//                    counterpart; the reference has primary rays only)
+//                    mirror ray per primary hit, in fixed slots per primary
+//                    ray (no reference counterpart; the reference has
+//                    primary rays only)
 //   Lambert shading  one directional light, spp averaged, replacing the
 //                    reference's debug shaders (testShading.cu:9-55)
 #include <cmath>
@@ -78,27 +79,35 @@ __device__ __forceinline__ ex::v3<float> facing_normal(const float* dpdu, const 
   return N;
 }
 
-// Secondary rays of primary hit r: shadow ray towards the light and mirror
-// ray, origins offset along the facing normal by eps * (1 + |S|).  Slots are
-// taken with one atomic per wave.
+// Secondary rays of primary ray r in slots per*r .. per*r+per-1 (shadow
+// first): shadow ray towards the light and mirror ray, origins offset along
+// the facing normal by eps * (1 + max|S|).  A miss leaves its slots empty
+// (kind -1), so the layout, and the Newton seeds that follow the ray index,
+// are the same on every run.  *count += valid rays (one atomic per wave).
 __global__ void k_secondaries(const float* ox, const float* oy, const float* oz, const float* dx, const float* dy,
                               const float* dz, const int* prim, const float* t, const float* dpdu,
                               const float* dpdv, int64_t n, float lx, float ly, float lz, float eps, int want_shadow,
                               int want_mirror, float* sox, float* soy, float* soz, float* sdx, float* sdy, float* sdz,
-                              int* parent, int* kind, unsigned long long* count) {
+                              int* parent, int* kind, unsigned long long* count, const int* ray_scene,
+                              int* out_scene) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool hit = r < n && prim[r] >= 0;
   const int per = want_shadow + want_mirror;
   const unsigned long long bal = __ballot(hit);
-  if (!bal) return;
   const int lane = threadIdx.x & 63;
-  const int leader = __ffsll((long long)bal) - 1;
-  unsigned long long base = 0;
-  if (lane == leader) base = atomicAdd(count, (unsigned long long)(__popcll(bal) * per));
-  base = __shfl(base, leader);
-  if (!hit) return;
-  const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
-  unsigned long long slot = base + (unsigned long long)(__popcll(bal & below) * per);
+  if (bal && lane == __ffsll((long long)bal) - 1) atomicAdd(count, (unsigned long long)(__popcll(bal) * per));
+  if (r >= n) return;
+  int64_t slot = (int64_t)per * r;
+  if (!hit) {
+    for (int k = 0; k < per; k++, slot++) {
+      sox[slot] = soy[slot] = soz[slot] = 0.0f;
+      sdx[slot] = sdy[slot] = sdz[slot] = 0.0f;
+      parent[slot] = (int)r;
+      kind[slot] = -1;
+      if (out_scene) out_scene[slot] = ray_scene[r];
+    }
+    return;
+  }
   const ex::v3<float> d{dx[r], dy[r], dz[r]};
   const float th = t[r];
   const ex::v3<float> S{ox[r] + th * d.x, oy[r] + th * d.y, oz[r] + th * d.z};
@@ -110,6 +119,7 @@ __global__ void k_secondaries(const float* ox, const float* oy, const float* oz,
     sdx[slot] = lx; sdy[slot] = ly; sdz[slot] = lz;
     parent[slot] = (int)r;
     kind[slot] = 0;
+    if (out_scene) out_scene[slot] = ray_scene[r];
     slot++;
   }
   if (want_mirror) {
@@ -119,6 +129,7 @@ __global__ void k_secondaries(const float* ox, const float* oy, const float* oz,
     sdx[slot] = m.x; sdy[slot] = m.y; sdz[slot] = m.z;
     parent[slot] = (int)r;
     kind[slot] = 1;
+    if (out_scene) out_scene[slot] = ray_scene[r];
   }
 }
 
@@ -141,6 +152,20 @@ __global__ void k_shade_lambert(const float* dx, const float* dy, const float* d
 }
 
 }  // namespace
+
+namespace rtn {
+// Secondary generation for the frame pipeline: counts into *count (device),
+// propagates the scene of the parent ray when ray_scene is given.
+void launch_secondaries(const float* const rays[6], const rtn_hits* hits, int64_t n, const float ldir[3], float eps,
+                        int want_shadow, int want_mirror, float* const out[6], int* parent, int* kind,
+                        unsigned long long* count, const int* ray_scene, int* out_scene, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_secondaries, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, rays[0], rays[1], rays[2],
+                     rays[3], rays[4], rays[5], hits->prim, hits->t, hits->dpdu, hits->dpdv, n, ldir[0], ldir[1],
+                     ldir[2], eps, want_shadow, want_mirror, out[0], out[1], out[2], out[3], out[4], out[5], parent,
+                     kind, count, ray_scene, out_scene);
+}
+}  // namespace rtn
 
 extern "C" int rtn_rays_silhouette(const rtn_scene* s, int64_t n, uint32_t seed, float max_deg, float t0_lo,
                                    float t0_hi, const rtn_ray_queue* rays, int* aim_patch, float* aim, void* stream) {
@@ -183,7 +208,8 @@ extern "C" int rtn_rays_secondary(const float* const rays[6], const rtn_hits* hi
   hipLaunchKernelGGL(k_secondaries, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, rays[0], rays[1], rays[2],
                      rays[3], rays[4], rays[5], hits->prim, hits->t, hits->dpdu, hits->dpdv, n, light[0] / ll,
                      light[1] / ll, light[2] / ll, eps, want_shadow, want_mirror, out->origin_x, out->origin_y,
-                     out->origin_z, out->direction_x, out->direction_y, out->direction_z, parent, kind, cnt.ptr);
+                     out->origin_z, out->direction_x, out->direction_y, out->direction_z, parent, kind, cnt.ptr,
+                     nullptr, nullptr);
   RTN_HIP(hipGetLastError());
   unsigned long long c = 0;
   RTN_HIP(hipMemcpyAsync(&c, cnt.ptr, sizeof(c), hipMemcpyDeviceToHost, st));

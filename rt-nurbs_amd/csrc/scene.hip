@@ -140,6 +140,17 @@ __global__ void k_headers(const SurfDesc* sd, int ns, int np, const float* brk, 
   kidx[g] = make_int2(span_u[d.u_off + i], span_v[d.v_off + j]);
 }
 
+// Nets repacked in BVH order: patch i copies the net of src = perm[i].
+__global__ void k_gather_nets(const float4* __restrict__ cp_in, const int* __restrict__ off_in,
+                              const int* __restrict__ pq_in, const uint32_t* __restrict__ perm,
+                              const int* __restrict__ off_out, int np, float4* __restrict__ cp_out) {
+  const int i = blockIdx.x;
+  if (i >= np) return;
+  const int src = (int)perm[i];
+  const int len = ((pq_in[src] & 0xff) + 1) * (((pq_in[src] >> 8) & 0xff) + 1);
+  for (int k = threadIdx.x; k < len; k += blockDim.x) cp_out[off_out[i] + k] = cp_in[off_in[src] + k];
+}
+
 // bounds (bezier_patch.h:154-169): per-coordinate true division by w.
 __global__ void k_bounds(const float4* cp, const int* cp_off, const int* pq, int np, float* bounds) {
   int g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -460,6 +471,135 @@ int build_sah_device(const float* d_pb, int n, std::vector<rtn_bvh_node>& nodes_
   return RTN_OK;
 }
 
+// Binned-SAH binary BVH on the host, for scenes too large for the exhaustive
+// split search (which is quadratic in the primitives of a node).  Per node:
+// centroid bounds, `bins` equal bins on each axis, the cheapest plane by the
+// surface-area heuristic (half areas, as bounds_ops.h:72-77), in-place
+// partition of the primitive order.  A node becomes a leaf when it holds one
+// primitive, or at most max_leaf primitives and no split beats testing them
+// all; a node whose centroids coincide is split at its median.  Output uses
+// the reference node layout (children adjacent, left_first = left child).
+int build_binned_host(const std::vector<float>& pb, int n, int bins, int max_leaf, std::vector<rtn_bvh_node>& nodes,
+                      std::vector<uint32_t>& order, int64_t* max_depth, int64_t* n_leaves) {
+  bins = std::max(2, std::min(bins, 256));
+  max_leaf = std::max(1, std::min(max_leaf, 128));
+  order.resize(n);
+  std::vector<float> cen(3 * (size_t)n);
+  for (int i = 0; i < n; i++) {
+    order[i] = (uint32_t)i;
+    for (int a = 0; a < 3; a++) cen[3 * (size_t)i + a] = 0.5f * (pb[6 * (size_t)i + a] + pb[6 * (size_t)i + 3 + a]);
+  }
+  nodes.clear();
+  nodes.reserve(2 * (size_t)n);
+  nodes.push_back(rtn_bvh_node{});
+  struct Task { int node, begin, end, depth; };
+  std::vector<Task> stack{{0, 0, n, 0}};
+  int64_t depth = 0, leaves = 0;
+  auto half_area = [](const float* b) {
+    const float ex = b[3] - b[0], ey = b[4] - b[1], ez = b[5] - b[2];
+    return ex * ey + ey * ez + ez * ex;
+  };
+  std::vector<float> bb(6 * bins), rb(6 * bins);
+  std::vector<int> bc(bins);
+  while (!stack.empty()) {
+    const Task t = stack.back();
+    stack.pop_back();
+    depth = std::max<int64_t>(depth, t.depth);
+    float box[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    float cb[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for (int k = t.begin; k < t.end; k++) {
+      const size_t i = order[k];
+      for (int a = 0; a < 3; a++) {
+        box[a] = std::min(box[a], pb[6 * i + a]);
+        box[3 + a] = std::max(box[3 + a], pb[6 * i + 3 + a]);
+        cb[a] = std::min(cb[a], cen[3 * i + a]);
+        cb[3 + a] = std::max(cb[3 + a], cen[3 * i + a]);
+      }
+    }
+    rtn_bvh_node& nd = nodes[t.node];
+    nd.p_min_x = box[0]; nd.p_min_y = box[1]; nd.p_min_z = box[2];
+    nd.p_max_x = box[3]; nd.p_max_y = box[4]; nd.p_max_z = box[5];
+    const int cnt = t.end - t.begin;
+    auto make_leaf = [&]() {
+      nodes[t.node].left_first = (uint32_t)t.begin;
+      nodes[t.node].prim_count = (uint32_t)cnt;
+      leaves++;
+    };
+    if (cnt == 1) { make_leaf(); continue; }
+    // best binned split over the three axes
+    float best_cost = INFINITY;
+    int best_axis = -1, best_bin = 0;
+    for (int a = 0; a < 3; a++) {
+      const float lo = cb[a], ext = cb[3 + a] - cb[a];
+      if (!(ext > 0.f)) continue;
+      const float scale = (float)bins / ext;
+      std::fill(bc.begin(), bc.end(), 0);
+      for (int b = 0; b < bins; b++) {
+        bb[6 * b] = bb[6 * b + 1] = bb[6 * b + 2] = INFINITY;
+        bb[6 * b + 3] = bb[6 * b + 4] = bb[6 * b + 5] = -INFINITY;
+      }
+      for (int k = t.begin; k < t.end; k++) {
+        const size_t i = order[k];
+        const int b = std::min(bins - 1, (int)((cen[3 * i + a] - lo) * scale));
+        bc[b]++;
+        for (int c = 0; c < 3; c++) {
+          bb[6 * b + c] = std::min(bb[6 * b + c], pb[6 * i + c]);
+          bb[6 * b + 3 + c] = std::max(bb[6 * b + 3 + c], pb[6 * i + 3 + c]);
+        }
+      }
+      // right-to-left sweep of accumulated boxes
+      float acc[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+      std::vector<int> rc(bins);
+      int rn = 0;
+      for (int b = bins - 1; b >= 1; b--) {
+        for (int c = 0; c < 3; c++) {
+          acc[c] = std::min(acc[c], bb[6 * b + c]);
+          acc[3 + c] = std::max(acc[3 + c], bb[6 * b + 3 + c]);
+        }
+        rn += bc[b];
+        rc[b] = rn;
+        for (int c = 0; c < 6; c++) rb[6 * b + c] = acc[c];
+      }
+      float lacc[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+      int ln = 0;
+      for (int b = 0; b < bins - 1; b++) {  // plane between bin b and b+1
+        for (int c = 0; c < 3; c++) {
+          lacc[c] = std::min(lacc[c], bb[6 * b + c]);
+          lacc[3 + c] = std::max(lacc[3 + c], bb[6 * b + 3 + c]);
+        }
+        ln += bc[b];
+        const int rn2 = rc[b + 1];
+        if (ln == 0 || rn2 == 0) continue;
+        const float cost = ln * half_area(lacc) + rn2 * half_area(&rb[6 * (b + 1)]);
+        if (cost < best_cost) { best_cost = cost; best_axis = a; best_bin = b; }
+      }
+    }
+    int mid;
+    if (best_axis < 0) {  // coincident centroids: median split
+      if (cnt <= max_leaf) { make_leaf(); continue; }
+      mid = t.begin + cnt / 2;
+    } else {
+      if (cnt <= max_leaf && best_cost >= cnt * half_area(box)) { make_leaf(); continue; }
+      const float lo = cb[best_axis], scale = (float)bins / (cb[3 + best_axis] - cb[best_axis]);
+      auto it = std::partition(order.begin() + t.begin, order.begin() + t.end, [&](uint32_t i) {
+        return std::min(bins - 1, (int)((cen[3 * (size_t)i + best_axis] - lo) * scale)) <= best_bin;
+      });
+      mid = (int)(it - order.begin());
+      if (mid == t.begin || mid == t.end) mid = t.begin + cnt / 2;
+    }
+    const int left = (int)nodes.size();
+    nodes.push_back(rtn_bvh_node{});
+    nodes.push_back(rtn_bvh_node{});
+    nodes[t.node].left_first = (uint32_t)left;
+    nodes[t.node].prim_count = 0;
+    stack.push_back({left + 1, mid, t.end, t.depth + 1});
+    stack.push_back({left, t.begin, mid, t.depth + 1});
+  }
+  if (max_depth) *max_depth = depth;
+  if (n_leaves) *n_leaves = leaves;
+  return RTN_OK;
+}
+
 // Collapse of the binary BVH into 4-wide nodes: each wide node takes a binary
 // interior node and repeatedly opens its largest-area interior child until it
 // has four children.  Leaves keep the binary leaf's primitive range.
@@ -595,8 +735,24 @@ namespace rtn {
 int scene_finish(rtn_scene* s);  // defined below
 }
 
+extern "C" void rtn_build_params_default(rtn_build_params* bp) {
+  if (!bp) return;
+  memset(bp, 0, sizeof(*bp));
+  bp->bvh = RTN_BVH_EXACT_SAH;
+  bp->bins = 32;
+  bp->max_leaf = 4;
+}
+
 extern "C" int rtn_scene_build(rtn_scene* s) {
-  RTN_CHECK(s, RTN_EINVAL, "rtn_scene_build: null scene");
+  rtn_build_params bp;
+  rtn_build_params_default(&bp);
+  return rtn_scene_build_ex(s, &bp);
+}
+
+extern "C" int rtn_scene_build_ex(rtn_scene* s, const rtn_build_params* bp) {
+  RTN_CHECK(s && bp, RTN_EINVAL, "rtn_scene_build: null argument");
+  RTN_CHECK(bp->bvh == RTN_BVH_EXACT_SAH || bp->bvh == RTN_BVH_BINNED_SAH, RTN_EINVAL,
+            "rtn_scene_build: unknown BVH builder %d", bp->bvh);
   RTN_CHECK(!s->built, RTN_EINVAL, "rtn_scene_build: scene already built");
   RTN_CHECK(!s->surfaces.empty() || !s->groups.empty(), RTN_EINVAL, "rtn_scene_build: empty scene");
   auto t0 = std::chrono::steady_clock::now();
@@ -696,15 +852,22 @@ extern "C" int rtn_scene_build(rtn_scene* s) {
   RTN_HIP(hipGetLastError());
   RTN_HIP(hipDeviceSynchronize());
   auto t1 = std::chrono::steady_clock::now();
-  // ---- BVH (reference exhaustive SAH), then reorder patches to BVH order
+  // ---- BVH (reference exhaustive SAH, or binned SAH), then reorder patches to BVH order
   std::vector<rtn_bvh_node> nodes;
   DevBuf<uint32_t> perm;
-  RTN_TRY(build_sah_device(d_pb.ptr, np, nodes, perm, &s->max_depth, &s->n_leaves));
+  if (bp->bvh == RTN_BVH_EXACT_SAH) {
+    RTN_TRY(build_sah_device(d_pb.ptr, np, nodes, perm, &s->max_depth, &s->n_leaves));
+  } else {
+    std::vector<float> pb_h(6 * (size_t)np);
+    RTN_TRY(d_pb.download(pb_h.data(), pb_h.size()));
+    std::vector<uint32_t> ord;
+    RTN_TRY(build_binned_host(pb_h, np, bp->bins, bp->max_leaf, nodes, ord, &s->max_depth, &s->n_leaves));
+    RTN_TRY(perm.upload(ord.data(), np));
+  }
   // BVH-order patch arrays; nets are repacked contiguously in BVH order
-  std::vector<int> cp_off_h(np), pq_h(np);
+  std::vector<int> pq_h(np);
   std::vector<uint32_t> perm_h(np);
   RTN_TRY(perm.download(perm_h.data(), np));
-  RTN_TRY(o_cp_off.download(cp_off_h.data(), np));
   RTN_TRY(o_pq.download(pq_h.data(), np));
   std::vector<int> new_off(np);
   std::vector<int64_t> hist((RTN_MAX_DEGREE + 1) * (RTN_MAX_DEGREE + 1), 0);
@@ -717,20 +880,10 @@ extern "C" int rtn_scene_build(rtn_scene* s) {
     hist[p * (RTN_MAX_DEGREE + 1) + q]++;
   }
   RTN_TRY(s->cp.alloc(total_cp));
-  for (int i = 0; i < np;) {  // copy runs of nets that stay contiguous
-    const int src = (int)perm_h[i];
-    int j = i + 1;
-    int64_t len = (int64_t)((pq_h[src] & 0xff) + 1) * (((pq_h[src] >> 8) & 0xff) + 1);
-    while (j < np && (int)perm_h[j] == (int)perm_h[j - 1] + 1) {
-      const int sj = (int)perm_h[j];
-      len += (int64_t)((pq_h[sj] & 0xff) + 1) * (((pq_h[sj] >> 8) & 0xff) + 1);
-      j++;
-    }
-    RTN_HIP(hipMemcpy(s->cp.ptr + new_off[i], d_cp.ptr + cp_off_h[src], len * sizeof(float4),
-                      hipMemcpyDeviceToDevice));
-    i = j;
-  }
   RTN_TRY(s->cp_off.upload(new_off.data(), np));
+  hipLaunchKernelGGL(k_gather_nets, dim3(np), dim3(64), 0, nullptr, d_cp.ptr, o_cp_off.ptr, o_pq.ptr, perm.ptr,
+                     s->cp_off.ptr, np, s->cp.ptr);
+  RTN_HIP(hipGetLastError());
   RTN_TRY(s->pq.alloc(np));
   RTN_TRY(s->uv.alloc(np));
   RTN_TRY(s->patch_id.alloc(np));

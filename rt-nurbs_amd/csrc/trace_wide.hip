@@ -26,48 +26,12 @@
 #include "mlp.hpp"
 #include "rtn_fast.hpp"
 #include "rtn_internal.hpp"
+#include "trace_wide.hpp"
 
 using namespace rtn;
 using namespace rtn::fast;
 
 namespace {
-
-constexpr int WB = 128;   // lanes per block (2 waves)
-constexpr int WSTK = 20;  // stack entries per lane: 20 KB of LDS per block -> 4 waves/SIMD
-
-struct WArgs {
-  const WideNode* nodes;
-  const float4* cp;
-  const int* cp_off;
-  const int* pq;
-  const float4* corners;  // dehomogenised corners q00,q10,q11,q01 (12 floats per patch)
-  const float *ox, *oy, *oz, *dx, *dy, *dz;
-  int64_t n_rays;
-  int max_iter;
-  float thr_det, thr_res;
-  uint32_t seed_base;
-  int hull_cull;
-  float cull_margin;
-  int mixed;
-  int any_hit;
-  float cond_thr;
-  // learned form: BF16 MFMA tiles and biases of the seed network, patch boxes
-  const mlp::bf16x8* mlp_tiles;
-  const float* mlp_bias;
-  const float* bounds;
-  int learned_iters;
-  // training pairs: every converged solve with t > 0 is appended
-  int* pair_ray;
-  int* pair_patch;
-  float2* pair_uv;
-  unsigned long long* pair_count;
-  int64_t pair_cap;
-  int* prim;
-  float *t, *u, *v, *dpdu, *dpdv;
-  uint32_t* counters;
-  unsigned long long* totals;
-  unsigned int* work;  // persistent: next ray to hand out
-};
 
 struct Cnt {  // per-ray work counters (counting builds only)
   uint32_t nc, ne, ncull, nbox, nleaf, npt, ovf, nfl, n64, nfb;
@@ -81,7 +45,13 @@ struct Lane {
   int best;
 };
 
-__device__ __forceinline__ void lane_start(Lane& L, const WArgs& A, uint32_t r) {
+// Starts ray r on this lane; returns its root node (the ray's scene in a
+// batch), or -1 for an empty secondary slot (recorded as a miss at once).
+__device__ __forceinline__ int lane_start(Lane& L, const WArgs& A, uint32_t r) {
+  if (A.ray_kind && A.ray_kind[r] < 0) {
+    A.prim[r] = -1;
+    return -1;
+  }
   L.r = r;
   L.o = {A.ox[r], A.oy[r], A.oz[r]};
   L.d = {A.dx[r], A.dy[r], A.dz[r]};
@@ -93,6 +63,7 @@ __device__ __forceinline__ void lane_start(Lane& L, const WArgs& A, uint32_t r) 
   L.rp = planes(L.o, L.d);
   L.t_max = INFINITY;
   L.best = -1;
+  return A.scene_root ? A.scene_root[A.ray_scene[r]] : 0;
 }
 
 template <bool COUNT>
@@ -106,7 +77,7 @@ __device__ __forceinline__ void lane_finish(const Lane& L, const WArgs& A, const
 
 // Expand one wide node: test the four child boxes (slab test with FMA,
 // t = b*rd - o*rd) and push the hits far-to-near so the nearest is on top.
-template <bool COUNT>
+template <int WB, int WSTK, bool COUNT>
 __device__ __forceinline__ void expand(Lane& L, Cnt& cn, const WideNode* __restrict__ nodes, int ni, int (*sc)[WB],
                                        float (*st)[WB], int tid, int& off) {
   const float4* base = reinterpret_cast<const float4*>(nodes + ni);
@@ -237,7 +208,7 @@ __device__ __forceinline__ LeafOut leaf_dispatch(int cls, const float4* net, int
   }
 }
 
-template <bool PERSISTENT, bool COUNT, bool LEARNED>
+template <int WB, int WSTK, bool PERSISTENT, bool COUNT, bool LEARNED>
 __global__ void __launch_bounds__(WB)
 k_trace_wide(const WArgs* __restrict__ args) {
   __shared__ int s_code[WSTK][WB];
@@ -261,9 +232,8 @@ k_trace_wide(const WArgs* __restrict__ args) {
   } else {
     const int64_t r = (int64_t)blockIdx.x * WB + tid;
     if (r < A.n_rays) {
-      lane_start(L, A, (uint32_t)r);
-      has_ray = true;
-      cur = 0;
+      cur = lane_start(L, A, (uint32_t)r);
+      has_ray = cur >= 0;
     }
   }
   while (true) {
@@ -282,10 +252,9 @@ k_trace_wide(const WArgs* __restrict__ args) {
           const unsigned rank = __popcll(need & ((1ull << lane) - 1));
           const int64_t r = (int64_t)base + rank;
           if (r < A.n_rays) {
-            lane_start(L, A, (uint32_t)r);
+            cur = lane_start(L, A, (uint32_t)r);
             if (COUNT) cn = Cnt{};
-            has_ray = true;
-            cur = 0;
+            has_ray = cur >= 0;
             off = 0;
             cand_left = 0;
           }
@@ -295,7 +264,7 @@ k_trace_wide(const WArgs* __restrict__ args) {
     // ---- traversal: advance every lane that has no pending leaf
     while (has_ray && cand_left == 0) {
       if (cur >= 0) {
-        expand<COUNT>(L, cn, A.nodes, cur, s_code, s_te, tid, off);
+        expand<WB, WSTK, COUNT>(L, cn, A.nodes, cur, s_code, s_te, tid, off);
         cur = -1;
         continue;
       }
@@ -427,7 +396,7 @@ k_trace_wide(const WArgs* __restrict__ args) {
       }
       if ((res.flags & 1) && 0.0f < res.t && res.t < L.t_max) {
         record(L, A, pi, res.t, res.u, res.v, res.Su, res.Sv);
-        if (A.any_hit) {  // occlusion query: the first hit ends the ray
+        if (A.any_hit || (A.ray_kind && A.ray_kind[L.r] == 0)) {  // occlusion query: the first hit ends the ray
           off = 0;
           cand_left = 0;
           cur = -1;
@@ -452,24 +421,77 @@ namespace rtn {
 const void* mlp_tiles(const rtn_mlp* net);
 const float* mlp_bias(const rtn_mlp* net);
 
-template <bool P, bool C>
-static void launch_wide(bool learned, unsigned blocks, hipStream_t stream, const WArgs* a) {
+namespace {
+template <int WB, int WSTK, bool P, bool C>
+void launch_shape(bool learned, unsigned blocks, hipStream_t stream, const WArgs* a) {
   if (learned)
-    hipLaunchKernelGGL((k_trace_wide<P, C, true>), dim3(blocks), dim3(WB), 0, stream, a);
+    hipLaunchKernelGGL((k_trace_wide<WB, WSTK, P, C, true>), dim3(blocks), dim3(WB), 0, stream, a);
   else
-    hipLaunchKernelGGL((k_trace_wide<P, C, false>), dim3(blocks), dim3(WB), 0, stream, a);
+    hipLaunchKernelGGL((k_trace_wide<WB, WSTK, P, C, false>), dim3(blocks), dim3(WB), 0, stream, a);
+}
+template <int WB, int WSTK>
+void launch_tuned(unsigned blocks, hipStream_t stream, const WArgs* a) {
+  hipLaunchKernelGGL((k_trace_wide<WB, WSTK, true, false, false>), dim3(blocks), dim3(WB), 0, stream, a);
+}
+}  // namespace
+
+const LaunchShape kDefaultShape{WB_DEFAULT, WSTK_DEFAULT, 8};
+const LaunchShape kTunedShapes[] = {{64, 16, 8},  {64, 24, 8},  {64, 32, 8},  {128, 16, 8}, {128, 24, 8},
+                                    {128, 32, 8}, {256, 16, 8}, {256, 24, 8}, {256, 32, 8}};
+const int kNumTunedShapes = 9;
+
+bool shape_available(const LaunchShape& sh, bool persistent, bool count, bool learned) {
+  if (sh.wb == WB_DEFAULT && sh.wstk == WSTK_DEFAULT) return true;
+  if (!persistent || count || learned) return false;
+  for (int i = 0; i < kNumTunedShapes; i++)
+    if (kTunedShapes[i].wb == sh.wb && kTunedShapes[i].wstk == sh.wstk) return true;
+  return false;
 }
 
-int trace_wide_scene(const rtn_scene* s, const float* const rays[6], int64_t n, const rtn_trace_params* prm,
-                     const rtn_hits* hits, unsigned long long* d_totals, hipStream_t stream,
-                     const PairOut* pairs) {
+int launch_wide_kernel(const LaunchShape& sh, bool persistent, bool count, bool learned, int64_t n_max,
+                       hipStream_t stream, const WArgs* d_args) {
+  RTN_CHECK(shape_available(sh, persistent, count, learned), RTN_EUNSUPPORTED,
+            "trace: launch shape %d lanes x %d stack entries is not built for this form", sh.wb, sh.wstk);
+  unsigned blocks;
+  if (persistent) {
+    int dev = 0, cus = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    blocks = (unsigned)std::max<int64_t>(
+        1, std::min<int64_t>((n_max + sh.wb - 1) / sh.wb, (int64_t)cus * std::max(1, sh.blocks_per_cu)));
+  } else {
+    blocks = (unsigned)std::max<int64_t>(1, (n_max + sh.wb - 1) / sh.wb);
+  }
+  if (sh.wb == WB_DEFAULT && sh.wstk == WSTK_DEFAULT) {
+    if (persistent) {
+      if (count) launch_shape<WB_DEFAULT, WSTK_DEFAULT, true, true>(learned, blocks, stream, d_args);
+      else launch_shape<WB_DEFAULT, WSTK_DEFAULT, true, false>(learned, blocks, stream, d_args);
+    } else {
+      if (count) launch_shape<WB_DEFAULT, WSTK_DEFAULT, false, true>(learned, blocks, stream, d_args);
+      else launch_shape<WB_DEFAULT, WSTK_DEFAULT, false, false>(learned, blocks, stream, d_args);
+    }
+  } else {
+#define RTN_SHAPE(W, S) \
+  else if (sh.wb == W && sh.wstk == S) launch_tuned<W, S>(blocks, stream, d_args);
+    if (false) {
+    }
+    RTN_SHAPE(64, 16) RTN_SHAPE(64, 24) RTN_SHAPE(64, 32) RTN_SHAPE(128, 16) RTN_SHAPE(128, 24) RTN_SHAPE(128, 32)
+    RTN_SHAPE(256, 16) RTN_SHAPE(256, 24) RTN_SHAPE(256, 32)
+#undef RTN_SHAPE
+  }
+  RTN_HIP(hipGetLastError());
+  return RTN_OK;
+}
+
+int fill_wide_args(const rtn_scene* s, const float* const rays[6], int64_t n, const rtn_trace_params* prm,
+                   const rtn_hits* hits, unsigned long long* d_totals, const PairOut* pairs, WArgs& A) {
   const bool learned = prm->seed_mode == RTN_SEED_LEARNED;
   RTN_CHECK(prm->seed_mode == RTN_SEED_PLAIN || learned, RTN_EINVAL, "rtn_trace: unknown seed mode %d",
             prm->seed_mode);
   RTN_CHECK(!learned || prm->seed_net, RTN_EINVAL, "rtn_trace: the learned seed needs seed_net");
   RTN_CHECK(prm->learned_iters >= 0 && prm->learned_iters <= 64, RTN_EINVAL, "rtn_trace: learned_iters outside 0..64");
   RTN_CHECK(prm->flat_threshold <= 0.f, RTN_EUNSUPPORTED, "rtn_trace: flat-skip runs in the exact form only");
-  WArgs A{};
+  A = WArgs{};
   A.nodes = s->wide.ptr;
   A.cp = s->cp.ptr;
   A.cp_off = s->cp_off.ptr;
@@ -502,6 +524,14 @@ int trace_wide_scene(const rtn_scene* s, const float* const rays[6], int64_t n, 
   A.prim = hits->prim; A.t = hits->t; A.u = hits->u; A.v = hits->v; A.dpdu = hits->dpdu; A.dpdv = hits->dpdv;
   A.counters = prm->count ? hits->counters : nullptr;
   A.totals = d_totals;
+  return RTN_OK;
+}
+
+int trace_wide_scene(const rtn_scene* s, const float* const rays[6], int64_t n, const rtn_trace_params* prm,
+                     const rtn_hits* hits, unsigned long long* d_totals, hipStream_t stream,
+                     const PairOut* pairs) {
+  WArgs A;
+  RTN_TRY(fill_wide_args(s, rays, n, prm, hits, d_totals, pairs, A));
   const bool persistent = prm->mode == RTN_MODE_PERSISTENT;
   // arguments and the ray counter live in one small device block (stream-ordered)
   struct Block { WArgs a; unsigned int work; unsigned int pad[3]; };
@@ -512,26 +542,10 @@ int trace_wide_scene(const rtn_scene* s, const float* const rays[6], int64_t n, 
   h.a = A;
   h.work = 0;
   RTN_HIP(hipMemcpyAsync(d_blk, &h, sizeof(Block), hipMemcpyHostToDevice, stream));
-  const bool count = prm->count != 0;
-  if (persistent) {
-    int dev = 0, cus = 256;
-    (void)hipGetDevice(&dev);
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + WB - 1) / WB, (int64_t)cus * 8));
-    if (count)
-      launch_wide<true, true>(learned, blocks, stream, &d_blk->a);
-    else
-      launch_wide<true, false>(learned, blocks, stream, &d_blk->a);
-  } else {
-    const unsigned blocks = (unsigned)((n + WB - 1) / WB);
-    if (count)
-      launch_wide<false, true>(learned, blocks, stream, &d_blk->a);
-    else
-      launch_wide<false, false>(learned, blocks, stream, &d_blk->a);
-  }
-  RTN_HIP(hipGetLastError());
+  const int rc = launch_wide_kernel(kDefaultShape, persistent, prm->count != 0,
+                                    prm->seed_mode == RTN_SEED_LEARNED, n, stream, &d_blk->a);
   RTN_HIP(hipFreeAsync(d_blk, stream));
-  return RTN_OK;
+  return rc;
 }
 
 }  // namespace rtn

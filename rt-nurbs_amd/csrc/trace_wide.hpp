@@ -1,0 +1,74 @@
+// Arguments and launch interface of the fast trace kernels (wide and
+// persistent forms), shared by the single-scene entry points and the
+// multi-scene frame (batched persistent launch, graph capture, autotuning).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "mlp.hpp"
+#include "rtn_internal.hpp"
+
+namespace rtn {
+
+constexpr int WB_DEFAULT = 128;   // lanes per block (2 waves)
+constexpr int WSTK_DEFAULT = 20;  // stack entries per lane: 20 KB of LDS per 128-lane block
+
+// Kernel arguments; they live in device memory (one small block per launch).
+struct WArgs {
+  const WideNode* nodes;
+  const float4* cp;
+  const int* cp_off;
+  const int* pq;
+  const float4* corners;  // dehomogenised corners q00,q10,q11,q01 (12 floats per patch)
+  const float *ox, *oy, *oz, *dx, *dy, *dz;
+  int64_t n_rays;         // read by the kernel at run time (a producer kernel may set it)
+  int max_iter;
+  float thr_det, thr_res;
+  uint32_t seed_base;
+  int hull_cull;
+  float cull_margin;
+  int mixed;
+  int any_hit;
+  float cond_thr;
+  // learned form: BF16 MFMA tiles and biases of the seed network, patch boxes
+  const mlp::bf16x8* mlp_tiles;
+  const float* mlp_bias;
+  const float* bounds;
+  int learned_iters;
+  // training pairs: every converged solve with t > 0 is appended
+  int* pair_ray;
+  int* pair_patch;
+  float2* pair_uv;
+  unsigned long long* pair_count;
+  int64_t pair_cap;
+  // multi-scene batch: root node of each scene and the scene of each ray
+  const int* scene_root;
+  const int* ray_scene;
+  const int* ray_kind;    // per ray, 0 = occlusion (any-hit) ray; null = all as any_hit says
+  int* prim;
+  float *t, *u, *v, *dpdu, *dpdv;
+  uint32_t* counters;
+  unsigned long long* totals;
+  unsigned int* work;     // persistent: next ray to hand out
+};
+
+// Launch shape: lanes per block (64, 128 or 256: 1, 2 or 4 waves), traversal
+// stack entries per lane (LDS = 8 bytes x entries x lanes per block) and, for
+// the persistent form, resident blocks per compute unit.
+struct LaunchShape {
+  int wb, wstk, blocks_per_cu;
+};
+
+extern const LaunchShape kDefaultShape;
+extern const LaunchShape kTunedShapes[];
+extern const int kNumTunedShapes;
+
+bool shape_available(const LaunchShape& sh, bool persistent, bool count, bool learned);
+int fill_wide_args(const rtn_scene* s, const float* const rays[6], int64_t n, const rtn_trace_params* prm,
+                   const rtn_hits* hits, unsigned long long* d_totals, const PairOut* pairs, WArgs& A);
+// n_max bounds the rays the launch may see (grid size); the kernel reads n_rays from d_args.
+int launch_wide_kernel(const LaunchShape& sh, bool persistent, bool count, bool learned, int64_t n_max,
+                       hipStream_t stream, const WArgs* d_args);
+
+}  // namespace rtn

@@ -36,8 +36,12 @@ EXPORTS = [
     "rtn_event_record", "rtn_event_elapsed_ms", "rtn_trace_pairs", "rtn_mlp_create", "rtn_mlp_destroy",
     "rtn_mlp_set_params", "rtn_mlp_get_params", "rtn_mlp_pair_features", "rtn_mlp_forward", "rtn_mlp_gradient",
     "rtn_mlp_train_params_default", "rtn_mlp_train", "rtn_rays_silhouette", "rtn_rays_secondary",
-    "rtn_shade_lambert",
+    "rtn_shade_lambert", "rtn_build_params_default", "rtn_scene_build_ex", "rtn_batch_create", "rtn_batch_destroy",
+    "rtn_batch_patch_base", "rtn_batch_trace", "rtn_frame_params_default", "rtn_frame_create", "rtn_frame_destroy",
+    "rtn_frame_run", "rtn_frame_capture", "rtn_frame_status", "rtn_frame_results", "rtn_frame_set_shape",
+    "rtn_frame_autotune",
 ]
+BVH_EXACT_SAH, BVH_BINNED_SAH = 0, 1
 MLP_NIN, MLP_HIDDEN, MLP_NPARAM = 20, 64, 9794
 
 
@@ -65,6 +69,26 @@ class MlpTrainParams(C.Structure):
     _fields_ = [("epochs", C.c_int), ("batch", C.c_int), ("lr", C.c_float), ("lr_final", C.c_float),
                 ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float), ("reserved", C.c_int),
                 ("seed", C.c_uint64)]
+
+
+class BuildParams(C.Structure):
+    _fields_ = [("bvh", C.c_int), ("bins", C.c_int), ("max_leaf", C.c_int), ("reserved", C.c_int * 5)]
+
+
+class FrameParams(C.Structure):
+    _fields_ = [("trace", TraceParams), ("light", C.c_float * 3), ("eps", C.c_float), ("shadow", C.c_int),
+                ("mirror", C.c_int), ("reserved", C.c_int * 4)]
+
+
+class FrameStatus(C.Structure):
+    _fields_ = [("n_primary", C.c_int64), ("n_secondary", C.c_int64), ("n_secondary_slots", C.c_int64),
+                ("overflow", C.c_ulonglong),
+                ("shape_lanes", C.c_int), ("shape_stack", C.c_int), ("shape_blocks_per_cu", C.c_int),
+                ("captured", C.c_int)]
+
+
+class TuneEntry(C.Structure):
+    _fields_ = [("lanes", C.c_int), ("stack", C.c_int), ("blocks_per_cu", C.c_int), ("ms", C.c_float)]
 
 
 class Hits(C.Structure):
@@ -113,6 +137,27 @@ def load():
     L.rtn_scene_add_nurbs.argtypes = [vp, i, i, vp, i, i, vp, vp]
     L.rtn_scene_add_patches.argtypes = [vp, i, i, i, vp]
     L.rtn_scene_build.argtypes = [vp]
+    L.rtn_build_params_default.argtypes = [C.POINTER(BuildParams)]
+    L.rtn_build_params_default.restype = None
+    L.rtn_scene_build_ex.argtypes = [vp, C.POINTER(BuildParams)]
+    L.rtn_batch_create.argtypes = [C.POINTER(vp), C.POINTER(vp), i]
+    L.rtn_batch_destroy.argtypes = [vp]
+    L.rtn_batch_destroy.restype = None
+    L.rtn_batch_patch_base.argtypes = [vp, vp]
+    L.rtn_batch_trace.argtypes = [vp, C.POINTER(vp * 6), vp, C.c_int64, C.POINTER(TraceParams), C.POINTER(Hits),
+                                  C.POINTER(TraceTotals), vp]
+    L.rtn_frame_params_default.argtypes = [C.POINTER(FrameParams)]
+    L.rtn_frame_params_default.restype = None
+    L.rtn_frame_create.argtypes = [C.POINTER(vp), vp, C.POINTER(vp * 6), vp, C.c_int64, C.POINTER(FrameParams)]
+    L.rtn_frame_destroy.argtypes = [vp]
+    L.rtn_frame_destroy.restype = None
+    L.rtn_frame_run.argtypes = [vp, vp]
+    L.rtn_frame_capture.argtypes = [vp, vp]
+    L.rtn_frame_status.argtypes = [vp, C.POINTER(FrameStatus)]
+    L.rtn_frame_results.argtypes = [vp, C.POINTER(Hits), C.POINTER(Hits), C.POINTER(vp), C.POINTER(vp),
+                                    C.POINTER(vp * 6)]
+    L.rtn_frame_set_shape.argtypes = [vp, i, i, i]
+    L.rtn_frame_autotune.argtypes = [vp, i, C.POINTER(TuneEntry), i, C.POINTER(i), vp]
     L.rtn_scene_stats_get.argtypes = [vp, C.POINTER(SceneStats)]
     L.rtn_scene_download.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
     L.rtn_trace_params_default.argtypes = [C.POINTER(TraceParams)]
@@ -279,10 +324,16 @@ class Rays:
         return r
 
 
-def generate_rays(cam, spp=1, jitter=False, seed=0, stream=None):
+def generate_rays(cam, spp=1, jitter=False, seed=0, stream=None, rays=None, offset=0):
+    """Pinhole rays of a camera (W*H*spp, samples of a pixel consecutive);
+    written at rays[offset:] when rays is given."""
     wfc, cfr = camera_matrices(cam)
-    rays = Rays(cam.W * cam.H * spp)
-    q = rays.queue()
+    n = cam.W * cam.H * spp
+    if rays is None:
+        rays = Rays(n)
+        offset = 0
+    assert offset + n <= rays.n
+    q = RayQueue(*[c.ptr + 4 * offset for c in rays.comp], rays.idx.ptr + 4 * offset if rays.idx else None)
     check(load().rtn_rays_perspective(wfc.ctypes.data, cfr.ctypes.data, cam.W, cam.H, spp, int(jitter), seed,
                                       C.byref(q), stream))
     return rays
@@ -309,8 +360,16 @@ class Scene:
         n = cp.size // (4 * (p + 1) * (q + 1))
         check(load().rtn_scene_add_patches(self.h, n, p, q, cp.ctypes.data))
 
-    def build(self):
-        check(load().rtn_scene_build(self.h))
+    def build(self, bvh=BVH_EXACT_SAH, bins=32, max_leaf=4):
+        """Decomposition, bounds and BVH on the device (exhaustive SAH by default;
+        BVH_BINNED_SAH for large scenes)."""
+        if bvh == BVH_EXACT_SAH:
+            check(load().rtn_scene_build(self.h))
+        else:
+            bp = BuildParams()
+            load().rtn_build_params_default(C.byref(bp))
+            bp.bvh, bp.bins, bp.max_leaf = bvh, bins, max_leaf
+            check(load().rtn_scene_build_ex(self.h, C.byref(bp)))
         return self
 
     def stats(self):
@@ -581,7 +640,8 @@ def silhouette_rays(scene, n, seed=0, max_deg=5.0, t0=(1.0, 3.0), rays=None, off
 
 
 def secondary_rays(rays, hits, light, eps=1e-4, shadow=True, mirror=True):
-    """Shadow and mirror rays of the primary hits: (Rays, parent, kind, count)."""
+    """Shadow and mirror rays of the primary hits in fixed slots (kind -1 =
+    empty): (Rays, parent, kind, count of non-empty slots)."""
     per = int(shadow) + int(mirror)
     cap = max(1, per * rays.n)
     out = Rays(cap, with_idx=False)
@@ -613,3 +673,115 @@ def write_pgm(path, img):
     with open(path, "wb") as f:
         f.write(b"P5\n%d %d\n255\n" % (a.shape[1], a.shape[0]))
         f.write(a.tobytes())
+
+
+# ------------------------------------------------------ batches and frames
+class Batch:
+    """Built scenes merged for one launch (rtn_batch); keeps the scenes alive."""
+
+    def __init__(self, scenes_):
+        self.scenes = list(scenes_)
+        arr = (C.c_void_p * len(self.scenes))(*[s.h for s in self.scenes])
+        h = C.c_void_p()
+        check(load().rtn_batch_create(C.byref(h), arr, len(self.scenes)))
+        self.h = h.value
+        base = np.zeros(len(self.scenes) + 1, np.int64)
+        check(load().rtn_batch_patch_base(self.h, base.ctypes.data))
+        self.patch_base = base
+
+    def trace(self, rays, ray_scene, prm, hits=None, totals=True):
+        if hits is None:
+            hits = HitBuffers(rays.n, counters=bool(prm.count))
+        hs = hits.struct()
+        tot = TraceTotals() if totals else None
+        arr = rays.ptr_array()
+        check(load().rtn_batch_trace(self.h, C.byref(arr), ray_scene.ptr, rays.n, C.byref(prm), C.byref(hs),
+                                     C.byref(tot) if tot is not None else None, None))
+        t = {k: getattr(tot, k) for k, _ in TraceTotals._fields_} if tot is not None else None
+        return hits, t
+
+    def __del__(self):
+        try:
+            if self.h:
+                load().rtn_batch_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def frame_params(light=(0.3, -0.4, 1.0), eps=1e-4, shadow=1, mirror=1, **trace_kw):
+    fp = FrameParams()
+    load().rtn_frame_params_default(C.byref(fp))
+    for k, v in trace_kw.items():
+        if not hasattr(fp.trace, k):
+            raise KeyError(k)
+        if k == "seed_net":
+            fp._net = v
+            v = v.h if v is not None else None
+        setattr(fp.trace, k, v)
+    fp.light[:] = list(light)
+    fp.eps, fp.shadow, fp.mirror = eps, shadow, mirror
+    return fp
+
+
+class Frame:
+    """Primary + secondary persistent traces over a batch (rtn_frame)."""
+
+    def __init__(self, batch, rays, ray_scene, fp):
+        self.batch, self.rays, self.ray_scene, self.fp = batch, rays, ray_scene, fp
+        h = C.c_void_p()
+        arr = rays.ptr_array()
+        check(load().rtn_frame_create(C.byref(h), batch.h, C.byref(arr), ray_scene.ptr, rays.n, C.byref(fp)))
+        self.h = h.value
+
+    def run(self, stream=None):
+        check(load().rtn_frame_run(self.h, stream))
+
+    def capture(self, stream):
+        check(load().rtn_frame_capture(self.h, stream))
+
+    def status(self):
+        st = FrameStatus()
+        check(load().rtn_frame_status(self.h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in FrameStatus._fields_}
+
+    def set_shape(self, lanes, stack, blocks_per_cu):
+        check(load().rtn_frame_set_shape(self.h, lanes, stack, blocks_per_cu))
+
+    def autotune(self, reps=2, stream=None):
+        table = (TuneEntry * 64)()
+        n = C.c_int()
+        check(load().rtn_frame_autotune(self.h, reps, table, 64, C.byref(n), stream))
+        return [dict(lanes=e.lanes, stack=e.stack, blocks_per_cu=e.blocks_per_cu, ms=e.ms) for e in table[:n.value]]
+
+    def results(self):
+        """Host copies: primary hits, secondary hits (first n_secondary), parent, kind."""
+        h1, h2 = Hits(), Hits()
+        par, knd = C.c_void_p(), C.c_void_p()
+        sr = (C.c_void_p * 6)()
+        check(load().rtn_frame_results(self.h, C.byref(h1), C.byref(h2), C.byref(par), C.byref(knd), C.byref(sr)))
+        st = self.status()
+        n1, n2 = st["n_primary"], st["n_secondary_slots"]
+
+        def grab(ptr, shape, dt):
+            out = np.empty(shape, dt)
+            if out.nbytes:
+                check(load().rtn_memcpy(out.ctypes.data, ptr, out.nbytes, 1))
+            return out
+
+        def hits(h, n):
+            return dict(hit=grab(h.prim, (n,), np.int32), t=grab(h.t, (n,), np.float32),
+                        u=grab(h.u, (n,), np.float32), v=grab(h.v, (n,), np.float32),
+                        dpdu=grab(h.dpdu, (n, 3), np.float32), dpdv=grab(h.dpdv, (n, 3), np.float32))
+        out = dict(primary=hits(h1, n1), secondary=hits(h2, n2), n_secondary=st["n_secondary"],
+                   parent=grab(par.value, (n2,), np.int32), kind=grab(knd.value, (n2,), np.int32),
+                   secondary_rays=[grab(sr[k], (n2,), np.float32) for k in range(6)])
+        return out
+
+    def __del__(self):
+        try:
+            if self.h:
+                load().rtn_frame_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass

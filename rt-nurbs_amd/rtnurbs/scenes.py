@@ -234,3 +234,42 @@ def config3_rays(scene, seed=0, W=256, H=256, max_deg=5.0):
     for a, b in zip(host, sh):
         a[mask] = b
     return lib.Rays.from_host(host), mask
+
+
+# ----------------------------------------------------------------- config 5
+C5_RES = (256, 512, 1024, 2048)
+
+
+def grid_surface(rng, target_patches, p, q) -> Nurbs:
+    """Config-1-style NURBS sized for about `target_patches` Bezier patches:
+    net on the unit grid, open-uniform knots with distinct interior values
+    (patches = segments_u x segments_v), w ~ U[0.5, 2], z ~ N(0, sigma) with
+    sigma = 0.2 * 7 / (net side - 1), i.e. config 1's roughness per net cell."""
+    su = max(1, int(round(np.sqrt(target_patches))))
+    sv = max(1, int(round(target_patches / su)))
+    nu, nv = su + p, sv + q
+
+    def knots(seg, deg):
+        inner = (np.arange(1, seg) / seg).astype(np.float32)
+        return np.concatenate([np.zeros(deg + 1, np.float32), inner, np.ones(deg + 1, np.float32)])
+    sigma = 0.2 * 7.0 / (max(nu, nv) - 1)
+    z = rng.normal(0.0, sigma, size=(nu, nv)).astype(np.float32)
+    w = rng.uniform(0.5, 2.0, size=(nu, nv)).astype(np.float32)
+    ii, jj = np.meshgrid(np.arange(nu), np.arange(nv), indexing="ij")
+    Pw = _homog((ii / np.float32(nu - 1)).astype(np.float32), (jj / np.float32(nv - 1)).astype(np.float32), z, w)
+    return Nurbs(nu - 1, nv - 1, p, q, knots(su, p), knots(sv, q), Pw.reshape(-1, 4)).check()
+
+
+def config5(seed: int = 0, n_scenes: int = 32, patch_range=(1000, 200000), res=C5_RES, scale=1):
+    """32 independent scenes of uneven size: patch counts log-uniform in
+    [1k, 200k], degrees uniform in 2..5 per direction, one config-1-style
+    surface each, seen by config 1's camera at a resolution drawn from
+    256^2..2048^2 (divided by `scale`).  Returns a list of Scene."""
+    rng = np.random.default_rng(50_000 + seed)
+    out = []
+    for k in range(n_scenes):
+        target = int(np.exp(rng.uniform(np.log(patch_range[0]), np.log(patch_range[1]))))
+        p, q = int(rng.integers(2, 6)), int(rng.integers(2, 6))
+        r = int(res[int(rng.integers(0, len(res)))]) // scale
+        out.append(Scene([grid_surface(rng, target, p, q)], c1_camera(r, r), f"c5-seed{seed}-{k}"))
+    return out

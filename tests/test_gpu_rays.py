@@ -72,10 +72,14 @@ def test_secondaries_any_hit_and_shading(rtl):
     sec, parent, kind, cnt = rtl.secondary_rays(rays, h, light, eps=1e-4)
     nh = int((g["hit"] >= 0).sum())
     assert cnt == 2 * nh
-    par, knd = parent.get()[:cnt], kind.get()[:cnt]
-    assert np.array_equal(np.sort(par[knd == 0]), np.flatnonzero(g["hit"] >= 0))
-    so = np.stack(sec.get()[:3], 1)[:cnt].astype(np.float64)
-    sd = np.stack(sec.get()[3:], 1)[:cnt].astype(np.float64)
+    par, knd = parent.get(), kind.get()
+    assert np.array_equal(par, np.repeat(np.arange(rays.n), 2))  # fixed slots 2r, 2r+1
+    assert np.array_equal(knd >= 0, np.repeat(g["hit"] >= 0, 2))
+    valid = knd >= 0
+    par, knd = par[valid], knd[valid]
+    assert np.array_equal(par[knd == 0], np.flatnonzero(g["hit"] >= 0))
+    so = np.stack(sec.get()[:3], 1)[valid].astype(np.float64)
+    sd = np.stack(sec.get()[3:], 1)[valid].astype(np.float64)
     ro = np.stack(rays.get()[:3], 1).astype(np.float64)
     rd = np.stack(rays.get()[3:], 1).astype(np.float64)
     S = ro[par] + g["t"][par, None] * rd[par]
@@ -89,7 +93,7 @@ def test_secondaries_any_hit_and_shading(rtl):
     refl = rd[par[m]] - 2 * (rd[par[m]] * N[m]).sum(1)[:, None] * N[m]
     np.testing.assert_allclose(sd[m], refl, atol=1e-5)
     # any-hit occlusion equals closest-hit hit/miss on the same rays
-    shadow = rtl.Rays.from_host([a[:cnt][knd == 0] for a in sec.get()])
+    shadow = rtl.Rays.from_host([a[valid][knd == 0] for a in sec.get()])
     a, ta = rtl.trace(scene, shadow, rtl.params(mode=rtl.MODE_WIDE, hull_cull=2, any_hit=1, count=1, **CFG))
     c, tc = rtl.trace(scene, shadow, rtl.params(mode=rtl.MODE_WIDE, hull_cull=2, count=1, **CFG))
     assert np.array_equal(a.get()["hit"] >= 0, c.get()["hit"] >= 0)
