@@ -1,14 +1,27 @@
 #!/usr/bin/env python3
-"""Benchmark: Mrays/s and mean Newton iterations per ray of the trace forms on
-the 10,018-patch 1920x1080x4-spp scene (64 seeded mixed-degree NURBS), with
-hit-t error against the FP64 reference trace, on one MI355X.
+"""Benchmark of the per-pixel BVH traversal + Newton rational-Bezier
+intersection on one MI355X: Mrays/s and mean Newton iterations per ray of
+each trace form, with hit-t / (u,v) / normal error against the FP64
+reference trace, the plain FP32 method (the drop-in) as the baseline.
 
-One step = one trace of the whole ray batch (8,294,400 rays) with scene and
-rays resident in HBM.  The headline value is the best form's Mrays/s; every
-form's Mrays/s, iterations per ray and FP64 error statistics are reported in
-the same JSON line, with the plain FP32 method (the drop-in) as the baseline.
+Configurations (--config):
+  c2  (default) 64 seeded random NURBS (degree 2-5) refined to 10,018
+      rational Bezier patches, 1920x1080, 4 jittered spp = 8,294,400 rays.
+      One step = one trace of the whole batch with scene and rays resident
+      in HBM.  The seed network of the learned forms is trained at start on
+      the GPU (plain-method pairs from training-seed scenes).
+  c1  one bicubic NURBS (25 patches), 256x256, 1 spp.
+  c3  config 1 with 20% silhouette rays; FP32 vs the FP64 correction step.
+  c5  32 scenes of 1k-200k patches at 256^2..2048^2 x 4 spp with one shadow
+      and one mirror ray per hit: one frame = persistent primary launch +
+      persistent secondary launch, hipGraph-captured, launch shape autotuned.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c1]
+The JSON line's `value` is the headline form's Mrays/s (c2: the fastest form
+running the learned seed and/or the FP64 correction step); every form is
+reported under `forms`.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c1|c3|c5]
+       python bench.py --calibrate-copy BYTES   (copy of known size, for PMC scaling)
 """
 import argparse
 import json
@@ -21,8 +34,10 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "rt-nurbs_amd"))
 
-PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 (AMD published; 78.6 without packed FP32)
+PEAK_FP32_TFLOPS = 157.3   # MI355X published FP32 peak
 PEAK_HBM_GBS = 8000.0
+BASE = dict(max_iter=8, thr_res=1e-5, thr_det=1e-5)
+METRIC = "Mrays/s + mean Newton iters/ray at ≤1e-5 hit-t err: learned/mixed vs plain"
 
 
 def log(*a):
@@ -33,16 +48,21 @@ def ray_flops(tot, n_rays, exact):
     """Algorithmic FP32 operations of a trace, priced by a fixed table on the
     work the run performed: 40 per ray (reciprocal direction and ray planes),
     33 per box test (66 per binary interior visit, whose two child boxes are
-    tested), 32 per binary leaf visit, 180 per patch test (corners and
-    bilinear seed), plus each Newton evaluation priced at its degree by the
-    reference algorithm's operation count (counted by the kernel).  Work a
-    form avoids (culled patches, fewer iterations) is not priced."""
+    tested; 4 per wide node), 32 per binary leaf visit, 180 per patch test
+    (corners and bilinear seed), plus each Newton evaluation priced at its
+    degree by the reference algorithm's operation count (counted by the
+    kernel).  Work a form avoids (rejected patches, fewer iterations) is not
+    priced."""
     boxes = 66.0 * tot["interior"] + 32.0 * tot["leaves"] if exact else 33.0 * tot["interior"]
     return 40.0 * n_rays + boxes + 180.0 * tot["patch_tests"] + float(tot["newton_flops"])
 
 
 def error_stats(g, ref, tie_rel=1e-5):
-    """Per-pixel comparison with the FP64 reference trace."""
+    """Per-pixel comparison with the FP64 reference trace.  A pixel is a tie
+    when the FP64 trace's second-nearest converged hit on another patch lies
+    within tie_rel of its nearest in t.  patch_mismatch_t_separated counts the
+    non-tie mismatches whose FP32 hit is also more than tie_rel away in t (a
+    different surface point, not the same point on a neighbouring patch)."""
     h32, h64 = g["hit"], ref["hit"]
     t64 = ref["t"]
     tie = (h64 >= 0) & np.isfinite(ref["t2"]) & ((ref["t2"] - t64) <= tie_rel * np.abs(t64))
@@ -50,6 +70,8 @@ def error_stats(g, ref, tie_rel=1e-5):
     same = both & (h32 == h64)
     out = {"rays": int(len(h32)), "hit_frac": float((h32 >= 0).mean()), "ties": int(tie.sum()),
            "patch_mismatch_non_tie": int((both & (h32 != h64) & ~tie).sum()),
+           "patch_mismatch_t_separated": int((both & (h32 != h64) & ~tie & (np.abs(g["t"].astype(np.float64) - t64)
+                                                                           > tie_rel * np.abs(t64))).sum()),
            "hit_miss_disagree": int(((h32 >= 0) != (h64 >= 0)).sum())}
     if same.any():
         te = np.abs(g["t"][same].astype(np.float64) - t64[same]) / np.abs(t64[same])
@@ -66,6 +88,7 @@ def error_stats(g, ref, tie_rel=1e-5):
 
 
 def time_form(L, scene, rays, prm, steps, warmup, stream):
+    """Kernel time per step from HIP events on the launch stream, and wall time."""
     hits = L.HitBuffers(rays.n)
     for _ in range(warmup):
         L.trace(scene, rays, prm, hits, totals=False, stream=stream)
@@ -83,8 +106,9 @@ def time_form(L, scene, rays, prm, steps, warmup, stream):
 
 
 def cpu_baseline(scene_dl, rays_host, target_s=10.0):
-    """The CPU restatement of the reference trace (oracle/) on the host cores,
-    over a bounded sample of the same rays and the same scene arrays."""
+    """The CPU restatement of the reference trace (oracle/, plain method,
+    FP32) on the host cores, over a bounded sample of the same rays and the
+    same scene arrays (the whole batch when it fits the time budget)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
     O.lib()
@@ -109,72 +133,31 @@ def cpu_baseline(scene_dl, rays_host, target_s=10.0):
             break
         n = min(n_all, int(n * min(8.0, target_s / max(dt, 1e-3))))
     return {"value": len(sel) / dt / 1e6, "unit": "Mrays/s", "cores": cores, "kind": "port",
-            "sample": f"{len(sel)} rays (every {stride}th of the batch), plain method FP32 (8 iterations, tol 1e-5), "
+            "sample": f"{len(sel)} of {n_all} rays (every {stride}th), plain method FP32 (8 iterations, tol 1e-5), "
                       f"{dt:.1f} s on {cores} threads"}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=["c2", "c1"])
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-f64", action="store_true", help="skip the FP64 error report")
-    ap.add_argument("--forms", default="all", help="comma list of forms to time (plain,wide,...)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(rays shard by image tile, see DESIGN.md)", file=sys.stderr)
-        sys.exit(2)
+def load_traffic(form, config):
+    tf = os.path.join(ROOT, "profiles", f"traffic_{form}_{config}.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            return json.load(f).get("bytes_per_launch")
+    return None
 
-    from rtnurbs import lib as L
-    from rtnurbs import scenes
-    L.load()
 
-    if args.config == "c2":
-        sc = scenes.config2(0)
-        spp, jitter = 4, True
-        workload = "c2: 64 random NURBS (deg 2-5) refined to 10,018 rational Bezier patches, 1920x1080, 4 jittered spp"
-    else:
-        sc = scenes.config1(0)
-        spp, jitter = 1, False
-        workload = "c1: bicubic 8x8 NURBS -> 25 patches, 256x256, 1 spp"
-    t0 = time.perf_counter()
-    scene = L.Scene(sc.surfaces).build()
-    st = scene.stats()
-    log(f"scene: {st['n_patches']} patches, {st['n_nodes']} nodes, depth {st['max_depth']}, "
-        f"build {time.perf_counter() - t0:.2f}s (decompose {st['build_ms_decompose']:.1f} ms, "
-        f"bvh {st['build_ms_bvh']:.1f} ms, wide {st['build_ms_wide']:.1f} ms)")
-    rays = L.generate_rays(sc.camera, spp=spp, jitter=jitter, seed=1)
-    n = rays.n
-    stream = L.stream_create()
+def roofline(tflops, form, config):
+    return {"bound": "mfma", "achieved": round(tflops, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(tflops / PEAK_FP32_TFLOPS, 4), "traffic": load_traffic(form, config),
+            "note": "compute roofline of FP32 vector work (Newton, de Casteljau, box tests), priced per ray by a "
+                    "fixed op table over the counted work; peak = MI355X published FP32"}
 
-    base = dict(max_iter=8, thr_res=1e-5, thr_det=1e-5)
-    forms = {
-        "plain": dict(mode=L.MODE_EXACT, **base),
-        "wide": dict(mode=L.MODE_WIDE, **base),
-        "wide_cull": dict(mode=L.MODE_WIDE, hull_cull=1, **base),
-        "wide_hull2": dict(mode=L.MODE_WIDE, hull_cull=2, **base),
-        "persistent_hull2": dict(mode=L.MODE_PERSISTENT, hull_cull=2, **base),
-        "wide_hull2_mixed": dict(mode=L.MODE_WIDE, hull_cull=2, precision=L.PREC_MIXED, cond_thr=1e4, **base),
-    }
-    if args.forms != "all":
-        keep = set(args.forms.split(","))
-        forms = {k: v for k, v in forms.items() if k in keep}
 
-    ref = None
-    if not args.no_f64:
-        t0 = time.perf_counter()
-        ref = L.trace_f64(scene, rays, max_iter=32, thr_det=1e-12, thr_res=1e-10).get()
-        log(f"fp64 reference: {time.perf_counter() - t0:.2f}s, hits {(ref['hit'] >= 0).mean():.3f}")
-
+def run_forms(L, scene, rays, forms, args, ref, stream):
     results = {}
+    n = rays.n
     for name, kw in forms.items():
         prm = L.params(**kw)
-        # counting run (same kernel, per-ray counters on) for work and iterations
-        prm_c = L.params(count=1, **kw)
-        hits_c, tot = L.trace(scene, rays, prm_c)
+        hits_c, tot = L.trace(scene, rays, L.params(count=1, **kw))   # counting run: work and iterations
         g = hits_c.get()
         hits, kern_ms, wall_ms = time_form(L, scene, rays, prm, args.steps, args.warmup, stream)
         flops = ray_flops(tot, n, kw["mode"] == L.MODE_EXACT)
@@ -182,14 +165,77 @@ def main():
              "newton_iters_per_ray": tot["newton_evals"] / n, "newton_calls_per_ray": tot["newton_calls"] / n,
              "patch_tests_per_ray": tot["patch_tests"] / n, "flops_per_ray": flops / n,
              "hull_culled_per_ray": tot["hull_culled"] / n, "fp64_steps": tot["fp64_steps"],
+             "learned_fallbacks_per_ray": tot["learned_fallbacks"] / n,
              "tflops": flops / (kern_ms * 1e-3) / 1e12}
         if ref is not None:
             r["fp64"] = error_stats(g, ref)
         results[name] = r
-        log(f"{name}: {r['mrays_s']:.1f} Mrays/s ({kern_ms:.2f} ms kernel, {wall_ms:.2f} ms wall), "
-            f"iters/ray {r['newton_iters_per_ray']:.2f}, {r['tflops']:.1f} TFLOP/s")
+        log(f"{name}: {r['mrays_s']:.1f} Mrays/s ({kern_ms:.3f} ms kernel, {wall_ms:.3f} ms wall), "
+            f"iters/ray {r['newton_iters_per_ray']:.3f}, {r['tflops']:.1f} TFLOP/s"
+            + (f", non-tie patch mismatches {r['fp64']['patch_mismatch_non_tie']}" if ref is not None else ""))
+    return results
 
-    head = max(results, key=lambda k: results[k]["mrays_s"])
+
+def make_line(args, value, ms, dtype, workload, extra):
+    out = {"metric": METRIC, "value": round(value, 3), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": dtype, "data": "synthetic (seeded NURBS scene generators)",
+           "config": workload}
+    out.update(extra)
+    return out
+
+
+def bench_c2_c1(args, L, scenes, learned):
+    if args.config == "c2":
+        sc = scenes.config2(0)
+        spp, jitter = 4, True
+        workload = "10k-patch 1080p: 64 random NURBS (deg 2-5) refined to 10,018 rational Bezier patches, " \
+                   "1920x1080, 4 jittered spp"
+    else:
+        sc = scenes.config1(0)
+        spp, jitter = 1, False
+        workload = "bicubic 8x8 NURBS -> 25 patches, 256x256, 1 spp"
+    t0 = time.perf_counter()
+    scene = L.Scene(sc.surfaces).build()
+    st = scene.stats()
+    log(f"scene: {st['n_patches']} patches, {st['n_nodes']} nodes, depth {st['max_depth']}, "
+        f"build {time.perf_counter() - t0:.2f}s (decompose {st['build_ms_decompose']:.1f} ms, "
+        f"bvh {st['build_ms_bvh']:.1f} ms, wide {st['build_ms_wide']:.1f} ms)")
+    rays = L.generate_rays(sc.camera, spp=spp, jitter=jitter, seed=1)
+    stream = L.stream_create()
+    net, train = None, None
+    if args.train_epochs > 0:
+        t0 = time.perf_counter()
+        net, losses, npairs = learned.build_seed_network(epochs=args.train_epochs)
+        train = {"pairs": int(npairs), "epochs": args.train_epochs, "seconds": round(time.perf_counter() - t0, 2),
+                 "epoch_loss": [round(float(x), 6) for x in losses]}
+        log(f"seed network: {npairs} pairs, {args.train_epochs} epochs, loss {losses[0]:.4f} -> {losses[-1]:.4f}, "
+            f"{train['seconds']} s")
+    P, W = L.MODE_PERSISTENT, L.MODE_WIDE
+    mixed = dict(precision=L.PREC_MIXED, cond_thr=1e4)
+    forms = {
+        "plain": dict(mode=L.MODE_EXACT, **BASE),
+        "wide": dict(mode=W, **BASE),
+        "wide_hull2": dict(mode=W, hull_cull=2, **BASE),
+        "persistent_hull2": dict(mode=P, hull_cull=2, **BASE),
+        "wide_hull2_mixed": dict(mode=W, hull_cull=2, **mixed, **BASE),
+        "persistent_hull2_mixed": dict(mode=P, hull_cull=2, **mixed, **BASE),
+    }
+    if net is not None:
+        lrn = dict(seed_mode=L.SEED_LEARNED, seed_net=net, learned_iters=4)
+        forms["wide_learned"] = dict(mode=W, hull_cull=2, **lrn, **BASE)
+        forms["persistent_learned_mixed"] = dict(mode=P, hull_cull=2, **lrn, **mixed, **BASE)
+    if args.forms != "all":
+        keep = set(args.forms.split(","))
+        forms = {k: v for k, v in forms.items() if k in keep}
+    ref = None
+    if not args.no_f64:
+        t0 = time.perf_counter()
+        ref = L.trace_f64(scene, rays, max_iter=32, thr_det=1e-12, thr_res=1e-10).get()
+        log(f"fp64 reference: {time.perf_counter() - t0:.2f}s, hits {(ref['hit'] >= 0).mean():.3f}")
+    results = run_forms(L, scene, rays, forms, args, ref, stream)
+    cand = [k for k in results if "mixed" in k or "learned" in k] or list(results)
+    head = max(cand, key=lambda k: results[k]["mrays_s"])
     H = results[head]
     cpu = None
     if not args.no_cpu:
@@ -199,32 +245,141 @@ def main():
             log(f"cpu baseline: {cpu['value']:.3f} Mrays/s on {cpu['cores']} threads")
         except Exception as e:  # the baseline is a report, never the measurement
             log(f"cpu baseline failed: {e!r}")
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", f"traffic_{head}_{args.config}.json")
-    if os.path.exists(tf):
-        with open(tf) as f:
-            traffic = json.load(f).get("bytes_per_launch")
-    out = {
-        "metric": "Mrays/s + mean Newton iters/ray at <=1e-5 hit-t err: learned/mixed vs plain",
-        "value": round(H["mrays_s"], 3),
-        "unit": "Mrays/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(H["kernel_ms"], 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (seeded NURBS scene generator)",
-        "config": {"workload": workload, "rays": n, "patches": int(st["n_patches"]), "headline_form": head},
-        "newton_iters_per_ray": round(H["newton_iters_per_ray"], 4),
-        "speedup_vs_plain": round(H["mrays_s"] / results["plain"]["mrays_s"], 3) if "plain" in results else None,
-        "forms": results,
-        "roofline": {"bound": "valu", "achieved": round(H["tflops"], 3), "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(H["tflops"] / PEAK_FP32_TFLOPS, 4), "traffic": traffic},
-        "cpu_baseline": cpu,
-    }
+    plain = results.get("plain", {}).get("mrays_s")
+    extra = {"newton_iters_per_ray": round(H["newton_iters_per_ray"], 4),
+             "speedup_vs_plain": round(H["mrays_s"] / plain, 3) if plain else None,
+             "speedups_vs_plain": {k: round(v["mrays_s"] / plain, 3) for k, v in results.items()} if plain else None,
+             "forms": results, "seed_network": train,
+             "roofline": roofline(H["tflops"], head, args.config), "cpu_baseline": cpu}
+    wl = {"workload": workload, "rays": rays.n, "patches": int(st["n_patches"]), "headline_form": head}
+    return make_line(args, H["mrays_s"], H["kernel_ms"], "f32", wl, extra)
+
+
+def bench_c3(args, L, scenes):
+    sc = scenes.config1(0)
+    scene = L.Scene(sc.surfaces).build()
+    rays, mask = scenes.config3_rays(scene, seed=0)
+    stream = L.stream_create()
+    ref = L.trace_f64(scene, rays).get()
+    forms = {"plain": dict(mode=L.MODE_EXACT, **BASE),
+             "wide_hull2": dict(mode=L.MODE_WIDE, hull_cull=2, **BASE),
+             "wide_hull2_mixed": dict(mode=L.MODE_WIDE, hull_cull=2, precision=L.PREC_MIXED, cond_thr=1e4, **BASE)}
+    for thr in (1e2, 1e1, 2.0):  # threshold sweep: how many rays the FP64 step needs for the t bar
+        forms[f"wide_hull2_mixed_cond{thr:g}"] = dict(mode=L.MODE_WIDE, hull_cull=2, precision=L.PREC_MIXED,
+                                                      cond_thr=thr, **BASE)
+    results = run_forms(L, scene, rays, forms, args, ref, stream)
+    for name, kw in forms.items():  # error on the silhouette rays alone
+        g = L.trace(scene, rays, L.params(**kw))[0].get()
+        sub = {k: v[mask] for k, v in g.items()}
+        rsub = {k: v[mask] for k, v in ref.items()}
+        results[name]["fp64_silhouette"] = error_stats(sub, rsub)
+    head = "wide_hull2_mixed"
+    H = results[head]
+    wl = {"workload": "config 1 scene, 65,536 rays of which 13,107 silhouette rays (grazing 0-5 deg)",
+          "rays": rays.n, "headline_form": head}
+    return make_line(args, H["mrays_s"], H["kernel_ms"], "f32+f64", wl,
+                     {"forms": results, "roofline": roofline(H["tflops"], head, "c3"), "cpu_baseline": None})
+
+
+def bench_c5(args, L, scenes):
+    t0 = time.perf_counter()
+    items = scenes.config5(0, n_scenes=args.c5_scenes, scale=args.c5_scale)
+    built = [L.Scene(s.surfaces).build(bvh=L.BVH_BINNED_SAH) for s in items]
+    n_patches = sum(b.stats()["n_patches"] for b in built)
+    batch = L.Batch(built)
+    log(f"c5: {len(built)} scenes, {n_patches} patches, built in {time.perf_counter() - t0:.1f}s")
+
+    def frame_for(scale):
+        cams = [scenes.Camera(s.camera.eye, s.camera.look, s.camera.up, s.camera.fov_deg,
+                              max(1, s.camera.W // scale), max(1, s.camera.H // scale)) for s in items]
+        counts = [c.W * c.H * 4 for c in cams]
+        rays = L.Rays(sum(counts), with_idx=False)
+        off = 0
+        for cam, nr in zip(cams, counts):
+            L.generate_rays(cam, spp=4, jitter=True, seed=1, rays=rays, offset=off)
+            off += nr
+        ids = L.DeviceArray.from_host(np.repeat(np.arange(len(items), dtype=np.int32), counts))
+        return L.Frame(batch, rays, ids, L.frame_params(**BASE, hull_cull=2)), rays, ids
+
+    proxy, _, _ = frame_for(4)
+    table = proxy.autotune(reps=2)
+    best = min((e for e in table if e["ms"] > 0), key=lambda e: e["ms"])
+    default = [e for e in table if (e["lanes"], e["stack"], e["blocks_per_cu"]) == (128, 20, 8)]
+    log(f"autotune (1/16 of the pixels): best {best}, default {default}")
+    del proxy
+    frame, rays, ids = frame_for(1)
+    frame.set_shape(best["lanes"], best["stack"], best["blocks_per_cu"])
+    stream = L.stream_create()
+    frame.capture(stream)
+    for _ in range(args.warmup):
+        frame.run(stream)
+    L.synchronize()
+    e0, e1 = L.Event(), L.Event()
+    e0.record(stream)
+    for _ in range(args.steps):
+        frame.run(stream)
+    e1.record(stream)
+    L.synchronize()
+    ms = e0.elapsed_ms(e1) / args.steps
+    st = frame.status()
+    total_rays = st["n_primary"] + st["n_secondary"]
+    log(f"c5 frame: {st['n_primary']} primary + {st['n_secondary']} secondary rays, {ms:.2f} ms, "
+        f"{total_rays / ms / 1e3:.1f} Mrays/s")
+    wl = {"workload": f"{len(built)} scenes, {n_patches} patches (1k-200k each, degree 2-5), 256^2-2048^2 x 4 spp "
+                      "+ shadow and mirror ray per hit, one graph-captured frame",
+          "primary_rays": st["n_primary"], "secondary_rays": st["n_secondary"], "patches": n_patches,
+          "launch_shape": {"lanes": st["shape_lanes"], "stack": st["shape_stack"],
+                           "blocks_per_cu": st["shape_blocks_per_cu"]}}
+    return make_line(args, total_rays / ms / 1e3, ms, "f32", wl,
+                     {"autotune": table, "roofline": None, "cpu_baseline": None,
+                      "note": "roofline not priced for the frame (no per-ray counters in the captured launches)"})
+
+
+def calibrate_copy(nbytes):
+    """Device-to-device copies of known size (PMC scaling of HBM counters)."""
+    from rtnurbs import lib as L
+    L.load()
+    a = L.DeviceArray((nbytes // 4,), np.float32)
+    b = L.DeviceArray((nbytes // 4,), np.float32)
+    a.fill_bytes(0)
+    L.synchronize()
+    for _ in range(3):
+        L.check(L.load().rtn_memcpy(b.ptr, a.ptr, a.nbytes, 2))
+    L.synchronize()
+    print(json.dumps({"copy_bytes": a.nbytes, "copies": 3}), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c2", choices=["c2", "c1", "c3", "c5"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-f64", action="store_true", help="skip the FP64 error report")
+    ap.add_argument("--forms", default="all", help="comma list of forms to time (plain,wide,...)")
+    ap.add_argument("--train-epochs", type=int, default=16, help="seed network epochs (0: no learned forms)")
+    ap.add_argument("--c5-scenes", type=int, default=32)
+    ap.add_argument("--c5-scale", type=int, default=1, help="divide config 5 resolutions by this")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(across GPUs the scene is replicated and rays shard by image tile or sample)",
+              file=sys.stderr)
+        sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
+    from rtnurbs import learned, scenes
+    from rtnurbs import lib as L
+    L.load()
+    if args.config in ("c2", "c1"):
+        out = bench_c2_c1(args, L, scenes, learned)
+    elif args.config == "c3":
+        out = bench_c3(args, L, scenes)
+    else:
+        out = bench_c5(args, L, scenes)
     print(json.dumps(out), flush=True)
 
 

@@ -35,7 +35,7 @@ def _orbit_camera(rng, center, radius, W, H):
     return scenes.Camera(tuple(eye), tuple(look), up, float(rng.uniform(35.0, 60.0)), W, H)
 
 
-def training_scenes(n_c1=48, n_c2=24, res=384, seed=0):
+def training_scenes(n_c1=48, n_c2=24, res=512, seed=0):
     """(surfaces, camera) pairs of the training set."""
     rng = np.random.default_rng(1234 + seed)
     out = []
