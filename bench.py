@@ -176,6 +176,31 @@ def run_forms(L, scene, rays, forms, args, ref, stream):
     return results
 
 
+def seed_ablation(L, scene, rays, net):
+    """On the (ray, patch) pairs the plain method solves on this scene (which
+    the network never saw): seed error and Newton evaluations from the plain
+    bilinear seed and from the network's seed."""
+    prm = L.params(mode=L.MODE_WIDE, hull_cull=2, **BASE)
+    pr, pp, puv, cnt = L.trace_pairs(scene, rays, prm, 2 * rays.n)
+    cnt = min(cnt, 2 * rays.n)
+    target = puv.get()[:cnt]
+    luv = net.forward(L.pair_features(scene, rays, pr, pp, cnt))
+    seed_p = L.solve_pairs(scene, rays, pr, pp, cnt, L.params(**{**BASE, "max_iter": 1}))["uv"]
+    a = L.solve_pairs(scene, rays, pr, pp, cnt, L.params(**BASE))
+    b = L.solve_pairs(scene, rays, pr, pp, cnt, L.params(**BASE), seed_uv=luv)
+    both = a["converged"] & b["converged"]
+    ep = np.abs(seed_p - target).max(1)
+    el = np.abs(luv.get() - target).max(1)
+    out = {"pairs": int(cnt), "seed_err_median_plain": float(np.median(ep)), "seed_err_median_learned": float(np.median(el)),
+           "iters_per_solve_plain": float(a["iters"][both].mean()), "iters_per_solve_learned": float(b["iters"][both].mean()),
+           "converged_plain": float(a["converged"].mean()), "converged_learned": float(b["converged"].mean()),
+           "converged_learned_or_fallback": float((a["converged"] | b["converged"]).mean())}
+    log(f"seed ablation on {cnt} solved pairs: seed error {out['seed_err_median_plain']:.3f} -> "
+        f"{out['seed_err_median_learned']:.3f}, evaluations per solve {out['iters_per_solve_plain']:.3f} -> "
+        f"{out['iters_per_solve_learned']:.3f}")
+    return out
+
+
 def make_line(args, value, ms, dtype, workload, extra):
     out = {"metric": METRIC, "value": round(value, 3), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
@@ -234,6 +259,7 @@ def bench_c2_c1(args, L, scenes, learned):
         ref = L.trace_f64(scene, rays, max_iter=32, thr_det=1e-12, thr_res=1e-10).get()
         log(f"fp64 reference: {time.perf_counter() - t0:.2f}s, hits {(ref['hit'] >= 0).mean():.3f}")
     results = run_forms(L, scene, rays, forms, args, ref, stream)
+    ablation = seed_ablation(L, scene, rays, net) if net is not None else None
     cand = [k for k in results if "mixed" in k or "learned" in k] or list(results)
     head = max(cand, key=lambda k: results[k]["mrays_s"])
     H = results[head]
@@ -249,7 +275,7 @@ def bench_c2_c1(args, L, scenes, learned):
     extra = {"newton_iters_per_ray": round(H["newton_iters_per_ray"], 4),
              "speedup_vs_plain": round(H["mrays_s"] / plain, 3) if plain else None,
              "speedups_vs_plain": {k: round(v["mrays_s"] / plain, 3) for k, v in results.items()} if plain else None,
-             "forms": results, "seed_network": train,
+             "forms": results, "seed_network": train, "seed_ablation": ablation,
              "roofline": roofline(H["tflops"], head, args.config), "cpu_baseline": cpu}
     wl = {"workload": workload, "rays": rays.n, "patches": int(st["n_patches"]), "headline_form": head}
     return make_line(args, H["mrays_s"], H["kernel_ms"], "f32", wl, extra)

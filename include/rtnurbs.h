@@ -239,6 +239,16 @@ int rtn_trace_pairs(const rtn_scene* scene, const float* const rays_soa[6], int6
                     const rtn_trace_params* prm, int* pair_ray, int* pair_patch, float* pair_uv, int64_t cap,
                     int64_t* count_host, void* stream);
 
+/* The plain method's Newton solve (bezier_patch.h:198-266, reference order)
+ * on given (ray, patch) pairs: from the plain bilinear seed (seed_uv NULL)
+ * or from seed_uv[2n] (e.g. the learned seed), max_iter / thr_* of prm, RNG
+ * seeded by ray index + prm->seed_base.  Device outputs: iters[n]
+ * (evaluations, -1 for an invalid pair), converged[n] (0/1), optional uv[2n]
+ * (solution, or the seed when not converged) and t[n] (-1 when not). */
+int rtn_solve_pairs(const rtn_scene* scene, const float* const rays_soa[6], int64_t n_rays, const int* pair_ray,
+                    const int* pair_patch, const float* seed_uv, int64_t n, const rtn_trace_params* prm, int* iters,
+                    int* converged, float* uv, float* t, void* stream);
+
 /* ------------------------------------------------------------ learned seed
  * No reference counterpart.  A 20 -> 64 -> 64 -> 64 -> 2 perceptron (GELU,
  * tanh form; sigmoid outputs) predicting Newton's starting (u,v) for a

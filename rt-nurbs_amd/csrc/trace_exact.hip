@@ -373,3 +373,65 @@ extern "C" int rtn_calculate_intersect(const rtn_ray_queue* rays, const rtn_bvh_
   (void)hipFree(tot);
   return rc;
 }
+
+// ------------------------------------------------------------ pair solves
+// The plain method's Newton solve (bezier_patch.h:198-266, reference
+// operation order) on given (ray, patch) pairs, from the plain bilinear seed
+// or from given seeds: isolates what a seed does to the iteration.
+namespace {
+__global__ void k_solve_pairs(const float4* cp, const int* cp_off, const int* pq, const float4* corners, int np,
+                              const float* ox, const float* oy, const float* oz, const float* dx, const float* dy,
+                              const float* dz, int64_t n_rays, const int* pair_ray, const int* pair_patch,
+                              const float* seed_uv, int64_t n, int max_iter, float thr_det, float thr_res,
+                              uint32_t seed_base, int* iters, int* converged, float* uv_out, float* t_out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int r = pair_ray[i], k = pair_patch[i];
+  if (r < 0 || r >= n_rays || k < 0 || k >= np) {
+    iters[i] = -1;
+    converged[i] = 0;
+    return;
+  }
+  const v3<float> o{ox[r], oy[r], oz[r]}, d{dx[r], dy[r], dz[r]};
+  const RayPlane<float> rp = ray_planes(o, d);
+  const float4* net = cp + cp_off[k];
+  const int P = pq[k] & 0xff, Q = (pq[k] >> 8) & 0xff;
+  float u0, v0;
+  if (seed_uv) {
+    u0 = seed_uv[2 * i];
+    v0 = seed_uv[2 * i + 1];
+  } else {
+    const float4 c0 = corners[3 * k], c1 = corners[3 * k + 1], c2 = corners[3 * k + 2];
+    bilinear_seed<float>({c0.x, c0.y, c0.z}, {c0.w, c1.x, c1.y}, {c1.z, c1.w, c2.x}, {c2.y, c2.z, c2.w}, o, d, u0, v0);
+  }
+  const NewtonRes<float> nr = newton<float>(
+      rp, [&](float uu, float vv, v4<float>& du, v4<float>& dv, v4<float>& pt) {
+        patch_eval_rt<RTN_MAX_DEGREE>(net, P, Q, uu, vv, du, dv, pt);
+      },
+      u0, v0, thr_det, thr_res, (uint32_t)r + seed_base, max_iter);
+  iters[i] = nr.evals;
+  converged[i] = nr.hit ? 1 : 0;
+  if (uv_out) {
+    uv_out[2 * i] = nr.hit ? nr.u : u0;
+    uv_out[2 * i + 1] = nr.hit ? nr.v : v0;
+  }
+  if (t_out) t_out[i] = nr.hit ? (nr.S.x - o.x) * d.x + (nr.S.y - o.y) * d.y + (nr.S.z - o.z) * d.z : -1.0f;
+}
+}  // namespace
+
+extern "C" int rtn_solve_pairs(const rtn_scene* s, const float* const rays[6], int64_t n_rays, const int* pair_ray,
+                               const int* pair_patch, const float* seed_uv, int64_t n, const rtn_trace_params* prm,
+                               int* iters, int* converged, float* uv, float* t, void* stream) {
+  RTN_CHECK(s && s->built && rays && pair_ray && pair_patch && prm && iters && converged && n >= 0, RTN_EINVAL,
+            "rtn_solve_pairs: bad arguments");
+  for (int k = 0; k < 6; k++) RTN_CHECK(rays[k], RTN_EINVAL, "rtn_solve_pairs: null ray component %d", k);
+  RTN_CHECK(prm->max_iter >= 1 && prm->max_iter <= 64, RTN_EINVAL, "rtn_solve_pairs: max_iter outside 1..64");
+  if (n == 0) return RTN_OK;
+  hipLaunchKernelGGL(k_solve_pairs, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, (hipStream_t)stream, s->cp.ptr,
+                     s->cp_off.ptr, s->pq.ptr, s->corners.ptr, (int)s->n_patches, rays[0], rays[1], rays[2], rays[3],
+                     rays[4], rays[5], n_rays, pair_ray, pair_patch, seed_uv, n, prm->max_iter, prm->thr_det,
+                     prm->thr_res, prm->seed_base, iters, converged, uv, t);
+  RTN_HIP(hipGetLastError());
+  if (!stream) RTN_HIP(hipStreamSynchronize(nullptr));
+  return RTN_OK;
+}

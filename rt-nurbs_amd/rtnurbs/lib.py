@@ -39,7 +39,7 @@ EXPORTS = [
     "rtn_shade_lambert", "rtn_build_params_default", "rtn_scene_build_ex", "rtn_batch_create", "rtn_batch_destroy",
     "rtn_batch_patch_base", "rtn_batch_trace", "rtn_frame_params_default", "rtn_frame_create", "rtn_frame_destroy",
     "rtn_frame_run", "rtn_frame_capture", "rtn_frame_status", "rtn_frame_results", "rtn_frame_set_shape",
-    "rtn_frame_autotune",
+    "rtn_frame_autotune", "rtn_solve_pairs",
 ]
 BVH_EXACT_SAH, BVH_BINNED_SAH = 0, 1
 MLP_NIN, MLP_HIDDEN, MLP_NPARAM = 20, 64, 9794
@@ -120,7 +120,7 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise RtnError(RTN_EUNSUPPORTED, f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
     L = C.CDLL(LIB_PATH)
-    vp, i, f, sz = C.c_void_p, C.c_int, C.c_float, C.c_size_t
+    vp, i, f, sz, i64 = C.c_void_p, C.c_int, C.c_float, C.c_size_t, C.c_int64
     L.rtn_last_error.restype = C.c_char_p
     L.rtn_malloc.argtypes = [C.POINTER(vp), sz]
     L.rtn_malloc_managed.argtypes = [C.POINTER(vp), sz]
@@ -158,6 +158,8 @@ def load():
                                     C.POINTER(vp * 6)]
     L.rtn_frame_set_shape.argtypes = [vp, i, i, i]
     L.rtn_frame_autotune.argtypes = [vp, i, C.POINTER(TuneEntry), i, C.POINTER(i), vp]
+    L.rtn_solve_pairs.argtypes = [vp, C.POINTER(vp * 6), i64, vp, vp, vp, i64, C.POINTER(TraceParams), vp, vp, vp,
+                                  vp, vp]
     L.rtn_scene_stats_get.argtypes = [vp, C.POINTER(SceneStats)]
     L.rtn_scene_download.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
     L.rtn_trace_params_default.argtypes = [C.POINTER(TraceParams)]
@@ -175,7 +177,6 @@ def load():
     L.rtn_event_destroy.argtypes = [vp]
     L.rtn_event_record.argtypes = [vp, vp]
     L.rtn_event_elapsed_ms.argtypes = [vp, vp, C.POINTER(C.c_float)]
-    i64 = C.c_int64
     L.rtn_trace_pairs.argtypes = [vp, C.POINTER(vp * 6), i64, C.POINTER(TraceParams), vp, vp, vp, i64,
                                   C.POINTER(i64), vp]
     L.rtn_mlp_create.argtypes = [C.POINTER(vp), C.c_uint64]
@@ -785,3 +786,18 @@ class Frame:
                 self.h = None
         except Exception:
             pass
+
+
+def solve_pairs(scene, rays, pair_ray, pair_patch, n, prm, seed_uv=None):
+    """Plain-method Newton on device pairs from the plain seed or seed_uv:
+    host dict of iters, converged, uv, t."""
+    it = DeviceArray((n,), np.int32)
+    cv = DeviceArray((n,), np.int32)
+    uv = DeviceArray((n, 2), np.float32)
+    t = DeviceArray((n,), np.float32)
+    arr = rays.ptr_array()
+    check(load().rtn_solve_pairs(scene.h, C.byref(arr), rays.n, pair_ray.ptr, pair_patch.ptr,
+                                 seed_uv.ptr if seed_uv is not None else None, n, C.byref(prm), it.ptr, cv.ptr, uv.ptr,
+                                 t.ptr, None))
+    synchronize()
+    return dict(iters=it.get(), converged=cv.get().astype(bool), uv=uv.get(), t=t.get())

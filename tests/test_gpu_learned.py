@@ -1,0 +1,74 @@
+"""What the trained seed network gains over the method without it.
+
+The network is trained on the GPU (its own forward/backward kernels) on
+plain-method pairs from training seeds of the config-1 generator, then
+evaluated on held-out seeds: on the (ray, patch) pairs the plain method
+solves, the network's seed is closer to the solution than the plain
+bilinear seed, and Newton from it needs fewer evaluations.  In the trace
+forms it runs as the first attempt with the plain seed as fallback, and the
+trace still finds what the plain method finds (near-identical hit sets)."""
+import numpy as np
+import pytest
+
+from rtnurbs import learned, scenes
+
+pytestmark = pytest.mark.gpu
+
+CFG = dict(max_iter=8, thr_res=1e-5, thr_det=1e-5)
+
+
+@pytest.fixture(scope="module")
+def net(rtl):
+    print("training seed network", flush=True)
+    sl = learned.training_scenes(n_c1=16, n_c2=0, res=384)
+    feats, targets, n = learned.collect_pairs(sl, max_pairs=3_000_000)
+    assert n > 1_000_000
+    net, losses = learned.train_seed_network(feats, targets, n, epochs=6)
+    assert losses[-1] < 0.7 * losses[0], losses
+    return net
+
+
+def test_learned_seed_beats_plain_seed_on_held_out_scenes(rtl, net):
+    prm1 = rtl.params(**{**CFG, "max_iter": 1})
+    prm = rtl.params(**CFG)
+    err_p, err_l, it_p, it_l, conv_p, conv_l = [], [], [], [], [], []
+    for seed in (2000, 2001, 2002):
+        s = scenes.bicubic_grid_surface(seed)
+        scene = rtl.Scene([s]).build()
+        rays = rtl.generate_rays(scenes.c1_camera(160, 160))
+        pr, pp, puv, cnt = rtl.trace_pairs(scene, rays, rtl.params(mode=rtl.MODE_WIDE, hull_cull=2, **CFG),
+                                           4 * rays.n)
+        target = puv.get()[:cnt]
+        feats = rtl.pair_features(scene, rays, pr, pp, cnt)
+        luv = net.forward(feats)
+        plain_seed = rtl.solve_pairs(scene, rays, pr, pp, cnt, prm1)["uv"]
+        err_p.append(np.abs(plain_seed - target).max(1))
+        err_l.append(np.abs(luv.get() - target).max(1))
+        a = rtl.solve_pairs(scene, rays, pr, pp, cnt, prm)
+        b = rtl.solve_pairs(scene, rays, pr, pp, cnt, prm, seed_uv=luv)
+        both = a["converged"] & b["converged"]
+        it_p.append(a["iters"][both])
+        it_l.append(b["iters"][both])
+        conv_p.append(a["converged"])
+        conv_l.append(b["converged"] | a["converged"])  # learned attempt, plain fallback
+    ep, el = np.concatenate(err_p), np.concatenate(err_l)
+    ip, il = np.concatenate(it_p), np.concatenate(it_l)
+    print(f"seed error median plain {np.median(ep):.4f} learned {np.median(el):.4f}; "
+          f"iterations plain {ip.mean():.3f} learned {il.mean():.3f}", flush=True)
+    assert np.median(el) < 0.6 * np.median(ep)
+    assert il.mean() < ip.mean()
+    assert np.concatenate(conv_l).mean() >= np.concatenate(conv_p).mean()
+
+
+def test_learned_trace_keeps_the_hits(rtl, net):
+    sc = scenes.config1(2000)
+    scene = rtl.Scene(sc.surfaces).build()
+    rays = rtl.generate_rays(sc.camera)
+    a, _ = rtl.trace(scene, rays, rtl.params(mode=rtl.MODE_WIDE, hull_cull=2, **CFG))
+    for mode in (rtl.MODE_WIDE, rtl.MODE_PERSISTENT):
+        b, tb = rtl.trace(scene, rays, rtl.params(mode=mode, hull_cull=2, seed_mode=rtl.SEED_LEARNED, seed_net=net,
+                                                  learned_iters=4, count=1, **CFG))
+        ga, gb = a.get(), b.get()
+        assert np.mean(ga["hit"] == gb["hit"]) > 0.99
+        assert (gb["hit"] >= 0).sum() >= 0.995 * (ga["hit"] >= 0).sum()
+        assert tb["learned_fallbacks"] > 0
