@@ -128,8 +128,10 @@ __device__ __forceinline__ void forward_wave(const bf16x8* __restrict__ tiles, c
     f32x4 d[4];
     bf16x8 h[2];
     const bf16x8 x = s_x[16 * nb + m][g];
+    const bf16x8* tp = tiles;
+    asm volatile("" : "+v"(tp));  // keep each layer's tile loads next to its products (register pressure)
 #pragma unroll
-    for (int ob = 0; ob < 4; ob++) d[ob] = mfma(tiles[(T_L1 + ob) * 64 + lane], x, bias4(bias, BB1 + 16 * ob + 4 * g));
+    for (int ob = 0; ob < 4; ob++) d[ob] = mfma(tp[(T_L1 + ob) * 64 + lane], x, bias4(bias, BB1 + 16 * ob + 4 * g));
 #pragma unroll
     for (int layer = 0; layer < 3; layer++) {
       if (KEEP) {
@@ -145,16 +147,18 @@ __device__ __forceinline__ void forward_wave(const bf16x8* __restrict__ tiles, c
       if (layer == 2) break;
       const int tl = layer == 0 ? T_L2 : T_L3;
       const int bb = layer == 0 ? BB2 : BB3;
+      asm volatile("" : "+v"(tp));
 #pragma unroll
       for (int ob = 0; ob < 4; ob++) {
         f32x4 acc = bias4(bias, bb + 16 * ob + 4 * g);
-        acc = mfma(tiles[(tl + ob * 2) * 64 + lane], h[0], acc);
-        d[ob] = mfma(tiles[(tl + ob * 2 + 1) * 64 + lane], h[1], acc);
+        acc = mfma(tp[(tl + ob * 2) * 64 + lane], h[0], acc);
+        d[ob] = mfma(tp[(tl + ob * 2 + 1) * 64 + lane], h[1], acc);
       }
     }
+    asm volatile("" : "+v"(tp));
     f32x4 o = bias4(bias, BB4 + 4 * g);
-    o = mfma(tiles[(T_L4 + 0) * 64 + lane], h[0], o);
-    o = mfma(tiles[(T_L4 + 1) * 64 + lane], h[1], o);
+    o = mfma(tp[(T_L4 + 0) * 64 + lane], h[0], o);
+    o = mfma(tp[(T_L4 + 1) * 64 + lane], h[1], o);
     if (g == 0) s_logit[16 * nb + m] = make_float2(o[0], o[1]);
   }
 }
@@ -171,6 +175,66 @@ __device__ __forceinline__ void store_row(bf16x8 (*s_x)[4], int row, const float
     }
 #pragma unroll
   for (int c = 0; c < 4; c++) s_x[row][c] = r[c];
+}
+
+// Feature row of one pair written straight into s_x[row] in 8-value chunks
+// (the same values as pair_features, without holding all 20 in registers).
+__device__ __forceinline__ void store_features(bf16x8 (*s_x)[4], int row, float ox, float oy, float oz, float dx,
+                                               float dy, float dz, const float* __restrict__ bnd, float4 c0,
+                                               float4 c1, float4 c2, int P, int Q) {
+  const float il = 1.0f / sqrtf(fmaf(dx, dx, fmaf(dy, dy, dz * dz)));
+  const float nx = dx * il, ny = dy * il, nz = dz * il;
+  const float cx = 0.5f * (bnd[0] + bnd[3]), cy = 0.5f * (bnd[1] + bnd[4]), cz = 0.5f * (bnd[2] + bnd[5]);
+  const float hs = fmaxf(fmaxf(bnd[3] - bnd[0], bnd[4] - bnd[1]), fmaxf(bnd[5] - bnd[2], 2e-20f)) * 0.5f;
+  const float is = 1.0f / hs;
+  const float tc = fmaxf(0.0f, fmaf(cx - ox, nx, fmaf(cy - oy, ny, (cz - oz) * nz)));
+  bf16x8 r;
+  r[0] = (__bf16)((fmaf(tc, nx, ox) - cx) * is);
+  r[1] = (__bf16)((fmaf(tc, ny, oy) - cy) * is);
+  r[2] = (__bf16)((fmaf(tc, nz, oz) - cz) * is);
+  r[3] = (__bf16)nx;
+  r[4] = (__bf16)ny;
+  r[5] = (__bf16)nz;
+  r[6] = (__bf16)(((float)P - 3.5f) * (1.0f / 1.5f));
+  r[7] = (__bf16)(((float)Q - 3.5f) * (1.0f / 1.5f));
+  s_x[row][0] = r;
+  r[0] = (__bf16)((c0.x - cx) * is); r[1] = (__bf16)((c0.y - cy) * is); r[2] = (__bf16)((c0.z - cz) * is);
+  r[3] = (__bf16)((c0.w - cx) * is); r[4] = (__bf16)((c1.x - cy) * is); r[5] = (__bf16)((c1.y - cz) * is);
+  r[6] = (__bf16)((c1.z - cx) * is); r[7] = (__bf16)((c1.w - cy) * is);
+  s_x[row][1] = r;
+  r[0] = (__bf16)((c2.x - cz) * is); r[1] = (__bf16)((c2.y - cx) * is); r[2] = (__bf16)((c2.z - cy) * is);
+  r[3] = (__bf16)((c2.w - cz) * is);
+  r[4] = r[5] = r[6] = r[7] = (__bf16)0.0f;
+  s_x[row][2] = r;
+  r[0] = r[1] = r[2] = r[3] = (__bf16)0.0f;
+  s_x[row][3] = r;
+}
+
+// Seeds for the active lanes of a wave from their feature rows: the active
+// pairs' rows were written compacted (rows 0..k-1, see seed_row), so a wave
+// with k active lanes runs ceil(k/16) pair blocks.  Called by all 64 lanes;
+// u, v are written for active lanes only.
+__device__ __forceinline__ void seed_wave_rows(const bf16x8* __restrict__ tiles, const float* __restrict__ bias,
+                                               bool active, int row, int n_act, float& u, float& v,
+                                               bf16x8 (*s_x)[4], float2* s_logit, int lane) {
+  wave_lds_sync();
+  forward_wave<false>(tiles, bias, s_x, (n_act + 15) >> 4, lane, s_logit, nullptr);
+  wave_lds_sync();
+  if (active) {
+    const float2 lg = s_logit[row];
+    u = fast_sigmoid(lg.x);
+    v = fast_sigmoid(lg.y);
+  }
+  wave_lds_sync();  // the next call's writes come after these reads
+}
+
+// Row of this lane in the compacted layout: active lanes first, in lane
+// order, then the inactive lanes; n_act = number of active lanes.
+__device__ __forceinline__ int seed_row(bool active, int lane, int& n_act) {
+  const unsigned long long bal = __ballot(active);
+  n_act = __popcll(bal);
+  const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  return active ? __popcll(bal & below) : n_act + __popcll(~bal & below);
 }
 
 // Seeds for the active lanes of a wave: active pairs are compacted to the

@@ -209,8 +209,7 @@ __device__ __forceinline__ LeafOut leaf_dispatch(int cls, const float4* net, int
 }
 
 template <int WB, int WSTK, bool PERSISTENT, bool COUNT, bool LEARNED>
-__global__ void __launch_bounds__(WB)
-k_trace_wide(const WArgs* __restrict__ args) {
+__device__ __forceinline__ void trace_wide_body(const WArgs* __restrict__ args) {
   __shared__ int s_code[WSTK][WB];
   __shared__ float s_te[WSTK][WB];
   // learned form: per-wave feature rows and output logits of the seed network
@@ -339,16 +338,19 @@ k_trace_wide(const WArgs* __restrict__ args) {
     in.pu = gu;
     in.pv = gv;
     if (LEARNED) {  // network seed for every pending pair of the wave (matrix cores), plain seed kept as fallback
-      float f[mlp::NIN];
+      auto rows = reinterpret_cast<mlp::bf16x8(*)[4]>(&s_x[tid & ~63][0]);
+      int n_act;
+      const int row = mlp::seed_row(pending, lane, n_act);
       if (pending) {
-        mlp::pair_features(L.o.x, L.o.y, L.o.z, L.d.x, L.d.y, L.d.z, A.bounds + 6 * (int64_t)pi, c0, c1, c2, p, q, f);
+        mlp::store_features(rows, row, L.o.x, L.o.y, L.o.z, L.d.x, L.d.y, L.d.z, A.bounds + 6 * (int64_t)pi, c0, c1,
+                            c2, p, q);
       } else {
+        const mlp::bf16x8 z = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
-        for (int k = 0; k < mlp::NIN; k++) f[k] = 0.0f;
+        for (int k = 0; k < 4; k++) rows[row][k] = z;
       }
-      const int wbase = tid & ~63;
-      mlp::seed_wave(A.mlp_tiles, A.mlp_bias, pending, f, in.gu, in.gv,
-                     reinterpret_cast<mlp::bf16x8(*)[4]>(&s_x[wbase][0]), &s_logit[wbase], lane);
+      mlp::seed_wave_rows(A.mlp_tiles, A.mlp_bias, pending, row, n_act, in.gu, in.gv, rows, &s_logit[tid & ~63],
+                          lane);
     }
     in.seed = L.r + A.seed_base;
     in.max_iter = A.max_iter;
@@ -414,6 +416,21 @@ k_trace_wide(const WArgs* __restrict__ args) {
   }
 }
 
+template <int WB, int WSTK, bool PERSISTENT, bool COUNT, bool LEARNED>
+__global__ void __launch_bounds__(WB) k_trace_wide(const WArgs* __restrict__ args) {
+  trace_wide_body<WB, WSTK, PERSISTENT, COUNT, LEARNED>(args);
+}
+
+// The one-ray-per-lane learned form held to two waves per SIMD (256 registers
+// per lane; left to itself the compiler goes past 256 and runs one wave,
+// at less than half the speed).  The persistent learned form fits two waves
+// without the constraint and is faster without it.
+template <int WB, int WSTK, bool COUNT>
+__global__ void __launch_bounds__(WB) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_trace_wide_learned2(const WArgs* __restrict__ args) {
+  trace_wide_body<WB, WSTK, false, COUNT, true>(args);
+}
+
 }  // namespace
 
 namespace rtn {
@@ -424,10 +441,13 @@ const float* mlp_bias(const rtn_mlp* net);
 namespace {
 template <int WB, int WSTK, bool P, bool C>
 void launch_shape(bool learned, unsigned blocks, hipStream_t stream, const WArgs* a) {
-  if (learned)
-    hipLaunchKernelGGL((k_trace_wide<WB, WSTK, P, C, true>), dim3(blocks), dim3(WB), 0, stream, a);
-  else
+  if (!learned) {
     hipLaunchKernelGGL((k_trace_wide<WB, WSTK, P, C, false>), dim3(blocks), dim3(WB), 0, stream, a);
+  } else if constexpr (P) {
+    hipLaunchKernelGGL((k_trace_wide<WB, WSTK, P, C, true>), dim3(blocks), dim3(WB), 0, stream, a);
+  } else {
+    hipLaunchKernelGGL((k_trace_wide_learned2<WB, WSTK, C>), dim3(blocks), dim3(WB), 0, stream, a);
+  }
 }
 template <int WB, int WSTK>
 void launch_tuned(unsigned blocks, hipStream_t stream, const WArgs* a) {
