@@ -137,6 +137,34 @@ def cpu_baseline(scene_dl, rays_host, target_s=10.0):
                       f"{dt:.1f} s on {cores} threads"}
 
 
+def cpu_reference(scene_dl, rays_host):
+    """The reference's own intersection code (its headers compiled on the host
+    by oracle/build_ref.sh, per-pixel loop of intersect.cu:33-193, its own
+    constants: 7 iterations, tolerance 1e-4), one thread, over the whole
+    batch.  Only for single-degree scenes of degree <= 4, as the reference."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    if not O.ref_available():
+        return None
+    R = O.ref()
+    pq = scene_dl["pq"]
+    p, q = int(pq[0, 0]), int(pq[0, 1])
+    if not (np.all(pq[:, 0] == p) and np.all(pq[:, 1] == q)) or max(p, q) > 4:
+        return None
+    n = len(rays_host[0])
+    f = [np.ascontiguousarray(a, np.float32) for a in rays_host]
+    hit = np.zeros(n, np.int32)
+    outs = [np.zeros(n, np.float32) for _ in range(3)] + [np.zeros(3 * n, np.float32) for _ in range(2)]
+    newton = np.zeros(n, np.int32)
+    cp = np.ascontiguousarray(scene_dl["cp"], np.float32)
+    t0 = time.perf_counter()
+    R.ref_trace(scene_dl["nodes"].ctypes.data, cp, len(pq), p, q, None, 0.0, 0.0, n, *f, hit, *outs, newton)
+    dt = time.perf_counter() - t0
+    return {"value": n / dt / 1e6, "unit": "Mrays/s", "cores": 1, "kind": "reference",
+            "sample": f"all {n} rays, the reference's trace code compiled on the host (its constants: 7 iterations, "
+                      f"tol 1e-4), {dt:.2f} s on one thread"}
+
+
 def load_traffic(form, config):
     tf = os.path.join(ROOT, "profiles", f"traffic_{form}_{config}.json")
     if os.path.exists(tf):
@@ -263,12 +291,18 @@ def bench_c2_c1(args, L, scenes, learned):
     cand = [k for k in results if "mixed" in k or "learned" in k] or list(results)
     head = max(cand, key=lambda k: results[k]["mrays_s"])
     H = results[head]
-    cpu = None
+    cpu, cpu_port = None, None
     if not args.no_cpu:
         try:
             dl = scene.download()
-            cpu = cpu_baseline(dl, [np.asarray(a) for a in rays.get()])
-            log(f"cpu baseline: {cpu['value']:.3f} Mrays/s on {cpu['cores']} threads")
+            rh = [np.asarray(a) for a in rays.get()]
+            cpu_port = cpu_baseline(dl, rh)
+            log(f"cpu baseline (port): {cpu_port['value']:.3f} Mrays/s on {cpu_port['cores']} threads")
+            cpu = cpu_reference(dl, rh) if args.config == "c1" else None
+            if cpu is not None:
+                log(f"cpu baseline (reference): {cpu['value']:.3f} Mrays/s on {cpu['cores']} thread")
+            else:
+                cpu = cpu_port
         except Exception as e:  # the baseline is a report, never the measurement
             log(f"cpu baseline failed: {e!r}")
     plain = results.get("plain", {}).get("mrays_s")
@@ -276,7 +310,8 @@ def bench_c2_c1(args, L, scenes, learned):
              "speedup_vs_plain": round(H["mrays_s"] / plain, 3) if plain else None,
              "speedups_vs_plain": {k: round(v["mrays_s"] / plain, 3) for k, v in results.items()} if plain else None,
              "forms": results, "seed_network": train, "seed_ablation": ablation,
-             "roofline": roofline(H["tflops"], head, args.config), "cpu_baseline": cpu}
+             "roofline": roofline(H["tflops"], head, args.config), "cpu_baseline": cpu,
+             "cpu_port_baseline": cpu_port if cpu is not cpu_port else None}
     wl = {"workload": workload, "rays": rays.n, "patches": int(st["n_patches"]), "headline_form": head}
     return make_line(args, H["mrays_s"], H["kernel_ms"], "f32", wl, extra)
 
@@ -330,11 +365,12 @@ def bench_c5(args, L, scenes):
     proxy, _, _ = frame_for(4)
     table = proxy.autotune(reps=2)
     best = min((e for e in table if e["ms"] > 0), key=lambda e: e["ms"])
-    default = [e for e in table if (e["lanes"], e["stack"], e["blocks_per_cu"]) == (128, 20, 8)]
+    default = [e for e in table if (e["lanes"], e["stack"], e["blocks_per_cu"], e["refill_min"]) == (128, 20, 8, 64)]
     log(f"autotune (1/16 of the pixels): best {best}, default {default}")
     del proxy
     frame, rays, ids = frame_for(1)
     frame.set_shape(best["lanes"], best["stack"], best["blocks_per_cu"])
+    frame.set_refill(best["refill_min"])
     stream = L.stream_create()
     frame.capture(stream)
     for _ in range(args.warmup):
@@ -355,7 +391,7 @@ def bench_c5(args, L, scenes):
                       "+ shadow and mirror ray per hit, one graph-captured frame",
           "primary_rays": st["n_primary"], "secondary_rays": st["n_secondary"], "patches": n_patches,
           "launch_shape": {"lanes": st["shape_lanes"], "stack": st["shape_stack"],
-                           "blocks_per_cu": st["shape_blocks_per_cu"]}}
+                           "blocks_per_cu": st["shape_blocks_per_cu"], "refill_min": st["refill_min"]}}
     return make_line(args, total_rays / ms / 1e3, ms, "f32", wl,
                      {"autotune": table, "roofline": None, "cpu_baseline": None,
                       "note": "roofline not priced for the frame (no per-ray counters in the captured launches)"})

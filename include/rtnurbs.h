@@ -201,7 +201,10 @@ typedef struct {
   int learned_iters;      /* learned seed: Newton iterations from the network's seed before the
                              plain-seed fallback (0 = max_iter)                  */
   int any_hit;            /* fast forms: a ray ends at its first converged hit (shadow rays) */
-  int reserved[2];
+  int refill_min;         /* persistent form: a wave takes new rays once this many of its 64 lanes
+                             are idle (0 = 64: whole-wave refill, the fastest for coherent
+                             primary rays; smaller values suit incoherent rays)     */
+  int reserved[1];
   const struct rtn_mlp* seed_net; /* RTN_SEED_LEARNED: the seed network          */
 } rtn_trace_params;
 
@@ -344,9 +347,9 @@ typedef struct {
   int64_t n_primary, n_secondary;  /* rays; n_secondary counts non-empty slots   */
   int64_t n_secondary_slots;       /* (shadow + mirror) x n_primary              */
   unsigned long long overflow;  /* rays whose traversal stack overflowed        */
-  int shape_lanes, shape_stack, shape_blocks_per_cu, captured;
+  int shape_lanes, shape_stack, shape_blocks_per_cu, captured, refill_min, reserved;
 } rtn_frame_status_t;
-typedef struct { int lanes, stack, blocks_per_cu; float ms; } rtn_tune_entry;
+typedef struct { int lanes, stack, blocks_per_cu, refill_min; float ms; } rtn_tune_entry;
 
 void rtn_frame_params_default(rtn_frame_params* fp);
 int rtn_frame_create(rtn_frame** frame, const rtn_batch* batch, const float* const rays_soa[6], const int* ray_scene,
@@ -362,10 +365,13 @@ int rtn_frame_results(const rtn_frame* frame, rtn_hits* primary, rtn_hits* secon
  * entries per lane (16/20/24/32; LDS = 8 B x entries x lanes), resident blocks
  * per compute unit.  Drops a captured graph. */
 int rtn_frame_set_shape(rtn_frame* frame, int lanes, int stack, int blocks_per_cu);
+/* Refill threshold of both persistent launches (see rtn_trace_params.refill_min). */
+int rtn_frame_set_refill(rtn_frame* frame, int refill_min);
 /* Autotuner: times the frame over every built launch shape at 2, 4 and 8
- * blocks per CU (reps timed runs each, stack overflow rejects a shape) and
- * keeps the fastest.  table (optional) receives each candidate, ms < 0 for
- * rejected ones.  Drops a captured graph. */
+ * blocks per CU (whole-wave refill; reps timed runs each, stack overflow
+ * rejects a shape), then the best shape at refill thresholds 1, 16 and 32
+ * idle lanes, and keeps the fastest.  table (optional) receives each
+ * candidate, ms < 0 for rejected ones.  Drops a captured graph. */
 int rtn_frame_autotune(rtn_frame* frame, int reps, rtn_tune_entry* table, int table_cap, int* n_tried,
                        void* stream);
 

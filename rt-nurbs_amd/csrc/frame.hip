@@ -226,6 +226,7 @@ struct rtn_frame {
   DevBuf<float> s_ray[6];
   DevBuf<FrameCtl> ctl;
   LaunchShape shape = kDefaultShape;
+  int refill = 64;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   ~rtn_frame() {
@@ -256,6 +257,14 @@ int frame_enqueue(rtn_frame* f, hipStream_t st) {
     RTN_HIP(hipGetLastError());
     RTN_TRY(launch_wide_kernel(f->shape, true, false, learned, f->cap2, st, &c->a2));
   }
+  return RTN_OK;
+}
+
+int frame_set_refill(rtn_frame* f, int refill) {
+  const int v = refill > 0 ? refill : 64;
+  RTN_HIP(hipMemcpy(&f->ctl.ptr->a1.refill_min, &v, sizeof(int), hipMemcpyHostToDevice));
+  RTN_HIP(hipMemcpy(&f->ctl.ptr->a2.refill_min, &v, sizeof(int), hipMemcpyHostToDevice));
+  f->refill = v;
   return RTN_OK;
 }
 
@@ -379,6 +388,7 @@ extern "C" int rtn_frame_status(const rtn_frame* f, rtn_frame_status_t* out) {
   out->shape_stack = f->shape.wstk;
   out->shape_blocks_per_cu = f->shape.blocks_per_cu;
   out->captured = f->exec != nullptr;
+  out->refill_min = f->refill;
   return RTN_OK;
 }
 
@@ -404,69 +414,96 @@ extern "C" int rtn_frame_set_shape(rtn_frame* f, int lanes, int stack, int block
   return RTN_OK;
 }
 
+extern "C" int rtn_frame_set_refill(rtn_frame* f, int refill_min) {
+  RTN_CHECK(f && refill_min >= 0 && refill_min <= 64, RTN_EINVAL, "rtn_frame_set_refill: bad arguments");
+  drop_graph(f);
+  return frame_set_refill(f, refill_min);
+}
+
 // Autotuner: every built launch shape (lanes per block x stack entries) at
-// 2, 4 and 8 resident blocks per compute unit; each candidate runs the frame
-// once to warm up and `reps` times under HIP events; candidates whose stack
-// overflows are rejected.  The fastest shape is kept (and any graph dropped).
+// 2, 4 and 8 resident blocks per compute unit with whole-wave refill, then
+// the best shape at refill thresholds 1, 16 and 32; each candidate runs the
+// frame once to warm up and `reps` times under HIP events; candidates whose
+// stack overflows are rejected.  The fastest is kept (and any graph dropped).
 extern "C" int rtn_frame_autotune(rtn_frame* f, int reps, rtn_tune_entry* table, int table_cap, int* n_tried,
                                   void* stream) {
   RTN_CHECK(f && reps >= 1, RTN_EINVAL, "rtn_frame_autotune: bad arguments");
   hipStream_t st = (hipStream_t)stream;
   drop_graph(f);
-  std::vector<LaunchShape> cand;
+  struct Cand { LaunchShape sh; int refill; };
+  std::vector<Cand> cand;
   const int bpcs[3] = {2, 4, 8};
   const bool learned = frame_learned(f);
   for (int b = 0; b < 3; b++) {
-    cand.push_back({kDefaultShape.wb, kDefaultShape.wstk, bpcs[b]});
+    cand.push_back({{kDefaultShape.wb, kDefaultShape.wstk, bpcs[b]}, 64});
     if (!learned)
-      for (int i = 0; i < kNumTunedShapes; i++) cand.push_back({kTunedShapes[i].wb, kTunedShapes[i].wstk, bpcs[b]});
+      for (int i = 0; i < kNumTunedShapes; i++)
+        cand.push_back({{kTunedShapes[i].wb, kTunedShapes[i].wstk, bpcs[b]}, 64});
   }
   hipEvent_t e0, e1;
   RTN_HIP(hipEventCreate(&e0));
   RTN_HIP(hipEventCreate(&e1));
-  LaunchShape best = f->shape;
+  Cand best{f->shape, f->refill};
   float best_ms = INFINITY;
   int tried = 0;
   int rc = RTN_OK;
-  for (const LaunchShape& sh : cand) {
-    f->shape = sh;
-    if ((rc = frame_enqueue(f, st))) break;
+  auto run_one = [&](const Cand& c) -> float {
+    f->shape = c.sh;
+    if ((rc = frame_set_refill(f, c.refill))) return -2.f;
+    if ((rc = frame_enqueue(f, st))) return -2.f;
     FrameCtl h;
     if (hipStreamSynchronize(st) != hipSuccess ||
         hipMemcpy(&h, f->ctl.ptr, sizeof(h), hipMemcpyDeviceToHost) != hipSuccess) {
       rc = RTN_EHIP;
-      break;
+      return -2.f;
     }
-    const bool ok = h.totals[6] == 0;
-    float ms = INFINITY;
-    if (ok) {
-      (void)hipEventRecord(e0, st);
-      for (int r = 0; r < reps && rc == RTN_OK; r++) rc = frame_enqueue(f, st);
-      (void)hipEventRecord(e1, st);
-      if (rc || hipEventSynchronize(e1) != hipSuccess) {
-        rc = rc ? rc : RTN_EHIP;
-        break;
-      }
-      (void)hipEventElapsedTime(&ms, e0, e1);
-      ms /= (float)reps;
-      if (ms < best_ms) {
-        best_ms = ms;
-        best = sh;
-      }
+    if (h.totals[6] != 0) return -1.f;  // stack overflow: rejected
+    (void)hipEventRecord(e0, st);
+    for (int r = 0; r < reps && rc == RTN_OK; r++) rc = frame_enqueue(f, st);
+    (void)hipEventRecord(e1, st);
+    if (rc || hipEventSynchronize(e1) != hipSuccess) {
+      rc = rc ? rc : RTN_EHIP;
+      return -2.f;
     }
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    return ms / (float)reps;
+  };
+  auto record = [&](const Cand& c, float ms) {
     if (table && tried < table_cap) {
-      table[tried].lanes = sh.wb;
-      table[tried].stack = sh.wstk;
-      table[tried].blocks_per_cu = sh.blocks_per_cu;
-      table[tried].ms = ok ? ms : -1.0f;
+      table[tried].lanes = c.sh.wb;
+      table[tried].stack = c.sh.wstk;
+      table[tried].blocks_per_cu = c.sh.blocks_per_cu;
+      table[tried].refill_min = c.refill;
+      table[tried].ms = ms;
     }
     tried++;
+    if (ms > 0.f && ms < best_ms) {
+      best_ms = ms;
+      best = c;
+    }
+  };
+  for (const Cand& c : cand) {
+    const float ms = run_one(c);
+    if (ms < -1.5f) break;
+    record(c, ms);
+  }
+  if (rc == RTN_OK && best_ms < INFINITY) {
+    const LaunchShape sh = best.sh;
+    for (int r : {1, 16, 32}) {
+      const Cand c{sh, r};
+      const float ms = run_one(c);
+      if (ms < -1.5f) break;
+      record(c, ms);
+    }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  f->shape = best;
+  f->shape = best.sh;
+  const int rc2 = frame_set_refill(f, best.refill);
   if (n_tried) *n_tried = tried;
   RTN_TRY(rc);
+  RTN_TRY(rc2);
   RTN_CHECK(best_ms < INFINITY, RTN_EOVERFLOW, "rtn_frame_autotune: every launch shape overflowed its stack");
   return RTN_OK;
 }

@@ -37,6 +37,7 @@ extern "C" int rtn_trace(const rtn_scene* s, const float* const rays[6], int64_t
   RTN_CHECK(n_rays >= 0 && n_rays < ((int64_t)1 << 32), RTN_EINVAL, "rtn_trace: bad ray count");
   RTN_CHECK(prm->max_iter >= 1 && prm->max_iter <= 64, RTN_EINVAL, "rtn_trace: max_iter outside 1..64");
   RTN_CHECK(prm->hull_cull >= 0 && prm->hull_cull <= 2, RTN_EINVAL, "rtn_trace: hull_cull outside 0..2");
+  RTN_CHECK(prm->refill_min >= 0 && prm->refill_min <= 64, RTN_EINVAL, "rtn_trace: refill_min outside 0..64");
   RTN_CHECK(!prm->count || hits->counters, RTN_EINVAL, "rtn_trace: count requested without a counter buffer");
   if (n_rays == 0) return RTN_OK;
   hipStream_t st = (hipStream_t)stream;

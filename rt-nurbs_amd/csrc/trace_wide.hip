@@ -239,8 +239,9 @@ __device__ __forceinline__ void trace_wide_body(const WArgs* __restrict__ args) 
     if (PERSISTENT && !exhausted) {
       // lanes without a ray take consecutive rays from the global counter,
       // one atomic per wave
+      // a wave takes new rays once `refill_min` of its lanes are idle (or all of them)
       const unsigned long long need = __ballot(!has_ray);
-      if (need) {
+      if (need && (__popcll(need) >= A.refill_min || need == ~0ull)) {
         const int cnt = __popcll(need);
         const int leader = __ffsll((long long)need) - 1;
         unsigned int base = 0;
@@ -528,6 +529,7 @@ int fill_wide_args(const rtn_scene* s, const float* const rays[6], int64_t n, co
   A.mixed = prm->precision == RTN_PREC_MIXED;
   A.cond_thr = prm->cond_thr;
   A.any_hit = prm->any_hit;
+  A.refill_min = prm->refill_min > 0 ? prm->refill_min : 64;
   if (learned) {
     A.mlp_tiles = reinterpret_cast<const mlp::bf16x8*>(mlp_tiles(prm->seed_net));
     A.mlp_bias = mlp_bias(prm->seed_net);

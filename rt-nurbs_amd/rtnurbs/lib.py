@@ -39,7 +39,7 @@ EXPORTS = [
     "rtn_shade_lambert", "rtn_build_params_default", "rtn_scene_build_ex", "rtn_batch_create", "rtn_batch_destroy",
     "rtn_batch_patch_base", "rtn_batch_trace", "rtn_frame_params_default", "rtn_frame_create", "rtn_frame_destroy",
     "rtn_frame_run", "rtn_frame_capture", "rtn_frame_status", "rtn_frame_results", "rtn_frame_set_shape",
-    "rtn_frame_autotune", "rtn_solve_pairs",
+    "rtn_frame_autotune", "rtn_solve_pairs", "rtn_frame_set_refill",
 ]
 BVH_EXACT_SAH, BVH_BINNED_SAH = 0, 1
 MLP_NIN, MLP_HIDDEN, MLP_NPARAM = 20, 64, 9794
@@ -62,7 +62,8 @@ class TraceParams(C.Structure):
                 ("flat_threshold", C.c_float), ("cam_k", C.c_float), ("seed_base", C.c_uint32),
                 ("seed_mode", C.c_int), ("precision", C.c_int), ("cond_thr", C.c_float),
                 ("hull_cull", C.c_int), ("count", C.c_int), ("learned_iters", C.c_int),
-                ("any_hit", C.c_int), ("reserved", C.c_int * 2), ("seed_net", C.c_void_p)]
+                ("any_hit", C.c_int), ("refill_min", C.c_int), ("reserved", C.c_int * 1),
+                ("seed_net", C.c_void_p)]
 
 
 class MlpTrainParams(C.Structure):
@@ -84,11 +85,12 @@ class FrameStatus(C.Structure):
     _fields_ = [("n_primary", C.c_int64), ("n_secondary", C.c_int64), ("n_secondary_slots", C.c_int64),
                 ("overflow", C.c_ulonglong),
                 ("shape_lanes", C.c_int), ("shape_stack", C.c_int), ("shape_blocks_per_cu", C.c_int),
-                ("captured", C.c_int)]
+                ("captured", C.c_int), ("refill_min", C.c_int), ("reserved", C.c_int)]
 
 
 class TuneEntry(C.Structure):
-    _fields_ = [("lanes", C.c_int), ("stack", C.c_int), ("blocks_per_cu", C.c_int), ("ms", C.c_float)]
+    _fields_ = [("lanes", C.c_int), ("stack", C.c_int), ("blocks_per_cu", C.c_int), ("refill_min", C.c_int),
+                ("ms", C.c_float)]
 
 
 class Hits(C.Structure):
@@ -157,6 +159,7 @@ def load():
     L.rtn_frame_results.argtypes = [vp, C.POINTER(Hits), C.POINTER(Hits), C.POINTER(vp), C.POINTER(vp),
                                     C.POINTER(vp * 6)]
     L.rtn_frame_set_shape.argtypes = [vp, i, i, i]
+    L.rtn_frame_set_refill.argtypes = [vp, i]
     L.rtn_frame_autotune.argtypes = [vp, i, C.POINTER(TuneEntry), i, C.POINTER(i), vp]
     L.rtn_solve_pairs.argtypes = [vp, C.POINTER(vp * 6), i64, vp, vp, vp, i64, C.POINTER(TraceParams), vp, vp, vp,
                                   vp, vp]
@@ -749,11 +752,15 @@ class Frame:
     def set_shape(self, lanes, stack, blocks_per_cu):
         check(load().rtn_frame_set_shape(self.h, lanes, stack, blocks_per_cu))
 
+    def set_refill(self, refill_min):
+        check(load().rtn_frame_set_refill(self.h, refill_min))
+
     def autotune(self, reps=2, stream=None):
         table = (TuneEntry * 64)()
         n = C.c_int()
         check(load().rtn_frame_autotune(self.h, reps, table, 64, C.byref(n), stream))
-        return [dict(lanes=e.lanes, stack=e.stack, blocks_per_cu=e.blocks_per_cu, ms=e.ms) for e in table[:n.value]]
+        return [dict(lanes=e.lanes, stack=e.stack, blocks_per_cu=e.blocks_per_cu, refill_min=e.refill_min, ms=e.ms)
+                for e in table[:n.value]]
 
     def results(self):
         """Host copies: primary hits, secondary hits (first n_secondary), parent, kind."""
