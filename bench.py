@@ -254,7 +254,9 @@ def bench_c2_c1(args, L, scenes, learned):
     log(f"scene: {st['n_patches']} patches, {st['n_nodes']} nodes, depth {st['max_depth']}, "
         f"build {time.perf_counter() - t0:.2f}s (decompose {st['build_ms_decompose']:.1f} ms, "
         f"bvh {st['build_ms_bvh']:.1f} ms, wide {st['build_ms_wide']:.1f} ms)")
-    rays = L.generate_rays(sc.camera, spp=spp, jitter=jitter, seed=1)
+    # tiled ray layout (8x8-pixel Morton tiles): the same camera rays, ordered so
+    # that a wave's 64 rays cover a compact pixel block; every form gets it
+    rays = L.generate_rays(sc.camera, spp=spp, jitter=jitter, seed=1, tiled=True)
     stream = L.stream_create()
     net, train = None, None
     if args.train_epochs > 0:
@@ -357,7 +359,7 @@ def bench_c5(args, L, scenes):
         rays = L.Rays(sum(counts), with_idx=False)
         off = 0
         for cam, nr in zip(cams, counts):
-            L.generate_rays(cam, spp=4, jitter=True, seed=1, rays=rays, offset=off)
+            L.generate_rays(cam, spp=4, jitter=True, seed=1, rays=rays, offset=off, tiled=True)
             off += nr
         ids = L.DeviceArray.from_host(np.repeat(np.arange(len(items), dtype=np.int32), counts))
         return L.Frame(batch, rays, ids, L.frame_params(**BASE, hull_cull=2)), rays, ids

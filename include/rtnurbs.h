@@ -94,12 +94,19 @@ int rtn_camera_perspective(const float eye[3], const float look[3], const float 
                            float camera_from_raster[16]);
 
 /* calculateRaysWrapper<perspective> (generateRaysKernel.cu:39-67; ray math
- * camera.h:162-174).  Writes W*H*spp rays into the device SoA queue, ray
- * index = (y*W + x)*spp + s, and idx_interact = -1.  jitter = 0: pixel
- * centres (x+0.5, y+0.5), the reference's rays; jitter = 1: per-ray raster
- * offsets from rtn_jitter (an extension for multi-sample configurations). */
+ * camera.h:162-174).  Writes W*H*spp rays into the device SoA queue and
+ * idx_interact = -1.  flags 0: pixel centres (x+0.5, y+0.5), ray index
+ * (y*W + x)*spp + s, the reference's rays.  RTN_RAYS_JITTER: per-ray raster
+ * offsets from a seeded hash of the ray index (multi-sample configurations).
+ * RTN_RAYS_TILED: ray index = (tiled position of the pixel)*spp + s (see the
+ * enum), the same rays in a layout that keeps a wave's rays together. */
+enum {
+  RTN_RAYS_JITTER = 1,  /* jittered sample positions (else pixel centres)                        */
+  RTN_RAYS_TILED = 2    /* ray layout by 8x8-pixel tiles (row-major tiles, Morton order inside),
+                           so a wave's 64 rays cover a compact pixel block; W, H multiples of 8   */
+};
 int rtn_rays_perspective(const float world_from_camera[16], const float camera_from_raster[16], int W, int H,
-                         int spp, int jitter, uint32_t jitter_seed, const rtn_ray_queue* rays, void* stream);
+                         int spp, int flags, uint32_t jitter_seed, const rtn_ray_queue* rays, void* stream);
 
 /* ----------------------------------------------------------- scene (GPU)
  * Scene assembly for the trace path: NURBS surfaces are decomposed on the GPU
@@ -311,7 +318,7 @@ int rtn_rays_secondary(const float* const rays_soa[6], const rtn_hits* hits, int
  * shaders, testShading.cu:9-55): pixel = mean over its spp consecutive rays of
  * albedo * max(0, N.L), N = unit(dpdu x dpdv) facing the ray (benchmark.cpp:91-92),
  * 0 on a miss or where shadow_hit[r] != 0 (optional).  image: device float[W*H]. */
-int rtn_shade_lambert(const float* const rays_soa[6], const rtn_hits* hits, int W, int H, int spp,
+int rtn_shade_lambert(const float* const rays_soa[6], const rtn_hits* hits, int W, int H, int spp, int ray_flags,
                       const float light[3], float albedo, const int* shadow_hit, float* image, void* stream);
 
 /* ---------------------------------------------- multi-scene batches, frames

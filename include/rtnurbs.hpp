@@ -46,7 +46,7 @@ inline void calculateIntersectWrapper(rtn_ray_queue& rays, rtn_bvh_node* nodes, 
  * __constant__ memory, generateRaysKernel.cu:42), resolution W x H. */
 inline void calculateRaysWrapper(rtn_ray_queue& rays, const float world_from_camera[16],
                                  const float camera_from_raster[16], int W, int H) {
-  report(rtn_rays_perspective(world_from_camera, camera_from_raster, W, H, 1, 0, 0, &rays, nullptr),
+  report(rtn_rays_perspective(world_from_camera, camera_from_raster, W, H, 1, 0, 0u, &rays, nullptr),
          "calculateRaysWrapper");
 }
 

@@ -202,13 +202,19 @@ struct CamPayload {  // camera_gpu_payload (camera.h:32-35)
 // camera.h:162-174).  One thread per ray, 1D over W*H*spp; rays of a pixel are
 // consecutive.  The camera travels as a kernel argument (SGPR-resident)
 // instead of a process-global __constant__ symbol, so calls are re-entrant.
-__global__ void __launch_bounds__(256) k_rays_perspective(CamPayload cam, int W, int H, int spp, int jit, uint32_t seed,
-                                                          rtn_ray_queue q) {
+__global__ void __launch_bounds__(256) k_rays_perspective(CamPayload cam, int W, int H, int spp, int jit, int tiled,
+                                                          uint32_t seed, rtn_ray_queue q) {
   const int64_t n = (int64_t)W * H * spp;
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
   const int64_t pix = r / spp;
-  const int x = (int)(pix % W), y = (int)(pix / W);
+  int x, y;
+  if (tiled) {
+    rtn::tiled_pixel(pix, W, x, y);
+  } else {
+    x = (int)(pix % W);
+    y = (int)(pix / W);
+  }
   const float ox = jit ? ex::jitter((uint32_t)r, seed, 0) : 0.5f;
   const float oy = jit ? ex::jitter((uint32_t)r, seed, 1) : 0.5f;
   const float e0 = float(x) + ox, e1 = float(y) + oy, e2 = 0.0f, e3 = 1.0f;
@@ -230,9 +236,13 @@ __global__ void __launch_bounds__(256) k_rays_perspective(CamPayload cam, int W,
   if (q.idx_interact) q.idx_interact[r] = -1;
 }
 
-extern "C" int rtn_rays_perspective(const float wfc[16], const float cfr[16], int W, int H, int spp, int jitter,
+extern "C" int rtn_rays_perspective(const float wfc[16], const float cfr[16], int W, int H, int spp, int flags,
                                     uint32_t jitter_seed, const rtn_ray_queue* rays, void* stream) {
   RTN_CHECK(wfc && cfr && rays && W > 0 && H > 0 && spp > 0, RTN_EINVAL, "rtn_rays_perspective: bad arguments");
+  RTN_CHECK((flags & ~(RTN_RAYS_JITTER | RTN_RAYS_TILED)) == 0, RTN_EINVAL, "rtn_rays_perspective: unknown flags");
+  const int jitter = (flags & RTN_RAYS_JITTER) ? 1 : 0, tiled = (flags & RTN_RAYS_TILED) ? 1 : 0;
+  RTN_CHECK(!tiled || (W % 8 == 0 && H % 8 == 0), RTN_EINVAL,
+            "rtn_rays_perspective: the tiled layout needs W and H multiples of 8");
   RTN_CHECK(rays->origin_x && rays->origin_y && rays->origin_z && rays->direction_x && rays->direction_y &&
                 rays->direction_z,
             RTN_EINVAL, "rtn_rays_perspective: null ray buffer");
@@ -243,7 +253,7 @@ extern "C" int rtn_rays_perspective(const float wfc[16], const float cfr[16], in
   memcpy(cam.cfr, cfr, 64);
   const int64_t blocks = (n + 255) / 256;
   hipLaunchKernelGGL(k_rays_perspective, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, cam, W, H, spp,
-                     jitter, jitter_seed, *rays);
+                     jitter, tiled, jitter_seed, *rays);
   RTN_HIP(hipGetLastError());
   if (!stream) RTN_HIP(hipStreamSynchronize(nullptr));
   return RTN_OK;

@@ -136,10 +136,16 @@ __global__ void k_secondaries(const float* ox, const float* oy, const float* oz,
 // Lambert shading of W*H pixels with spp consecutive rays each:
 // albedo * max(0, N.L) on hits (N facing the ray), 0 on misses; spp mean.
 __global__ void k_shade_lambert(const float* dx, const float* dy, const float* dz, const int* prim,
-                                const float* dpdu, const float* dpdv, int64_t n_pix, int spp, float lx, float ly,
-                                float lz, float albedo, const int* shadow_hit, float* out) {
+                                const float* dpdu, const float* dpdv, int64_t n_pix, int W, int spp, int tiled,
+                                float lx, float ly, float lz, float albedo, const int* shadow_hit, float* out) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n_pix) return;
+  int64_t pix = p;  // image index of ray group p
+  if (tiled) {
+    int x, y;
+    rtn::tiled_pixel(p, W, x, y);
+    pix = (int64_t)y * W + x;
+  }
   float acc = 0.0f;
   for (int s = 0; s < spp; s++) {
     const int64_t r = p * spp + s;
@@ -148,7 +154,7 @@ __global__ void k_shade_lambert(const float* dx, const float* dy, const float* d
     const ex::v3<float> N = facing_normal(dpdu, dpdv, r, ex::v3<float>{dx[r], dy[r], dz[r]});
     acc += albedo * fmaxf(0.0f, N.x * lx + N.y * ly + N.z * lz);
   }
-  out[p] = acc / (float)spp;
+  out[pix] = acc / (float)spp;
 }
 
 }  // namespace
@@ -219,16 +225,18 @@ extern "C" int rtn_rays_secondary(const float* const rays[6], const rtn_hits* hi
 }
 
 extern "C" int rtn_shade_lambert(const float* const rays[6], const rtn_hits* hits, int W, int H, int spp,
-                                 const float light[3], float albedo, const int* shadow_hit, float* image,
-                                 void* stream) {
+                                 int ray_flags, const float light[3], float albedo, const int* shadow_hit,
+                                 float* image, void* stream) {
   RTN_CHECK(rays && hits && hits->prim && hits->dpdu && hits->dpdv && light && image && W > 0 && H > 0 && spp > 0,
             RTN_EINVAL, "rtn_shade_lambert: bad arguments");
+  const int tiled = (ray_flags & RTN_RAYS_TILED) ? 1 : 0;
+  RTN_CHECK(!tiled || (W % 8 == 0 && H % 8 == 0), RTN_EINVAL, "rtn_shade_lambert: tiled layout needs W, H % 8 == 0");
   const float ll = std::sqrt(light[0] * light[0] + light[1] * light[1] + light[2] * light[2]);
   RTN_CHECK(ll > 0.f, RTN_EINVAL, "rtn_shade_lambert: zero light direction");
   const int64_t np = (int64_t)W * H;
   hipLaunchKernelGGL(k_shade_lambert, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, (hipStream_t)stream, rays[3],
-                     rays[4], rays[5], hits->prim, hits->dpdu, hits->dpdv, np, spp, light[0] / ll, light[1] / ll,
-                     light[2] / ll, albedo, shadow_hit, image);
+                     rays[4], rays[5], hits->prim, hits->dpdu, hits->dpdv, np, W, spp, tiled, light[0] / ll,
+                     light[1] / ll, light[2] / ll, albedo, shadow_hit, image);
   RTN_HIP(hipGetLastError());
   if (!stream) RTN_HIP(hipStreamSynchronize(nullptr));
   return RTN_OK;

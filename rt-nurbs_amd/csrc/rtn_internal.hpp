@@ -90,6 +90,19 @@ struct alignas(16) WideNode {
 };
 static_assert(sizeof(WideNode) == 128, "wide node is 128 bytes");
 
+// Pixel of position p in the tiled ray layout: 8x8-pixel tiles in row-major
+// tile order, Morton (Z) order inside a tile, so 16 consecutive pixels form a
+// 4x4 block (W a multiple of 8).
+__host__ __device__ inline void tiled_pixel(int64_t p, int W, int& x, int& y) {
+  const int64_t tile = p >> 6;
+  const int in = (int)(p & 63);
+  const int tiles_x = W >> 3;
+  const int mx = (in & 1) | ((in >> 1) & 2) | ((in >> 2) & 4);
+  const int my = ((in >> 1) & 1) | ((in >> 2) & 2) | ((in >> 3) & 4);
+  x = (int)(tile % tiles_x) * 8 + mx;
+  y = (int)(tile / tiles_x) * 8 + my;
+}
+
 // Training-pair output of the wide kernels (converged plain-method solves).
 struct PairOut {
   int* ray;

@@ -196,7 +196,7 @@ def load():
     L.rtn_rays_silhouette.argtypes = [vp, i64, C.c_uint32, f, f, f, C.POINTER(RayQueue), vp, vp, vp]
     L.rtn_rays_secondary.argtypes = [C.POINTER(vp * 6), C.POINTER(Hits), i64, vp, f, i, i, C.POINTER(RayQueue), vp,
                                      vp, i64, C.POINTER(i64), vp]
-    L.rtn_shade_lambert.argtypes = [C.POINTER(vp * 6), C.POINTER(Hits), i, i, i, vp, f, vp, vp, vp]
+    L.rtn_shade_lambert.argtypes = [C.POINTER(vp * 6), C.POINTER(Hits), i, i, i, i, vp, f, vp, vp, vp]
     _lib = L
     return L
 
@@ -328,9 +328,12 @@ class Rays:
         return r
 
 
-def generate_rays(cam, spp=1, jitter=False, seed=0, stream=None, rays=None, offset=0):
-    """Pinhole rays of a camera (W*H*spp, samples of a pixel consecutive);
-    written at rays[offset:] when rays is given."""
+RAYS_JITTER, RAYS_TILED = 1, 2
+
+
+def generate_rays(cam, spp=1, jitter=False, seed=0, stream=None, rays=None, offset=0, tiled=False):
+    """Pinhole rays of a camera (W*H*spp, samples of a pixel consecutive, pixels
+    row-major or in 8x8 Morton tiles); written at rays[offset:] when rays is given."""
     wfc, cfr = camera_matrices(cam)
     n = cam.W * cam.H * spp
     if rays is None:
@@ -338,7 +341,8 @@ def generate_rays(cam, spp=1, jitter=False, seed=0, stream=None, rays=None, offs
         offset = 0
     assert offset + n <= rays.n
     q = RayQueue(*[c.ptr + 4 * offset for c in rays.comp], rays.idx.ptr + 4 * offset if rays.idx else None)
-    check(load().rtn_rays_perspective(wfc.ctypes.data, cfr.ctypes.data, cam.W, cam.H, spp, int(jitter), seed,
+    flags = (RAYS_JITTER if jitter else 0) | (RAYS_TILED if tiled else 0)
+    check(load().rtn_rays_perspective(wfc.ctypes.data, cfr.ctypes.data, cam.W, cam.H, spp, flags, seed,
                                       C.byref(q), stream))
     return rays
 
@@ -661,13 +665,13 @@ def secondary_rays(rays, hits, light, eps=1e-4, shadow=True, mirror=True):
     return out, parent, kind, cnt.value
 
 
-def shade_lambert(rays, hits, W, H, spp, light, albedo=0.8, shadow_hit=None):
+def shade_lambert(rays, hits, W, H, spp, light, albedo=0.8, shadow_hit=None, tiled=False):
     img = DeviceArray((H, W), np.float32)
     lt = np.asarray(light, np.float32)
     hs = hits.struct()
     arr = rays.ptr_array()
-    check(load().rtn_shade_lambert(C.byref(arr), C.byref(hs), W, H, spp, lt.ctypes.data, albedo,
-                                   shadow_hit.ptr if shadow_hit is not None else None, img.ptr, None))
+    check(load().rtn_shade_lambert(C.byref(arr), C.byref(hs), W, H, spp, RAYS_TILED if tiled else 0, lt.ctypes.data,
+                                   albedo, shadow_hit.ptr if shadow_hit is not None else None, img.ptr, None))
     return img
 
 
