@@ -211,7 +211,9 @@ typedef struct {
   int refill_min;         /* persistent form: a wave takes new rays once this many of its 64 lanes
                              are idle (0 = 64: whole-wave refill, the fastest for coherent
                              primary rays; smaller values suit incoherent rays)     */
-  int reserved[1];
+  int fallback_iters;     /* learned seed: Newton iterations of the plain-seed fallback attempt
+                             (0 = max_iter; max_iter - learned_iters keeps the per-pair budget
+                             of the plain method)                                 */
   const struct rtn_mlp* seed_net; /* RTN_SEED_LEARNED: the seed network          */
 } rtn_trace_params;
 

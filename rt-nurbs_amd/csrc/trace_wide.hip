@@ -149,7 +149,7 @@ struct LeafIn {
   f3 o, d;
   float gu, gv, pu, pv;
   uint32_t seed;
-  int max_iter, learned_iters, hull_cull, mixed, learned;
+  int max_iter, learned_iters, fallback_iters, hull_cull, mixed, learned;
   float thr_det, thr_res, cull_margin, cond_thr;
 };
 
@@ -172,7 +172,7 @@ __device__ __forceinline__ LeafOut leaf_q(const float4* __restrict__ net, int P,
 #pragma unroll 1
   for (int attempt = 0;; attempt++) {
     h = newton_q<Q>(net, P, in.rp, su, sv, in.thr_det, in.thr_res, in.seed,
-                    attempt == 0 && in.learned ? in.learned_iters : in.max_iter);
+                    !in.learned ? in.max_iter : attempt == 0 ? in.learned_iters : in.fallback_iters);
     o.evals += h.evals;
     if (h.hit || !in.learned || attempt == 1) break;
     o.flags |= 8;
@@ -356,6 +356,7 @@ __device__ __forceinline__ void trace_wide_body(const WArgs* __restrict__ args) 
     in.seed = L.r + A.seed_base;
     in.max_iter = A.max_iter;
     in.learned_iters = A.learned_iters;
+    in.fallback_iters = A.fallback_iters;
     in.hull_cull = A.hull_cull;
     in.mixed = A.mixed;
     in.learned = LEARNED;
@@ -511,6 +512,8 @@ int fill_wide_args(const rtn_scene* s, const float* const rays[6], int64_t n, co
             prm->seed_mode);
   RTN_CHECK(!learned || prm->seed_net, RTN_EINVAL, "rtn_trace: the learned seed needs seed_net");
   RTN_CHECK(prm->learned_iters >= 0 && prm->learned_iters <= 64, RTN_EINVAL, "rtn_trace: learned_iters outside 0..64");
+  RTN_CHECK(prm->fallback_iters >= 0 && prm->fallback_iters <= 64, RTN_EINVAL,
+            "rtn_trace: fallback_iters outside 0..64");
   RTN_CHECK(prm->flat_threshold <= 0.f, RTN_EUNSUPPORTED, "rtn_trace: flat-skip runs in the exact form only");
   A = WArgs{};
   A.nodes = s->wide.ptr;
@@ -535,6 +538,7 @@ int fill_wide_args(const rtn_scene* s, const float* const rays[6], int64_t n, co
     A.mlp_bias = mlp_bias(prm->seed_net);
     A.bounds = s->bounds.ptr;
     A.learned_iters = prm->learned_iters > 0 ? prm->learned_iters : prm->max_iter;
+    A.fallback_iters = prm->fallback_iters > 0 ? prm->fallback_iters : prm->max_iter;
   }
   if (pairs) {
     A.pair_ray = pairs->ray;

@@ -37,6 +37,7 @@ struct WArgs {
   const float* mlp_bias;
   const float* bounds;
   int learned_iters;
+  int fallback_iters;
   // training pairs: every converged solve with t > 0 is appended
   int* pair_ray;
   int* pair_patch;
