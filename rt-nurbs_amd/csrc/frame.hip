@@ -72,7 +72,6 @@ int copy_into(DevBuf<T>& dst, size_t at, const DevBuf<T>& src, size_t n) {
 
 // Device control block of a frame.
 struct FrameCtl {
-  WArgs a1, a2;                   // primary and secondary launches
   unsigned int work1, work2;      // persistent ray counters
   unsigned long long sec_count;   // secondaries written
   unsigned long long totals[12];  // trace totals (overflow at [6])
@@ -189,16 +188,17 @@ extern "C" int rtn_batch_trace(const rtn_batch* b, const float* const rays[6], c
   if (n == 0) return RTN_OK;
   hipStream_t st = (hipStream_t)stream;
   RTN_TRY(check_scene_ids(b, ray_scene, n, st));
-  struct Block { WArgs a; unsigned int work; unsigned int pad[3]; unsigned long long tot[12]; };
+  struct Block { unsigned int work; unsigned int pad[3]; unsigned long long tot[12]; };
   Block* d = nullptr;
   RTN_HIP(hipMallocAsync((void**)&d, sizeof(Block), st));
   Block h{};
-  int rc = batch_args(b, rays, ray_scene, n, prm, hits, &d->tot[0], h.a);
+  WArgs a;
+  int rc = batch_args(b, rays, ray_scene, n, prm, hits, &d->tot[0], a);
   if (rc == RTN_OK) {
-    h.a.work = &d->work;
+    a.work = &d->work;
     RTN_HIP(hipMemcpyAsync(d, &h, sizeof(Block), hipMemcpyHostToDevice, st));
     rc = launch_wide_kernel(kDefaultShape, prm->mode == RTN_MODE_PERSISTENT, prm->count != 0,
-                            prm->seed_mode == RTN_SEED_LEARNED, n, st, &d->a);
+                            prm->seed_mode == RTN_SEED_LEARNED, n, st, a);
   }
   unsigned long long tot[12] = {0};
   if (rc == RTN_OK) {
@@ -225,6 +225,7 @@ struct rtn_frame {
   DevBuf<float> t, u, v, dpdu, dpdv, s_t, s_u, s_v, s_dpdu, s_dpdv;
   DevBuf<float> s_ray[6];
   DevBuf<FrameCtl> ctl;
+  WArgs a1{}, a2{};  // kernel arguments of the primary and secondary launches
   LaunchShape shape = kDefaultShape;
   int refill = 64;
   hipGraph_t graph = nullptr;
@@ -247,7 +248,7 @@ int frame_enqueue(rtn_frame* f, hipStream_t st) {
   hipLaunchKernelGGL(k_frame_reset, dim3(1), dim3(64), 0, st, c);
   RTN_HIP(hipGetLastError());
   const bool learned = frame_learned(f);
-  RTN_TRY(launch_wide_kernel(f->shape, true, false, learned, f->n, st, &c->a1));
+  RTN_TRY(launch_wide_kernel(f->shape, true, false, learned, f->n, st, f->a1));
   if (f->cap2 > 0) {
     const rtn_hits h1 = f->hits1();
     float* outs[6];
@@ -255,15 +256,15 @@ int frame_enqueue(rtn_frame* f, hipStream_t st) {
     launch_secondaries(f->rays, &h1, f->n, f->fp.light, f->fp.eps, f->fp.shadow, f->fp.mirror, outs,
                        f->s_parent.ptr, f->s_kind.ptr, &c->sec_count, f->ray_scene, f->s_scene.ptr, st);
     RTN_HIP(hipGetLastError());
-    RTN_TRY(launch_wide_kernel(f->shape, true, false, learned, f->cap2, st, &c->a2));
+    RTN_TRY(launch_wide_kernel(f->shape, true, false, learned, f->cap2, st, f->a2));
   }
   return RTN_OK;
 }
 
 int frame_set_refill(rtn_frame* f, int refill) {
   const int v = refill > 0 ? refill : 64;
-  RTN_HIP(hipMemcpy(&f->ctl.ptr->a1.refill_min, &v, sizeof(int), hipMemcpyHostToDevice));
-  RTN_HIP(hipMemcpy(&f->ctl.ptr->a2.refill_min, &v, sizeof(int), hipMemcpyHostToDevice));
+  f->a1.refill_min = v;  // launch arguments: a captured graph must be captured again
+  f->a2.refill_min = v;
   f->refill = v;
   return RTN_OK;
 }
@@ -331,16 +332,16 @@ extern "C" int rtn_frame_create(rtn_frame** out, const rtn_batch* b, const float
   }
   FrameCtl h{};
   const rtn_hits h1 = f->hits1();
-  if ((rc = batch_args(b, f->rays, ray_scene, n, &f->fp.trace, &h1, f->ctl.ptr->totals, h.a1))) return fail(rc);
-  h.a1.work = &f->ctl.ptr->work1;
+  if ((rc = batch_args(b, f->rays, ray_scene, n, &f->fp.trace, &h1, f->ctl.ptr->totals, f->a1))) return fail(rc);
+  f->a1.work = &f->ctl.ptr->work1;
   if (f->cap2 > 0) {
     const rtn_hits h2 = f->hits2();
     const float* srays[6];
     for (int k = 0; k < 6; k++) srays[k] = f->s_ray[k].ptr;
-    if ((rc = batch_args(b, srays, f->s_scene.ptr, f->cap2, &f->fp.trace, &h2, f->ctl.ptr->totals, h.a2)))
+    if ((rc = batch_args(b, srays, f->s_scene.ptr, f->cap2, &f->fp.trace, &h2, f->ctl.ptr->totals, f->a2)))
       return fail(rc);
-    h.a2.ray_kind = f->s_kind.ptr;
-    h.a2.work = &f->ctl.ptr->work2;
+    f->a2.ray_kind = f->s_kind.ptr;
+    f->a2.work = &f->ctl.ptr->work2;
   }
   if (hipMemcpy(f->ctl.ptr, &h, sizeof(h), hipMemcpyHostToDevice) != hipSuccess) return fail(RTN_EHIP);
   *out = f;

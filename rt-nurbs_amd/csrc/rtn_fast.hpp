@@ -398,26 +398,6 @@ __device__ __forceinline__ Hit newton_q(const float4* __restrict__ net, int P, c
   return h;
 }
 
-template <int Q>
-__device__ __forceinline__ bool hull_culled_q(const float4* __restrict__ net, int P, const Plane2& rp, float margin) {
-  bool pos1 = true, neg1 = true, pos2 = true, neg2 = true;
-#pragma unroll 1
-  for (int i = 0; i <= P; i++) {
-#pragma unroll
-    for (int j = 0; j <= Q; j++) {
-      const float4 c = net[i * (Q + 1) + j];
-      const float s1 = fmaf(rp.n1.x, c.x, fmaf(rp.n1.y, c.y, fmaf(rp.n1.z, c.z, rp.d1 * c.w)));
-      const float s2 = fmaf(rp.n2.x, c.x, fmaf(rp.n2.y, c.y, fmaf(rp.n2.z, c.z, rp.d2 * c.w)));
-      const float mw = margin * c.w;
-      pos1 = pos1 && (s1 > mw);
-      neg1 = neg1 && (s1 < -mw);
-      pos2 = pos2 && (s2 > mw);
-      neg2 = neg2 && (s2 < -mw);
-    }
-  }
-  return pos1 || neg1 || pos2 || neg2;
-}
-
 // Bernstein polynomial B_i^n(t) in FP64 from its closed form C(n,i) t^i (1-t)^(n-i),
 // evaluated with scalar loops (no arrays, so the rare FP64 step adds no
 // register arrays to the kernel); zero outside 0 <= i <= n.
@@ -503,9 +483,10 @@ namespace fast {
 // rejected when a line separates the projected points from the ball: tested
 // along the eight directions (+-1,0), (0,+-1), (+-1,+-1), whose L1-ball support
 // is the margin itself, as min(x - m*w) > 0 or max(x + m*w) < 0 with
-// x in {c1, c2, c1+c2, c1-c2}.  Runtime degrees; no divisions.
+// x in {c1, c2, c1+c2, c1-c2}; without `diagonals`, the four axis directions
+// only (the per-plane sign test).  Runtime degrees; no divisions.
 __device__ __forceinline__ bool hull2d_culled(const float4* __restrict__ net, int n_cp, const Plane2& rp,
-                                              float margin) {
+                                              float margin, bool diagonals) {
   float lo0 = INFINITY, lo1 = INFINITY, lo2 = INFINITY, lo3 = INFINITY;
   float hi0 = -INFINITY, hi1 = -INFINITY, hi2 = -INFINITY, hi3 = -INFINITY;
 #pragma unroll 4
@@ -519,6 +500,10 @@ __device__ __forceinline__ bool hull2d_culled(const float4* __restrict__ net, in
     lo1 = fminf(lo1, c2 - mw); hi1 = fmaxf(hi1, c2 + mw);
     lo2 = fminf(lo2, s - mw); hi2 = fmaxf(hi2, s + mw);
     lo3 = fminf(lo3, d - mw); hi3 = fmaxf(hi3, d + mw);
+  }
+  if (!diagonals) {
+    lo2 = lo3 = -INFINITY;
+    hi2 = hi3 = INFINITY;
   }
   return fmaxf(fmaxf(lo0, lo1), fmaxf(lo2, lo3)) > 0.0f || fminf(fminf(hi0, hi1), fminf(hi2, hi3)) < 0.0f;
 }

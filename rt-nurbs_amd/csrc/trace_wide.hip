@@ -141,7 +141,7 @@ struct LeafOut {
   float t, u, v;
   f3 S, Su, Sv;
   int evals;     // surface evaluations (both attempts when a learned seed falls back)
-  int flags;     // 1 hit, 2 culled, 4 FP64 step done, 8 learned fallback, 16 FP64 step needed
+  int flags;     // 1 hit, 4 FP64 step done, 8 learned fallback, 16 FP64 step needed
 };
 
 struct LeafIn {
@@ -149,22 +149,18 @@ struct LeafIn {
   f3 o, d;
   float gu, gv, pu, pv;
   uint32_t seed;
-  int max_iter, learned_iters, fallback_iters, hull_cull, mixed, learned;
-  float thr_det, thr_res, cull_margin, cond_thr;
+  int max_iter, learned_iters, fallback_iters, mixed, learned;
+  float thr_det, thr_res, cond_thr;
 };
 
 // Leaf work for one patch with compile-time row length Q+1 and runtime row
-// count P+1: optional hull rejection, Newton from the seed (plain-seed
-// fallback for the learned form), optional FP64 correction.
+// count P+1: Newton from the seed (plain-seed fallback for the learned form),
+// flagging an FP64 correction.
 template <int Q>
 __device__ __forceinline__ LeafOut leaf_q(const float4* __restrict__ net, int P, const LeafIn& in) {
   LeafOut o;
   o.flags = 0;
   o.evals = 0;
-  if (in.hull_cull == 1 && hull_culled_q<Q>(net, P, in.rp, in.cull_margin)) {
-    o.flags = 2;
-    return o;
-  }
   // attempt 0 from the given seed; for the learned form a failed attempt is
   // repeated from the plain seed
   Hit h;
@@ -209,13 +205,12 @@ __device__ __forceinline__ LeafOut leaf_dispatch(int cls, const float4* net, int
 }
 
 template <int WB, int WSTK, bool PERSISTENT, bool COUNT, bool LEARNED>
-__device__ __forceinline__ void trace_wide_body(const WArgs* __restrict__ args) {
+__device__ __forceinline__ void trace_wide_body(const WArgs& A) {
   __shared__ int s_code[WSTK][WB];
   __shared__ float s_te[WSTK][WB];
   // learned form: per-wave feature rows and output logits of the seed network
   __shared__ mlp::bf16x8 s_x[LEARNED ? WB : 1][4];
   __shared__ float2 s_logit[LEARNED ? WB : 1];
-  const WArgs& A = *args;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   Lane L;
@@ -224,18 +219,23 @@ __device__ __forceinline__ void trace_wide_body(const WArgs* __restrict__ args) 
   int off = 0;            // stack depth
   int cur = -1;           // wide node to expand, or -1
   int cand_first = 0, cand_left = 0;  // primitives of the current leaf still to test
-  bool exhausted = true;  // no more rays to hand out
-  unsigned long long acc[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-  if (PERSISTENT) {
-    exhausted = false;
-  } else {
-    const int64_t r = (int64_t)blockIdx.x * WB + tid;
-    if (r < A.n_rays) {
-      cur = lane_start(L, A, (uint32_t)r);
-      has_ray = cur >= 0;
-    }
-  }
+  bool exhausted;         // no more rays to hand out
+  // work totals of the lane: all eleven when counting, otherwise only the
+  // stack-overflow count (total 6), so no idle accumulators occupy registers
+  constexpr int NACC = COUNT ? 11 : 1;
+  unsigned long long acc[NACC] = {};
+  exhausted = false;
+  // One loop shape for both forms (it also keeps the one-ray form's register
+  // allocation as small as the persistent one's).
   while (true) {
+    if (!PERSISTENT && !exhausted) {  // the lane's one ray, taken on the first pass
+      exhausted = true;
+      const int64_t r = (int64_t)blockIdx.x * WB + tid;
+      if (r < A.n_rays) {
+        cur = lane_start(L, A, (uint32_t)r);
+        has_ray = cur >= 0;
+      }
+    }
     if (PERSISTENT && !exhausted) {
       // lanes without a ray take consecutive rays from the global counter,
       // one atomic per wave
@@ -270,11 +270,11 @@ __device__ __forceinline__ void trace_wide_body(const WArgs* __restrict__ args) 
       }
       if (off == 0) {  // traversal finished: the ray is done
         lane_finish<COUNT>(L, A, cn);
-        if (COUNT) {
+        if constexpr (COUNT) {
           acc[0] += cn.nc; acc[1] += cn.ne; acc[3] += cn.nbox; acc[4] += cn.nleaf; acc[5] += cn.npt;
           acc[6] += cn.ovf; acc[7] += cn.nfl; acc[8] += cn.n64; acc[9] += cn.ncull; acc[10] += cn.nfb;
         } else if (cn.ovf) {
-          acc[6] += 1;
+          acc[0] += 1;
           cn.ovf = 0;
         }
         has_ray = false;
@@ -288,11 +288,13 @@ __device__ __forceinline__ void trace_wide_body(const WArgs* __restrict__ args) 
         cand_first = code & 0xffffff;
         cand_left = ((code >> 24) & 0x7f) + 1;
         if (COUNT) cn.nleaf++;
-        if (A.hull_cull == 2) {  // reject from the projected control points before any Newton pass
+        if (A.hull_cull) {  // reject from the projected control points before any Newton pass
           while (cand_left > 0) {
             const int pqv = A.pq[cand_first];
             const int np = ((pqv & 0xff) + 1) * (((pqv >> 8) & 0xff) + 1);
-            if (!(pqv & 0x10000) || !hull2d_culled(A.cp + A.cp_off[cand_first], np, L.rp, A.cull_margin)) break;
+            if (!(pqv & 0x10000) ||
+                !hull2d_culled(A.cp + A.cp_off[cand_first], np, L.rp, A.cull_margin, A.hull_cull == 2))
+              break;
             if (COUNT) {
               cn.ncull++;
               cn.npt++;
@@ -357,12 +359,10 @@ __device__ __forceinline__ void trace_wide_body(const WArgs* __restrict__ args) 
     in.max_iter = A.max_iter;
     in.learned_iters = A.learned_iters;
     in.fallback_iters = A.fallback_iters;
-    in.hull_cull = A.hull_cull;
     in.mixed = A.mixed;
     in.learned = LEARNED;
     in.thr_det = A.thr_det;
     in.thr_res = A.thr_res;
-    in.cull_margin = A.cull_margin;
     in.cond_thr = A.cond_thr;
     const float4* net = A.cp + A.cp_off[pi];
     LeafOut res;
@@ -380,15 +380,11 @@ __device__ __forceinline__ void trace_wide_body(const WArgs* __restrict__ args) 
         if (correct64_rt(net, p, q, L.o, L.d, res.u, res.v, res.S, res.Su, res.Sv, res.t)) res.flags |= 4;
       }
       if (COUNT) {
-        if (res.flags & 2) {
-          cn.ncull++;
-        } else {
-          cn.nc++;
-          cn.ne += res.evals;
-          cn.nfl += res.evals * ex::eval_flops(p, q);
-          if (res.flags & 4) cn.n64++;
-          if (res.flags & 8) cn.nfb++;
-        }
+        cn.nc++;
+        cn.ne += res.evals;
+        cn.nfl += res.evals * ex::eval_flops(p, q);
+        if (res.flags & 4) cn.n64++;
+        if (res.flags & 8) cn.nfb++;
       }
       if (A.pair_count && (res.flags & 1) && 0.0f < res.t) {
         const unsigned long long k = atomicAdd(A.pair_count, 1ull);
@@ -410,17 +406,20 @@ __device__ __forceinline__ void trace_wide_body(const WArgs* __restrict__ args) 
   }
   if (A.totals) {  // every lane of the wave arrives here together
 #pragma unroll
-    for (int k = 0; k < 11; k++) {
+    for (int k = 0; k < NACC; k++) {
       unsigned long long x = acc[k];
       for (int sft = 32; sft > 0; sft >>= 1) x += __shfl_xor(x, sft);
-      if (lane == 0 && x) atomicAdd(A.totals + k, x);
+      if (lane == 0 && x) atomicAdd(A.totals + (COUNT ? k : 6), x);
     }
   }
 }
 
+// The arguments travel by value in the kernel-argument segment: pointers read
+// from there are known to address global memory, so every load and store of
+// the kernel is a global (not flat) access and LDS waits never wait on them.
 template <int WB, int WSTK, bool PERSISTENT, bool COUNT, bool LEARNED>
-__global__ void __launch_bounds__(WB) k_trace_wide(const WArgs* __restrict__ args) {
-  trace_wide_body<WB, WSTK, PERSISTENT, COUNT, LEARNED>(args);
+__global__ void __launch_bounds__(WB) k_trace_wide(const WArgs A) {
+  trace_wide_body<WB, WSTK, PERSISTENT, COUNT, LEARNED>(A);
 }
 
 // The one-ray-per-lane learned form held to two waves per SIMD (256 registers
@@ -429,8 +428,8 @@ __global__ void __launch_bounds__(WB) k_trace_wide(const WArgs* __restrict__ arg
 // without the constraint and is faster without it.
 template <int WB, int WSTK, bool COUNT>
 __global__ void __launch_bounds__(WB) __attribute__((amdgpu_waves_per_eu(2, 2)))
-k_trace_wide_learned2(const WArgs* __restrict__ args) {
-  trace_wide_body<WB, WSTK, false, COUNT, true>(args);
+k_trace_wide_learned2(const WArgs A) {
+  trace_wide_body<WB, WSTK, false, COUNT, true>(A);
 }
 
 }  // namespace
@@ -441,19 +440,35 @@ const void* mlp_tiles(const rtn_mlp* net);
 const float* mlp_bias(const rtn_mlp* net);
 
 namespace {
-template <int WB, int WSTK, bool P, bool C>
-void launch_shape(bool learned, unsigned blocks, hipStream_t stream, const WArgs* a) {
-  if (!learned) {
-    hipLaunchKernelGGL((k_trace_wide<WB, WSTK, P, C, false>), dim3(blocks), dim3(WB), 0, stream, a);
-  } else if constexpr (P) {
-    hipLaunchKernelGGL((k_trace_wide<WB, WSTK, P, C, true>), dim3(blocks), dim3(WB), 0, stream, a);
-  } else {
-    hipLaunchKernelGGL((k_trace_wide_learned2<WB, WSTK, C>), dim3(blocks), dim3(WB), 0, stream, a);
+// One launch: the one-ray form covers n_max rays; the persistent form launches
+// blocks_per_cu blocks per compute unit, capped at the blocks the kernel's
+// registers and LDS let stay resident (a block that could not start before the
+// queue drains would only add a tail).
+template <int WB>
+void launch_one(void (*kernel)(const WArgs), bool persistent, int blocks_per_cu, int64_t n_max,
+                hipStream_t stream, const WArgs& a) {
+  const int64_t need = std::max<int64_t>(1, (n_max + WB - 1) / WB);
+  unsigned blocks = (unsigned)need;
+  if (persistent) {
+    int dev = 0, cus = 256, occ = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, WB, 0) != hipSuccess || occ < 1)
+      occ = blocks_per_cu;
+    const int per_cu = std::max(1, std::min(blocks_per_cu, occ));
+    blocks = (unsigned)std::min<int64_t>(need, (int64_t)cus * per_cu);
   }
+  hipLaunchKernelGGL(kernel, dim3(blocks), dim3(WB), 0, stream, a);
 }
-template <int WB, int WSTK>
-void launch_tuned(unsigned blocks, hipStream_t stream, const WArgs* a) {
-  hipLaunchKernelGGL((k_trace_wide<WB, WSTK, true, false, false>), dim3(blocks), dim3(WB), 0, stream, a);
+template <int WB, int WSTK, bool P, bool C>
+void launch_shape(bool learned, int bpc, int64_t n_max, hipStream_t stream, const WArgs& a) {
+  if (!learned) {
+    launch_one<WB>(k_trace_wide<WB, WSTK, P, C, false>, P, bpc, n_max, stream, a);
+  } else if constexpr (P) {
+    launch_one<WB>(k_trace_wide<WB, WSTK, P, C, true>, P, bpc, n_max, stream, a);
+  } else {
+    launch_one<WB>(k_trace_wide_learned2<WB, WSTK, C>, P, bpc, n_max, stream, a);
+  }
 }
 }  // namespace
 
@@ -471,30 +486,21 @@ bool shape_available(const LaunchShape& sh, bool persistent, bool count, bool le
 }
 
 int launch_wide_kernel(const LaunchShape& sh, bool persistent, bool count, bool learned, int64_t n_max,
-                       hipStream_t stream, const WArgs* d_args) {
+                       hipStream_t stream, const WArgs& args) {
   RTN_CHECK(shape_available(sh, persistent, count, learned), RTN_EUNSUPPORTED,
             "trace: launch shape %d lanes x %d stack entries is not built for this form", sh.wb, sh.wstk);
-  unsigned blocks;
-  if (persistent) {
-    int dev = 0, cus = 256;
-    (void)hipGetDevice(&dev);
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    blocks = (unsigned)std::max<int64_t>(
-        1, std::min<int64_t>((n_max + sh.wb - 1) / sh.wb, (int64_t)cus * std::max(1, sh.blocks_per_cu)));
-  } else {
-    blocks = (unsigned)std::max<int64_t>(1, (n_max + sh.wb - 1) / sh.wb);
-  }
+  const int bpc = std::max(1, sh.blocks_per_cu);
   if (sh.wb == WB_DEFAULT && sh.wstk == WSTK_DEFAULT) {
     if (persistent) {
-      if (count) launch_shape<WB_DEFAULT, WSTK_DEFAULT, true, true>(learned, blocks, stream, d_args);
-      else launch_shape<WB_DEFAULT, WSTK_DEFAULT, true, false>(learned, blocks, stream, d_args);
+      if (count) launch_shape<WB_DEFAULT, WSTK_DEFAULT, true, true>(learned, bpc, n_max, stream, args);
+      else launch_shape<WB_DEFAULT, WSTK_DEFAULT, true, false>(learned, bpc, n_max, stream, args);
     } else {
-      if (count) launch_shape<WB_DEFAULT, WSTK_DEFAULT, false, true>(learned, blocks, stream, d_args);
-      else launch_shape<WB_DEFAULT, WSTK_DEFAULT, false, false>(learned, blocks, stream, d_args);
+      if (count) launch_shape<WB_DEFAULT, WSTK_DEFAULT, false, true>(learned, bpc, n_max, stream, args);
+      else launch_shape<WB_DEFAULT, WSTK_DEFAULT, false, false>(learned, bpc, n_max, stream, args);
     }
   } else {
 #define RTN_SHAPE(W, S) \
-  else if (sh.wb == W && sh.wstk == S) launch_tuned<W, S>(blocks, stream, d_args);
+  else if (sh.wb == W && sh.wstk == S) launch_one<W>(k_trace_wide<W, S, true, false, false>, true, bpc, n_max, stream, args);
     if (false) {
     }
     RTN_SHAPE(64, 16) RTN_SHAPE(64, 24) RTN_SHAPE(64, 32) RTN_SHAPE(128, 16) RTN_SHAPE(128, 24) RTN_SHAPE(128, 32)
@@ -559,18 +565,17 @@ int trace_wide_scene(const rtn_scene* s, const float* const rays[6], int64_t n, 
   WArgs A;
   RTN_TRY(fill_wide_args(s, rays, n, prm, hits, d_totals, pairs, A));
   const bool persistent = prm->mode == RTN_MODE_PERSISTENT;
-  // arguments and the ray counter live in one small device block (stream-ordered)
-  struct Block { WArgs a; unsigned int work; unsigned int pad[3]; };
-  Block* d_blk = nullptr;
-  RTN_HIP(hipMallocAsync((void**)&d_blk, sizeof(Block), stream));
-  A.work = persistent ? &d_blk->work : nullptr;
-  Block h{};
-  h.a = A;
-  h.work = 0;
-  RTN_HIP(hipMemcpyAsync(d_blk, &h, sizeof(Block), hipMemcpyHostToDevice, stream));
+  // the persistent form's ray counter lives in a small stream-ordered block
+  unsigned int* d_work = nullptr;
+  if (persistent) {
+    RTN_HIP(hipMallocAsync((void**)&d_work, 4 * sizeof(unsigned int), stream));
+    const unsigned int zero[4] = {0, 0, 0, 0};
+    RTN_HIP(hipMemcpyAsync(d_work, zero, sizeof(zero), hipMemcpyHostToDevice, stream));
+  }
+  A.work = d_work;
   const int rc = launch_wide_kernel(kDefaultShape, persistent, prm->count != 0,
-                                    prm->seed_mode == RTN_SEED_LEARNED, n, stream, &d_blk->a);
-  RTN_HIP(hipFreeAsync(d_blk, stream));
+                                    prm->seed_mode == RTN_SEED_LEARNED, n, stream, A);
+  if (d_work) RTN_HIP(hipFreeAsync(d_work, stream));
   return rc;
 }
 

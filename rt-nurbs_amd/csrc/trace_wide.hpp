@@ -14,7 +14,7 @@ namespace rtn {
 constexpr int WB_DEFAULT = 128;   // lanes per block (2 waves)
 constexpr int WSTK_DEFAULT = 20;  // stack entries per lane: 20 KB of LDS per 128-lane block
 
-// Kernel arguments; they live in device memory (one small block per launch).
+// Kernel arguments, passed by value.
 struct WArgs {
   const WideNode* nodes;
   const float4* cp;
@@ -22,7 +22,7 @@ struct WArgs {
   const int* pq;
   const float4* corners;  // dehomogenised corners q00,q10,q11,q01 (12 floats per patch)
   const float *ox, *oy, *oz, *dx, *dy, *dz;
-  int64_t n_rays;         // read by the kernel at run time (a producer kernel may set it)
+  int64_t n_rays;
   int max_iter;
   float thr_det, thr_res;
   uint32_t seed_base;
@@ -69,8 +69,8 @@ extern const int kNumTunedShapes;
 bool shape_available(const LaunchShape& sh, bool persistent, bool count, bool learned);
 int fill_wide_args(const rtn_scene* s, const float* const rays[6], int64_t n, const rtn_trace_params* prm,
                    const rtn_hits* hits, unsigned long long* d_totals, const PairOut* pairs, WArgs& A);
-// n_max bounds the rays the launch may see (grid size); the kernel reads n_rays from d_args.
+// n_max bounds the rays the launch may see (grid size).
 int launch_wide_kernel(const LaunchShape& sh, bool persistent, bool count, bool learned, int64_t n_max,
-                       hipStream_t stream, const WArgs* d_args);
+                       hipStream_t stream, const WArgs& args);
 
 }  // namespace rtn
