@@ -57,8 +57,9 @@ __device__ __forceinline__ float fast_sigmoid(float z) {
   return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-z * LOG2E));
 }
 __device__ __forceinline__ float gelu(float x) {
-  const float z = 2.0f * GK0 * x * fmaf(GK1, x * x, 1.0f);
-  return x * fast_sigmoid(z);
+  constexpr float C = -2.0f * GK0 * LOG2E;  // sigmoid(z) = 1 / (1 + 2^(-z log2 e))
+  const float e = __builtin_amdgcn_exp2f(x * C * fmaf(GK1, x * x, 1.0f));
+  return x * __builtin_amdgcn_rcpf(1.0f + e);
 }
 // d gelu / dx = s + 2 x s (1 - s) k0 (1 + 3 k1 x^2), s = sigmoid(2 k0 (x + k1 x^3))
 __device__ __forceinline__ float gelu_grad(float x) {
@@ -77,11 +78,11 @@ __device__ __forceinline__ float bf16r(float x) { return (float)(__bf16)x; }
 __device__ __forceinline__ void pair_features(float ox, float oy, float oz, float dx, float dy, float dz,
                                               const float* __restrict__ bnd, float4 c0, float4 c1, float4 c2, int P,
                                               int Q, float f[NIN]) {
-  const float il = 1.0f / sqrtf(fmaf(dx, dx, fmaf(dy, dy, dz * dz)));
+  const float il = __builtin_amdgcn_rsqf(fmaf(dx, dx, fmaf(dy, dy, dz * dz)));  // features end in BF16
   const float nx = dx * il, ny = dy * il, nz = dz * il;
   const float cx = 0.5f * (bnd[0] + bnd[3]), cy = 0.5f * (bnd[1] + bnd[4]), cz = 0.5f * (bnd[2] + bnd[5]);
   const float hs = fmaxf(fmaxf(bnd[3] - bnd[0], bnd[4] - bnd[1]), fmaxf(bnd[5] - bnd[2], 2e-20f)) * 0.5f;
-  const float is = 1.0f / hs;
+  const float is = __builtin_amdgcn_rcpf(hs);
   const float tc = fmaxf(0.0f, fmaf(cx - ox, nx, fmaf(cy - oy, ny, (cz - oz) * nz)));
   f[0] = (fmaf(tc, nx, ox) - cx) * is;
   f[1] = (fmaf(tc, ny, oy) - cy) * is;
@@ -128,8 +129,12 @@ __device__ __forceinline__ void forward_wave(const bf16x8* __restrict__ tiles, c
     f32x4 d[4];
     bf16x8 h[2];
     const bf16x8 x = s_x[16 * nb + m][g];
-    const bf16x8* tp = tiles;
-    asm volatile("" : "+v"(tp));  // keep each layer's tile loads next to its products (register pressure)
+    // keep each layer's tile loads next to its products (register pressure): an
+    // opaque offset stops the loads from being hoisted, and offsetting (rather
+    // than hiding the pointer) keeps them global-memory loads
+    int toff = 0;
+    asm volatile("" : "+v"(toff));
+    const bf16x8* tp = tiles + toff;
 #pragma unroll
     for (int ob = 0; ob < 4; ob++) d[ob] = mfma(tp[(T_L1 + ob) * 64 + lane], x, bias4(bias, BB1 + 16 * ob + 4 * g));
 #pragma unroll
@@ -147,7 +152,8 @@ __device__ __forceinline__ void forward_wave(const bf16x8* __restrict__ tiles, c
       if (layer == 2) break;
       const int tl = layer == 0 ? T_L2 : T_L3;
       const int bb = layer == 0 ? BB2 : BB3;
-      asm volatile("" : "+v"(tp));
+      asm volatile("" : "+v"(toff));
+      tp = tiles + toff;
 #pragma unroll
       for (int ob = 0; ob < 4; ob++) {
         f32x4 acc = bias4(bias, bb + 16 * ob + 4 * g);
@@ -155,11 +161,73 @@ __device__ __forceinline__ void forward_wave(const bf16x8* __restrict__ tiles, c
         d[ob] = mfma(tp[(tl + ob * 2 + 1) * 64 + lane], h[1], acc);
       }
     }
-    asm volatile("" : "+v"(tp));
+    asm volatile("" : "+v"(toff));
+    tp = tiles + toff;
     f32x4 o = bias4(bias, BB4 + 4 * g);
     o = mfma(tp[(T_L4 + 0) * 64 + lane], h[0], o);
     o = mfma(tp[(T_L4 + 1) * 64 + lane], h[1], o);
     if (g == 0) s_logit[16 * nb + m] = make_float2(o[0], o[1]);
+  }
+}
+
+// Inference pass of the seed network for nblk blocks of 16 pair rows (rows of
+// three 8-value groups), each lane receiving the logits of pair row `row`.
+// The weight loads are issued ahead of their products: the first layer's
+// tiles and biases once for all blocks, every later layer's before the
+// previous layer's GELU, so their latency overlaps that work.  Must be called
+// by all 64 lanes with nblk wave-uniform.
+__device__ __forceinline__ void forward_seed(const bf16x8* __restrict__ tiles, const float* __restrict__ bias,
+                                             const bf16x8 (*s_x)[3], int nblk, int lane, int row, float& lu,
+                                             float& lv) {
+  const int m = lane & 15, g = lane >> 4;
+  bf16x8 w1[4];
+  f32x4 b1[4];
+#pragma unroll
+  for (int ob = 0; ob < 4; ob++) {
+    w1[ob] = tiles[(T_L1 + ob) * 64 + lane];
+    b1[ob] = bias4(bias, BB1 + 16 * ob + 4 * g);
+  }
+#pragma unroll 1
+  for (int nb = 0; nb < nblk; nb++) {
+    int toff = 0;  // opaque offset: later layers' tiles are loaded per block, not held across blocks
+    asm volatile("" : "+v"(toff));
+    const bf16x8* tp = tiles + toff;
+    bf16x8 x = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (g < 3) x = s_x[16 * nb + m][g];
+    f32x4 d[4];
+#pragma unroll
+    for (int ob = 0; ob < 4; ob++) d[ob] = mfma(w1[ob], x, b1[ob]);
+    bf16x8 h[2];
+#pragma unroll
+    for (int layer = 0; layer < 2; layer++) {
+      const int tl = layer == 0 ? T_L2 : T_L3;
+      const int bb = layer == 0 ? BB2 : BB3;
+      bf16x8 w[8];
+      f32x4 b[4];
+#pragma unroll
+      for (int j = 0; j < 8; j++) w[j] = tp[(tl + j) * 64 + lane];
+#pragma unroll
+      for (int ob = 0; ob < 4; ob++) b[ob] = bias4(bias, bb + 16 * ob + 4 * g);
+#pragma unroll
+      for (int kb = 0; kb < 2; kb++)
+#pragma unroll
+        for (int i = 0; i < 8; i++) h[kb][i] = (__bf16)gelu(d[2 * kb + (i >> 2)][i & 3]);
+#pragma unroll
+      for (int ob = 0; ob < 4; ob++) d[ob] = mfma(w[2 * ob + 1], h[1], mfma(w[2 * ob], h[0], b[ob]));
+    }
+    const bf16x8 w40 = tp[(T_L4 + 0) * 64 + lane], w41 = tp[(T_L4 + 1) * 64 + lane];
+    const f32x4 b4 = bias4(bias, BB4 + 4 * g);
+#pragma unroll
+    for (int kb = 0; kb < 2; kb++)
+#pragma unroll
+      for (int i = 0; i < 8; i++) h[kb][i] = (__bf16)gelu(d[2 * kb + (i >> 2)][i & 3]);
+    const f32x4 o = mfma(w41, h[1], mfma(w40, h[0], b4));
+    // lane m (< 16) holds the logits of pair 16 nb + m
+    const float a = __shfl(o[0], row & 15), bv = __shfl(o[1], row & 15);
+    if ((row >> 4) == nb) {
+      lu = a;
+      lv = bv;
+    }
   }
 }
 
@@ -179,14 +247,14 @@ __device__ __forceinline__ void store_row(bf16x8 (*s_x)[4], int row, const float
 
 // Feature row of one pair written straight into s_x[row] in 8-value chunks
 // (the same values as pair_features, without holding all 20 in registers).
-__device__ __forceinline__ void store_features(bf16x8 (*s_x)[4], int row, float ox, float oy, float oz, float dx,
+__device__ __forceinline__ void store_features(bf16x8 (*s_x)[3], int row, float ox, float oy, float oz, float dx,
                                                float dy, float dz, const float* __restrict__ bnd, float4 c0,
                                                float4 c1, float4 c2, int P, int Q) {
-  const float il = 1.0f / sqrtf(fmaf(dx, dx, fmaf(dy, dy, dz * dz)));
+  const float il = __builtin_amdgcn_rsqf(fmaf(dx, dx, fmaf(dy, dy, dz * dz)));  // features end in BF16
   const float nx = dx * il, ny = dy * il, nz = dz * il;
   const float cx = 0.5f * (bnd[0] + bnd[3]), cy = 0.5f * (bnd[1] + bnd[4]), cz = 0.5f * (bnd[2] + bnd[5]);
   const float hs = fmaxf(fmaxf(bnd[3] - bnd[0], bnd[4] - bnd[1]), fmaxf(bnd[5] - bnd[2], 2e-20f)) * 0.5f;
-  const float is = 1.0f / hs;
+  const float is = __builtin_amdgcn_rcpf(hs);
   const float tc = fmaxf(0.0f, fmaf(cx - ox, nx, fmaf(cy - oy, ny, (cz - oz) * nz)));
   bf16x8 r;
   r[0] = (__bf16)((fmaf(tc, nx, ox) - cx) * is);
@@ -206,8 +274,6 @@ __device__ __forceinline__ void store_features(bf16x8 (*s_x)[4], int row, float 
   r[3] = (__bf16)((c2.w - cz) * is);
   r[4] = r[5] = r[6] = r[7] = (__bf16)0.0f;
   s_x[row][2] = r;
-  r[0] = r[1] = r[2] = r[3] = (__bf16)0.0f;
-  s_x[row][3] = r;
 }
 
 // Seeds for the active lanes of a wave from their feature rows: the active
@@ -216,16 +282,15 @@ __device__ __forceinline__ void store_features(bf16x8 (*s_x)[4], int row, float 
 // u, v are written for active lanes only.
 __device__ __forceinline__ void seed_wave_rows(const bf16x8* __restrict__ tiles, const float* __restrict__ bias,
                                                bool active, int row, int n_act, float& u, float& v,
-                                               bf16x8 (*s_x)[4], float2* s_logit, int lane) {
+                                               const bf16x8 (*s_x)[3], int lane) {
   wave_lds_sync();
-  forward_wave<false>(tiles, bias, s_x, (n_act + 15) >> 4, lane, s_logit, nullptr);
-  wave_lds_sync();
+  float lu = 0.0f, lv = 0.0f;
+  forward_seed(tiles, bias, s_x, (n_act + 15) >> 4, lane, row, lu, lv);
   if (active) {
-    const float2 lg = s_logit[row];
-    u = fast_sigmoid(lg.x);
-    v = fast_sigmoid(lg.y);
+    u = fast_sigmoid(lu);
+    v = fast_sigmoid(lv);
   }
-  wave_lds_sync();  // the next call's writes come after these reads
+  wave_lds_sync();  // the next call's row writes come after this call's reads
 }
 
 // Row of this lane in the compacted layout: active lanes first, in lane

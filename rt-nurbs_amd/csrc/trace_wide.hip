@@ -208,9 +208,8 @@ template <int WB, int WSTK, bool PERSISTENT, bool COUNT, bool LEARNED>
 __device__ __forceinline__ void trace_wide_body(const WArgs& A) {
   __shared__ int s_code[WSTK][WB];
   __shared__ float s_te[WSTK][WB];
-  // learned form: per-wave feature rows and output logits of the seed network
-  __shared__ mlp::bf16x8 s_x[LEARNED ? WB : 1][4];
-  __shared__ float2 s_logit[LEARNED ? WB : 1];
+  // learned form: per-wave feature rows of the seed network (24 BF16 per row)
+  __shared__ mlp::bf16x8 s_x[LEARNED ? WB : 1][3];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   Lane L;
@@ -341,7 +340,7 @@ __device__ __forceinline__ void trace_wide_body(const WArgs& A) {
     in.pu = gu;
     in.pv = gv;
     if (LEARNED) {  // network seed for every pending pair of the wave (matrix cores), plain seed kept as fallback
-      auto rows = reinterpret_cast<mlp::bf16x8(*)[4]>(&s_x[tid & ~63][0]);
+      auto rows = reinterpret_cast<mlp::bf16x8(*)[3]>(&s_x[tid & ~63][0]);
       int n_act;
       const int row = mlp::seed_row(pending, lane, n_act);
       if (pending) {
@@ -350,10 +349,9 @@ __device__ __forceinline__ void trace_wide_body(const WArgs& A) {
       } else {
         const mlp::bf16x8 z = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
-        for (int k = 0; k < 4; k++) rows[row][k] = z;
+        for (int k = 0; k < 3; k++) rows[row][k] = z;
       }
-      mlp::seed_wave_rows(A.mlp_tiles, A.mlp_bias, pending, row, n_act, in.gu, in.gv, rows, &s_logit[tid & ~63],
-                          lane);
+      mlp::seed_wave_rows(A.mlp_tiles, A.mlp_bias, pending, row, n_act, in.gu, in.gv, rows, lane);
     }
     in.seed = L.r + A.seed_base;
     in.max_iter = A.max_iter;
@@ -422,14 +420,13 @@ __global__ void __launch_bounds__(WB) k_trace_wide(const WArgs A) {
   trace_wide_body<WB, WSTK, PERSISTENT, COUNT, LEARNED>(A);
 }
 
-// The one-ray-per-lane learned form held to two waves per SIMD (256 registers
-// per lane; left to itself the compiler goes past 256 and runs one wave,
-// at less than half the speed).  The persistent learned form fits two waves
-// without the constraint and is faster without it.
-template <int WB, int WSTK, bool COUNT>
-__global__ void __launch_bounds__(WB) __attribute__((amdgpu_waves_per_eu(2, 2)))
-k_trace_wide_learned2(const WArgs A) {
-  trace_wide_body<WB, WSTK, false, COUNT, true>(A);
+// The learned forms held to three waves per SIMD (168 registers per lane
+// with the matrix accumulators, a few bytes of spill; left to itself the
+// compiler takes about 200 and runs two).
+template <int WB, int WSTK, bool PERSISTENT, bool COUNT>
+__global__ void __launch_bounds__(WB) __attribute__((amdgpu_waves_per_eu(3, 3)))
+k_trace_wide_learned(const WArgs A) {
+  trace_wide_body<WB, WSTK, PERSISTENT, COUNT, true>(A);
 }
 
 }  // namespace
@@ -464,10 +461,8 @@ template <int WB, int WSTK, bool P, bool C>
 void launch_shape(bool learned, int bpc, int64_t n_max, hipStream_t stream, const WArgs& a) {
   if (!learned) {
     launch_one<WB>(k_trace_wide<WB, WSTK, P, C, false>, P, bpc, n_max, stream, a);
-  } else if constexpr (P) {
-    launch_one<WB>(k_trace_wide<WB, WSTK, P, C, true>, P, bpc, n_max, stream, a);
   } else {
-    launch_one<WB>(k_trace_wide_learned2<WB, WSTK, C>, P, bpc, n_max, stream, a);
+    launch_one<WB>(k_trace_wide_learned<WB, WSTK, P, C>, P, bpc, n_max, stream, a);
   }
 }
 }  // namespace

@@ -67,7 +67,7 @@ def test_learned_trace_keeps_the_hits(rtl, net):
     a, _ = rtl.trace(scene, rays, rtl.params(mode=rtl.MODE_WIDE, hull_cull=2, **CFG))
     for mode in (rtl.MODE_WIDE, rtl.MODE_PERSISTENT):
         b, tb = rtl.trace(scene, rays, rtl.params(mode=mode, hull_cull=2, seed_mode=rtl.SEED_LEARNED, seed_net=net,
-                                                  learned_iters=3, fallback_iters=8, count=1, **CFG))
+                                                  learned_iters=2, fallback_iters=8, count=1, **CFG))
         ga, gb = a.get(), b.get()
         assert np.mean(ga["hit"] == gb["hit"]) > 0.99
         assert (gb["hit"] >= 0).sum() >= 0.995 * (ga["hit"] >= 0).sum()
