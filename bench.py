@@ -277,10 +277,11 @@ def bench_c2_c1(args, L, scenes, learned):
         "persistent_hull2_mixed": dict(mode=P, hull_cull=2, **mixed, **BASE),
     }
     if net is not None:
-        # a 2-iteration attempt from the network's seed, then the plain seed's full 8: the
-        # plain method's accuracy (measured budget sweep: longer attempts converge to other
-        # roots more often, shared budgets lose hits)
-        lrn = dict(seed_mode=L.SEED_LEARNED, seed_net=net, learned_iters=2, fallback_iters=8)
+        # a 3-evaluation attempt from the network's seed (two Newton steps: the fewest that
+        # converge from its typical error), then the plain seed's full 8; the four lanes of a
+        # quad testing the same patch share one network evaluation (measured sweep: longer
+        # attempts converge to other roots more often, shared iteration budgets lose hits)
+        lrn = dict(seed_mode=L.SEED_LEARNED, seed_net=net, learned_iters=3, fallback_iters=8, seed_share=1)
         forms["wide_learned"] = dict(mode=W, hull_cull=2, **lrn, **BASE)
         forms["persistent_learned_mixed"] = dict(mode=P, hull_cull=2, **lrn, **mixed, **BASE)
     if args.forms != "all":

@@ -214,6 +214,10 @@ typedef struct {
   int fallback_iters;     /* learned seed: Newton iterations of the plain-seed fallback attempt
                              (0 = max_iter; max_iter - learned_iters keeps the per-pair budget
                              of the plain method)                                 */
+  int seed_share;         /* learned seed: 1 = the four lanes of an aligned lane quad (four consecutive
+                             rays: a pixel's samples, or neighbouring pixels) that test the same patch
+                             in the same pass start from one network evaluation, the quad's first
+                             ray's; 0 = one evaluation per (ray, patch) pair     */
   const struct rtn_mlp* seed_net; /* RTN_SEED_LEARNED: the seed network          */
 } rtn_trace_params;
 
@@ -264,8 +268,9 @@ int rtn_solve_pairs(const rtn_scene* scene, const float* const rays_soa[6], int6
 /* ------------------------------------------------------------ learned seed
  * No reference counterpart.  A 20 -> 64 -> 64 -> 64 -> 2 perceptron (GELU,
  * tanh form; sigmoid outputs) predicting Newton's starting (u,v) for a
- * (ray, patch) pair from the pair's features (ray and corner points in the
- * patch bounding-box frame, degrees).  Inference and training run BF16
+ * (ray, patch) pair from the pair's features: a 3 x 3 sample of the control
+ * net projected on the ray's two planes (the ray at the origin), in units of
+ * the patch box's half extent, and the degrees.  Inference and training run BF16
  * MFMA with FP32 accumulation; FP32 master parameters, Adam.
  * Parameter layout (RTN_MLP_NPARAM floats): W1[64][20] b1[64] W2[64][64]
  * b2[64] W3[64][64] b3[64] W4[2][64] b4[2], weights row-major (out x in). */

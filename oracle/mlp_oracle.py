@@ -4,8 +4,10 @@ The seed network has no counterpart in the reference: it is this project's
 design, and this module restates it in numpy FP64 so the GPU kernels can be
 checked against an independent implementation:
 
-  features(...)       the 20 pair features (ray and corner points in the
-                      patch bounding-box frame, degrees)
+  ray_planes(d)       the two unit ray-plane normals (the reference's
+                      ray-plane construction, math_utils.h:33-54)
+  features(...)       the 20 pair features (a 3 x 3 sample of the control
+                      net projected on the ray planes, degrees)
   forward(...)        20 -> 64 -> 64 -> 64 -> 2, GELU (tanh form), sigmoid out
   loss_and_grad(...)  mean over pairs of |uv - target|^2 and its gradient
 
@@ -17,7 +19,9 @@ FP64.  With quantize=False everything is FP64.
 
 Pinning (tests/test_mlp_oracle.py): the FP64 gradient is checked against
 torch.autograd in float64 and against central finite differences; GELU
-against torch's tanh-approximate GELU; features against closed forms.
+against torch's tanh-approximate GELU; features against closed forms
+(axis-aligned rays over flat nets, odd/even middle indices, similarity
+invariance).
 """
 from __future__ import annotations
 
@@ -75,18 +79,50 @@ def sigmoid(z):
     return 1.0 / (1.0 + np.exp(-z))
 
 
-def features(o, d, bounds, corners, p, q):
+def ray_planes(d):
+    """Unit normals n1, n2 ((n,3) each) of the two planes through a ray with
+    direction d (n,3): n1 from the two largest-magnitude components' axis
+    pair, n2 = n1 x d/|d| (math_utils.h:33-54)."""
+    d = np.asarray(d, np.float64)
+    ax, ay, az = np.abs(d).T
+    xd = (ax > ay) & (ax > az)
+    b0 = np.where(xd, d[:, 0], d[:, 1])
+    b1 = np.where(xd, d[:, 1], d[:, 2])
+    den = 1.0 / np.sqrt(b0 * b0 + b1 * b1)
+    z = np.zeros_like(den)
+    n1 = np.where(xd[:, None], np.stack([d[:, 1] * den, -d[:, 0] * den, z], 1),
+                  np.stack([z, d[:, 2] * den, -d[:, 1] * den], 1))
+    n2 = np.cross(n1, d / np.linalg.norm(d, axis=1, keepdims=True))
+    return n1, n2
+
+
+def features(o, d, bounds, nets, p, q):
     """o, d: (n,3) ray origin/direction; bounds: (n,6) min xyz, max xyz of the
-    patch; corners: (n,4,3) dehomogenised q00, q10, q11, q01; p, q: (n,)."""
-    o, d, bounds, corners = (np.asarray(a, np.float64) for a in (o, d, bounds, corners))
-    dn = d / np.linalg.norm(d, axis=1, keepdims=True)
-    c = 0.5 * (bounds[:, :3] + bounds[:, 3:])
+    patch; nets: n homogeneous control nets, each ((p+1)(q+1), 4) row-major
+    (rows = u index); p, q: (n,).  For sample rows {0, p/2, p} x columns
+    {0, q/2, q} (an odd degree's middle = its two central indices), the mean
+    of the dehomogenised points at those indices, x, gives
+    (n1.(x - o), n2.(x - o)) / hs, hs = largest half extent of the box;
+    then the degrees (p - 3.5)/1.5, (q - 3.5)/1.5."""
+    o, d, bounds = (np.asarray(a, np.float64) for a in (o, d, bounds))
+    p, q = np.asarray(p), np.asarray(q)
+    n1, n2 = ray_planes(d)
     hs = 0.5 * np.maximum((bounds[:, 3:] - bounds[:, :3]).max(axis=1), 2e-20)
-    tc = np.maximum(0.0, ((c - o) * dn).sum(axis=1))
-    e = (o + tc[:, None] * dn - c) / hs[:, None]
-    cr = (corners - c[:, None, :]) / hs[:, None, None]
-    deg = np.stack([(np.asarray(p) - 3.5) / 1.5, (np.asarray(q) - 3.5) / 1.5], axis=1)
-    return np.concatenate([e, dn, deg, cr.reshape(-1, 12)], axis=1)
+    out = np.zeros((len(o), NIN))
+    for i in range(len(o)):
+        pi, qi = int(p[i]), int(q[i])
+        net = np.asarray(nets[i], np.float64).reshape(pi + 1, qi + 1, 4)
+        x = net[..., :3] / net[..., 3:]
+        rows = ((0, 0), (pi // 2, (pi + 1) // 2), (pi, pi))
+        cols = ((0, 0), (qi // 2, (qi + 1) // 2), (qi, qi))
+        for a, (r0, r1) in enumerate(rows):
+            for b, (c0, c1) in enumerate(cols):
+                m = 0.25 * (x[r0, c0] + x[r0, c1] + x[r1, c0] + x[r1, c1]) - o[i]
+                out[i, 2 * (3 * a + b)] = m @ n1[i] / hs[i]
+                out[i, 2 * (3 * a + b) + 1] = m @ n2[i] / hs[i]
+        out[i, 18] = (pi - 3.5) / 1.5
+        out[i, 19] = (qi - 3.5) / 1.5
+    return out
 
 
 def forward(w, X, quantize=False):

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "mlp.hpp"
+#include "rtn_fast.hpp"
 #include "rtn_internal.hpp"
 
 using namespace rtn;
@@ -73,8 +74,9 @@ __global__ void k_pack(const float* __restrict__ w, bf16x8* __restrict__ tiles, 
   tiles[gid] = r;
 }
 
-__global__ void k_features(const float* __restrict__ bounds, const float4* __restrict__ corners,
-                           const int* __restrict__ pq, int np, const float* ox, const float* oy, const float* oz,
+__global__ void k_features(const float* __restrict__ bounds, const float4* __restrict__ cp,
+                           const int* __restrict__ cp_off, const int* __restrict__ pq, int np, const float* ox,
+                           const float* oy, const float* oz,
                            const float* dx, const float* dy, const float* dz, int64_t n_rays,
                            const int* __restrict__ pair_ray, const int* __restrict__ pair_patch, int64_t n,
                            float* __restrict__ feats, int* __restrict__ bad) {
@@ -87,8 +89,10 @@ __global__ void k_features(const float* __restrict__ bounds, const float4* __res
   }
   float f[NIN];
   const int v = pq[p];
-  pair_features(ox[r], oy[r], oz[r], dx[r], dy[r], dz[r], bounds + 6 * (int64_t)p, corners[3 * p], corners[3 * p + 1],
-                corners[3 * p + 2], v & 0xff, (v >> 8) & 0xff, f);
+  const rtn::fast::f3 o{ox[r], oy[r], oz[r]};
+  const rtn::fast::Plane2 rp = rtn::fast::planes(o, rtn::fast::f3{dx[r], dy[r], dz[r]});
+  pair_features(o.x, o.y, o.z, rp.n1.x, rp.n1.y, rp.n1.z, rp.n2.x, rp.n2.y, rp.n2.z, cp + cp_off[p], v & 0xff,
+                (v >> 8) & 0xff, bounds + 6 * (int64_t)p, f);
 #pragma unroll
   for (int k = 0; k < NIN; k++) feats[i * NIN + k] = f[k];
 }
@@ -377,8 +381,8 @@ extern "C" int rtn_mlp_pair_features(const rtn_scene* s, const float* const rays
   DevBuf<int> bad;
   RTN_TRY(bad.alloc(1));
   RTN_HIP(hipMemsetAsync(bad.ptr, 0, sizeof(int), st));
-  hipLaunchKernelGGL(k_features, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, s->bounds.ptr, s->corners.ptr,
-                     s->pq.ptr, (int)s->n_patches, rays[0], rays[1], rays[2], rays[3], rays[4], rays[5], n_rays,
+  hipLaunchKernelGGL(k_features, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, s->bounds.ptr, s->cp.ptr,
+                     s->cp_off.ptr, s->pq.ptr, (int)s->n_patches, rays[0], rays[1], rays[2], rays[3], rays[4], rays[5], n_rays,
                      pair_ray, pair_patch, n, feats, bad.ptr);
   RTN_HIP(hipGetLastError());
   int hb = 0;

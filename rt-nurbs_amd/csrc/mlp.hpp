@@ -69,33 +69,47 @@ __device__ __forceinline__ float gelu_grad(float x) {
 }
 __device__ __forceinline__ float bf16r(float x) { return (float)(__bf16)x; }
 
-// Input features of a (ray, patch) pair in the frame of the patch's bounding
-// box: centre c = box centre, scale s = largest half extent.
-//   f[0..2]  ray point closest to c (ray parameter clamped at 0), (p - c) / s
-//   f[3..5]  unit ray direction
-//   f[6..7]  degrees, (p - 3.5) / 1.5 and (q - 3.5) / 1.5
-//   f[8..19] dehomogenised corner points q00, q10, q11, q01, (q - c) / s
-__device__ __forceinline__ void pair_features(float ox, float oy, float oz, float dx, float dy, float dz,
-                                              const float* __restrict__ bnd, float4 c0, float4 c1, float4 c2, int P,
-                                              int Q, float f[NIN]) {
-  const float il = __builtin_amdgcn_rsqf(fmaf(dx, dx, fmaf(dy, dy, dz * dz)));  // features end in BF16
-  const float nx = dx * il, ny = dy * il, nz = dz * il;
-  const float cx = 0.5f * (bnd[0] + bnd[3]), cy = 0.5f * (bnd[1] + bnd[4]), cz = 0.5f * (bnd[2] + bnd[5]);
+// Input features of a (ray, patch) pair.  On the two ray planes the ray is
+// the origin and the patch a 2D map of (u,v) whose control points are the
+// projected control points: the seed is where that map reaches the origin.
+// The features sample the projected net on a 3 x 3 grid of control indices,
+// rows {0, P/2, P} by columns {0, Q/2, Q} (an odd degree's middle averages its
+// two central indices), each sample the mean of the dehomogenised points at
+// those indices, projected as (n1.(x - o), n2.(x - o)) and divided by the
+// patch box's largest half extent hs:
+//   f[2 (3a + b)], f[2 (3a + b) + 1]  sample (row a, column b), a, b = 0..2
+//   f[18], f[19]                      degrees, (p - 3.5) / 1.5, (q - 3.5) / 1.5
+// n1, n2: unit ray-plane normals (perpendicular to the ray direction).
+// Reciprocals are hardware approximations: the features end in BF16.
+__device__ __forceinline__ void pair_features(float ox, float oy, float oz, float n1x, float n1y, float n1z,
+                                              float n2x, float n2y, float n2z, const float4* __restrict__ net,
+                                              int P, int Q, const float* __restrict__ bnd, float f[NIN]) {
   const float hs = fmaxf(fmaxf(bnd[3] - bnd[0], bnd[4] - bnd[1]), fmaxf(bnd[5] - bnd[2], 2e-20f)) * 0.5f;
   const float is = __builtin_amdgcn_rcpf(hs);
-  const float tc = fmaxf(0.0f, fmaf(cx - ox, nx, fmaf(cy - oy, ny, (cz - oz) * nz)));
-  f[0] = (fmaf(tc, nx, ox) - cx) * is;
-  f[1] = (fmaf(tc, ny, oy) - cy) * is;
-  f[2] = (fmaf(tc, nz, oz) - cz) * is;
-  f[3] = nx;
-  f[4] = ny;
-  f[5] = nz;
-  f[6] = ((float)P - 3.5f) * (1.0f / 1.5f);
-  f[7] = ((float)Q - 3.5f) * (1.0f / 1.5f);
-  const float q[12] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w, c2.x, c2.y, c2.z, c2.w};
-  const float cc[3] = {cx, cy, cz};
+  const int ri[4] = {0, P >> 1, (P + 1) >> 1, P}, ci[4] = {0, Q >> 1, (Q + 1) >> 1, Q};
+  float a1[4][4], a2[4][4];  // projected points at rows ri x columns ci
 #pragma unroll
-  for (int k = 0; k < 12; k++) f[8 + k] = (q[k] - cc[k % 3]) * is;
+  for (int r = 0; r < 4; r++)
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const float4 x = net[ri[r] * (Q + 1) + ci[c]];
+      const float iw = __builtin_amdgcn_rcpf(x.w);
+      const float ex = fmaf(x.x, iw, -ox), ey = fmaf(x.y, iw, -oy), ez = fmaf(x.z, iw, -oz);
+      a1[r][c] = fmaf(n1x, ex, fmaf(n1y, ey, n1z * ez)) * is;
+      a2[r][c] = fmaf(n2x, ex, fmaf(n2y, ey, n2z * ez)) * is;
+    }
+  // sample rows/columns as index pairs into ri / ci: {0,0}, {1,2}, {3,3}
+  const int lo[3] = {0, 1, 3}, hi[3] = {0, 2, 3};
+#pragma unroll
+  for (int sa = 0; sa < 3; sa++)
+#pragma unroll
+    for (int sb = 0; sb < 3; sb++) {
+      const int k = 2 * (3 * sa + sb);
+      f[k] = 0.25f * ((a1[lo[sa]][lo[sb]] + a1[lo[sa]][hi[sb]]) + (a1[hi[sa]][lo[sb]] + a1[hi[sa]][hi[sb]]));
+      f[k + 1] = 0.25f * ((a2[lo[sa]][lo[sb]] + a2[lo[sa]][hi[sb]]) + (a2[hi[sa]][lo[sb]] + a2[hi[sa]][hi[sb]]));
+    }
+  f[18] = ((float)P - 3.5f) * (1.0f / 1.5f);
+  f[19] = ((float)Q - 3.5f) * (1.0f / 1.5f);
 }
 
 __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
@@ -245,35 +259,21 @@ __device__ __forceinline__ void store_row(bf16x8 (*s_x)[4], int row, const float
   for (int c = 0; c < 4; c++) s_x[row][c] = r[c];
 }
 
-// Feature row of one pair written straight into s_x[row] in 8-value chunks
-// (the same values as pair_features, without holding all 20 in registers).
-__device__ __forceinline__ void store_features(bf16x8 (*s_x)[3], int row, float ox, float oy, float oz, float dx,
-                                               float dy, float dz, const float* __restrict__ bnd, float4 c0,
-                                               float4 c1, float4 c2, int P, int Q) {
-  const float il = __builtin_amdgcn_rsqf(fmaf(dx, dx, fmaf(dy, dy, dz * dz)));  // features end in BF16
-  const float nx = dx * il, ny = dy * il, nz = dz * il;
-  const float cx = 0.5f * (bnd[0] + bnd[3]), cy = 0.5f * (bnd[1] + bnd[4]), cz = 0.5f * (bnd[2] + bnd[5]);
-  const float hs = fmaxf(fmaxf(bnd[3] - bnd[0], bnd[4] - bnd[1]), fmaxf(bnd[5] - bnd[2], 2e-20f)) * 0.5f;
-  const float is = __builtin_amdgcn_rcpf(hs);
-  const float tc = fmaxf(0.0f, fmaf(cx - ox, nx, fmaf(cy - oy, ny, (cz - oz) * nz)));
-  bf16x8 r;
-  r[0] = (__bf16)((fmaf(tc, nx, ox) - cx) * is);
-  r[1] = (__bf16)((fmaf(tc, ny, oy) - cy) * is);
-  r[2] = (__bf16)((fmaf(tc, nz, oz) - cz) * is);
-  r[3] = (__bf16)nx;
-  r[4] = (__bf16)ny;
-  r[5] = (__bf16)nz;
-  r[6] = (__bf16)(((float)P - 3.5f) * (1.0f / 1.5f));
-  r[7] = (__bf16)(((float)Q - 3.5f) * (1.0f / 1.5f));
-  s_x[row][0] = r;
-  r[0] = (__bf16)((c0.x - cx) * is); r[1] = (__bf16)((c0.y - cy) * is); r[2] = (__bf16)((c0.z - cz) * is);
-  r[3] = (__bf16)((c0.w - cx) * is); r[4] = (__bf16)((c1.x - cy) * is); r[5] = (__bf16)((c1.y - cz) * is);
-  r[6] = (__bf16)((c1.z - cx) * is); r[7] = (__bf16)((c1.w - cy) * is);
-  s_x[row][1] = r;
-  r[0] = (__bf16)((c2.x - cz) * is); r[1] = (__bf16)((c2.y - cx) * is); r[2] = (__bf16)((c2.z - cy) * is);
-  r[3] = (__bf16)((c2.w - cz) * is);
-  r[4] = r[5] = r[6] = r[7] = (__bf16)0.0f;
-  s_x[row][2] = r;
+// Feature row of one pair (pair_features) written into s_x[row] as three
+// 8-value BF16 groups (values 20..23 zero).
+__device__ __forceinline__ void store_features(bf16x8 (*s_x)[3], int row, float ox, float oy, float oz, float n1x,
+                                               float n1y, float n1z, float n2x, float n2y, float n2z,
+                                               const float4* __restrict__ net, int P, int Q,
+                                               const float* __restrict__ bnd) {
+  float f[NIN];
+  pair_features(ox, oy, oz, n1x, n1y, n1z, n2x, n2y, n2z, net, P, Q, bnd, f);
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    bf16x8 r;
+#pragma unroll
+    for (int i = 0; i < 8; i++) r[i] = (__bf16)(8 * c + i < NIN ? f[8 * c + i < NIN ? 8 * c + i : 0] : 0.0f);
+    s_x[row][c] = r;
+  }
 }
 
 // Seeds for the active lanes of a wave from their feature rows: the active

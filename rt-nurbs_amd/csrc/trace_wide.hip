@@ -340,18 +340,30 @@ __device__ __forceinline__ void trace_wide_body(const WArgs& A) {
     in.pu = gu;
     in.pv = gv;
     if (LEARNED) {  // network seed for every pending pair of the wave (matrix cores), plain seed kept as fallback
+      // with seed sharing, a lane whose quad leader (lane & ~3) tests the same
+      // patch takes the leader's seed instead of an evaluation of its own
+      const int lead = lane & ~3;
+      const int lead_pi = __shfl(pending ? pi : -1, lead);
+      const bool own = pending && (!A.seed_share || lane == lead || lead_pi != pi);
       auto rows = reinterpret_cast<mlp::bf16x8(*)[3]>(&s_x[tid & ~63][0]);
       int n_act;
-      const int row = mlp::seed_row(pending, lane, n_act);
-      if (pending) {
-        mlp::store_features(rows, row, L.o.x, L.o.y, L.o.z, L.d.x, L.d.y, L.d.z, A.bounds + 6 * (int64_t)pi, c0, c1,
-                            c2, p, q);
+      const int row = mlp::seed_row(own, lane, n_act);
+      if (own) {
+        mlp::store_features(rows, row, L.o.x, L.o.y, L.o.z, L.rp.n1.x, L.rp.n1.y, L.rp.n1.z, L.rp.n2.x, L.rp.n2.y,
+                            L.rp.n2.z, A.cp + A.cp_off[pi], p, q, A.bounds + 6 * (int64_t)pi);
       } else {
         const mlp::bf16x8 z = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
         for (int k = 0; k < 3; k++) rows[row][k] = z;
       }
-      mlp::seed_wave_rows(A.mlp_tiles, A.mlp_bias, pending, row, n_act, in.gu, in.gv, rows, lane);
+      mlp::seed_wave_rows(A.mlp_tiles, A.mlp_bias, own, row, n_act, in.gu, in.gv, rows, lane);
+      if (A.seed_share) {
+        const float lu = __shfl(in.gu, lead), lv = __shfl(in.gv, lead);
+        if (pending && !own) {
+          in.gu = lu;
+          in.gv = lv;
+        }
+      }
     }
     in.seed = L.r + A.seed_base;
     in.max_iter = A.max_iter;
@@ -515,6 +527,7 @@ int fill_wide_args(const rtn_scene* s, const float* const rays[6], int64_t n, co
   RTN_CHECK(prm->learned_iters >= 0 && prm->learned_iters <= 64, RTN_EINVAL, "rtn_trace: learned_iters outside 0..64");
   RTN_CHECK(prm->fallback_iters >= 0 && prm->fallback_iters <= 64, RTN_EINVAL,
             "rtn_trace: fallback_iters outside 0..64");
+  RTN_CHECK(prm->seed_share == 0 || prm->seed_share == 1, RTN_EINVAL, "rtn_trace: seed_share is 0 or 1");
   RTN_CHECK(prm->flat_threshold <= 0.f, RTN_EUNSUPPORTED, "rtn_trace: flat-skip runs in the exact form only");
   A = WArgs{};
   A.nodes = s->wide.ptr;
@@ -540,6 +553,7 @@ int fill_wide_args(const rtn_scene* s, const float* const rays[6], int64_t n, co
     A.bounds = s->bounds.ptr;
     A.learned_iters = prm->learned_iters > 0 ? prm->learned_iters : prm->max_iter;
     A.fallback_iters = prm->fallback_iters > 0 ? prm->fallback_iters : prm->max_iter;
+    A.seed_share = prm->seed_share;
   }
   if (pairs) {
     A.pair_ray = pairs->ray;

@@ -38,6 +38,7 @@ struct WArgs {
   const float* bounds;
   int learned_iters;
   int fallback_iters;
+  int seed_share;
   // training pairs: every converged solve with t > 0 is appended
   int* pair_ray;
   int* pair_patch;

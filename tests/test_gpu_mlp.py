@@ -74,19 +74,18 @@ def test_gradient_matches_oracle(rtl):
 
 
 def _patch_geometry(dl):
-    """Bounds (true division by w) and dehomogenised corners per patch, from a scene download."""
+    """Bounds (true division by w) and control nets per patch, from a scene download."""
     cp, off, pq = dl["cp"].astype(np.float32), dl["off"], dl["pq"]
     n = len(off)
     bounds = np.zeros((n, 6), np.float32)
-    corners = np.zeros((n, 4, 3), np.float64)
+    nets = []
     for i in range(n):
         p, q = pq[i]
         net = cp[off[i]:off[i] + (p + 1) * (q + 1)]
         xyz = net[:, :3] / net[:, 3:4]
         bounds[i, :3], bounds[i, 3:] = xyz.min(0), xyz.max(0)
-        for k, j in enumerate((0, p * (q + 1), p * (q + 1) + q, q)):
-            corners[i, k] = net[j, :3] / net[j, 3]
-    return bounds, corners
+        nets.append(net)
+    return bounds, nets
 
 
 @pytest.mark.parametrize("name,mk", [("c1", lambda: scenes.config1(0, 96, 96)),
@@ -111,8 +110,10 @@ def test_pairs_and_features(rtl, name, mk):
     # features against the oracle
     f = rtl.pair_features(scene, rays, rtl.DeviceArray.from_host(pr), rtl.DeviceArray.from_host(pp), cnt).get()
     dl = scene.download()
-    bounds, corners = _patch_geometry(dl)
+    bounds, nets = _patch_geometry(dl)
     o = np.stack([a for a in rays.get()[:3]], 1)[pr]
     d = np.stack([a for a in rays.get()[3:]], 1)[pr]
-    ref = M.features(o, d, bounds[pp], corners[pp], dl["pq"][pp, 0], dl["pq"][pp, 1])
-    np.testing.assert_allclose(f, ref, rtol=1e-5, atol=2e-5)
+    ref = M.features(o, d, bounds[pp], [nets[k] for k in pp], dl["pq"][pp, 0], dl["pq"][pp, 1])
+    # FP32 projections with hardware reciprocals against FP64: far below the
+    # BF16 step (2^-8 relative) the features are rounded to
+    np.testing.assert_allclose(f, ref, rtol=1e-4, atol=1e-4)

@@ -99,30 +99,63 @@ def test_bf16_rounding_matches_torch():
     assert M.bf16(1 + 2 ** -8) == 1.0 and M.bf16(1 + 3 * 2 ** -8) == 1 + 2 ** -6
 
 
+def _flat_net(p, q, x0=0.5, y0=1.5, z=3.0, w=None):
+    """A flat (p, q) net over [x0, x0+1] x [y0, y0+1] at height z, rows along x."""
+    net = np.zeros((p + 1, q + 1, 4))
+    for i in range(p + 1):
+        for j in range(q + 1):
+            net[i, j] = [x0 + i / p, y0 + j / q, z, 1.0]
+    if w is not None:  # rational: scale the homogeneous point, same position
+        net *= w[..., None]
+    return net.reshape(-1, 4)
+
+
 def test_features_closed_forms():
-    # unit box centred at (1,2,3), half extent 0.5; a ray through the centre
-    bounds = np.array([[0.5, 1.5, 2.5, 1.5, 2.5, 3.5]])
-    corners = np.array([[[0.5, 1.5, 3.0], [1.5, 1.5, 3.0], [1.5, 2.5, 3.0], [0.5, 2.5, 3.0]]])
-    o = np.array([[1.0, 2.0, -7.0]])
-    d = np.array([[0.0, 0.0, 2.0]])
-    f = M.features(o, d, bounds, corners, np.array([3]), np.array([2]))[0]
-    np.testing.assert_allclose(f[:3], 0.0, atol=1e-15)           # closest point = centre
-    np.testing.assert_allclose(f[3:6], [0, 0, 1])                # unit direction
-    np.testing.assert_allclose(f[6:8], [-1 / 3, -1.0])           # degrees (p - 3.5) / 1.5
-    np.testing.assert_allclose(f[8:].reshape(4, 3), [[-1, -1, 0], [1, -1, 0], [1, 1, 0], [-1, 1, 0]])
-    # ray pointing away from the box: the closest point is the origin itself
-    f2 = M.features(np.array([[1.0, 2.0, 5.0]]), np.array([[0.0, 0.0, 1.0]]), bounds, corners, [3], [3])[0]
-    np.testing.assert_allclose(f2[:3], [0, 0, 4.0])
+    # ray planes: for a ray along +z, n1 = (0, 1, 0), n2 = n1 x z = (1, 0, 0)
+    n1, n2 = M.ray_planes(np.array([[0.0, 0.0, 2.0], [3.0, 0.0, 0.0]]))
+    np.testing.assert_allclose(n1[0], [0, 1, 0])
+    np.testing.assert_allclose(n2[0], [1, 0, 0])
+    np.testing.assert_allclose(n1[1], [0, -1, 0])   # x-dominant: (d.y, -d.x, 0)/|.|
+    np.testing.assert_allclose(n2[1], [0, 0, 1])    # (0, -1, 0) x (1, 0, 0)
+    # flat unit net centred at (1, 2, 3), box half extent 0.5; a ray along +z
+    # through (1, 2): the samples are the net's corners, edge middles and
+    # centre at (n1, n2) = (y - 2, x - 1) / 0.5
+    bounds = np.array([[0.5, 1.5, 3.0, 1.5, 2.5, 3.0]])
+    o, d = np.array([[1.0, 2.0, -7.0]]), np.array([[0.0, 0.0, 2.0]])
+    for p, q in ((2, 2), (3, 2), (3, 5), (4, 3)):
+        f = M.features(o, d, bounds, [_flat_net(p, q)], [p], [q])[0]
+        s = f[:18].reshape(3, 3, 2)
+        for a, xa in enumerate((-1.0, 0.0, 1.0)):
+            for b, yb in enumerate((-1.0, 0.0, 1.0)):
+                np.testing.assert_allclose(s[a, b], [yb, xa], atol=1e-12)
+        np.testing.assert_allclose(f[18:], [(p - 3.5) / 1.5, (q - 3.5) / 1.5])
+    # weights change the homogeneous net, not the dehomogenised points
+    w = np.random.default_rng(3).uniform(0.5, 2.0, (4, 3))
+    np.testing.assert_allclose(M.features(o, d, bounds, [_flat_net(3, 2, w=w)], [3], [2]),
+                               M.features(o, d, bounds, [_flat_net(3, 2)], [3], [2]), atol=1e-12)
+    # moving the ray by (dx, dy) shifts every sample by -(dy, dx) / 0.5
+    f0 = M.features(o, d, bounds, [_flat_net(3, 3)], [3], [3])[0]
+    f1 = M.features(o + [0.1, -0.2, 0.0], d, bounds, [_flat_net(3, 3)], [3], [3])[0]
+    np.testing.assert_allclose((f1 - f0)[:18].reshape(9, 2), np.tile([0.4, -0.2], (9, 1)), atol=1e-12)
+    # odd degree: the middle sample averages the two central rows
+    net = _flat_net(3, 2).reshape(4, 3, 4)
+    net[1, :, 2] += 0.3   # lift row 1 along z: no effect on the projection along z
+    net[2, :, 0] += 0.2   # move row 2 along x by 0.2: the middle row moves by 0.1
+    f2 = M.features(o, d, bounds, [net.reshape(-1, 4)], [3], [2])[0]
+    g0 = M.features(o, d, bounds, [_flat_net(3, 2)], [3], [2])[0]
+    np.testing.assert_allclose((f2 - g0)[:18].reshape(3, 3, 2)[1, :, 1], 0.1 / 0.5, atol=1e-12)
+    np.testing.assert_allclose((f2 - g0)[:18].reshape(3, 3, 2)[[0, 2]], 0.0, atol=1e-12)
     # similarity invariance: translate and scale the whole configuration
     rng = np.random.default_rng(2)
     o, d = rng.normal(size=(5, 3)), rng.normal(size=(5, 3))
     lo = rng.normal(size=(5, 3))
     b = np.concatenate([lo, lo + rng.uniform(0.1, 2, (5, 3))], axis=1)
-    cr = rng.normal(size=(5, 4, 3))
     p = np.array([2, 3, 4, 5, 3])
-    a = M.features(o, d, b, cr, p, p)
+    nets = [np.concatenate([rng.normal(size=((k + 1) * (k + 1), 3)), np.ones(((k + 1) * (k + 1), 1))], 1) for k in p]
+    a = M.features(o, d, b, nets, p, p)
     s, t = 3.7, np.array([1.0, -2.0, 0.5])
-    a2 = M.features(o * s + t, d, b * s + np.tile(t, 2), cr * s + t, p, p)
+    nets2 = [np.concatenate([n[:, :3] * s + t, n[:, 3:]], 1) for n in nets]
+    a2 = M.features(o * s + t, d, b * s + np.tile(t, 2), nets2, p, p)
     np.testing.assert_allclose(a, a2, rtol=1e-12, atol=1e-12)
 
 
