@@ -76,4 +76,5 @@ def test_learned_trace_keeps_the_hits(rtl, net):
             assert (gb["hit"] >= 0).sum() >= 0.995 * (ga["hit"] >= 0).sum(), (mode, share)
             # the learned attempt converges for most hits: fewer fallbacks than Newton calls
             assert 0 < tb["learned_fallbacks"] < 0.8 * tb["newton_calls"], (mode, share)
-            assert tb["newton_calls"] == ta["newton_calls"]
+            # the same patch tests, up to the pruning that different closest hits cause
+            assert abs(tb["newton_calls"] - ta["newton_calls"]) < 0.01 * ta["newton_calls"]
