@@ -275,6 +275,9 @@ def bench_c2_c1(args, L, scenes, learned):
         "persistent_hull2": dict(mode=P, hull_cull=2, **BASE),
         "wide_hull2_mixed": dict(mode=W, hull_cull=2, **mixed, **BASE),
         "persistent_hull2_mixed": dict(mode=P, hull_cull=2, **mixed, **BASE),
+        # abandon solves stuck against the patch boundary (misses) from the 4th evaluation
+        "wide_hull2_mixed_stall": dict(mode=W, hull_cull=2, stall_exit=1, **mixed, **BASE),
+        "persistent_hull2_mixed_stall": dict(mode=P, hull_cull=2, stall_exit=1, **mixed, **BASE),
     }
     if net is not None:
         # a 3-evaluation attempt from the network's seed (two Newton steps: the fewest that
@@ -284,6 +287,7 @@ def bench_c2_c1(args, L, scenes, learned):
         lrn = dict(seed_mode=L.SEED_LEARNED, seed_net=net, learned_iters=3, fallback_iters=8, seed_share=1)
         forms["wide_learned"] = dict(mode=W, hull_cull=2, **lrn, **BASE)
         forms["persistent_learned_mixed"] = dict(mode=P, hull_cull=2, **lrn, **mixed, **BASE)
+        forms["persistent_learned_mixed_stall"] = dict(mode=P, hull_cull=2, stall_exit=1, **lrn, **mixed, **BASE)
     if args.forms != "all":
         keep = set(args.forms.split(","))
         forms = {k: v for k, v in forms.items() if k in keep}

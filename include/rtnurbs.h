@@ -218,6 +218,12 @@ typedef struct {
                              rays: a pixel's samples, or neighbouring pixels) that test the same patch
                              in the same pass start from one network evaluation, the quad's first
                              ray's; 0 = one evaluation per (ray, patch) pair     */
+  int stall_exit;         /* fast forms: 1 = abandon a Newton solve as a miss from its 4th evaluation
+                             on when two consecutive updates leave the (u,v) domain on the same
+                             side(s) and the residual fell by less than 10% (stuck against the
+                             patch boundary); 0 = always run the full iteration count. The
+                             reference's Newton carries a disabled divergence exit
+                             (bezier_patch.h:231-232)                             */
   const struct rtn_mlp* seed_net; /* RTN_SEED_LEARNED: the seed network          */
 } rtn_trace_params;
 

@@ -356,13 +356,19 @@ __device__ __forceinline__ void eval_q(const float4* __restrict__ net, int P, fl
   }
 }
 
+// Newton on one patch (compile-time row length Q+1).  With `stall`, a solve
+// is abandoned as a miss from its 4th evaluation on when two consecutive
+// updates leave the parameter domain on the same side(s) and the residual fell
+// by less than 10%: the iterate is stuck against the patch boundary.
 template <int Q>
 __device__ __forceinline__ Hit newton_q(const float4* __restrict__ net, int P, const Plane2& rp, float u0, float v0,
-                                        float thr_det, float thr_res, uint32_t seed, int max_iter) {
+                                        float thr_det, float thr_res, uint32_t seed, int max_iter, bool stall) {
   float u = u0, v = v0;
   Hit h;
   h.hit = false;
   h.evals = 0;
+  int prev_side = 0;
+  float prev_err = INFINITY;
 #pragma unroll 1
   for (int it = 1; it <= max_iter; it++) {
     float4 pt, du, dv;
@@ -373,7 +379,8 @@ __device__ __forceinline__ Hit newton_q(const float4* __restrict__ net, int P, c
     const float r0 = dot3(rp.n1, S) + rp.d1;
     const float r1 = dot3(rp.n2, S) + rp.d2;
     const float m0 = dot3(rp.n1, Su), m1 = dot3(rp.n2, Su), m2 = dot3(rp.n1, Sv), m3 = dot3(rp.n2, Sv);
-    if (fabsf(r0) + fabsf(r1) < thr_res) {
+    const float err = fabsf(r0) + fabsf(r1);
+    if (err < thr_res) {
       h.hit = true;
       h.u = u; h.v = v; h.S = S; h.Su = Su; h.Sv = Sv;
       h.m0 = m0; h.m1 = m1; h.m2 = m2; h.m3 = m3;
@@ -391,6 +398,12 @@ __device__ __forceinline__ Hit newton_q(const float4* __restrict__ net, int P, c
       const float id = 1.0f / det;
       u = u - fmaf(m3, r0, -m2 * r1) * id;
       v = v - fmaf(-m1, r0, m0 * r1) * id;
+    }
+    if (stall) {
+      const int side = (u < 0.0f) | (u > 1.0f) << 1 | (v < 0.0f) << 2 | (v > 1.0f) << 3;
+      if (it >= 4 && side != 0 && side == prev_side && err > 0.9f * prev_err) return h;
+      prev_side = side;
+      prev_err = err;
     }
     u = fminf(fmaxf(u, 0.0f), 1.0f);
     v = fminf(fmaxf(v, 0.0f), 1.0f);

@@ -149,7 +149,7 @@ struct LeafIn {
   f3 o, d;
   float gu, gv, pu, pv;
   uint32_t seed;
-  int max_iter, learned_iters, fallback_iters, mixed, learned;
+  int max_iter, learned_iters, fallback_iters, mixed, learned, stall;
   float thr_det, thr_res, cond_thr;
 };
 
@@ -168,7 +168,7 @@ __device__ __forceinline__ LeafOut leaf_q(const float4* __restrict__ net, int P,
 #pragma unroll 1
   for (int attempt = 0;; attempt++) {
     h = newton_q<Q>(net, P, in.rp, su, sv, in.thr_det, in.thr_res, in.seed,
-                    !in.learned ? in.max_iter : attempt == 0 ? in.learned_iters : in.fallback_iters);
+                    !in.learned ? in.max_iter : attempt == 0 ? in.learned_iters : in.fallback_iters, in.stall != 0);
     o.evals += h.evals;
     if (h.hit || !in.learned || attempt == 1) break;
     o.flags |= 8;
@@ -370,6 +370,7 @@ __device__ __forceinline__ void trace_wide_body(const WArgs& A) {
     in.learned_iters = A.learned_iters;
     in.fallback_iters = A.fallback_iters;
     in.mixed = A.mixed;
+    in.stall = A.stall_exit;
     in.learned = LEARNED;
     in.thr_det = A.thr_det;
     in.thr_res = A.thr_res;
@@ -528,6 +529,7 @@ int fill_wide_args(const rtn_scene* s, const float* const rays[6], int64_t n, co
   RTN_CHECK(prm->fallback_iters >= 0 && prm->fallback_iters <= 64, RTN_EINVAL,
             "rtn_trace: fallback_iters outside 0..64");
   RTN_CHECK(prm->seed_share == 0 || prm->seed_share == 1, RTN_EINVAL, "rtn_trace: seed_share is 0 or 1");
+  RTN_CHECK(prm->stall_exit == 0 || prm->stall_exit == 1, RTN_EINVAL, "rtn_trace: stall_exit is 0 or 1");
   RTN_CHECK(prm->flat_threshold <= 0.f, RTN_EUNSUPPORTED, "rtn_trace: flat-skip runs in the exact form only");
   A = WArgs{};
   A.nodes = s->wide.ptr;
@@ -546,6 +548,7 @@ int fill_wide_args(const rtn_scene* s, const float* const rays[6], int64_t n, co
   A.mixed = prm->precision == RTN_PREC_MIXED;
   A.cond_thr = prm->cond_thr;
   A.any_hit = prm->any_hit;
+  A.stall_exit = prm->stall_exit;
   A.refill_min = prm->refill_min > 0 ? prm->refill_min : 64;
   if (learned) {
     A.mlp_tiles = reinterpret_cast<const mlp::bf16x8*>(mlp_tiles(prm->seed_net));

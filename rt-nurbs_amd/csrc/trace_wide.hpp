@@ -30,6 +30,7 @@ struct WArgs {
   float cull_margin;
   int mixed;
   int any_hit;
+  int stall_exit;         // abandon Newton solves stuck against the patch boundary
   int refill_min;         // persistent: lanes idle before a wave takes new rays
   float cond_thr;
   // learned form: BF16 MFMA tiles and biases of the seed network, patch boxes

@@ -63,7 +63,7 @@ class TraceParams(C.Structure):
                 ("seed_mode", C.c_int), ("precision", C.c_int), ("cond_thr", C.c_float),
                 ("hull_cull", C.c_int), ("count", C.c_int), ("learned_iters", C.c_int),
                 ("any_hit", C.c_int), ("refill_min", C.c_int), ("fallback_iters", C.c_int),
-                ("seed_share", C.c_int), ("seed_net", C.c_void_p)]
+                ("seed_share", C.c_int), ("stall_exit", C.c_int), ("seed_net", C.c_void_p)]
 
 
 class MlpTrainParams(C.Structure):
