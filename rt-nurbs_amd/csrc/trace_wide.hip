@@ -498,14 +498,12 @@ int launch_wide_kernel(const LaunchShape& sh, bool persistent, bool count, bool 
   RTN_CHECK(shape_available(sh, persistent, count, learned), RTN_EUNSUPPORTED,
             "trace: launch shape %d lanes x %d stack entries is not built for this form", sh.wb, sh.wstk);
   const int bpc = std::max(1, sh.blocks_per_cu);
-  if (sh.wb == WB_DEFAULT && sh.wstk == WSTK_DEFAULT) {
-    if (persistent) {
-      if (count) launch_shape<WB_DEFAULT, WSTK_DEFAULT, true, true>(learned, bpc, n_max, stream, args);
-      else launch_shape<WB_DEFAULT, WSTK_DEFAULT, true, false>(learned, bpc, n_max, stream, args);
-    } else {
-      if (count) launch_shape<WB_DEFAULT, WSTK_DEFAULT, false, true>(learned, bpc, n_max, stream, args);
-      else launch_shape<WB_DEFAULT, WSTK_DEFAULT, false, false>(learned, bpc, n_max, stream, args);
-    }
+  if (!persistent) {  // the default shape's stack depth in single-wave blocks
+    if (count) launch_shape<WB_ONE_RAY, WSTK_DEFAULT, false, true>(learned, bpc, n_max, stream, args);
+    else launch_shape<WB_ONE_RAY, WSTK_DEFAULT, false, false>(learned, bpc, n_max, stream, args);
+  } else if (sh.wb == WB_DEFAULT && sh.wstk == WSTK_DEFAULT) {
+    if (count) launch_shape<WB_DEFAULT, WSTK_DEFAULT, true, true>(learned, bpc, n_max, stream, args);
+    else launch_shape<WB_DEFAULT, WSTK_DEFAULT, true, false>(learned, bpc, n_max, stream, args);
   } else {
 #define RTN_SHAPE(W, S) \
   else if (sh.wb == W && sh.wstk == S) launch_one<W>(k_trace_wide<W, S, true, false, false>, true, bpc, n_max, stream, args);

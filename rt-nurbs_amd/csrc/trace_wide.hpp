@@ -11,8 +11,11 @@
 
 namespace rtn {
 
-constexpr int WB_DEFAULT = 128;   // lanes per block (2 waves)
+constexpr int WB_DEFAULT = 128;   // lanes per block (2 waves) of the persistent form
 constexpr int WSTK_DEFAULT = 20;  // stack entries per lane: 20 KB of LDS per 128-lane block
+// The one-ray-per-lane form runs single-wave blocks: a finished wave frees its
+// slot at once instead of waiting for the other waves of its block (+5%).
+constexpr int WB_ONE_RAY = 64;
 
 // Kernel arguments, passed by value.
 struct WArgs {
