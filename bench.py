@@ -370,7 +370,8 @@ def bench_c5(args, L, scenes):
             L.generate_rays(cam, spp=4, jitter=True, seed=1, rays=rays, offset=off, tiled=True)
             off += nr
         ids = L.DeviceArray.from_host(np.repeat(np.arange(len(items), dtype=np.int32), counts))
-        return L.Frame(batch, rays, ids, L.frame_params(**BASE, hull_cull=2)), rays, ids
+        fp = L.frame_params(**BASE, hull_cull=2, stall_exit=1, precision=L.PREC_MIXED, cond_thr=1e4)
+        return L.Frame(batch, rays, ids, fp), rays, ids
 
     proxy, _, _ = frame_for(4)
     table = proxy.autotune(reps=2)
@@ -399,6 +400,7 @@ def bench_c5(args, L, scenes):
         f"{total_rays / ms / 1e3:.1f} Mrays/s")
     wl = {"workload": f"{len(built)} scenes, {n_patches} patches (1k-200k each, degree 2-5), 256^2-2048^2 x 4 spp "
                       "+ shadow and mirror ray per hit, one graph-captured frame",
+          "form": "persistent, projected-hull rejection, mixed precision, boundary-stall exit",
           "primary_rays": st["n_primary"], "secondary_rays": st["n_secondary"], "patches": n_patches,
           "launch_shape": {"lanes": st["shape_lanes"], "stack": st["shape_stack"],
                            "blocks_per_cu": st["shape_blocks_per_cu"], "refill_min": st["refill_min"]}}

@@ -387,11 +387,12 @@ int rtn_frame_results(const rtn_frame* frame, rtn_hits* primary, rtn_hits* secon
 int rtn_frame_set_shape(rtn_frame* frame, int lanes, int stack, int blocks_per_cu);
 /* Refill threshold of both persistent launches (see rtn_trace_params.refill_min). */
 int rtn_frame_set_refill(rtn_frame* frame, int refill_min);
-/* Autotuner: times the frame over every built launch shape at 2, 4 and 8
- * blocks per CU (whole-wave refill; reps timed runs each, stack overflow
- * rejects a shape), then the best shape at refill thresholds 1, 16 and 32
- * idle lanes, and keeps the fastest.  table (optional) receives each
- * candidate, ms < 0 for rejected ones.  Drops a captured graph. */
+/* Autotuner: times the frame over every built launch shape at 2, 4, 8 and 12
+ * blocks per CU (capped at the blocks that stay resident; whole-wave refill;
+ * reps timed runs each; stack overflow rejects a shape), then the best shape
+ * at refill thresholds 1, 16 and 32 idle lanes, and keeps the fastest.  table
+ * (optional) receives each candidate, ms < 0 for rejected ones.  Drops a
+ * captured graph. */
 int rtn_frame_autotune(rtn_frame* frame, int reps, rtn_tune_entry* table, int table_cap, int* n_tried,
                        void* stream);
 

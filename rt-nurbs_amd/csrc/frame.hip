@@ -422,7 +422,7 @@ extern "C" int rtn_frame_set_refill(rtn_frame* f, int refill_min) {
 }
 
 // Autotuner: every built launch shape (lanes per block x stack entries) at
-// 2, 4 and 8 resident blocks per compute unit with whole-wave refill, then
+// 2, 4, 8 and 12 resident blocks per compute unit with whole-wave refill, then
 // the best shape at refill thresholds 1, 16 and 32; each candidate runs the
 // frame once to warm up and `reps` times under HIP events; candidates whose
 // stack overflows are rejected.  The fastest is kept (and any graph dropped).
@@ -433,9 +433,9 @@ extern "C" int rtn_frame_autotune(rtn_frame* f, int reps, rtn_tune_entry* table,
   drop_graph(f);
   struct Cand { LaunchShape sh; int refill; };
   std::vector<Cand> cand;
-  const int bpcs[3] = {2, 4, 8};
+  const int bpcs[4] = {2, 4, 8, 12};  // capped at the resident blocks when launched
   const bool learned = frame_learned(f);
-  for (int b = 0; b < 3; b++) {
+  for (int b = 0; b < 4; b++) {
     cand.push_back({{kDefaultShape.wb, kDefaultShape.wstk, bpcs[b]}, 64});
     if (!learned)
       for (int i = 0; i < kNumTunedShapes; i++)

@@ -15,12 +15,12 @@
 // the seed is the reference's bilinear seed in its own operation order; the
 // iteration keeps the plain method's convergence test, PCG perturbation on a
 // singular Jacobian and clamping.  Options: exact rejection of primitives from
-// their projected control points (at leaf entry, or per class before Newton),
-// the learned seed (network on the matrix cores for all pending pairs of the
-// wave, plain seed as fallback), and an FP64 correction step for hits whose
-// Jacobian condition number exceeds a threshold (mixed precision).  Improving
-// hits are written through to the output arrays, as the reference writes its
-// record (intersect.cu:133-138).
+// their projected control points at leaf entry, the learned seed (network on
+// the matrix cores for all pending pairs of the wave, plain seed as fallback),
+// an exit for solves stuck against the patch boundary, and an FP64 correction
+// step for hits whose Jacobian condition number exceeds a threshold (mixed
+// precision).  Improving hits are written through to the output arrays, as the
+// reference writes its record (intersect.cu:133-138).
 #include <algorithm>
 
 #include "mlp.hpp"
