@@ -369,6 +369,7 @@ __device__ __forceinline__ Hit newton_q(const float4* __restrict__ net, int P, c
   h.evals = 0;
   int prev_side = 0;
   float prev_err = INFINITY;
+  int pend = 0;  // PCG draws owed: taken only on the rare singular step
 #pragma unroll 1
   for (int it = 1; it <= max_iter; it++) {
     float4 pt, du, dv;
@@ -386,12 +387,17 @@ __device__ __forceinline__ Hit newton_q(const float4* __restrict__ net, int P, c
       h.m0 = m0; h.m1 = m1; h.m2 = m2; h.m3 = m3;
       return h;
     }
-    seed = pcg_hash(seed);
-    const float rand1 = (float)seed * 2.3283064e-10f;
-    seed = pcg_hash(seed);
-    const float rand2 = (float)seed * 2.3283064e-10f;
+    // two PCG draws per iteration, as the plain method; the state is only
+    // advanced when a singular step uses them (same values, same results)
+    pend += 2;
     const float det = fmaf(m0, m3, -m2 * m1);
     if (fabsf(det) < thr_det) {
+      for (; pend > 2; pend--) seed = pcg_hash(seed);
+      seed = pcg_hash(seed);
+      const float rand1 = (float)seed * 2.3283064e-10f;
+      seed = pcg_hash(seed);
+      const float rand2 = (float)seed * 2.3283064e-10f;
+      pend = 0;
       u = fmaf(0.1f, (u0 - u) * rand1, u);
       v = fmaf(0.1f, (v0 - v) * rand2, v);
     } else {
