@@ -135,7 +135,8 @@ extern "C" int rtn_batch_create(rtn_batch** out, rtn_scene* const* scenes, int k
   auto fail = [&](int rc) { delete b; return rc; };
   int rc;
   if ((rc = m.cp.alloc(ncp)) || (rc = m.cp_off.alloc(np)) || (rc = m.pq.alloc(np)) || (rc = m.bounds.alloc(6 * np)) ||
-      (rc = m.delta.alloc(np)) || (rc = m.corners.alloc(3 * np)) || (rc = m.wide.alloc(nw)))
+      (rc = m.delta.alloc(np)) || (rc = m.corners.alloc(3 * np)) || (rc = m.obb.alloc(4 * np)) ||
+      (rc = m.wide.alloc(nw)))
     return fail(rc);
   int64_t pb = 0, cb = 0, wb = 0;
   b->patch_base.push_back(0);
@@ -144,7 +145,8 @@ extern "C" int rtn_batch_create(rtn_batch** out, rtn_scene* const* scenes, int k
     const int spn = (int)s->n_patches, sw = (int)s->n_wide;
     if ((rc = copy_into(m.cp, cb, s->cp, s->n_cp)) || (rc = copy_into(m.pq, pb, s->pq, spn)) ||
         (rc = copy_into(m.bounds, 6 * pb, s->bounds, 6 * (size_t)spn)) || (rc = copy_into(m.delta, pb, s->delta, spn)) ||
-        (rc = copy_into(m.corners, 3 * pb, s->corners, 3 * (size_t)spn)))
+        (rc = copy_into(m.corners, 3 * pb, s->corners, 3 * (size_t)spn)) ||
+        (rc = copy_into(m.obb, 4 * pb, s->obb, 4 * (size_t)spn)))
       return fail(rc);
     hipLaunchKernelGGL(k_add_i32, dim3((spn + 255) / 256), dim3(256), 0, nullptr, s->cp_off.ptr, spn, (int)cb,
                        m.cp_off.ptr + pb);

@@ -494,6 +494,33 @@ __device__ __forceinline__ bool correct64_rt(const float4* __restrict__ net, int
 namespace rtn {
 namespace fast {
 
+// Exact rejection of a (ray, patch) pair from the patch's oriented box (the
+// box holds the dehomogenised control points, hence the surface, when the
+// weights are positive).  Newton converges only where |r0| + |r1| < thr, and
+// |r0| + |r1| >= the distance from the ray's line, so a line that misses the
+// box widened by the margin cannot converge on the patch.  Line (not ray)
+// against box, slab test in the box frame with hardware reciprocals; the
+// widening also covers the FP32 rounding of the transform.
+__device__ __forceinline__ bool obb_culled(const float4* __restrict__ box, f3 o, f3 d, float margin) {
+  const float4 a0 = box[0], a1 = box[1], a2 = box[2], c = box[3];
+  const float ox = o.x - c.x, oy = o.y - c.y, oz = o.z - c.z;
+  const float l0 = fmaf(a0.x, ox, fmaf(a0.y, oy, a0.z * oz)), l1 = fmaf(a1.x, ox, fmaf(a1.y, oy, a1.z * oz)),
+              l2 = fmaf(a2.x, ox, fmaf(a2.y, oy, a2.z * oz));
+  const float d0 = fmaf(a0.x, d.x, fmaf(a0.y, d.y, a0.z * d.z)), d1 = fmaf(a1.x, d.x, fmaf(a1.y, d.y, a1.z * d.z)),
+              d2 = fmaf(a2.x, d.x, fmaf(a2.y, d.y, a2.z * d.z));
+  const float slack = margin + 1e-6f * (fabsf(o.x) + fabsf(o.y) + fabsf(o.z) + fabsf(c.x) + fabsf(c.y) + fabsf(c.z));
+  const float e0 = a0.w + slack, e1 = a1.w + slack, e2 = a2.w + slack;
+  const float i0 = __builtin_amdgcn_rcpf(fabsf(d0) > 1e-30f ? d0 : copysignf(1e-30f, d0));
+  const float i1 = __builtin_amdgcn_rcpf(fabsf(d1) > 1e-30f ? d1 : copysignf(1e-30f, d1));
+  const float i2 = __builtin_amdgcn_rcpf(fabsf(d2) > 1e-30f ? d2 : copysignf(1e-30f, d2));
+  const float p0 = (-e0 - l0) * i0, q0 = (e0 - l0) * i0;
+  const float p1 = (-e1 - l1) * i1, q1 = (e1 - l1) * i1;
+  const float p2 = (-e2 - l2) * i2, q2 = (e2 - l2) * i2;
+  const float tn = fmaxf(fmaxf(fminf(p0, q0), fminf(p1, q1)), fminf(p2, q2));
+  const float tf = fminf(fminf(fmaxf(p0, q0), fmaxf(p1, q1)), fmaxf(p2, q2));
+  return tn > tf + 1e-5f * (fabsf(tn) + fabsf(tf));
+}
+
 // Exact rejection of a (ray, patch) pair from its control points projected on
 // the ray planes.  With positive weights, the residual vector (r0, r1) of any
 // surface point is a convex combination of the projected control points

@@ -135,6 +135,7 @@ struct rtn_scene {
   rtn::DevBuf<float> bounds;       // 6 per patch, BVH order
   rtn::DevBuf<float> delta;        // patchDelta, BVH order
   rtn::DevBuf<float4> corners;     // dehomogenised corners q00,q10,q11,q01 (xyz) packed 3 float4
+  rtn::DevBuf<float4> obb;         // oriented box of the dehomogenised net: (axis k, half extent k) x3, (centre, 0)
   rtn::DevBuf<rtn_bvh_node> nodes; // reference layout
   rtn::DevBuf<rtn::WideNode> wide; // 4-wide SoA nodes
   std::vector<int64_t> deg_hist;   // (RTN_MAX_DEGREE+1)^2

@@ -201,6 +201,65 @@ __global__ void k_prep(const float4* cp, const int* cp_off, int* pq, int np, flo
   pq[g] = (pq[g] & 0xffff) | (pos ? 0x10000 : 0);
 }
 
+// Oriented bounding box of each patch's dehomogenised control points (they
+// bound the surface when the weights are positive): the normal axis w from the
+// cross product of the corner diagonals, u along the mean of the two u-edges
+// made perpendicular to w, v = w x u (the coordinate axes when a patch's
+// corners are degenerate).  Stored as (u, e_u), (v, e_v), (w, e_w), (centre, 0)
+// with half extents widened by a relative slack that covers the FP32 rounding
+// of this projection and of the ray's transform into the box frame.
+__global__ void k_obb(const float4* cp, const int* cp_off, const int* pq, int np, float4* obb) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= np) return;
+  const int p = pq[g] & 0xff, q = (pq[g] >> 8) & 0xff, row = q + 1, n = (p + 1) * row;
+  const float4* c = cp + cp_off[g];
+  auto dh = [](float4 a) { return make_float3(a.x / a.w, a.y / a.w, a.z / a.w); };
+  auto sub = [](float3 a, float3 b) { return make_float3(a.x - b.x, a.y - b.y, a.z - b.z); };
+  auto dot = [](float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; };
+  auto cross = [](float3 a, float3 b) {
+    return make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+  };
+  const float3 q00 = dh(c[0]), q10 = dh(c[p * row]), q11 = dh(c[p * row + q]), q01 = dh(c[q]);
+  float3 w = cross(sub(q11, q00), sub(q01, q10));
+  float wl = sqrtf(dot(w, w));
+  float3 ax[3] = {make_float3(1.f, 0.f, 0.f), make_float3(0.f, 1.f, 0.f), make_float3(0.f, 0.f, 1.f)};
+  if (wl > 1e-30f && wl < INFINITY) {
+    w = make_float3(w.x / wl, w.y / wl, w.z / wl);
+    float3 u = make_float3(q10.x - q00.x + q11.x - q01.x, q10.y - q00.y + q11.y - q01.y, q10.z - q00.z + q11.z - q01.z);
+    const float uw = dot(u, w);
+    u = make_float3(u.x - w.x * uw, u.y - w.y * uw, u.z - w.z * uw);
+    float ul = sqrtf(dot(u, u));
+    if (!(ul > 1e-30f && ul < INFINITY)) {  // any unit vector perpendicular to w
+      u = fabsf(w.x) < 0.9f ? make_float3(0.f, w.z, -w.y) : make_float3(-w.z, 0.f, w.x);
+      ul = sqrtf(dot(u, u));
+    }
+    u = make_float3(u.x / ul, u.y / ul, u.z / ul);
+    ax[0] = u;
+    ax[1] = cross(w, u);
+    ax[2] = w;
+  }
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  float mag = 0.0f;
+  for (int i = 0; i < n; i++) {
+    const float3 x = dh(c[i]);
+    mag = fmaxf(mag, fabsf(x.x) + fabsf(x.y) + fabsf(x.z));
+    for (int k = 0; k < 3; k++) {
+      const float y = dot(ax[k], x);
+      lo[k] = fminf(lo[k], y);
+      hi[k] = fmaxf(hi[k], y);
+    }
+  }
+  float3 ctr = make_float3(0.f, 0.f, 0.f);
+  float e[3];
+  for (int k = 0; k < 3; k++) {
+    const float m = 0.5f * (lo[k] + hi[k]);
+    ctr = make_float3(ctr.x + ax[k].x * m, ctr.y + ax[k].y * m, ctr.z + ax[k].z * m);
+    e[k] = 0.5f * (hi[k] - lo[k]) + 4e-6f * mag + 1e-6f;
+  }
+  for (int k = 0; k < 3; k++) obb[4 * g + k] = make_float4(ax[k].x, ax[k].y, ax[k].z, e[k]);
+  obb[4 * g + 3] = make_float4(ctr.x, ctr.y, ctr.z, 0.0f);
+}
+
 // Gather an array of T by permutation: out[i] = in[perm[i]] (n items of `w` elements).
 template <class T>
 __global__ void k_gather(const T* in, const uint32_t* perm, int n, int w, T* out) {
@@ -903,6 +962,10 @@ extern "C" int rtn_scene_build_ex(rtn_scene* s, const rtn_build_params* bp) {
   RTN_TRY(s->delta.alloc(np));
   hipLaunchKernelGGL(k_prep, dim3(nblk(np, 256)), dim3(256), 0, nullptr, s->cp.ptr, s->cp_off.ptr, s->pq.ptr, np,
                      s->corners.ptr, s->delta.ptr);
+  RTN_HIP(hipGetLastError());
+  RTN_TRY(s->obb.alloc(4 * (size_t)np));
+  hipLaunchKernelGGL(k_obb, dim3(nblk(np, 256)), dim3(256), 0, nullptr, s->cp.ptr, s->cp_off.ptr, s->pq.ptr, np,
+                     s->obb.ptr);
   RTN_HIP(hipGetLastError());
   RTN_TRY(s->nodes.upload(nodes.data(), nodes.size()));
   RTN_HIP(hipDeviceSynchronize());

@@ -287,13 +287,14 @@ __device__ __forceinline__ void trace_wide_body(const WArgs& A) {
         cand_first = code & 0xffffff;
         cand_left = ((code >> 24) & 0x7f) + 1;
         if (COUNT) cn.nleaf++;
-        if (A.hull_cull) {  // reject from the projected control points before any Newton pass
+        if (A.hull_cull) {  // reject from the oriented box, then the projected control points, before Newton
           while (cand_left > 0) {
             const int pqv = A.pq[cand_first];
-            const int np = ((pqv & 0xff) + 1) * (((pqv >> 8) & 0xff) + 1);
-            if (!(pqv & 0x10000) ||
-                !hull2d_culled(A.cp + A.cp_off[cand_first], np, L.rp, A.cull_margin, A.hull_cull == 2))
-              break;
+            if (!(pqv & 0x10000)) break;
+            if (!obb_culled(A.obb + 4 * (int64_t)cand_first, L.o, L.d, A.cull_margin)) {
+              const int np = ((pqv & 0xff) + 1) * (((pqv >> 8) & 0xff) + 1);
+              if (!hull2d_culled(A.cp + A.cp_off[cand_first], np, L.rp, A.cull_margin, A.hull_cull == 2)) break;
+            }
             if (COUNT) {
               cn.ncull++;
               cn.npt++;
@@ -535,6 +536,7 @@ int fill_wide_args(const rtn_scene* s, const float* const rays[6], int64_t n, co
   A.cp_off = s->cp_off.ptr;
   A.pq = s->pq.ptr;
   A.corners = s->corners.ptr;
+  A.obb = s->obb.ptr;
   A.ox = rays[0]; A.oy = rays[1]; A.oz = rays[2]; A.dx = rays[3]; A.dy = rays[4]; A.dz = rays[5];
   A.n_rays = n;
   A.max_iter = prm->max_iter;

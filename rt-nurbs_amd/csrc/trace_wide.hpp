@@ -24,6 +24,7 @@ struct WArgs {
   const int* cp_off;
   const int* pq;
   const float4* corners;  // dehomogenised corners q00,q10,q11,q01 (12 floats per patch)
+  const float4* obb;      // oriented box of each net (4 float4 per patch)
   const float *ox, *oy, *oz, *dx, *dy, *dz;
   int64_t n_rays;
   int max_iter;
