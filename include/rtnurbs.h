@@ -201,9 +201,11 @@ typedef struct {
   int seed_mode;          /* RTN_SEED_*                                          */
   int precision;          /* RTN_PREC_*                                          */
   float cond_thr;         /* condition-number threshold of the FP64 step         */
-  int hull_cull;          /* exact rejection at leaf entry, before Newton (fast forms): 0 off; 1 sign
-                             test of the net on each ray plane; 2 projected-hull test (8 directions);
-                             both need positive weights: nets with a weight <= 0 are never rejected */
+  int hull_cull;          /* exact rejection at leaf entry, before Newton (fast forms): 0 off; 1 the
+                             ray's line against the patch's oriented box, then a sign test of the net
+                             on each ray plane; 2 the box, then the projected-hull test (8
+                             directions); both need positive weights: nets with a weight <= 0 are
+                             never rejected                                       */
   int count;              /* 1: fill the per-ray counters                        */
   int learned_iters;      /* learned seed: Newton iterations from the network's seed before the
                              plain-seed fallback (0 = max_iter)                  */
