@@ -127,6 +127,21 @@ int rtn_scene_add_patches(rtn_scene* scene, int np, int p, int q, const float* c
 /* Runs decomposition, bounds and the BVH build on the device. */
 int rtn_scene_build(rtn_scene* scene);
 
+/* Scene files (no reference counterpart: the reference builds its scenes in
+ * code, main.cpp:70-98).  A file holds the surfaces and patch groups added to
+ * an unbuilt scene, exactly as rtn_scene_add_nurbs / rtn_scene_add_patches
+ * took them, and optionally a perspective camera; little-endian binary,
+ * magic "RTNSCENE", version 1 (layout in rt-nurbs_amd/csrc/scene_io.hip).
+ * Host-only calls.  rtn_scene_load returns a new, unbuilt scene; every count is
+ * checked against the file size and every surface through rtn_scene_add_nurbs. */
+typedef struct {
+  float eye[3], look[3], up[3];
+  float fov_deg;
+  int width, height;
+} rtn_camera_desc;
+int rtn_scene_save(const rtn_scene* scene, const rtn_camera_desc* camera_or_null, const char* path);
+int rtn_scene_load(const char* path, rtn_scene** scene, rtn_camera_desc* camera_or_null, int* has_camera);
+
 /* Build options.  RTN_BVH_EXACT_SAH is buildBVH's exhaustive SAH (bvh.h:85-143,
  * reference numbering; quadratic in the primitives of a node); RTN_BVH_BINNED_SAH
  * is a binned-SAH build for large scenes (no reference counterpart): `bins`

@@ -39,7 +39,7 @@ EXPORTS = [
     "rtn_shade_lambert", "rtn_build_params_default", "rtn_scene_build_ex", "rtn_batch_create", "rtn_batch_destroy",
     "rtn_batch_patch_base", "rtn_batch_trace", "rtn_frame_params_default", "rtn_frame_create", "rtn_frame_destroy",
     "rtn_frame_run", "rtn_frame_capture", "rtn_frame_status", "rtn_frame_results", "rtn_frame_set_shape",
-    "rtn_frame_autotune", "rtn_solve_pairs", "rtn_frame_set_refill",
+    "rtn_frame_autotune", "rtn_solve_pairs", "rtn_frame_set_refill", "rtn_scene_save", "rtn_scene_load",
 ]
 BVH_EXACT_SAH, BVH_BINNED_SAH = 0, 1
 MLP_NIN, MLP_HIDDEN, MLP_NPARAM = 20, 64, 9794
@@ -104,6 +104,11 @@ class TraceTotals(C.Structure):
                                              "hull_culled", "learned_fallbacks", "reserved")]
 
 
+class CameraDesc(C.Structure):
+    _fields_ = [("eye", C.c_float * 3), ("look", C.c_float * 3), ("up", C.c_float * 3), ("fov_deg", C.c_float),
+                ("width", C.c_int), ("height", C.c_int)]
+
+
 class SceneStats(C.Structure):
     _fields_ = [("n_surfaces", C.c_int64), ("n_patches", C.c_int64), ("n_cp", C.c_int64), ("n_nodes", C.c_int64),
                 ("n_wide_nodes", C.c_int64), ("max_depth", C.c_int64), ("n_leaves", C.c_int64),
@@ -139,6 +144,8 @@ def load():
     L.rtn_scene_add_nurbs.argtypes = [vp, i, i, vp, i, i, vp, vp]
     L.rtn_scene_add_patches.argtypes = [vp, i, i, i, vp]
     L.rtn_scene_build.argtypes = [vp]
+    L.rtn_scene_save.argtypes = [vp, C.POINTER(CameraDesc), C.c_char_p]
+    L.rtn_scene_load.argtypes = [C.c_char_p, C.POINTER(vp), C.POINTER(CameraDesc), C.POINTER(i)]
     L.rtn_build_params_default.argtypes = [C.POINTER(BuildParams)]
     L.rtn_build_params_default.restype = None
     L.rtn_scene_build_ex.argtypes = [vp, C.POINTER(BuildParams)]
@@ -367,6 +374,31 @@ class Scene:
         cp = np.ascontiguousarray(cp, np.float32)
         n = cp.size // (4 * (p + 1) * (q + 1))
         check(load().rtn_scene_add_patches(self.h, n, p, q, cp.ctypes.data))
+
+    def save(self, path, camera=None):
+        """Writes the surfaces and patch groups added so far (and the camera, a
+        scenes.Camera, when given) to a scene file (rtn_scene_save)."""
+        cd = None
+        if camera is not None:
+            cd = CameraDesc()
+            cd.eye[:], cd.look[:], cd.up[:] = list(camera.eye), list(camera.look), list(camera.up)
+            cd.fov_deg, cd.width, cd.height = camera.fov_deg, camera.W, camera.H
+        check(load().rtn_scene_save(self.h, C.byref(cd) if cd is not None else None, os.fsencode(path)))
+
+    @classmethod
+    def load(cls, path):
+        """(Scene, scenes.Camera or None) from a scene file (rtn_scene_load); the scene is unbuilt."""
+        from .scenes import Camera
+        L = load()
+        h, cd, has = C.c_void_p(), CameraDesc(), C.c_int()
+        check(L.rtn_scene_load(os.fsencode(path), C.byref(h), C.byref(cd), C.byref(has)))
+        sc = cls.__new__(cls)
+        sc.h = h.value
+        cam = None
+        if has.value:
+            cam = Camera(tuple(cd.eye), tuple(cd.look), tuple(cd.up), float(cd.fov_deg), int(cd.width),
+                         int(cd.height))
+        return sc, cam
 
     def build(self, bvh=BVH_EXACT_SAH, bins=32, max_leaf=4):
         """Decomposition, bounds and BVH on the device (exhaustive SAH by default;

@@ -51,6 +51,11 @@ class Scene:
     camera: Camera | None = None
     name: str = ""
 
+    def save(self, path):
+        """Writes the surfaces and camera to a scene file (rtn_scene_save)."""
+        from . import lib
+        lib.Scene(self.surfaces).save(path, self.camera)
+
 
 def _homog(x, y, z, w):
     x = np.asarray(x, np.float32); y = np.asarray(y, np.float32)
