@@ -453,6 +453,18 @@ int rtn_patch_bounds(const float* cp, int np, int p, int q, float* bounds);
 int rtn_refine(int dir, int n, int p, const float* U, int m, const float* Pw, float C, int* inserted,
                float* knots_out, float* Pw_out);
 
+/* NurbsOps::curvatureRefinementU then curvatureRefinementV (nurbs.h:290-486)
+ * of every NURBS surface added to the scene, on the GPU: one launch per
+ * direction counts the knots of all spans of all surfaces, one more inserts
+ * them (A5.1, nurbs.h:97-140), each column curve on its own lane.  The
+ * scene's surfaces are replaced by the refined ones; call before
+ * rtn_scene_build.  A constant <= 0 skips its direction.  Synchronous.
+ * Results equal rtn_refine's bit for bit, except that Δ^1.5 in knotsToAdd is
+ * the correctly rounded power where the host calls powf (a one-ulp difference
+ * on rare inputs, which moves a knot count only when the count's real value
+ * lies within that ulp of an integer). */
+int rtn_scene_refine(rtn_scene* scene, float C_u, float C_v);
+
 #ifdef __cplusplus
 }
 #endif

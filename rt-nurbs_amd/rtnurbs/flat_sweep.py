@@ -5,7 +5,7 @@ the sphere at 1920x1080 from its benchmark camera, refines each surface with
 curvatureRefinementU/V at C in {0, 1, 2, 4, 8, 16}, and for every flat-skip
 threshold eps (pixels) in {0.25 .. 8} traces the plain method against a
 ground truth at eps = 0.  This harness produces the same table through this
-library: refinement (rtn_refine), GPU decomposition and BVH (rtn_scene_build),
+library: GPU refinement (rtn_scene_refine), GPU decomposition and BVH (rtn_scene_build),
 the exact trace form with `flat_threshold` and `cam_k` (the reference's
 flat-skip, intersect.cu:104-123), HIP-event timing.
 
@@ -63,10 +63,10 @@ def sweep(res=(1920, 1080), reps=5, cs=(0, 1, 2, 4, 8, 16), epss=(0.25, 0.5, 1.0
     print(HEADER, file=out, flush=True)
     for name, make in (("torus", scenes.torus), ("sphere", scenes.sphere)):
         for Cc in cs:
-            surf = make()
+            scene = L.Scene([make()])
             if Cc > 0:
-                surf = L.refine(surf, Cc, Cc)
-            scene = L.Scene([surf]).build()
+                scene.refine(Cc, Cc)
+            scene.build()
             patches = scene.stats()["n_patches"]
             base = dict(mode=L.MODE_EXACT, cam_k=cam_k)  # reference constants: 7 iterations, 1e-5 / 1e-4
             ref, _ = L.trace(scene, rays, L.params(**base, flat_threshold=0.0))

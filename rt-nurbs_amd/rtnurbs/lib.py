@@ -32,7 +32,7 @@ EXPORTS = [
     "rtn_camera_perspective", "rtn_rays_perspective", "rtn_scene_create", "rtn_scene_destroy",
     "rtn_scene_add_nurbs", "rtn_scene_add_patches", "rtn_scene_build", "rtn_scene_stats_get",
     "rtn_scene_download", "rtn_trace_params_default", "rtn_trace", "rtn_calculate_intersect", "rtn_get_patch",
-    "rtn_build_bvh", "rtn_patch_bounds", "rtn_refine", "rtn_trace_f64", "rtn_event_create", "rtn_event_destroy",
+    "rtn_build_bvh", "rtn_patch_bounds", "rtn_refine", "rtn_scene_refine", "rtn_trace_f64", "rtn_event_create", "rtn_event_destroy",
     "rtn_event_record", "rtn_event_elapsed_ms", "rtn_trace_pairs", "rtn_mlp_create", "rtn_mlp_destroy",
     "rtn_mlp_set_params", "rtn_mlp_get_params", "rtn_mlp_pair_features", "rtn_mlp_forward", "rtn_mlp_gradient",
     "rtn_mlp_train_params_default", "rtn_mlp_train", "rtn_rays_silhouette", "rtn_rays_secondary",
@@ -181,6 +181,7 @@ def load():
     L.rtn_build_bvh.argtypes = [vp, i, vp, vp, C.POINTER(i)]
     L.rtn_patch_bounds.argtypes = [vp, i, i, i, vp]
     L.rtn_refine.argtypes = [i, i, i, vp, i, vp, f, C.POINTER(i), vp, vp]
+    L.rtn_scene_refine.argtypes = [vp, f, f]
     L.rtn_trace_f64.argtypes = [vp, C.POINTER(vp * 6), C.c_int64, i, C.c_double, C.c_double, C.c_uint32,
                                 vp, vp, vp, vp, vp, vp, vp, vp, vp]
     L.rtn_event_create.argtypes = [C.POINTER(vp)]
@@ -399,6 +400,12 @@ class Scene:
             cam = Camera(tuple(cd.eye), tuple(cd.look), tuple(cd.up), float(cd.fov_deg), int(cd.width),
                          int(cd.height))
         return sc, cam
+
+    def refine(self, C_u, C_v=None):
+        """curvatureRefinementU then V (nurbs.h:290-486) of every surface added
+        so far, on the GPU (rtn_scene_refine); C_v defaults to C_u."""
+        check(load().rtn_scene_refine(self.h, C_u, C_u if C_v is None else C_v))
+        return self
 
     def build(self, bvh=BVH_EXACT_SAH, bins=32, max_leaf=4):
         """Decomposition, bounds and BVH on the device (exhaustive SAH by default;
