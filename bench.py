@@ -68,10 +68,11 @@ def error_stats(g, ref, tie_rel=1e-5):
     tie = (h64 >= 0) & np.isfinite(ref["t2"]) & ((ref["t2"] - t64) <= tie_rel * np.abs(t64))
     both = (h32 >= 0) & (h64 >= 0)
     same = both & (h32 == h64)
+    mm = both & (h32 != h64) & ~tie
     out = {"rays": int(len(h32)), "hit_frac": float((h32 >= 0).mean()), "ties": int(tie.sum()),
-           "patch_mismatch_non_tie": int((both & (h32 != h64) & ~tie).sum()),
-           "patch_mismatch_t_separated": int((both & (h32 != h64) & ~tie & (np.abs(g["t"].astype(np.float64) - t64)
-                                                                           > tie_rel * np.abs(t64))).sum()),
+           "patch_mismatch_non_tie": int(mm.sum()),
+           "patch_mismatch_t_separated": int((np.abs(g["t"][mm].astype(np.float64) - t64[mm])
+                                              > tie_rel * np.abs(t64[mm])).sum()),
            "hit_miss_disagree": int(((h32 >= 0) != (h64 >= 0)).sum())}
     if same.any():
         te = np.abs(g["t"][same].astype(np.float64) - t64[same]) / np.abs(t64[same])
@@ -87,22 +88,42 @@ def error_stats(g, ref, tie_rel=1e-5):
     return out
 
 
-def time_form(L, scene, rays, prm, steps, warmup, stream):
-    """Kernel time per step from HIP events on the launch stream, and wall time."""
+def time_form(L, scene, rays, prm, steps, warmup, stream, per_launch=None):
+    """Kernel time per step from HIP events on the launch stream (mean over the
+    timed launches), and wall time.  An event between consecutive launches
+    also gives each launch's own time; `per_launch` (a dict) receives their
+    median and minimum."""
     hits = L.HitBuffers(rays.n)
     for _ in range(warmup):
         L.trace(scene, rays, prm, hits, totals=False, stream=stream)
     L.synchronize()
-    e0, e1 = L.Event(), L.Event()
+    ev = [L.Event() for _ in range(steps + 1)]
     t0 = time.perf_counter()
-    e0.record(stream)
-    for _ in range(steps):
+    ev[0].record(stream)
+    for i in range(steps):
         L.trace(scene, rays, prm, hits, totals=False, stream=stream)
-    e1.record(stream)
+        ev[i + 1].record(stream)
     L.synchronize()
     wall = (time.perf_counter() - t0) / steps * 1e3
-    kern = e0.elapsed_ms(e1) / steps
+    kern = ev[0].elapsed_ms(ev[steps]) / steps
+    if per_launch is not None and steps > 0:
+        each = np.array([ev[i].elapsed_ms(ev[i + 1]) for i in range(steps)])
+        per_launch.update(median=float(np.median(each)), min=float(each.min()))
     return hits, kern, wall
+
+
+def host_info():
+    """CPU model and logical core count of the machine the baseline runs on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count()}
 
 
 def cpu_baseline(scene_dl, rays_host, target_s=10.0):
@@ -132,9 +153,18 @@ def cpu_baseline(scene_dl, rays_host, target_s=10.0):
         if dt >= target_s * 0.4 or n >= n_all:
             break
         n = min(n_all, int(n * min(8.0, target_s / max(dt, 1e-3))))
+    # one thread over a 1/16 sample of the same rays
+    sel1 = sel[::16]
+    s1 = [np.ascontiguousarray(a[sel1]) for a in rays_host]
+    t0 = time.perf_counter()
+    O.trace_f32(scene_dl["nodes"], ps, s1, max_iter=8, thr_res=1e-5, thr_det=1e-5, seed_base=0, threads=1)
+    dt1 = time.perf_counter() - t0
     return {"value": len(sel) / dt / 1e6, "unit": "Mrays/s", "cores": cores, "kind": "port",
             "sample": f"{len(sel)} of {n_all} rays (every {stride}th), plain method FP32 (8 iterations, tol 1e-5), "
-                      f"{dt:.1f} s on {cores} threads"}
+                      f"{dt:.1f} s on {cores} threads",
+            "single_thread": {"value": len(sel1) / dt1 / 1e6, "unit": "Mrays/s",
+                              "sample": f"{len(sel1)} rays (every {16 * stride}th), {dt1:.2f} s on 1 thread"},
+            "host": host_info()}
 
 
 def cpu_reference(scene_dl, rays_host):
@@ -187,9 +217,11 @@ def run_forms(L, scene, rays, forms, args, ref, stream):
         prm = L.params(**kw)
         hits_c, tot = L.trace(scene, rays, L.params(count=1, **kw))   # counting run: work and iterations
         g = hits_c.get()
-        hits, kern_ms, wall_ms = time_form(L, scene, rays, prm, args.steps, args.warmup, stream)
+        each = {}
+        hits, kern_ms, wall_ms = time_form(L, scene, rays, prm, args.steps, args.warmup, stream, each)
         flops = ray_flops(tot, n, kw["mode"] == L.MODE_EXACT)
         r = {"mrays_s": n / (kern_ms * 1e-3) / 1e6, "kernel_ms": kern_ms, "wall_ms": wall_ms,
+             "kernel_ms_median": each.get("median"), "kernel_ms_min": each.get("min"),
              "newton_iters_per_ray": tot["newton_evals"] / n, "newton_calls_per_ray": tot["newton_calls"] / n,
              "patch_tests_per_ray": tot["patch_tests"] / n, "flops_per_ray": flops / n,
              "hull_culled_per_ray": tot["hull_culled"] / n, "fp64_steps": tot["fp64_steps"],
@@ -307,7 +339,8 @@ def bench_c2_c1(args, L, scenes, learned):
             dl = scene.download()
             rh = [np.asarray(a) for a in rays.get()]
             cpu_port = cpu_baseline(dl, rh)
-            log(f"cpu baseline (port): {cpu_port['value']:.3f} Mrays/s on {cpu_port['cores']} threads")
+            log(f"cpu baseline (port): {cpu_port['value']:.3f} Mrays/s on {cpu_port['cores']} threads, "
+                f"{cpu_port['single_thread']['value']:.3f} on one ({cpu_port['host']['cpu_model']})")
             cpu = cpu_reference(dl, rh) if args.config == "c1" else None
             if cpu is not None:
                 log(f"cpu baseline (reference): {cpu['value']:.3f} Mrays/s on {cpu['cores']} thread")
@@ -322,6 +355,8 @@ def bench_c2_c1(args, L, scenes, learned):
              "forms": results, "seed_network": train, "seed_ablation": ablation,
              "roofline": roofline(H["tflops"], head, args.config), "cpu_baseline": cpu,
              "cpu_port_baseline": cpu_port if cpu is not cpu_port else None}
+    extra["kernel_ms_per_launch"] = {"mean": round(H["kernel_ms"], 4), "median": H["kernel_ms_median"],
+                                     "min": H["kernel_ms_min"]}
     wl = {"workload": workload, "rays": rays.n, "patches": int(st["n_patches"]), "headline_form": head}
     return make_line(args, H["mrays_s"], H["kernel_ms"], "f32", wl, extra)
 
