@@ -62,6 +62,9 @@ typedef struct {                                         /* SoAQueue::ray_queue,
   int* idx_interact;
 } rtn_ray_queue;
 
+/* Error message of the calling thread's last failure, and the library
+ * version (no reference counterpart: the reference prints errors and goes on,
+ * memory.h:32-57). */
 const char* rtn_last_error(void);
 int rtn_version(void);
 
@@ -122,7 +125,9 @@ int rtn_scene_destroy(rtn_scene* scene);
  * homogeneous, u-major.  Degrees 1..RTN_MAX_DEGREE. */
 int rtn_scene_add_nurbs(rtn_scene* scene, int n, int p, const float* U, int m, int q, const float* V,
                         const float* Pw);
-/* Adds already-decomposed patches of one degree: cp[np*(p+1)*(q+1)*4]. */
+/* Adds already-decomposed patches of one degree: cp[np*(p+1)*(q+1)*4], each
+ * net row-major homogeneous as getPatch writes it (bezier_patch.h:119-134).
+ * No reference counterpart: the reference decomposes one surface per BVH. */
 int rtn_scene_add_patches(rtn_scene* scene, int np, int p, int q, const float* cp);
 /* Runs decomposition, bounds and the BVH build on the device. */
 int rtn_scene_build(rtn_scene* scene);
@@ -266,6 +271,12 @@ typedef struct {
   unsigned long long fp64_steps, hull_culled, learned_fallbacks, reserved;
 } rtn_trace_totals;
 
+/* The trace of calculateIntersectWrapper -> calculateIntersect
+ * (intersect.cu:20-219) over a built scene, writing SoA hit buffers: the
+ * exact mode is the reference's method and operation order (bit-exact), the
+ * wide and persistent modes the same method in the fast forms.  Traces n_rays
+ * (0 <= n_rays < 2^32; the persistent form takes at most 2^32 - 2^24, the
+ * range of its ray queue) in the form prm->mode selects. */
 int rtn_trace(const rtn_scene* scene, const float* const rays_soa[6], int64_t n_rays, const rtn_trace_params* prm,
               const rtn_hits* hits, rtn_trace_totals* totals_host, void* stream);
 

@@ -498,6 +498,10 @@ int launch_wide_kernel(const LaunchShape& sh, bool persistent, bool count, bool 
                        hipStream_t stream, const WArgs& args) {
   RTN_CHECK(shape_available(sh, persistent, count, learned), RTN_EUNSUPPORTED,
             "trace: launch shape %d lanes x %d stack entries is not built for this form", sh.wb, sh.wstk);
+  // the persistent queue is a 32-bit counter that every wave may advance by
+  // up to 64 once past the end: keep it below 2^32 with room for the grid
+  RTN_CHECK(!persistent || n_max <= (int64_t)0xFFFFFFFFu - ((int64_t)1 << 24), RTN_EINVAL,
+            "trace: %lld rays exceed the persistent queue's range", (long long)n_max);
   const int bpc = std::max(1, sh.blocks_per_cu);
   if (!persistent) {  // the default shape's stack depth in single-wave blocks
     if (count) launch_shape<WB_ONE_RAY, WSTK_DEFAULT, false, true>(learned, bpc, n_max, stream, args);
