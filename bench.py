@@ -298,11 +298,13 @@ def bench_c2_c1(args, L, scenes, learned):
                  "epoch_loss": [round(float(x), 6) for x in losses]}
         log(f"seed network: {npairs} pairs, {args.train_epochs} epochs, loss {losses[0]:.4f} -> {losses[-1]:.4f}, "
             f"{train['seconds']} s")
-    P, W = L.MODE_PERSISTENT, L.MODE_WIDE
+    P, W, K = L.MODE_PERSISTENT, L.MODE_WIDE, L.MODE_PACKET
     mixed = dict(precision=L.PREC_MIXED, cond_thr=1e4)
     forms = {
         "plain": dict(mode=L.MODE_EXACT, **BASE),
         "wide": dict(mode=W, **BASE),
+        # one wave traverses the BVH for its 64 rays together (packet form)
+        "packet": dict(mode=K, **BASE),
         "wide_hull2": dict(mode=W, hull_cull=2, **BASE),
         "persistent_hull2": dict(mode=P, hull_cull=2, **BASE),
         "wide_hull2_mixed": dict(mode=W, hull_cull=2, **mixed, **BASE),
@@ -310,6 +312,8 @@ def bench_c2_c1(args, L, scenes, learned):
         # abandon solves stuck against the patch boundary (misses) from the 4th evaluation
         "wide_hull2_mixed_stall": dict(mode=W, hull_cull=2, stall_exit=1, **mixed, **BASE),
         "persistent_hull2_mixed_stall": dict(mode=P, hull_cull=2, stall_exit=1, **mixed, **BASE),
+        "packet_hull2_mixed": dict(mode=K, hull_cull=2, **mixed, **BASE),
+        "packet_hull2_mixed_stall": dict(mode=K, hull_cull=2, stall_exit=1, **mixed, **BASE),
     }
     if net is not None:
         # a 3-evaluation attempt from the network's seed (two Newton steps: the fewest that

@@ -199,7 +199,10 @@ int rtn_rays_silhouette(const rtn_scene* scene, int64_t n, uint32_t seed, float 
 enum {
   RTN_MODE_EXACT = 0,     /* reference binary BVH, reference op order            */
   RTN_MODE_WIDE = 1,      /* 4-wide SoA BVH, FMA arithmetic, degree-binned Newton */
-  RTN_MODE_PERSISTENT = 2 /* persistent work-queue kernel over the wide BVH       */
+  RTN_MODE_PERSISTENT = 2,/* persistent work-queue kernel over the wide BVH       */
+  RTN_MODE_PACKET = 3     /* one wave traverses the wide BVH for its 64 rays together
+                             (coherent rays: primary rays in the tiled layout); plain
+                             seed, single scenes                                  */
 };
 enum {
   RTN_SEED_PLAIN = 0,     /* the reference's bilinear seed                      */

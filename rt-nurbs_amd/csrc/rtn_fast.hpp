@@ -18,6 +18,23 @@ using ex::pcg_hash;
 
 struct f3 { float x, y, z; };
 
+// Control nets and boxes are read through plain global pointers, or through
+// the constant address space when every lane of a wave reads the same
+// address: then the compiler fetches them with scalar loads into scalar
+// registers (the trace never writes them).
+template <class T>
+using cptr = const __attribute__((address_space(4))) T*;
+template <class T>
+__device__ __forceinline__ cptr<T> as_const(const T* p) {
+  return (cptr<T>)(p);
+}
+typedef float f4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 ld4(const float4* p, int64_t i) { return p[i]; }
+__device__ __forceinline__ float4 ld4(cptr<float4> p, int64_t i) {
+  const f4v v = ((cptr<f4v>)(p))[i];
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+
 __device__ __forceinline__ float dot3(f3 a, f3 b) { return fmaf(a.x, b.x, fmaf(a.y, b.y, a.z * b.z)); }
 
 // Bernstein basis of degree N at t and its derivative, by the scalar
@@ -324,8 +341,8 @@ __device__ __forceinline__ void bernstein_rt(int n, float t, float (&b)[MAXD + 1
 // Point and partials of a patch with runtime row count P+1 and compile-time
 // row length Q+1: rows stream from memory; the u-direction weights rotate
 // through registers (weight of row i is always in bu[0]).
-template <int Q>
-__device__ __forceinline__ void eval_q(const float4* __restrict__ net, int P, float u, float v, float4& pt,
+template <int Q, class NP = const float4*>
+__device__ __forceinline__ void eval_q(NP net, int P, float u, float v, float4& pt,
                                        float4& du, float4& dv) {
   float bu[MAXD + 1], dbu[MAXD + 1], bv[Q + 1], dbv[Q + 1];
   bernstein_rt(P, u, bu, dbu);
@@ -335,11 +352,11 @@ __device__ __forceinline__ void eval_q(const float4* __restrict__ net, int P, fl
   dv = pt;
 #pragma unroll 1
   for (int i = 0; i <= P; i++) {
-    const float4* row = net + i * (Q + 1);
+    const NP row = net + i * (Q + 1);
     float4 r = make_float4(0.f, 0.f, 0.f, 0.f), rv = r;
 #pragma unroll
     for (int j = 0; j <= Q; j++) {
-      const float4 c = row[j];
+      const float4 c = ld4(row, j);
       r.x = fmaf(bv[j], c.x, r.x); r.y = fmaf(bv[j], c.y, r.y); r.z = fmaf(bv[j], c.z, r.z); r.w = fmaf(bv[j], c.w, r.w);
       rv.x = fmaf(dbv[j], c.x, rv.x); rv.y = fmaf(dbv[j], c.y, rv.y);
       rv.z = fmaf(dbv[j], c.z, rv.z); rv.w = fmaf(dbv[j], c.w, rv.w);
@@ -360,8 +377,8 @@ __device__ __forceinline__ void eval_q(const float4* __restrict__ net, int P, fl
 // is abandoned as a miss from its 4th evaluation on when two consecutive
 // updates leave the parameter domain on the same side(s) and the residual fell
 // by less than 10%: the iterate is stuck against the patch boundary.
-template <int Q>
-__device__ __forceinline__ Hit newton_q(const float4* __restrict__ net, int P, const Plane2& rp, float u0, float v0,
+template <int Q, class NP = const float4*>
+__device__ __forceinline__ Hit newton_q(NP net, int P, const Plane2& rp, float u0, float v0,
                                         float thr_det, float thr_res, uint32_t seed, int max_iter, bool stall) {
   float u = u0, v = v0;
   Hit h;
@@ -430,14 +447,15 @@ __device__ __forceinline__ double bern64(int n, int i, double t) {
 }
 
 // FP64 point and partials, runtime degrees (rows and columns streamed).
-__device__ __forceinline__ void eval64_rt(const float4* __restrict__ net, int P, int Q, double u, double v, D3& S,
+template <class NP = const float4*>
+__device__ __forceinline__ void eval64_rt(NP net, int P, int Q, double u, double v, D3& S,
                                           D3& Su, D3& Sv) {
   double pt[4] = {0, 0, 0, 0}, du[4] = {0, 0, 0, 0}, dv[4] = {0, 0, 0, 0};
   for (int i = 0; i <= P; i++) {
     const double bu = bern64(P, i, u), dbu = P * (bern64(P - 1, i - 1, u) - bern64(P - 1, i, u));
     for (int j = 0; j <= Q; j++) {
       const double bv = bern64(Q, j, v), dbv = Q * (bern64(Q - 1, j - 1, v) - bern64(Q - 1, j, v));
-      const float4 c = net[i * (Q + 1) + j];
+      const float4 c = ld4(net, i * (Q + 1) + j);
       const double cc[4] = {c.x, c.y, c.z, c.w};
 #pragma unroll
       for (int k = 0; k < 4; k++) {
@@ -455,7 +473,8 @@ __device__ __forceinline__ void eval64_rt(const float4* __restrict__ net, int P,
 
 // One FP64 Newton step from the FP32 (u,v) on the FP64 ray planes, then the
 // corrected point: the mixed-precision form's step for cond(J) > threshold.
-__device__ __forceinline__ bool correct64_rt(const float4* __restrict__ net, int P, int Q, f3 o, f3 d, float& uo,
+template <class NP = const float4*>
+__device__ __forceinline__ bool correct64_rt(NP net, int P, int Q, f3 o, f3 d, float& uo,
                                           float& vo, f3& So, f3& Suo, f3& Svo, float& t_out) {
   const double ox = o.x, oy = o.y, oz = o.z, dx = d.x, dy = d.y, dz = d.z;
   const double ax = fabs(dx), ay = fabs(dy), az = fabs(dz);
@@ -501,8 +520,9 @@ namespace fast {
 // box widened by the margin cannot converge on the patch.  Line (not ray)
 // against box, slab test in the box frame with hardware reciprocals; the
 // widening also covers the FP32 rounding of the transform.
-__device__ __forceinline__ bool obb_culled(const float4* __restrict__ box, f3 o, f3 d, float margin) {
-  const float4 a0 = box[0], a1 = box[1], a2 = box[2], c = box[3];
+template <class NP = const float4*>
+__device__ __forceinline__ bool obb_culled(NP box, f3 o, f3 d, float margin) {
+  const float4 a0 = ld4(box, 0), a1 = ld4(box, 1), a2 = ld4(box, 2), c = ld4(box, 3);
   const float ox = o.x - c.x, oy = o.y - c.y, oz = o.z - c.z;
   const float l0 = fmaf(a0.x, ox, fmaf(a0.y, oy, a0.z * oz)), l1 = fmaf(a1.x, ox, fmaf(a1.y, oy, a1.z * oz)),
               l2 = fmaf(a2.x, ox, fmaf(a2.y, oy, a2.z * oz));
@@ -531,13 +551,14 @@ __device__ __forceinline__ bool obb_culled(const float4* __restrict__ box, f3 o,
 // is the margin itself, as min(x - m*w) > 0 or max(x + m*w) < 0 with
 // x in {c1, c2, c1+c2, c1-c2}; without `diagonals`, the four axis directions
 // only (the per-plane sign test).  Runtime degrees; no divisions.
-__device__ __forceinline__ bool hull2d_culled(const float4* __restrict__ net, int n_cp, const Plane2& rp,
+template <class NP = const float4*>
+__device__ __forceinline__ bool hull2d_culled(NP net, int n_cp, const Plane2& rp,
                                               float margin, bool diagonals) {
   float lo0 = INFINITY, lo1 = INFINITY, lo2 = INFINITY, lo3 = INFINITY;
   float hi0 = -INFINITY, hi1 = -INFINITY, hi2 = -INFINITY, hi3 = -INFINITY;
 #pragma unroll 4
   for (int k = 0; k < n_cp; k++) {
-    const float4 c = net[k];
+    const float4 c = ld4(net, k);
     const float c1 = fmaf(rp.n1.x, c.x, fmaf(rp.n1.y, c.y, fmaf(rp.n1.z, c.z, rp.d1 * c.w)));
     const float c2 = fmaf(rp.n2.x, c.x, fmaf(rp.n2.y, c.y, fmaf(rp.n2.z, c.z, rp.d2 * c.w)));
     const float mw = margin * c.w;

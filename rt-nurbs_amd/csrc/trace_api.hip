@@ -1,4 +1,4 @@
-// rtn_trace: dispatch over the trace forms (exact / wide / persistent).
+// rtn_trace: dispatch over the trace forms (exact / wide / persistent / packet).
 #include <cstring>
 
 #include "rtn_internal.hpp"
@@ -8,6 +8,8 @@ int trace_exact_scene(const rtn_scene* s, const float* const rays[6], int64_t n,
                       const rtn_hits* hits, unsigned long long* d_totals, hipStream_t stream);
 int trace_wide_scene(const rtn_scene* s, const float* const rays[6], int64_t n, const rtn_trace_params* prm,
                      const rtn_hits* hits, unsigned long long* d_totals, hipStream_t stream, const PairOut* pairs);
+int trace_packet_scene(const rtn_scene* s, const float* const rays[6], int64_t n, const rtn_trace_params* prm,
+                       const rtn_hits* hits, unsigned long long* d_totals, hipStream_t stream);
 }  // namespace rtn
 
 using namespace rtn;
@@ -57,6 +59,9 @@ extern "C" int rtn_trace(const rtn_scene* s, const float* const rays[6], int64_t
     case RTN_MODE_WIDE:
     case RTN_MODE_PERSISTENT:
       rc = trace_wide_scene(s, rays, n_rays, prm, hits, d_tot, st, nullptr);
+      break;
+    case RTN_MODE_PACKET:
+      rc = trace_packet_scene(s, rays, n_rays, prm, hits, d_tot, st);
       break;
     default:
       set_error("rtn_trace: unknown mode %d", prm->mode);

@@ -156,8 +156,8 @@ struct LeafIn {
 // Leaf work for one patch with compile-time row length Q+1 and runtime row
 // count P+1: Newton from the seed (plain-seed fallback for the learned form),
 // flagging an FP64 correction.
-template <int Q>
-__device__ __forceinline__ LeafOut leaf_q(const float4* __restrict__ net, int P, const LeafIn& in) {
+template <int Q, class NP = const float4*>
+__device__ __forceinline__ LeafOut leaf_q(NP net, int P, const LeafIn& in) {
   LeafOut o;
   o.flags = 0;
   o.evals = 0;
@@ -194,7 +194,8 @@ __device__ __forceinline__ LeafOut leaf_q(const float4* __restrict__ net, int P,
 // the row count P+1 is a runtime loop.
 __device__ __forceinline__ int degree_class(int p, int q) { return q - 1; }
 
-__device__ __forceinline__ LeafOut leaf_dispatch(int cls, const float4* net, int p, const LeafIn& in) {
+template <class NP = const float4*>
+__device__ __forceinline__ LeafOut leaf_dispatch(int cls, NP net, int p, const LeafIn& in) {
   switch (cls) {  // wave-uniform: a scalar branch
     case 0: return leaf_q<1>(net, p, in);
     case 1: return leaf_q<2>(net, p, in);
@@ -443,6 +444,191 @@ k_trace_wide_learned(const WArgs A) {
   trace_wide_body<WB, WSTK, PERSISTENT, COUNT, true>(A);
 }
 
+// ------------------------------------------------------------ packet form
+// One wave traverses the 4-wide BVH for its 64 rays together.  The wave
+// visits a node when any of its lanes enters one of the node's child boxes;
+// each lane tests the child boxes against its own ray and closest hit, so a
+// lane enters exactly the boxes it would enter alone.  The stack is the
+// wave's: entry i lives in lane i of one register (a select and readlane), and
+// holds wide-node indices or, for a leaf, its parent node and slot, so that on
+// the pop every lane re-tests the leaf box against its closest hit exactly as
+// the one-ray form tests the entry distance it pushed.  A leaf's patches are
+// then tested one at a time by the lanes that entered it: node, box and
+// control-point addresses are the same on every lane (one cache line per
+// wave), and every Newton solve of a pass has the patch's degree, so there is
+// no degree-class divergence.  Children are ordered near-first by the entry
+// distance of the first lane that enters each.
+//
+// Per (ray, patch) the arithmetic is the one-ray form's; a ray tests every
+// patch whose box it enters before its closest hit, so the closest hit is the
+// one-ray form's except where two patches give the same t (the first tested
+// is kept).  Plain seed; no multi-scene batches.
+__device__ __forceinline__ int rl_int(int x, int l) { return __builtin_amdgcn_readlane(x, l); }
+__device__ __forceinline__ float rl_float(float x, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l)); }
+
+// Entry and exit distances of the lane's ray through child slot k of a node
+// (the one-ray form's slab test, expand()).
+__device__ __forceinline__ bool child_box(cptr<float> f, int k, const Lane& L, float& tmin) {
+  const float ox = L.o.x * L.rd.x, oy = L.o.y * L.rd.y, oz = L.o.z * L.rd.z;
+  const float tx0 = fmaf(f[k], L.rd.x, -ox), tx1 = fmaf(f[4 + k], L.rd.x, -ox);
+  const float ty0 = fmaf(f[8 + k], L.rd.y, -oy), ty1 = fmaf(f[12 + k], L.rd.y, -oy);
+  const float tz0 = fmaf(f[16 + k], L.rd.z, -oz), tz1 = fmaf(f[20 + k], L.rd.z, -oz);
+  tmin = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+  const float tmax = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1)) * 1.0000005f;
+  return tmin <= tmax && tmin < L.t_max;
+}
+
+constexpr int PKT_STACK = 64;  // one entry per lane of the stack register
+
+template <bool COUNT>
+__global__ void __launch_bounds__(64) k_trace_packet(const WArgs A) {
+  const int lane = threadIdx.x;
+  Lane L;
+  Cnt cn{};
+  bool active = false;
+  {
+    const int64_t r = (int64_t)blockIdx.x * 64 + lane;
+    if (r < A.n_rays) active = lane_start(L, A, (uint32_t)r) >= 0;
+  }
+  const bool had_ray = active;
+  constexpr int NACC = COUNT ? 11 : 1;
+  unsigned long long acc[NACC] = {};
+  int stk = 0;  // the wave's stack: entry i in lane i
+  int sp = 0;   // wave-uniform depth
+  int e = 0;    // entry being processed: wide node (>= 0) or (parent << 2 | slot) | sign bit for a leaf
+  while (__ballot(active)) {
+    if (e >= 0) {
+      const cptr<float> base = as_const(reinterpret_cast<const float*>(A.nodes + e));
+      const cptr<int> chp = as_const(reinterpret_cast<const int*>(A.nodes + e) + 24);
+      const int cha[4] = {chp[0], chp[1], chp[2], chp[3]};
+      float key[4];
+      int code[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        float tmin;
+        const bool hit = active && cha[k] != 0x7fffffff && child_box(base, k, L, tmin);
+        const unsigned long long m = __ballot(hit);
+        key[k] = INFINITY;
+        if (m) key[k] = rl_float(tmin, __ffsll((long long)m) - 1);
+        // leaves are pushed as their parent and slot, re-tested on the pop
+        code[k] = cha[k] >= 0 ? cha[k] : (int)(0x80000000u | ((uint32_t)e << 2) | (uint32_t)k);
+      }
+      if (COUNT && active) cn.nbox += 4;
+#define RTN_CSWAP(a, b)                                   \
+  if (key[a] < key[b]) {                                  \
+    float tt = key[a]; key[a] = key[b]; key[b] = tt;      \
+    int cc = code[a]; code[a] = code[b]; code[b] = cc;    \
+  }
+      RTN_CSWAP(0, 1) RTN_CSWAP(2, 3) RTN_CSWAP(0, 2) RTN_CSWAP(1, 3) RTN_CSWAP(1, 2)
+#undef RTN_CSWAP
+      // descending keys: push far to near so the nearest is on top
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        if (key[k] < INFINITY) {
+          if (sp < PKT_STACK) {
+            stk = lane == sp ? code[k] : stk;
+            sp++;
+          } else if (active) {
+            cn.ovf = 1;
+          }
+        }
+      }
+    } else {
+      const int parent = (e & 0x7fffffff) >> 2, slot = e & 3;
+      const cptr<float> base = as_const(reinterpret_cast<const float*>(A.nodes + parent));
+      const int leaf = as_const(reinterpret_cast<const int*>(A.nodes + parent) + 24)[slot];
+      float tmin;
+      bool in_leaf = active && child_box(base, slot, L, tmin);
+      if (COUNT && in_leaf) cn.nleaf++;
+      const int first = leaf & 0xffffff, count = ((leaf >> 24) & 0x7f) + 1;
+#pragma unroll 1
+      for (int pi = first; pi < first + count && __ballot(in_leaf); pi++) {
+        bool act = in_leaf;
+        const int pqv = as_const(A.pq)[pi];
+        const cptr<float4> net = as_const(A.cp + as_const(A.cp_off)[pi]);
+        if (act && A.hull_cull && (pqv & 0x10000)) {
+          bool culled = obb_culled(as_const(A.obb + 4 * (int64_t)pi), L.o, L.d, A.cull_margin);
+          if (!culled) {
+            const int np = ((pqv & 0xff) + 1) * (((pqv >> 8) & 0xff) + 1);
+            culled = hull2d_culled(net, np, L.rp, A.cull_margin, A.hull_cull == 2);
+          }
+          if (culled) {
+            act = false;
+            if (COUNT) {
+              cn.ncull++;
+              cn.npt++;
+            }
+          }
+        }
+        if (!__ballot(act)) continue;
+        const int p = pqv & 0xff, q = (pqv >> 8) & 0xff;
+        LeafOut res;
+        res.flags = 0;
+        if (act) {
+          if (COUNT) cn.npt++;
+          const cptr<float4> cn4 = as_const(A.corners + 3 * (int64_t)pi);
+          const float4 c0 = ld4(cn4, 0), c1 = ld4(cn4, 1), c2 = ld4(cn4, 2);
+          LeafIn in;
+          in.rp = L.rp;
+          in.o = L.o;
+          in.d = L.d;
+          ex::bilinear_seed<float>({c0.x, c0.y, c0.z}, {c0.w, c1.x, c1.y}, {c1.z, c1.w, c2.x}, {c2.y, c2.z, c2.w},
+                                   {L.o.x, L.o.y, L.o.z}, {L.d.x, L.d.y, L.d.z}, in.gu, in.gv);
+          in.pu = in.gu;
+          in.pv = in.gv;
+          in.seed = L.r + A.seed_base;
+          in.max_iter = A.max_iter;
+          in.learned_iters = A.max_iter;
+          in.fallback_iters = A.max_iter;
+          in.mixed = A.mixed;
+          in.stall = A.stall_exit;
+          in.learned = 0;
+          in.thr_det = A.thr_det;
+          in.thr_res = A.thr_res;
+          in.cond_thr = A.cond_thr;
+          res = leaf_dispatch(degree_class(p, q), net, p, in);  // the patch's degree: a wave-uniform branch
+          if (res.flags & 16) {  // FP64 correction of an ill-conditioned hit
+            if (correct64_rt(net, p, q, L.o, L.d, res.u, res.v, res.S, res.Su, res.Sv, res.t)) res.flags |= 4;
+          }
+          if (COUNT) {
+            cn.nc++;
+            cn.ne += res.evals;
+            cn.nfl += res.evals * ex::eval_flops(p, q);
+            if (res.flags & 4) cn.n64++;
+          }
+          if ((res.flags & 1) && 0.0f < res.t && res.t < L.t_max) {
+            record(L, A, pi, res.t, res.u, res.v, res.Su, res.Sv);
+            if (A.any_hit || (A.ray_kind && A.ray_kind[L.r] == 0)) {  // occlusion query: the first hit ends the ray
+              active = false;
+              in_leaf = false;
+            }
+          }
+        }
+      }
+    }
+    if (sp == 0) break;
+    sp--;
+    e = rl_int(stk, sp);
+  }
+  if (had_ray) {
+    lane_finish<COUNT>(L, A, cn);
+    if constexpr (COUNT) {
+      acc[0] += cn.nc; acc[1] += cn.ne; acc[3] += cn.nbox; acc[4] += cn.nleaf; acc[5] += cn.npt;
+      acc[6] += cn.ovf; acc[7] += cn.nfl; acc[8] += cn.n64; acc[9] += cn.ncull; acc[10] += cn.nfb;
+    } else if (cn.ovf) {
+      acc[0] += 1;
+    }
+  }
+  if (A.totals) {  // every lane of the wave arrives here together
+#pragma unroll
+    for (int k = 0; k < NACC; k++) {
+      unsigned long long x = acc[k];
+      for (int sft = 32; sft > 0; sft >>= 1) x += __shfl_xor(x, sft);
+      if (lane == 0 && x) atomicAdd(A.totals + (COUNT ? k : 6), x);
+    }
+  }
+}
+
 }  // namespace
 
 namespace rtn {
@@ -593,6 +779,19 @@ int trace_wide_scene(const rtn_scene* s, const float* const rays[6], int64_t n, 
                                     prm->seed_mode == RTN_SEED_LEARNED, n, stream, A);
   if (d_work) RTN_HIP(hipFreeAsync(d_work, stream));
   return rc;
+}
+
+int trace_packet_scene(const rtn_scene* s, const float* const rays[6], int64_t n, const rtn_trace_params* prm,
+                       const rtn_hits* hits, unsigned long long* d_totals, hipStream_t stream) {
+  RTN_CHECK(prm->seed_mode == RTN_SEED_PLAIN, RTN_EUNSUPPORTED, "rtn_trace: the packet form runs the plain seed");
+  WArgs A;
+  RTN_TRY(fill_wide_args(s, rays, n, prm, hits, d_totals, nullptr, A));
+  A.work = nullptr;
+  const unsigned blocks = (unsigned)((n + 63) / 64);
+  if (prm->count) hipLaunchKernelGGL(k_trace_packet<true>, dim3(blocks), dim3(64), 0, stream, A);
+  else hipLaunchKernelGGL(k_trace_packet<false>, dim3(blocks), dim3(64), 0, stream, A);
+  RTN_HIP(hipGetLastError());
+  return RTN_OK;
 }
 
 }  // namespace rtn

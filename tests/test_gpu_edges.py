@@ -10,13 +10,15 @@
 
 The exact form is compared with the oracle bit for bit (hit patch, t, u, v,
 dpdu, dpdv); the fast forms with one another bit for bit (persistent = wide,
-hull-rejected = unrejected) and with the exact form on the hit patches."""
+hull-rejected = unrejected, packet = wide but for equal-t ties) and with the
+exact form on the hit patches."""
 import numpy as np
 import pytest
 
 from helpers import c2_small
 from rtnurbs import scenes
 from test_gpu_exact import _compare_trace, _oracle_scene
+from test_gpu_wide import same_but_ties
 
 pytestmark = pytest.mark.gpu
 
@@ -27,7 +29,8 @@ def _forms(rtl):
     W, P = rtl.MODE_WIDE, rtl.MODE_PERSISTENT
     return {"wide": dict(mode=W), "persistent": dict(mode=P), "wide_hull2": dict(mode=W, hull_cull=2),
             "persistent_hull2_stall": dict(mode=P, hull_cull=2, stall_exit=1),
-            "wide_hull2_stall": dict(mode=W, hull_cull=2, stall_exit=1)}
+            "wide_hull2_stall": dict(mode=W, hull_cull=2, stall_exit=1), "packet": dict(mode=rtl.MODE_PACKET),
+            "packet_hull2_stall": dict(mode=rtl.MODE_PACKET, hull_cull=2, stall_exit=1)}
 
 
 def _check_forms(rtl, scene, rays, ex):
@@ -46,6 +49,8 @@ def _check_forms(rtl, scene, rays, ex):
         k = y["hit"] >= 0
         for f in ("t", "u", "v"):
             assert np.array_equal(x[f][k].view(np.uint32), y[f][k].view(np.uint32)), (a, b, f)
+    same_but_ties(out["packet"], wd)
+    same_but_ties(out["packet_hull2_stall"], out["wide_hull2_stall"])
     if len(ex["hit"]):
         assert np.mean(ex["hit"] == wd["hit"]) > 0.99
         same = m & (ex["hit"] == wd["hit"])

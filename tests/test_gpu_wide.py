@@ -1,5 +1,6 @@
-"""GPU fast forms (wide 4-ary BVH, persistent work queue, hull rejection,
-mixed precision) against the exact plain method and the FP64 reference.
+"""GPU fast forms (wide 4-ary BVH, persistent work queue, packet traversal,
+hull rejection, mixed precision) against the exact plain method and the FP64
+reference.
 
 The fast forms change the arithmetic (FMA, tensor-Bernstein evaluation) and
 the traversal order, not the method: per ray the same patches are tested
@@ -28,6 +29,20 @@ CASES = [
 def _run(rtl, scene, rays, **kw):
     h, tot = rtl.trace(scene, rays, rtl.params(**CFG, **kw))
     return h.get(), tot
+
+
+def same_but_ties(a, b):
+    """The packet form against the one-ray form: per (ray, patch) the same
+    arithmetic, and a ray tests every patch whose box it enters before its
+    closest hit, so the closest hits agree bit for bit except where two
+    patches give the same t (the order of the tests then picks one)."""
+    d = a["hit"] != b["hit"]
+    assert d.sum() <= max(2, 1e-4 * len(d)), d.sum()
+    assert np.all((a["hit"][d] >= 0) & (b["hit"][d] >= 0))
+    assert np.all(np.abs(a["t"][d] - b["t"][d]) <= 1e-6 * np.abs(b["t"][d]))
+    m = ~d & (b["hit"] >= 0)
+    for k in ("t", "u", "v"):
+        assert np.array_equal(a[k][m].view(np.uint32), b[k][m].view(np.uint32)), k
 
 
 @pytest.mark.parametrize("name,mk", CASES, ids=[c[0] for c in CASES])
@@ -60,6 +75,14 @@ def test_wide_forms_agree(rtl, name, mk):
         for k in ("t", "u", "v"):
             assert np.array_equal(h2[k][m].view(np.uint32), wd[k][m].view(np.uint32)), k
     assert culled[2] > culled[1]  # the diagonal directions reject more
+    # packet form (one traversal per wave) = one-ray form but for ties, with and without rejection
+    pk, tk = _run(rtl, scene, rays, mode=rtl.MODE_PACKET, count=1)
+    # the wave's test order (nearest first for its first lane) can test a patch
+    # before a nearer hit of this lane would have pruned it: a few % more solves
+    assert tk["overflow"] == 0 and abs(tk["newton_evals"] - tw["newton_evals"]) <= 0.05 * tw["newton_evals"]
+    same_but_ties(pk, wd)
+    pk2, _ = _run(rtl, scene, rays, mode=rtl.MODE_PACKET, hull_cull=2)
+    same_but_ties(pk2, wd)
     # wide vs exact: hit patches agree except near ties / boundary solves
     both = (ex["hit"] >= 0) & (wd["hit"] >= 0)
     assert np.mean(ex["hit"] == wd["hit"]) > 0.995
@@ -102,6 +125,8 @@ def test_stall_exit(rtl, name, mk):
     assert ts["newton_calls"] == tf["newton_calls"]
     assert ts["newton_evals"] < 0.95 * tf["newton_evals"]
     assert np.array_equal(ps["hit"], st["hit"]) and tp["newton_evals"] == ts["newton_evals"]
+    pk, _ = _run(rtl, scene, rays, mode=rtl.MODE_PACKET, hull_cull=2, stall_exit=1)
+    same_but_ties(pk, st)
     assert np.mean(st["hit"] == full["hit"]) > 0.9995
     # every abandoned solve was a miss of the full count or a hit another patch also provides
     assert (st["hit"] >= 0).sum() >= 0.9995 * (full["hit"] >= 0).sum()
