@@ -298,7 +298,7 @@ def bench_c2_c1(args, L, scenes, learned):
                  "epoch_loss": [round(float(x), 6) for x in losses]}
         log(f"seed network: {npairs} pairs, {args.train_epochs} epochs, loss {losses[0]:.4f} -> {losses[-1]:.4f}, "
             f"{train['seconds']} s")
-    P, W, K = L.MODE_PERSISTENT, L.MODE_WIDE, L.MODE_PACKET
+    P, W, K, KP = L.MODE_PERSISTENT, L.MODE_WIDE, L.MODE_PACKET, L.MODE_PERSISTENT_PACKET
     mixed = dict(precision=L.PREC_MIXED, cond_thr=1e4)
     forms = {
         "plain": dict(mode=L.MODE_EXACT, **BASE),
@@ -324,6 +324,9 @@ def bench_c2_c1(args, L, scenes, learned):
         forms["wide_learned"] = dict(mode=W, hull_cull=2, **lrn, **BASE)
         forms["persistent_learned_mixed"] = dict(mode=P, hull_cull=2, **lrn, **mixed, **BASE)
         forms["persistent_learned_mixed_stall"] = dict(mode=P, hull_cull=2, stall_exit=1, **lrn, **mixed, **BASE)
+        forms["packet_learned_mixed_stall"] = dict(mode=K, hull_cull=2, stall_exit=1, **lrn, **mixed, **BASE)
+        forms["persistent_packet_learned_mixed"] = dict(mode=KP, hull_cull=2, **lrn, **mixed, **BASE)
+        forms["persistent_packet_learned_mixed_stall"] = dict(mode=KP, hull_cull=2, stall_exit=1, **lrn, **mixed, **BASE)
     if args.forms != "all":
         keep = set(args.forms.split(","))
         forms = {k: v for k, v in forms.items() if k in keep}

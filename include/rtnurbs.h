@@ -200,9 +200,11 @@ enum {
   RTN_MODE_EXACT = 0,     /* reference binary BVH, reference op order            */
   RTN_MODE_WIDE = 1,      /* 4-wide SoA BVH, FMA arithmetic, degree-binned Newton */
   RTN_MODE_PERSISTENT = 2,/* persistent work-queue kernel over the wide BVH       */
-  RTN_MODE_PACKET = 3     /* one wave traverses the wide BVH for its 64 rays together
-                             (coherent rays: primary rays in the tiled layout); plain
-                             seed, single scenes                                  */
+  RTN_MODE_PACKET = 3,    /* one wave traverses the wide BVH for its 64 rays together
+                             (coherent rays: primary rays in the tiled layout);
+                             single scenes                                        */
+  RTN_MODE_PERSISTENT_PACKET = 4 /* persistent waves taking 64-ray chunks from a work
+                             queue, each chunk traced as a packet                 */
 };
 enum {
   RTN_SEED_PLAIN = 0,     /* the reference's bilinear seed                      */

@@ -19,6 +19,8 @@
 
 #include <cstdint>
 
+#include "rtn_mem.hpp"
+
 namespace rtn {
 namespace mlp {
 
@@ -81,10 +83,12 @@ __device__ __forceinline__ float bf16r(float x) { return (float)(__bf16)x; }
 //   f[18], f[19]                      degrees, (p - 3.5) / 1.5, (q - 3.5) / 1.5
 // n1, n2: unit ray-plane normals (perpendicular to the ray direction).
 // Reciprocals are hardware approximations: the features end in BF16.
+template <class NP = const float4*, class BP = const float*>
 __device__ __forceinline__ void pair_features(float ox, float oy, float oz, float n1x, float n1y, float n1z,
-                                              float n2x, float n2y, float n2z, const float4* __restrict__ net,
-                                              int P, int Q, const float* __restrict__ bnd, float f[NIN]) {
-  const float hs = fmaxf(fmaxf(bnd[3] - bnd[0], bnd[4] - bnd[1]), fmaxf(bnd[5] - bnd[2], 2e-20f)) * 0.5f;
+                                              float n2x, float n2y, float n2z, NP net, int P, int Q, BP bnd,
+                                              float f[NIN]) {
+  const float hs = fmaxf(fmaxf(ld1(bnd, 3) - ld1(bnd, 0), ld1(bnd, 4) - ld1(bnd, 1)),
+                         fmaxf(ld1(bnd, 5) - ld1(bnd, 2), 2e-20f)) * 0.5f;
   const float is = __builtin_amdgcn_rcpf(hs);
   const int ri[4] = {0, P >> 1, (P + 1) >> 1, P}, ci[4] = {0, Q >> 1, (Q + 1) >> 1, Q};
   float a1[4][4], a2[4][4];  // projected points at rows ri x columns ci
@@ -92,7 +96,7 @@ __device__ __forceinline__ void pair_features(float ox, float oy, float oz, floa
   for (int r = 0; r < 4; r++)
 #pragma unroll
     for (int c = 0; c < 4; c++) {
-      const float4 x = net[ri[r] * (Q + 1) + ci[c]];
+      const float4 x = ld4(net, ri[r] * (Q + 1) + ci[c]);
       const float iw = __builtin_amdgcn_rcpf(x.w);
       const float ex = fmaf(x.x, iw, -ox), ey = fmaf(x.y, iw, -oy), ez = fmaf(x.z, iw, -oz);
       a1[r][c] = fmaf(n1x, ex, fmaf(n1y, ey, n1z * ez)) * is;
@@ -261,10 +265,10 @@ __device__ __forceinline__ void store_row(bf16x8 (*s_x)[4], int row, const float
 
 // Feature row of one pair (pair_features) written into s_x[row] as three
 // 8-value BF16 groups (values 20..23 zero).
+template <class NP = const float4*, class BP = const float*>
 __device__ __forceinline__ void store_features(bf16x8 (*s_x)[3], int row, float ox, float oy, float oz, float n1x,
-                                               float n1y, float n1z, float n2x, float n2y, float n2z,
-                                               const float4* __restrict__ net, int P, int Q,
-                                               const float* __restrict__ bnd) {
+                                               float n1y, float n1z, float n2x, float n2y, float n2z, NP net,
+                                               int P, int Q, BP bnd) {
   float f[NIN];
   pair_features(ox, oy, oz, n1x, n1y, n1z, n2x, n2y, n2z, net, P, Q, bnd, f);
 #pragma unroll

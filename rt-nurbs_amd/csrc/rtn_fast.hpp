@@ -10,6 +10,7 @@
 #include <cstdint>
 
 #include "rtn_exact.hpp"
+#include "rtn_mem.hpp"
 
 namespace rtn {
 namespace fast {
@@ -18,22 +19,6 @@ using ex::pcg_hash;
 
 struct f3 { float x, y, z; };
 
-// Control nets and boxes are read through plain global pointers, or through
-// the constant address space when every lane of a wave reads the same
-// address: then the compiler fetches them with scalar loads into scalar
-// registers (the trace never writes them).
-template <class T>
-using cptr = const __attribute__((address_space(4))) T*;
-template <class T>
-__device__ __forceinline__ cptr<T> as_const(const T* p) {
-  return (cptr<T>)(p);
-}
-typedef float f4v __attribute__((ext_vector_type(4)));
-__device__ __forceinline__ float4 ld4(const float4* p, int64_t i) { return p[i]; }
-__device__ __forceinline__ float4 ld4(cptr<float4> p, int64_t i) {
-  const f4v v = ((cptr<f4v>)(p))[i];
-  return make_float4(v.x, v.y, v.z, v.w);
-}
 
 __device__ __forceinline__ float dot3(f3 a, f3 b) { return fmaf(a.x, b.x, fmaf(a.y, b.y, a.z * b.z)); }
 

@@ -61,6 +61,7 @@ extern "C" int rtn_trace(const rtn_scene* s, const float* const rays[6], int64_t
       rc = trace_wide_scene(s, rays, n_rays, prm, hits, d_tot, st, nullptr);
       break;
     case RTN_MODE_PACKET:
+    case RTN_MODE_PERSISTENT_PACKET:
       rc = trace_packet_scene(s, rays, n_rays, prm, hits, d_tot, st);
       break;
     default:

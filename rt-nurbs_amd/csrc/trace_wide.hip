@@ -462,7 +462,7 @@ k_trace_wide_learned(const WArgs A) {
 // Per (ray, patch) the arithmetic is the one-ray form's; a ray tests every
 // patch whose box it enters before its closest hit, so the closest hit is the
 // one-ray form's except where two patches give the same t (the first tested
-// is kept).  Plain seed; no multi-scene batches.
+// is kept).  Plain or learned seed; single scenes (no multi-scene batches).
 __device__ __forceinline__ int rl_int(int x, int l) { return __builtin_amdgcn_readlane(x, l); }
 __device__ __forceinline__ float rl_float(float x, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l)); }
 
@@ -480,19 +480,20 @@ __device__ __forceinline__ bool child_box(cptr<float> f, int k, const Lane& L, f
 
 constexpr int PKT_STACK = 64;  // one entry per lane of the stack register
 
-template <bool COUNT>
-__global__ void __launch_bounds__(64) k_trace_packet(const WArgs A) {
-  const int lane = threadIdx.x;
+// The 64 rays r0 .. r0+63 (those below n_rays) traced as one packet by the
+// calling wave; work totals accumulate into acc.
+template <bool COUNT, bool LEARNED, int NACC>
+__device__ __forceinline__ void packet_chunk(const WArgs& A, int64_t r0, unsigned long long (&acc)[NACC],
+                                             mlp::bf16x8 (*s_x)[3]) {
+  const int lane = threadIdx.x & 63;
   Lane L;
   Cnt cn{};
   bool active = false;
   {
-    const int64_t r = (int64_t)blockIdx.x * 64 + lane;
+    const int64_t r = r0 + lane;
     if (r < A.n_rays) active = lane_start(L, A, (uint32_t)r) >= 0;
   }
   const bool had_ray = active;
-  constexpr int NACC = COUNT ? 11 : 1;
-  unsigned long long acc[NACC] = {};
   int stk = 0;  // the wave's stack: entry i in lane i
   int sp = 0;   // wave-uniform depth
   int e = 0;    // entry being processed: wide node (>= 0) or (parent << 2 | slot) | sign bit for a leaf
@@ -564,11 +565,10 @@ __global__ void __launch_bounds__(64) k_trace_packet(const WArgs A) {
         const int p = pqv & 0xff, q = (pqv >> 8) & 0xff;
         LeafOut res;
         res.flags = 0;
+        LeafIn in;
         if (act) {
-          if (COUNT) cn.npt++;
           const cptr<float4> cn4 = as_const(A.corners + 3 * (int64_t)pi);
           const float4 c0 = ld4(cn4, 0), c1 = ld4(cn4, 1), c2 = ld4(cn4, 2);
-          LeafIn in;
           in.rp = L.rp;
           in.o = L.o;
           in.d = L.d;
@@ -576,13 +576,40 @@ __global__ void __launch_bounds__(64) k_trace_packet(const WArgs A) {
                                    {L.o.x, L.o.y, L.o.z}, {L.d.x, L.d.y, L.d.z}, in.gu, in.gv);
           in.pu = in.gu;
           in.pv = in.gv;
+        }
+        if (LEARNED) {  // network seeds of the wave's pairs on this patch (matrix cores), plain seed as fallback
+          // with seed sharing every lane of an aligned quad whose first lane tests the patch takes that lane's seed
+          const int lead = lane & ~3;
+          const bool lead_act = __shfl((int)act, lead) != 0;
+          const bool own = act && (!A.seed_share || lane == lead || !lead_act);
+          int n_act;
+          const int row = mlp::seed_row(own, lane, n_act);
+          if (own) {
+            mlp::store_features(s_x, row, L.o.x, L.o.y, L.o.z, L.rp.n1.x, L.rp.n1.y, L.rp.n1.z, L.rp.n2.x, L.rp.n2.y,
+                                L.rp.n2.z, net, p, q, as_const(A.bounds + 6 * (int64_t)pi));
+          } else {
+            const mlp::bf16x8 z = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+            for (int k = 0; k < 3; k++) s_x[row][k] = z;
+          }
+          mlp::seed_wave_rows(A.mlp_tiles, A.mlp_bias, own, row, n_act, in.gu, in.gv, s_x, lane);
+          if (A.seed_share) {
+            const float lu = __shfl(in.gu, lead), lv = __shfl(in.gv, lead);
+            if (act && !own) {
+              in.gu = lu;
+              in.gv = lv;
+            }
+          }
+        }
+        if (act) {
+          if (COUNT) cn.npt++;
           in.seed = L.r + A.seed_base;
           in.max_iter = A.max_iter;
-          in.learned_iters = A.max_iter;
-          in.fallback_iters = A.max_iter;
+          in.learned_iters = A.learned_iters;
+          in.fallback_iters = A.fallback_iters;
           in.mixed = A.mixed;
           in.stall = A.stall_exit;
-          in.learned = 0;
+          in.learned = LEARNED;
           in.thr_det = A.thr_det;
           in.thr_res = A.thr_res;
           in.cond_thr = A.cond_thr;
@@ -595,6 +622,7 @@ __global__ void __launch_bounds__(64) k_trace_packet(const WArgs A) {
             cn.ne += res.evals;
             cn.nfl += res.evals * ex::eval_flops(p, q);
             if (res.flags & 4) cn.n64++;
+            if (res.flags & 8) cn.nfb++;
           }
           if ((res.flags & 1) && 0.0f < res.t && res.t < L.t_max) {
             record(L, A, pi, res.t, res.u, res.v, res.Su, res.Sv);
@@ -619,6 +647,29 @@ __global__ void __launch_bounds__(64) k_trace_packet(const WArgs A) {
       acc[0] += 1;
     }
   }
+}
+
+// One packet per single-wave block, or (persistent) waves that take 64-ray
+// chunks from a global counter until the rays run out: a wave finishes its
+// whole chunk before it takes the next, so chunks keep the ray layout's
+// coherence.
+template <bool PERSISTENT, bool COUNT, bool LEARNED>
+__device__ __forceinline__ void packet_body(const WArgs& A) {
+  __shared__ mlp::bf16x8 s_x[LEARNED ? 64 : 1][3];
+  const int lane = threadIdx.x & 63;
+  constexpr int NACC = COUNT ? 11 : 1;
+  unsigned long long acc[NACC] = {};
+  if (!PERSISTENT) {
+    packet_chunk<COUNT, LEARNED>(A, (int64_t)blockIdx.x * 64, acc, s_x);
+  } else {
+    while (true) {
+      unsigned int base = 0;
+      if (lane == 0) base = atomicAdd(A.work, 64u);
+      base = __shfl(base, 0);
+      if ((int64_t)base >= A.n_rays) break;
+      packet_chunk<COUNT, LEARNED>(A, (int64_t)base, acc, s_x);
+    }
+  }
   if (A.totals) {  // every lane of the wave arrives here together
 #pragma unroll
     for (int k = 0; k < NACC; k++) {
@@ -627,6 +678,16 @@ __global__ void __launch_bounds__(64) k_trace_packet(const WArgs A) {
       if (lane == 0 && x) atomicAdd(A.totals + (COUNT ? k : 6), x);
     }
   }
+}
+
+template <bool PERSISTENT, bool COUNT>
+__global__ void __launch_bounds__(64) k_trace_packet(const WArgs A) {
+  packet_body<PERSISTENT, COUNT, false>(A);
+}
+
+template <bool PERSISTENT, bool COUNT>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) k_trace_packet_learned(const WArgs A) {
+  packet_body<PERSISTENT, COUNT, true>(A);
 }
 
 }  // namespace
@@ -668,6 +729,9 @@ void launch_shape(bool learned, int bpc, int64_t n_max, hipStream_t stream, cons
 }  // namespace
 
 const LaunchShape kDefaultShape{WB_DEFAULT, WSTK_DEFAULT, 8};
+// persistent packet form: resident single-wave blocks per compute unit (capped
+// by what the kernel's registers keep resident)
+constexpr int kPacketBlocksPerCU = 16;
 const LaunchShape kTunedShapes[] = {{64, 16, 8},  {64, 24, 8},  {64, 32, 8},  {128, 16, 8}, {128, 24, 8},
                                     {128, 32, 8}, {256, 16, 8}, {256, 24, 8}, {256, 32, 8}};
 const int kNumTunedShapes = 9;
@@ -783,14 +847,33 @@ int trace_wide_scene(const rtn_scene* s, const float* const rays[6], int64_t n, 
 
 int trace_packet_scene(const rtn_scene* s, const float* const rays[6], int64_t n, const rtn_trace_params* prm,
                        const rtn_hits* hits, unsigned long long* d_totals, hipStream_t stream) {
-  RTN_CHECK(prm->seed_mode == RTN_SEED_PLAIN, RTN_EUNSUPPORTED, "rtn_trace: the packet form runs the plain seed");
+  const bool persistent = prm->mode == RTN_MODE_PERSISTENT_PACKET;
+  const bool learned = prm->seed_mode == RTN_SEED_LEARNED;
+  RTN_CHECK(!persistent || n <= (int64_t)0xFFFFFFFFu - ((int64_t)1 << 24), RTN_EINVAL,
+            "trace: %lld rays exceed the persistent queue's range", (long long)n);
   WArgs A;
   RTN_TRY(fill_wide_args(s, rays, n, prm, hits, d_totals, nullptr, A));
-  A.work = nullptr;
-  const unsigned blocks = (unsigned)((n + 63) / 64);
-  if (prm->count) hipLaunchKernelGGL(k_trace_packet<true>, dim3(blocks), dim3(64), 0, stream, A);
-  else hipLaunchKernelGGL(k_trace_packet<false>, dim3(blocks), dim3(64), 0, stream, A);
-  RTN_HIP(hipGetLastError());
+  A.bounds = s->bounds.ptr;
+  unsigned int* d_work = nullptr;
+  if (persistent) {
+    RTN_HIP(hipMallocAsync((void**)&d_work, 4 * sizeof(unsigned int), stream));
+    const unsigned int zero[4] = {0, 0, 0, 0};
+    RTN_HIP(hipMemcpyAsync(d_work, zero, sizeof(zero), hipMemcpyHostToDevice, stream));
+  }
+  A.work = d_work;
+  const bool count = prm->count != 0;
+  void (*kernel)(const WArgs);
+  if (persistent) {
+    kernel = learned ? (count ? k_trace_packet_learned<true, true> : k_trace_packet_learned<true, false>)
+                     : (count ? k_trace_packet<true, true> : k_trace_packet<true, false>);
+  } else {
+    kernel = learned ? (count ? k_trace_packet_learned<false, true> : k_trace_packet_learned<false, false>)
+                     : (count ? k_trace_packet<false, true> : k_trace_packet<false, false>);
+  }
+  launch_one<64>(kernel, persistent, kPacketBlocksPerCU, n, stream, A);
+  const hipError_t e = hipGetLastError();
+  if (d_work) RTN_HIP(hipFreeAsync(d_work, stream));
+  RTN_HIP(e);
   return RTN_OK;
 }
 

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(PKG_DIR, "lib", "librtnurbs.so")
 
 RTN_OK, RTN_EINVAL, RTN_EHIP, RTN_ENOMEM, RTN_EUNSUPPORTED, RTN_EOVERFLOW = 0, -1, -2, -3, -4, -5
 MAX_DEGREE = 5
-MODE_EXACT, MODE_WIDE, MODE_PERSISTENT, MODE_PACKET = 0, 1, 2, 3
+MODE_EXACT, MODE_WIDE, MODE_PERSISTENT, MODE_PACKET, MODE_PERSISTENT_PACKET = 0, 1, 2, 3, 4
 SEED_PLAIN, SEED_LEARNED = 0, 1
 PREC_FP32, PREC_MIXED = 0, 1
 

@@ -78,3 +78,32 @@ def test_learned_trace_keeps_the_hits(rtl, net):
             assert 0 < tb["learned_fallbacks"] < 0.8 * tb["newton_calls"], (mode, share)
             # the same patch tests, up to the pruning that different closest hits cause
             assert abs(tb["newton_calls"] - ta["newton_calls"]) < 0.01 * ta["newton_calls"]
+
+
+def test_learned_packet_forms(rtl, net):
+    """The packet form with the learned seed: without seed sharing every pair
+    gets its own network evaluation, so it equals the one-ray form but for
+    equal-t ties; the persistent packet form traces the same 64-ray chunks
+    with the same code, so it equals the packet form bit for bit."""
+    from test_gpu_wide import same_but_ties
+    sc = scenes.config1(2001)
+    scene = rtl.Scene(sc.surfaces).build()
+    rays = rtl.generate_rays(sc.camera, spp=4, jitter=True, seed=5, tiled=True)
+    lrn = dict(hull_cull=2, seed_mode=rtl.SEED_LEARNED, seed_net=net, learned_iters=3, fallback_iters=8, **CFG)
+    out = {}
+    for name, kw in (("wide0", dict(mode=rtl.MODE_WIDE, seed_share=0)), ("packet0", dict(mode=rtl.MODE_PACKET, seed_share=0)),
+                     ("packet1", dict(mode=rtl.MODE_PACKET, seed_share=1, stall_exit=1, precision=rtl.PREC_MIXED)),
+                     ("ppacket1", dict(mode=rtl.MODE_PERSISTENT_PACKET, seed_share=1, stall_exit=1,
+                                       precision=rtl.PREC_MIXED))):
+        print(f"case learned {name}", flush=True)
+        h, tot = rtl.trace(scene, rays, rtl.params(count=1, **lrn, **kw))
+        assert tot["overflow"] == 0 and 0 < tot["learned_fallbacks"] < 0.8 * tot["newton_calls"], name
+        out[name] = h.get()
+    same_but_ties(out["packet0"], out["wide0"])
+    a, b = out["ppacket1"], out["packet1"]
+    assert np.array_equal(a["hit"], b["hit"])
+    m = b["hit"] >= 0
+    for k in ("t", "u", "v"):
+        assert np.array_equal(a[k][m].view(np.uint32), b[k][m].view(np.uint32)), k
+    # sharing and the stall exit keep the hits of the unshared form
+    assert np.mean(b["hit"] == out["packet0"]["hit"]) > 0.99
