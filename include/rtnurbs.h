@@ -394,7 +394,10 @@ typedef struct {
   float light[3];         /* direction towards the light                        */
   float eps;              /* secondary origin offset, eps * (1 + max|S|)        */
   int shadow, mirror;     /* 0/1: generate shadow / mirror rays                 */
-  int reserved[4];
+  int packet_primary;     /* 1: trace the primary rays with the packet form (one wave per 64
+                             coherent rays) instead of the persistent per-lane form  */
+  int packet_secondary;   /* 1: the same for the shadow and mirror rays         */
+  int reserved[2];
 } rtn_frame_params;
 typedef struct {
   int64_t n_primary, n_secondary;  /* rays; n_secondary counts non-empty slots   */
@@ -420,6 +423,9 @@ int rtn_frame_results(const rtn_frame* frame, rtn_hits* primary, rtn_hits* secon
 int rtn_frame_set_shape(rtn_frame* frame, int lanes, int stack, int blocks_per_cu);
 /* Refill threshold of both persistent launches (see rtn_trace_params.refill_min). */
 int rtn_frame_set_refill(rtn_frame* frame, int refill_min);
+/* Packet form for the primary and/or the secondary launch (see
+ * rtn_frame_params.packet_primary).  Drops a captured graph. */
+int rtn_frame_set_packet(rtn_frame* frame, int packet_primary, int packet_secondary);
 /* Autotuner: times the frame over every built launch shape at 2, 4, 8 and 12
  * blocks per CU (capped at the blocks that stay resident; whole-wave refill;
  * reps timed runs each; stack overflow rejects a shape), then the best shape

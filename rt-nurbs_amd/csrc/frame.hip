@@ -108,9 +108,11 @@ int check_scene_ids(const rtn_batch* b, const int* ids, int64_t n, hipStream_t s
 
 int batch_args(const rtn_batch* b, const float* const rays[6], const int* ray_scene, int64_t n,
                const rtn_trace_params* prm, const rtn_hits* hits, unsigned long long* d_totals, WArgs& A) {
-  RTN_CHECK(prm->mode == RTN_MODE_WIDE || prm->mode == RTN_MODE_PERSISTENT, RTN_EUNSUPPORTED,
-            "batch trace: runs the wide or persistent form");
+  RTN_CHECK(prm->mode == RTN_MODE_WIDE || prm->mode == RTN_MODE_PERSISTENT || prm->mode == RTN_MODE_PACKET ||
+                prm->mode == RTN_MODE_PERSISTENT_PACKET,
+            RTN_EUNSUPPORTED, "batch trace: runs the wide, persistent or packet forms");
   RTN_TRY(fill_wide_args(&b->merged, rays, n, prm, hits, d_totals, nullptr, A));
+  A.bounds = b->merged.bounds.ptr;
   A.scene_root = b->roots.ptr;
   A.ray_scene = ray_scene;
   return RTN_OK;
@@ -199,8 +201,11 @@ extern "C" int rtn_batch_trace(const rtn_batch* b, const float* const rays[6], c
   if (rc == RTN_OK) {
     a.work = &d->work;
     RTN_HIP(hipMemcpyAsync(d, &h, sizeof(Block), hipMemcpyHostToDevice, st));
-    rc = launch_wide_kernel(kDefaultShape, prm->mode == RTN_MODE_PERSISTENT, prm->count != 0,
-                            prm->seed_mode == RTN_SEED_LEARNED, n, st, a);
+    const bool packet = prm->mode == RTN_MODE_PACKET || prm->mode == RTN_MODE_PERSISTENT_PACKET;
+    rc = packet ? launch_packet_kernel(prm->mode == RTN_MODE_PERSISTENT_PACKET, prm->count != 0,
+                                       prm->seed_mode == RTN_SEED_LEARNED, n, st, a)
+                : launch_wide_kernel(kDefaultShape, prm->mode == RTN_MODE_PERSISTENT, prm->count != 0,
+                                     prm->seed_mode == RTN_SEED_LEARNED, n, st, a);
   }
   unsigned long long tot[12] = {0};
   if (rc == RTN_OK) {
@@ -250,7 +255,8 @@ int frame_enqueue(rtn_frame* f, hipStream_t st) {
   hipLaunchKernelGGL(k_frame_reset, dim3(1), dim3(64), 0, st, c);
   RTN_HIP(hipGetLastError());
   const bool learned = frame_learned(f);
-  RTN_TRY(launch_wide_kernel(f->shape, true, false, learned, f->n, st, f->a1));
+  if (f->fp.packet_primary) RTN_TRY(launch_packet_kernel(false, false, learned, f->n, st, f->a1));
+  else RTN_TRY(launch_wide_kernel(f->shape, true, false, learned, f->n, st, f->a1));
   if (f->cap2 > 0) {
     const rtn_hits h1 = f->hits1();
     float* outs[6];
@@ -258,7 +264,8 @@ int frame_enqueue(rtn_frame* f, hipStream_t st) {
     launch_secondaries(f->rays, &h1, f->n, f->fp.light, f->fp.eps, f->fp.shadow, f->fp.mirror, outs,
                        f->s_parent.ptr, f->s_kind.ptr, &c->sec_count, f->ray_scene, f->s_scene.ptr, st);
     RTN_HIP(hipGetLastError());
-    RTN_TRY(launch_wide_kernel(f->shape, true, false, learned, f->cap2, st, f->a2));
+    if (f->fp.packet_secondary) RTN_TRY(launch_packet_kernel(false, false, learned, f->cap2, st, f->a2));
+    else RTN_TRY(launch_wide_kernel(f->shape, true, false, learned, f->cap2, st, f->a2));
   }
   return RTN_OK;
 }
@@ -306,6 +313,9 @@ extern "C" int rtn_frame_create(rtn_frame** out, const rtn_batch* b, const float
   RTN_CHECK(!fp->trace.count && !fp->trace.any_hit, RTN_EINVAL, "rtn_frame_create: no counters, closest-hit primaries");
   RTN_CHECK((fp->shadow == 0 || fp->shadow == 1) && (fp->mirror == 0 || fp->mirror == 1), RTN_EINVAL,
             "rtn_frame_create: shadow and mirror are 0 or 1");
+  RTN_CHECK((fp->packet_primary == 0 || fp->packet_primary == 1) &&
+                (fp->packet_secondary == 0 || fp->packet_secondary == 1),
+            RTN_EINVAL, "rtn_frame_create: packet_primary and packet_secondary are 0 or 1");
   const float ll = std::sqrt(fp->light[0] * fp->light[0] + fp->light[1] * fp->light[1] + fp->light[2] * fp->light[2]);
   RTN_CHECK(ll > 0.f, RTN_EINVAL, "rtn_frame_create: zero light direction");
   *out = nullptr;
@@ -421,6 +431,15 @@ extern "C" int rtn_frame_set_refill(rtn_frame* f, int refill_min) {
   RTN_CHECK(f && refill_min >= 0 && refill_min <= 64, RTN_EINVAL, "rtn_frame_set_refill: bad arguments");
   drop_graph(f);
   return frame_set_refill(f, refill_min);
+}
+
+extern "C" int rtn_frame_set_packet(rtn_frame* f, int packet_primary, int packet_secondary) {
+  RTN_CHECK(f && (packet_primary == 0 || packet_primary == 1) && (packet_secondary == 0 || packet_secondary == 1),
+            RTN_EINVAL, "rtn_frame_set_packet: bad arguments");
+  drop_graph(f);
+  f->fp.packet_primary = packet_primary;
+  f->fp.packet_secondary = packet_secondary;
+  return RTN_OK;
 }
 
 // Autotuner: every built launch shape (lanes per block x stack entries) at

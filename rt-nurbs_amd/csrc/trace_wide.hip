@@ -488,15 +488,24 @@ __device__ __forceinline__ void packet_chunk(const WArgs& A, int64_t r0, unsigne
   const int lane = threadIdx.x & 63;
   Lane L;
   Cnt cn{};
-  bool active = false;
+  int root = -1;
   {
     const int64_t r = r0 + lane;
-    if (r < A.n_rays) active = lane_start(L, A, (uint32_t)r) >= 0;
+    if (r < A.n_rays) root = lane_start(L, A, (uint32_t)r);
   }
-  const bool had_ray = active;
+  bool pending = root >= 0;  // rays whose traversal has not run yet
+  const bool had_ray = pending;
+  // one packet per distinct scene root among the wave's rays (in a batch the
+  // rays of one scene are contiguous, so a wave almost always has one)
+  while (true) {
+  const unsigned long long todo = __ballot(pending);
+  if (!todo) break;
+  int e = rl_int(root, __ffsll((long long)todo) - 1);  // entry being processed: wide node (>= 0) or
+                                                       // (parent << 2 | slot) | sign bit for a leaf
+  bool active = pending && root == e;
+  pending = pending && !active;
   int stk = 0;  // the wave's stack: entry i in lane i
   int sp = 0;   // wave-uniform depth
-  int e = 0;    // entry being processed: wide node (>= 0) or (parent << 2 | slot) | sign bit for a leaf
   while (__ballot(active)) {
     if (e >= 0) {
       const cptr<float> base = as_const(reinterpret_cast<const float*>(A.nodes + e));
@@ -637,6 +646,7 @@ __device__ __forceinline__ void packet_chunk(const WArgs& A, int64_t r0, unsigne
     if (sp == 0) break;
     sp--;
     e = rl_int(stk, sp);
+  }
   }
   if (had_ray) {
     lane_finish<COUNT>(L, A, cn);
@@ -849,12 +859,27 @@ int trace_wide_scene(const rtn_scene* s, const float* const rays[6], int64_t n, 
   return rc;
 }
 
+int launch_packet_kernel(bool persistent, bool count, bool learned, int64_t n_max, hipStream_t stream,
+                         const WArgs& args) {
+  RTN_CHECK(!persistent || n_max <= (int64_t)0xFFFFFFFFu - ((int64_t)1 << 24), RTN_EINVAL,
+            "trace: %lld rays exceed the persistent queue's range", (long long)n_max);
+  RTN_CHECK(args.bounds, RTN_EINVAL, "trace: the packet form needs the patch boxes");
+  void (*kernel)(const WArgs);
+  if (persistent) {
+    kernel = learned ? (count ? k_trace_packet_learned<true, true> : k_trace_packet_learned<true, false>)
+                     : (count ? k_trace_packet<true, true> : k_trace_packet<true, false>);
+  } else {
+    kernel = learned ? (count ? k_trace_packet_learned<false, true> : k_trace_packet_learned<false, false>)
+                     : (count ? k_trace_packet<false, true> : k_trace_packet<false, false>);
+  }
+  launch_one<64>(kernel, persistent, kPacketBlocksPerCU, n_max, stream, args);
+  RTN_HIP(hipGetLastError());
+  return RTN_OK;
+}
+
 int trace_packet_scene(const rtn_scene* s, const float* const rays[6], int64_t n, const rtn_trace_params* prm,
                        const rtn_hits* hits, unsigned long long* d_totals, hipStream_t stream) {
   const bool persistent = prm->mode == RTN_MODE_PERSISTENT_PACKET;
-  const bool learned = prm->seed_mode == RTN_SEED_LEARNED;
-  RTN_CHECK(!persistent || n <= (int64_t)0xFFFFFFFFu - ((int64_t)1 << 24), RTN_EINVAL,
-            "trace: %lld rays exceed the persistent queue's range", (long long)n);
   WArgs A;
   RTN_TRY(fill_wide_args(s, rays, n, prm, hits, d_totals, nullptr, A));
   A.bounds = s->bounds.ptr;
@@ -865,20 +890,9 @@ int trace_packet_scene(const rtn_scene* s, const float* const rays[6], int64_t n
     RTN_HIP(hipMemcpyAsync(d_work, zero, sizeof(zero), hipMemcpyHostToDevice, stream));
   }
   A.work = d_work;
-  const bool count = prm->count != 0;
-  void (*kernel)(const WArgs);
-  if (persistent) {
-    kernel = learned ? (count ? k_trace_packet_learned<true, true> : k_trace_packet_learned<true, false>)
-                     : (count ? k_trace_packet<true, true> : k_trace_packet<true, false>);
-  } else {
-    kernel = learned ? (count ? k_trace_packet_learned<false, true> : k_trace_packet_learned<false, false>)
-                     : (count ? k_trace_packet<false, true> : k_trace_packet<false, false>);
-  }
-  launch_one<64>(kernel, persistent, kPacketBlocksPerCU, n, stream, A);
-  const hipError_t e = hipGetLastError();
+  const int rc = launch_packet_kernel(persistent, prm->count != 0, prm->seed_mode == RTN_SEED_LEARNED, n, stream, A);
   if (d_work) RTN_HIP(hipFreeAsync(d_work, stream));
-  RTN_HIP(e);
-  return RTN_OK;
+  return rc;
 }
 
 }  // namespace rtn

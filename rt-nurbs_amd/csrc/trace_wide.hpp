@@ -78,5 +78,9 @@ int fill_wide_args(const rtn_scene* s, const float* const rays[6], int64_t n, co
 // n_max bounds the rays the launch may see (grid size).
 int launch_wide_kernel(const LaunchShape& sh, bool persistent, bool count, bool learned, int64_t n_max,
                        hipStream_t stream, const WArgs& args);
+// The packet form (one wave traverses for its 64 rays; persistent: 64-ray
+// chunks from args.work); args.bounds must be set.
+int launch_packet_kernel(bool persistent, bool count, bool learned, int64_t n_max, hipStream_t stream,
+                         const WArgs& args);
 
 }  // namespace rtn

@@ -39,7 +39,7 @@ EXPORTS = [
     "rtn_shade_lambert", "rtn_build_params_default", "rtn_scene_build_ex", "rtn_batch_create", "rtn_batch_destroy",
     "rtn_batch_patch_base", "rtn_batch_trace", "rtn_frame_params_default", "rtn_frame_create", "rtn_frame_destroy",
     "rtn_frame_run", "rtn_frame_capture", "rtn_frame_status", "rtn_frame_results", "rtn_frame_set_shape",
-    "rtn_frame_autotune", "rtn_solve_pairs", "rtn_frame_set_refill", "rtn_scene_save", "rtn_scene_load",
+    "rtn_frame_autotune", "rtn_solve_pairs", "rtn_frame_set_refill", "rtn_frame_set_packet", "rtn_scene_save", "rtn_scene_load",
 ]
 BVH_EXACT_SAH, BVH_BINNED_SAH = 0, 1
 MLP_NIN, MLP_HIDDEN, MLP_NPARAM = 20, 64, 9794
@@ -78,7 +78,8 @@ class BuildParams(C.Structure):
 
 class FrameParams(C.Structure):
     _fields_ = [("trace", TraceParams), ("light", C.c_float * 3), ("eps", C.c_float), ("shadow", C.c_int),
-                ("mirror", C.c_int), ("reserved", C.c_int * 4)]
+                ("mirror", C.c_int), ("packet_primary", C.c_int), ("packet_secondary", C.c_int),
+                ("reserved", C.c_int * 2)]
 
 
 class FrameStatus(C.Structure):
@@ -167,6 +168,7 @@ def load():
                                     C.POINTER(vp * 6)]
     L.rtn_frame_set_shape.argtypes = [vp, i, i, i]
     L.rtn_frame_set_refill.argtypes = [vp, i]
+    L.rtn_frame_set_packet.argtypes = [vp, i, i]
     L.rtn_frame_autotune.argtypes = [vp, i, C.POINTER(TuneEntry), i, C.POINTER(i), vp]
     L.rtn_solve_pairs.argtypes = [vp, C.POINTER(vp * 6), i64, vp, vp, vp, i64, C.POINTER(TraceParams), vp, vp, vp,
                                   vp, vp]
@@ -756,7 +758,8 @@ class Batch:
             pass
 
 
-def frame_params(light=(0.3, -0.4, 1.0), eps=1e-4, shadow=1, mirror=1, **trace_kw):
+def frame_params(light=(0.3, -0.4, 1.0), eps=1e-4, shadow=1, mirror=1, packet_primary=0, packet_secondary=0,
+                 **trace_kw):
     fp = FrameParams()
     load().rtn_frame_params_default(C.byref(fp))
     for k, v in trace_kw.items():
@@ -768,6 +771,7 @@ def frame_params(light=(0.3, -0.4, 1.0), eps=1e-4, shadow=1, mirror=1, **trace_k
         setattr(fp.trace, k, v)
     fp.light[:] = list(light)
     fp.eps, fp.shadow, fp.mirror = eps, shadow, mirror
+    fp.packet_primary, fp.packet_secondary = packet_primary, packet_secondary
     return fp
 
 
@@ -797,6 +801,9 @@ class Frame:
 
     def set_refill(self, refill_min):
         check(load().rtn_frame_set_refill(self.h, refill_min))
+
+    def set_packet(self, primary, secondary):
+        check(load().rtn_frame_set_packet(self.h, int(primary), int(secondary)))
 
     def autotune(self, reps=2, stream=None):
         table = (TuneEntry * 64)()

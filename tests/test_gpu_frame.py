@@ -82,6 +82,22 @@ def test_batch_equals_single_scenes(rtl):
             m = s1["hit"] >= 0
             assert np.array_equal(g["t"][off:off + n][m].view(np.uint32), s1["t"][m].view(np.uint32))
             off += n
+    # packet forms over a batch: one traversal per scene root present in a wave;
+    # the same hits as the wide form but for equal-t ties, persistent = one-shot
+    from test_gpu_wide import same_but_ties
+    wb, _ = batch.trace(rays, ids, rtl.params(mode=rtl.MODE_WIDE, **CFG))
+    pb, _ = batch.trace(rays, ids, rtl.params(mode=rtl.MODE_PACKET, **CFG))
+    qb, _ = batch.trace(rays, ids, rtl.params(mode=rtl.MODE_PERSISTENT_PACKET, **CFG))
+    same_but_ties(pb.get(), wb.get())
+    assert np.array_equal(qb.get()["hit"], pb.get()["hit"])
+    # rays of different scenes interleaved within every wave
+    perm = np.random.default_rng(0).permutation(rays.n)
+    rp = rtl.Rays.from_host([a[perm] for a in rays.get()])
+    ip = rtl.DeviceArray.from_host(ids.get()[perm])
+    pw, _ = batch.trace(rp, ip, rtl.params(mode=rtl.MODE_WIDE, **CFG))
+    pk, tk = batch.trace(rp, ip, rtl.params(mode=rtl.MODE_PACKET, count=1, **CFG))
+    assert tk["overflow"] == 0
+    same_but_ties(pk.get(), pw.get())
     bad = rtl.DeviceArray.from_host(np.full(rays.n, len(items), np.int32))
     with pytest.raises(rtl.RtnError):
         batch.trace(rays, bad, rtl.params(mode=rtl.MODE_WIDE, **CFG))
@@ -126,6 +142,25 @@ def test_frame_graph_replay_and_autotune(rtl):
             m = ref[part]["hit"] >= 0
             assert np.array_equal(got[part]["t"][m].view(np.uint32), ref[part]["t"][m].view(np.uint32))
         batch.trace(rays, ids, rtl.params(mode=rtl.MODE_WIDE, **CFG))  # eager work between replays
+    # packet form for both launches: the primaries equal the persistent form's
+    # but for equal-t ties, and a captured graph replays the eager run
+    print("frame packet", flush=True)
+    from test_gpu_wide import same_but_ties
+    fr.set_packet(1, 1)
+    fr.run()
+    pk = fr.results()
+    same_but_ties(pk["primary"], ref["primary"])
+    assert abs(pk["n_secondary"] - ref["n_secondary"]) <= 4
+    assert np.mean(pk["secondary"]["hit"] == ref["secondary"]["hit"]) > 0.999
+    fr.capture(stream)
+    for rep in range(2):
+        fr.run(stream)
+        rtl.synchronize()
+        got = fr.results()
+        for part in ("primary", "secondary"):
+            assert np.array_equal(got[part]["hit"], pk[part]["hit"]), (rep, part)
+        batch.trace(rays, ids, rtl.params(mode=rtl.MODE_WIDE, **CFG))
+    fr.set_packet(0, 0)
     # autotuner: tries every built shape, keeps one, results unchanged
     print("autotune", flush=True)
     table = fr.autotune(reps=1)
