@@ -212,7 +212,9 @@ enum {
 };
 enum {
   RTN_PREC_FP32 = 0,      /* FP32 Newton                                         */
-  RTN_PREC_MIXED = 1      /* FP32 Newton + FP64 step when cond(J) > cond_thr     */
+  RTN_PREC_MIXED = 1      /* FP32 Newton; after the trace, one FP64 Newton step on each ray's
+                             closest hit whose cond(J) > cond_thr (fast forms; needs the
+                             t, u and v outputs)                                  */
 };
 
 typedef struct {

@@ -34,7 +34,7 @@ using namespace rtn::fast;
 namespace {
 
 struct Cnt {  // per-ray work counters (counting builds only)
-  uint32_t nc, ne, ncull, nbox, nleaf, npt, ovf, nfl, n64, nfb;
+  uint32_t nc, ne, ncull, nbox, nleaf, npt, ovf, nfl, n64, nfb;  // n64 stays 0: the FP64 steps run in k_fix64
 };
 
 struct Lane {
@@ -43,6 +43,7 @@ struct Lane {
   uint32_t r;
   float t_max;
   int best;
+  bool fix;  // mixed precision: the closest hit so far is ill-conditioned
 };
 
 // Starts ray r on this lane; returns its root node (the ray's scene in a
@@ -63,12 +64,15 @@ __device__ __forceinline__ int lane_start(Lane& L, const WArgs& A, uint32_t r) {
   L.rp = planes(L.o, L.d);
   L.t_max = INFINITY;
   L.best = -1;
+  L.fix = false;
   return A.scene_root ? A.scene_root[A.ray_scene[r]] : 0;
 }
 
 template <bool COUNT>
 __device__ __forceinline__ void lane_finish(const Lane& L, const WArgs& A, const Cnt& c) {
-  A.prim[L.r] = L.best;
+  // a closest hit that wants the FP64 step is marked -2 - patch until the
+  // correction pass (k_fix64) has run it
+  A.prim[L.r] = L.fix ? -2 - L.best : L.best;
   if (COUNT && A.counters) {
     uint32_t* o = A.counters + 8 * (int64_t)L.r;
     o[0] = c.nc; o[1] = c.ne; o[2] = c.ncull; o[3] = c.nbox; o[4] = c.nleaf; o[5] = c.npt; o[6] = c.ovf; o[7] = c.nfl;
@@ -125,9 +129,11 @@ __device__ __forceinline__ void expand(Lane& L, Cnt& cn, const WideNode* __restr
   }
 }
 
-__device__ __forceinline__ void record(Lane& L, const WArgs& A, int pi, float t, float u, float v, f3 Su, f3 Sv) {
+__device__ __forceinline__ void record(Lane& L, const WArgs& A, int pi, float t, float u, float v, f3 Su, f3 Sv,
+                                       bool fix) {
   L.t_max = t;
   L.best = pi;
+  L.fix = fix;
   const uint32_t r = L.r;
   if (A.t) A.t[r] = t;
   if (A.u) A.u[r] = u;
@@ -141,7 +147,7 @@ struct LeafOut {
   float t, u, v;
   f3 S, Su, Sv;
   int evals;     // surface evaluations (both attempts when a learned seed falls back)
-  int flags;     // 1 hit, 4 FP64 step done, 8 learned fallback, 16 FP64 step needed
+  int flags;     // 1 hit, 8 learned fallback, 16 FP64 step wanted (ill-conditioned hit)
 };
 
 struct LeafIn {
@@ -177,8 +183,9 @@ __device__ __forceinline__ LeafOut leaf_q(NP net, int P, const LeafIn& in) {
   }
   if (!h.hit) return o;
   const float t = fmaf(h.S.x - in.o.x, in.d.x, fmaf(h.S.y - in.o.y, in.d.y, (h.S.z - in.o.z) * in.d.z));
-  // ill-conditioned hit: the FP64 step runs after the class loop (flag 16),
-  // in one runtime-degree copy shared by all classes
+  // ill-conditioned hit (flag 16): if it ends as the ray's closest hit, the
+  // correction pass k_fix64 runs the FP64 step on it after the trace, so the
+  // FP64 code and its registers stay out of the trace kernels
   if (in.mixed && cond2(h.m0, h.m1, h.m2, h.m3) > in.cond_thr) o.flags |= 16;
   o.flags |= 1;
   o.t = t;
@@ -389,14 +396,10 @@ __device__ __forceinline__ void trace_wide_body(const WArgs& A) {
       todo &= ~__ballot(mine);
     }
     if (pending) {
-      if (res.flags & 16) {  // FP64 correction of an ill-conditioned hit
-        if (correct64_rt(net, p, q, L.o, L.d, res.u, res.v, res.S, res.Su, res.Sv, res.t)) res.flags |= 4;
-      }
       if (COUNT) {
         cn.nc++;
         cn.ne += res.evals;
         cn.nfl += res.evals * ex::eval_flops(p, q);
-        if (res.flags & 4) cn.n64++;
         if (res.flags & 8) cn.nfb++;
       }
       if (A.pair_count && (res.flags & 1) && 0.0f < res.t) {
@@ -408,7 +411,7 @@ __device__ __forceinline__ void trace_wide_body(const WArgs& A) {
         }
       }
       if ((res.flags & 1) && 0.0f < res.t && res.t < L.t_max) {
-        record(L, A, pi, res.t, res.u, res.v, res.Su, res.Sv);
+        record(L, A, pi, res.t, res.u, res.v, res.Su, res.Sv, (res.flags & 16) != 0);
         if (A.any_hit || (A.ray_kind && A.ray_kind[L.r] == 0)) {  // occlusion query: the first hit ends the ray
           off = 0;
           cand_left = 0;
@@ -623,18 +626,14 @@ __device__ __forceinline__ void packet_chunk(const WArgs& A, int64_t r0, unsigne
           in.thr_res = A.thr_res;
           in.cond_thr = A.cond_thr;
           res = leaf_dispatch(degree_class(p, q), net, p, in);  // the patch's degree: a wave-uniform branch
-          if (res.flags & 16) {  // FP64 correction of an ill-conditioned hit
-            if (correct64_rt(net, p, q, L.o, L.d, res.u, res.v, res.S, res.Su, res.Sv, res.t)) res.flags |= 4;
-          }
           if (COUNT) {
             cn.nc++;
             cn.ne += res.evals;
             cn.nfl += res.evals * ex::eval_flops(p, q);
-            if (res.flags & 4) cn.n64++;
             if (res.flags & 8) cn.nfb++;
           }
           if ((res.flags & 1) && 0.0f < res.t && res.t < L.t_max) {
-            record(L, A, pi, res.t, res.u, res.v, res.Su, res.Sv);
+            record(L, A, pi, res.t, res.u, res.v, res.Su, res.Sv, (res.flags & 16) != 0);
             if (A.any_hit || (A.ray_kind && A.ray_kind[L.r] == 0)) {  // occlusion query: the first hit ends the ray
               active = false;
               in_leaf = false;
@@ -690,18 +689,50 @@ __device__ __forceinline__ void packet_body(const WArgs& A) {
   }
 }
 
-// Four waves per SIMD: 128 registers per lane and about 50 bytes of spill,
-// measured 12% faster than three waves unspilled (five waves spill 160 bytes
-// and fall back).  The learned kernel stays at three: at four it spills 330
-// bytes and runs a third slower.
+// Five waves per SIMD: the kernel needs 111 registers per lane (four waves);
+// held to 96 it spills 64 bytes and runs 5% faster.  The learned kernel stays
+// at three waves: at four it spills 330 bytes and runs a third slower.
 template <bool PERSISTENT, bool COUNT>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 4))) k_trace_packet(const WArgs A) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 5))) k_trace_packet(const WArgs A) {
   packet_body<PERSISTENT, COUNT, false>(A);
 }
 
 template <bool PERSISTENT, bool COUNT>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) k_trace_packet_learned(const WArgs A) {
   packet_body<PERSISTENT, COUNT, true>(A);
+}
+
+// Mixed precision, after the trace: every ray whose closest hit was
+// ill-conditioned (cond2(J) > cond_thr, marked prim = -2 - patch by the trace)
+// gets one FP64 Newton step from its FP32 (u,v) on FP64 ray planes; t, u, v,
+// dpdu, dpdv are replaced and prim restored.  A pass over prim, so the trace
+// kernels carry no FP64 code.
+__global__ void __launch_bounds__(256) k_fix64(const WArgs A) {
+  unsigned long long done = 0;
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  for (int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x; r < A.n_rays; r += stride) {
+    const int code = A.prim[r];
+    if (code > -2) continue;
+    const int pi = -2 - code;
+    const int pqv = A.pq[pi];
+    const int p = pqv & 0xff, q = (pqv >> 8) & 0xff;
+    const f3 o{A.ox[r], A.oy[r], A.oz[r]}, d{A.dx[r], A.dy[r], A.dz[r]};
+    float u = A.u[r], v = A.v[r], t;
+    f3 S, Su, Sv;
+    if (correct64_rt(A.cp + A.cp_off[pi], p, q, o, d, u, v, S, Su, Sv, t)) {
+      done++;
+      A.t[r] = t;
+      A.u[r] = u;
+      A.v[r] = v;
+      if (A.dpdu) { A.dpdu[3 * r] = Su.x; A.dpdu[3 * r + 1] = Su.y; A.dpdu[3 * r + 2] = Su.z; }
+      if (A.dpdv) { A.dpdv[3 * r] = Sv.x; A.dpdv[3 * r + 1] = Sv.y; A.dpdv[3 * r + 2] = Sv.z; }
+    }
+    A.prim[r] = pi;
+  }
+  if (A.totals) {
+    for (int sft = 32; sft > 0; sft >>= 1) done += __shfl_xor(done, sft);
+    if ((threadIdx.x & 63) == 0 && done) atomicAdd(A.totals + 8, done);
+  }
 }
 
 }  // namespace
@@ -783,7 +814,7 @@ int launch_wide_kernel(const LaunchShape& sh, bool persistent, bool count, bool 
 #undef RTN_SHAPE
   }
   RTN_HIP(hipGetLastError());
-  return RTN_OK;
+  return launch_fix64(n_max, stream, args);
 }
 
 int fill_wide_args(const rtn_scene* s, const float* const rays[6], int64_t n, const rtn_trace_params* prm,
@@ -797,6 +828,8 @@ int fill_wide_args(const rtn_scene* s, const float* const rays[6], int64_t n, co
             "rtn_trace: fallback_iters outside 0..64");
   RTN_CHECK(prm->seed_share == 0 || prm->seed_share == 1, RTN_EINVAL, "rtn_trace: seed_share is 0 or 1");
   RTN_CHECK(prm->stall_exit == 0 || prm->stall_exit == 1, RTN_EINVAL, "rtn_trace: stall_exit is 0 or 1");
+  RTN_CHECK(prm->precision != RTN_PREC_MIXED || (hits->t && hits->u && hits->v), RTN_EINVAL,
+            "rtn_trace: mixed precision needs the t, u and v outputs");
   RTN_CHECK(prm->flat_threshold <= 0.f, RTN_EUNSUPPORTED, "rtn_trace: flat-skip runs in the exact form only");
   A = WArgs{};
   A.nodes = s->wide.ptr;
@@ -859,6 +892,18 @@ int trace_wide_scene(const rtn_scene* s, const float* const rays[6], int64_t n, 
   return rc;
 }
 
+int launch_fix64(int64_t n_max, hipStream_t stream, const WArgs& args) {
+  if (!args.mixed || n_max <= 0) return RTN_OK;
+  int dev = 0, cus = 256;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  const int64_t need = (n_max + 255) / 256;
+  const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)cus * 8));
+  hipLaunchKernelGGL(k_fix64, dim3(blocks), dim3(256), 0, stream, args);
+  RTN_HIP(hipGetLastError());
+  return RTN_OK;
+}
+
 int launch_packet_kernel(bool persistent, bool count, bool learned, int64_t n_max, hipStream_t stream,
                          const WArgs& args) {
   RTN_CHECK(!persistent || n_max <= (int64_t)0xFFFFFFFFu - ((int64_t)1 << 24), RTN_EINVAL,
@@ -874,7 +919,7 @@ int launch_packet_kernel(bool persistent, bool count, bool learned, int64_t n_ma
   }
   launch_one<64>(kernel, persistent, kPacketBlocksPerCU, n_max, stream, args);
   RTN_HIP(hipGetLastError());
-  return RTN_OK;
+  return launch_fix64(n_max, stream, args);
 }
 
 int trace_packet_scene(const rtn_scene* s, const float* const rays[6], int64_t n, const rtn_trace_params* prm,

@@ -82,5 +82,9 @@ int launch_wide_kernel(const LaunchShape& sh, bool persistent, bool count, bool 
 // chunks from args.work); args.bounds must be set.
 int launch_packet_kernel(bool persistent, bool count, bool learned, int64_t n_max, hipStream_t stream,
                          const WArgs& args);
+// Both launchers end with the mixed-precision correction pass (k_fix64) when
+// args.mixed is set: the FP64 step of every ray whose closest hit is
+// ill-conditioned.
+int launch_fix64(int64_t n_max, hipStream_t stream, const WArgs& args);
 
 }  // namespace rtn
