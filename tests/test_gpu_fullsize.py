@@ -1,0 +1,69 @@
+"""The 10k-patch 1080p scene at its full size (10,018 patches, 1920x1080 x 4
+jittered spp = 8,294,400 rays), through size-independent properties:
+
+- the patch count is the frozen closed form;
+- the packet form equals the one-ray wide form but for equal-t ties, and the
+  persistent packet form equals the packet form bit for bit;
+- exact rejection (oriented box + projected hull) changes nothing;
+- against the FP64 reference, every FP32 form keeps hit t within 1e-5 on at
+  least 99.8% of the same-patch pixels and has at most 0.3% non-tie patch
+  mismatches (the plain method's own level)."""
+import numpy as np
+import pytest
+
+from rtnurbs import scenes
+from test_gpu_wide import same_but_ties
+
+pytestmark = pytest.mark.gpu
+
+CFG = dict(max_iter=8, thr_res=1e-5, thr_det=1e-5)
+
+
+@pytest.fixture(scope="module")
+def c2(rtl):
+    print("case 10k-patch 1080p x 4 spp", flush=True)
+    sc = scenes.config2(0)
+    scene = rtl.Scene(sc.surfaces).build()
+    rays = rtl.generate_rays(sc.camera, spp=4, jitter=True, seed=1, tiled=True)
+    ref = rtl.trace_f64(scene, rays, max_iter=32, thr_det=1e-12, thr_res=1e-10).get()
+    return scene, rays, ref
+
+
+def _fp64_check(g, ref, what):
+    h32, h64 = g["hit"], ref["hit"]
+    tie = (h64 >= 0) & np.isfinite(ref["t2"]) & ((ref["t2"] - ref["t"]) <= 1e-5 * np.abs(ref["t"]))
+    both = (h32 >= 0) & (h64 >= 0)
+    mism = (both & (h32 != h64) & ~tie).sum()
+    assert mism <= 0.003 * len(h32), (what, mism)
+    same = both & (h32 == h64)
+    rel = np.abs(g["t"][same].astype(np.float64) - ref["t"][same]) / np.abs(ref["t"][same])
+    assert np.mean(rel <= 1e-5) >= 0.998, (what, np.mean(rel <= 1e-5))
+
+
+def test_fullsize_forms(rtl, c2):
+    scene, rays, ref = c2
+    assert rays.n == 8_294_400 and scene.stats()["n_patches"] == scenes.C2_PATCHES
+    out = {}
+    for name, kw in (("wide", dict(mode=rtl.MODE_WIDE)),
+                     ("wide_hull2", dict(mode=rtl.MODE_WIDE, hull_cull=2)),
+                     ("packet_hull2_mixed_stall", dict(mode=rtl.MODE_PACKET, hull_cull=2, stall_exit=1,
+                                                       precision=rtl.PREC_MIXED, cond_thr=1e4)),
+                     ("wide_hull2_mixed_stall", dict(mode=rtl.MODE_WIDE, hull_cull=2, stall_exit=1,
+                                                     precision=rtl.PREC_MIXED, cond_thr=1e4)),
+                     ("ppacket_hull2_mixed_stall", dict(mode=rtl.MODE_PERSISTENT_PACKET, hull_cull=2, stall_exit=1,
+                                                        precision=rtl.PREC_MIXED, cond_thr=1e4))):
+        print(f"form {name}", flush=True)
+        h, tot = rtl.trace(rays=rays, scene=scene, prm=rtl.params(count=1, **CFG, **kw))
+        assert tot["overflow"] == 0, name
+        out[name] = h.get()
+        _fp64_check(out[name], ref, name)
+    a, b = out["wide_hull2"], out["wide"]
+    assert np.array_equal(a["hit"], b["hit"])
+    m = b["hit"] >= 0
+    assert np.array_equal(a["t"][m].view(np.uint32), b["t"][m].view(np.uint32))
+    same_but_ties(out["packet_hull2_mixed_stall"], out["wide_hull2_mixed_stall"])
+    a, b = out["ppacket_hull2_mixed_stall"], out["packet_hull2_mixed_stall"]
+    assert np.array_equal(a["hit"], b["hit"])
+    m = b["hit"] >= 0
+    for k in ("t", "u", "v"):
+        assert np.array_equal(a[k][m].view(np.uint32), b[k][m].view(np.uint32)), k
