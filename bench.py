@@ -418,7 +418,8 @@ def bench_c5(args, L, scenes):
     proxy, _, _ = frame_for(4)
     table = proxy.autotune(reps=2)
     best = min((e for e in table if e["ms"] > 0), key=lambda e: e["ms"])
-    default = [e for e in table if (e["lanes"], e["stack"], e["blocks_per_cu"], e["refill_min"]) == (128, 20, 8, 64)]
+    default = [e for e in table if (e["lanes"], e["stack"], e["blocks_per_cu"], e["refill_min"], e["packet_primary"],
+                                    e["packet_secondary"]) == (128, 20, 8, 64, 0, 0)]
     log(f"autotune (1/16 of the pixels): best {best}, default {default}")
     del proxy
     frame, rays, ids = frame_for(1)
@@ -446,6 +447,7 @@ def bench_c5(args, L, scenes):
         total_rays = st["n_primary"] + st["n_secondary"]
         log(f"c5 frame (packet primary {pp}, secondary {ps}): {st['n_primary']} primary + {st['n_secondary']} "
             f"secondary rays, {variants[(pp, ps)]:.2f} ms, {total_rays / variants[(pp, ps)] / 1e3:.1f} Mrays/s")
+    log(f"autotuner's choice on the proxy: packet primary {best['packet_primary']}, secondary {best['packet_secondary']}")
     (pp, ps), ms = min(variants.items(), key=lambda kv: kv[1])
     names = {0: "persistent per-lane", 1: "packet"}
     wl = {"workload": f"{len(built)} scenes, {n_patches} patches (1k-200k each, degree 2-5), 256^2-2048^2 x 4 spp "

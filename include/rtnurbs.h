@@ -405,9 +405,10 @@ typedef struct {
   int64_t n_primary, n_secondary;  /* rays; n_secondary counts non-empty slots   */
   int64_t n_secondary_slots;       /* (shadow + mirror) x n_primary              */
   unsigned long long overflow;  /* rays whose traversal stack overflowed        */
-  int shape_lanes, shape_stack, shape_blocks_per_cu, captured, refill_min, reserved;
+  int shape_lanes, shape_stack, shape_blocks_per_cu, captured, refill_min;
+  int packet;                      /* 1: packet primary launch, 2: packet secondary launch */
 } rtn_frame_status_t;
-typedef struct { int lanes, stack, blocks_per_cu, refill_min; float ms; } rtn_tune_entry;
+typedef struct { int lanes, stack, blocks_per_cu, refill_min; float ms; int packet_primary, packet_secondary; } rtn_tune_entry;
 
 void rtn_frame_params_default(rtn_frame_params* fp);
 int rtn_frame_create(rtn_frame** frame, const rtn_batch* batch, const float* const rays_soa[6], const int* ray_scene,
@@ -431,9 +432,10 @@ int rtn_frame_set_packet(rtn_frame* frame, int packet_primary, int packet_second
 /* Autotuner: times the frame over every built launch shape at 2, 4, 8 and 12
  * blocks per CU (capped at the blocks that stay resident; whole-wave refill;
  * reps timed runs each; stack overflow rejects a shape), then the best shape
- * at refill thresholds 1, 16 and 32 idle lanes, and keeps the fastest.  table
- * (optional) receives each candidate, ms < 0 for rejected ones.  Drops a
- * captured graph. */
+ * at refill thresholds 1, 16 and 32 idle lanes, then the packet form for the
+ * primary launch and for both launches, and keeps the fastest (shape, refill
+ * threshold and packet flags).  table (optional) receives each candidate,
+ * ms < 0 for rejected ones.  Drops a captured graph. */
 int rtn_frame_autotune(rtn_frame* frame, int reps, rtn_tune_entry* table, int table_cap, int* n_tried,
                        void* stream);
 

@@ -402,6 +402,7 @@ extern "C" int rtn_frame_status(const rtn_frame* f, rtn_frame_status_t* out) {
   out->shape_blocks_per_cu = f->shape.blocks_per_cu;
   out->captured = f->exec != nullptr;
   out->refill_min = f->refill;
+  out->packet = (f->fp.packet_primary ? 1 : 0) | (f->fp.packet_secondary ? 2 : 0);
   return RTN_OK;
 }
 
@@ -444,7 +445,8 @@ extern "C" int rtn_frame_set_packet(rtn_frame* f, int packet_primary, int packet
 
 // Autotuner: every built launch shape (lanes per block x stack entries) at
 // 2, 4, 8 and 12 resident blocks per compute unit with whole-wave refill, then
-// the best shape at refill thresholds 1, 16 and 32; each candidate runs the
+// the best shape at refill thresholds 1, 16 and 32, then the packet form for
+// the primary launch and for both launches; each candidate runs the
 // frame once to warm up and `reps` times under HIP events; candidates whose
 // stack overflows are rejected.  The fastest is kept (and any graph dropped).
 extern "C" int rtn_frame_autotune(rtn_frame* f, int reps, rtn_tune_entry* table, int table_cap, int* n_tried,
@@ -452,25 +454,27 @@ extern "C" int rtn_frame_autotune(rtn_frame* f, int reps, rtn_tune_entry* table,
   RTN_CHECK(f && reps >= 1, RTN_EINVAL, "rtn_frame_autotune: bad arguments");
   hipStream_t st = (hipStream_t)stream;
   drop_graph(f);
-  struct Cand { LaunchShape sh; int refill; };
+  struct Cand { LaunchShape sh; int refill; int pp, ps; };
   std::vector<Cand> cand;
   const int bpcs[4] = {2, 4, 8, 12};  // capped at the resident blocks when launched
   const bool learned = frame_learned(f);
   for (int b = 0; b < 4; b++) {
-    cand.push_back({{kDefaultShape.wb, kDefaultShape.wstk, bpcs[b]}, 64});
+    cand.push_back({{kDefaultShape.wb, kDefaultShape.wstk, bpcs[b]}, 64, 0, 0});
     if (!learned)
       for (int i = 0; i < kNumTunedShapes; i++)
-        cand.push_back({{kTunedShapes[i].wb, kTunedShapes[i].wstk, bpcs[b]}, 64});
+        cand.push_back({{kTunedShapes[i].wb, kTunedShapes[i].wstk, bpcs[b]}, 64, 0, 0});
   }
   hipEvent_t e0, e1;
   RTN_HIP(hipEventCreate(&e0));
   RTN_HIP(hipEventCreate(&e1));
-  Cand best{f->shape, f->refill};
+  Cand best{f->shape, f->refill, f->fp.packet_primary, f->fp.packet_secondary};
   float best_ms = INFINITY;
   int tried = 0;
   int rc = RTN_OK;
   auto run_one = [&](const Cand& c) -> float {
     f->shape = c.sh;
+    f->fp.packet_primary = c.pp;
+    f->fp.packet_secondary = c.ps;
     if ((rc = frame_set_refill(f, c.refill))) return -2.f;
     if ((rc = frame_enqueue(f, st))) return -2.f;
     FrameCtl h;
@@ -498,6 +502,8 @@ extern "C" int rtn_frame_autotune(rtn_frame* f, int reps, rtn_tune_entry* table,
       table[tried].blocks_per_cu = c.sh.blocks_per_cu;
       table[tried].refill_min = c.refill;
       table[tried].ms = ms;
+      table[tried].packet_primary = c.pp;
+      table[tried].packet_secondary = c.ps;
     }
     tried++;
     if (ms > 0.f && ms < best_ms) {
@@ -513,7 +519,17 @@ extern "C" int rtn_frame_autotune(rtn_frame* f, int reps, rtn_tune_entry* table,
   if (rc == RTN_OK && best_ms < INFINITY) {
     const LaunchShape sh = best.sh;
     for (int r : {1, 16, 32}) {
-      const Cand c{sh, r};
+      const Cand c{sh, r, 0, 0};
+      const float ms = run_one(c);
+      if (ms < -1.5f) break;
+      record(c, ms);
+    }
+  }
+  // then the packet form for the primary launch, and for both launches
+  if (rc == RTN_OK && best_ms < INFINITY) {
+    const Cand base = best;
+    for (int ps = 0; ps <= 1; ps++) {
+      const Cand c{base.sh, base.refill, 1, ps};
       const float ms = run_one(c);
       if (ms < -1.5f) break;
       record(c, ms);
@@ -522,6 +538,8 @@ extern "C" int rtn_frame_autotune(rtn_frame* f, int reps, rtn_tune_entry* table,
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   f->shape = best.sh;
+  f->fp.packet_primary = best.pp;
+  f->fp.packet_secondary = best.ps;
   const int rc2 = frame_set_refill(f, best.refill);
   if (n_tried) *n_tried = tried;
   RTN_TRY(rc);

@@ -86,12 +86,12 @@ class FrameStatus(C.Structure):
     _fields_ = [("n_primary", C.c_int64), ("n_secondary", C.c_int64), ("n_secondary_slots", C.c_int64),
                 ("overflow", C.c_ulonglong),
                 ("shape_lanes", C.c_int), ("shape_stack", C.c_int), ("shape_blocks_per_cu", C.c_int),
-                ("captured", C.c_int), ("refill_min", C.c_int), ("reserved", C.c_int)]
+                ("captured", C.c_int), ("refill_min", C.c_int), ("packet", C.c_int)]
 
 
 class TuneEntry(C.Structure):
     _fields_ = [("lanes", C.c_int), ("stack", C.c_int), ("blocks_per_cu", C.c_int), ("refill_min", C.c_int),
-                ("ms", C.c_float)]
+                ("ms", C.c_float), ("packet_primary", C.c_int), ("packet_secondary", C.c_int)]
 
 
 class Hits(C.Structure):
@@ -809,8 +809,8 @@ class Frame:
         table = (TuneEntry * 64)()
         n = C.c_int()
         check(load().rtn_frame_autotune(self.h, reps, table, 64, C.byref(n), stream))
-        return [dict(lanes=e.lanes, stack=e.stack, blocks_per_cu=e.blocks_per_cu, refill_min=e.refill_min, ms=e.ms)
-                for e in table[:n.value]]
+        return [dict(lanes=e.lanes, stack=e.stack, blocks_per_cu=e.blocks_per_cu, refill_min=e.refill_min, ms=e.ms,
+                     packet_primary=e.packet_primary, packet_secondary=e.packet_secondary) for e in table[:n.value]]
 
     def results(self):
         """Host copies: primary hits, secondary hits (first n_secondary), parent, kind."""
