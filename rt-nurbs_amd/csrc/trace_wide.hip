@@ -34,7 +34,7 @@ using namespace rtn::fast;
 namespace {
 
 struct Cnt {  // per-ray work counters (counting builds only)
-  uint32_t nc, ne, ncull, nbox, nleaf, npt, ovf, nfl, n64, nfb;  // n64 stays 0: the FP64 steps run in k_fix64
+  uint32_t nc, ne, ncull, nbox, nleaf, npt, ovf, nfl, nfb;  // (FP64 steps are counted by k_fix64)
 };
 
 struct Lane {
@@ -279,7 +279,7 @@ __device__ __forceinline__ void trace_wide_body(const WArgs& A) {
         lane_finish<COUNT>(L, A, cn);
         if constexpr (COUNT) {
           acc[0] += cn.nc; acc[1] += cn.ne; acc[3] += cn.nbox; acc[4] += cn.nleaf; acc[5] += cn.npt;
-          acc[6] += cn.ovf; acc[7] += cn.nfl; acc[8] += cn.n64; acc[9] += cn.ncull; acc[10] += cn.nfb;
+          acc[6] += cn.ovf; acc[7] += cn.nfl; acc[9] += cn.ncull; acc[10] += cn.nfb;
         } else if (cn.ovf) {
           acc[0] += 1;
           cn.ovf = 0;
@@ -651,7 +651,7 @@ __device__ __forceinline__ void packet_chunk(const WArgs& A, int64_t r0, unsigne
     lane_finish<COUNT>(L, A, cn);
     if constexpr (COUNT) {
       acc[0] += cn.nc; acc[1] += cn.ne; acc[3] += cn.nbox; acc[4] += cn.nleaf; acc[5] += cn.npt;
-      acc[6] += cn.ovf; acc[7] += cn.nfl; acc[8] += cn.n64; acc[9] += cn.ncull; acc[10] += cn.nfb;
+      acc[6] += cn.ovf; acc[7] += cn.nfl; acc[9] += cn.ncull; acc[10] += cn.nfb;
     } else if (cn.ovf) {
       acc[0] += 1;
     }

@@ -98,6 +98,10 @@ def test_secondaries_any_hit_and_shading(rtl):
     c, tc = rtl.trace(scene, shadow, rtl.params(mode=rtl.MODE_WIDE, hull_cull=2, count=1, **CFG))
     assert np.array_equal(a.get()["hit"] >= 0, c.get()["hit"] >= 0)
     assert ta["newton_calls"] <= tc["newton_calls"]
+    # the same in the packet form (a lane leaves the packet at its first hit)
+    ap, tap = rtl.trace(scene, shadow, rtl.params(mode=rtl.MODE_PACKET, hull_cull=2, any_hit=1, count=1, **CFG))
+    assert np.array_equal(ap.get()["hit"] >= 0, c.get()["hit"] >= 0)
+    assert tap["newton_calls"] <= tc["newton_calls"] * 1.05
     # Lambert shading (no shadows) against numpy
     img = rtl.shade_lambert(rays, h, 128, 128, 1, light, albedo=0.8).get().ravel()
     hit = g["hit"] >= 0
