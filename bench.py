@@ -52,7 +52,8 @@ def ray_flops(tot, n_rays, exact):
     (corners and bilinear seed), plus each Newton evaluation priced at its
     degree by the reference algorithm's operation count (counted by the
     kernel).  Work a form avoids (rejected patches, fewer iterations) is not
-    priced."""
+    priced, and neither is work a form adds beyond the per-ray algorithm (the
+    packet forms are priced by the one-ray form's counts)."""
     boxes = 66.0 * tot["interior"] + 32.0 * tot["leaves"] if exact else 33.0 * tot["interior"]
     return 40.0 * n_rays + boxes + 180.0 * tot["patch_tests"] + float(tot["newton_flops"])
 
@@ -219,8 +220,15 @@ def run_forms(L, scene, rays, forms, args, ref, stream):
         g = hits_c.get()
         each = {}
         hits, kern_ms, wall_ms = time_form(L, scene, rays, prm, args.steps, args.warmup, stream, each)
-        flops = ray_flops(tot, n, kw["mode"] == L.MODE_EXACT)
+        alg = tot
+        if kw["mode"] in (L.MODE_PACKET, L.MODE_PERSISTENT_PACKET):
+            # a packet lane also tests the child boxes of nodes the wave visits for
+            # other lanes: the algorithmic work of a ray is what the one-ray form
+            # counts for it, so the packet forms are priced by that form's counts
+            _, alg = L.trace(scene, rays, L.params(count=1, **{**kw, "mode": L.MODE_WIDE}))
+        flops = ray_flops(alg, n, kw["mode"] == L.MODE_EXACT)
         r = {"mrays_s": n / (kern_ms * 1e-3) / 1e6, "kernel_ms": kern_ms, "wall_ms": wall_ms,
+             "box_tests_per_ray": tot["interior"] / n, "box_tests_per_ray_priced": alg["interior"] / n,
              "kernel_ms_median": each.get("median"), "kernel_ms_min": each.get("min"),
              "newton_iters_per_ray": tot["newton_evals"] / n, "newton_calls_per_ray": tot["newton_calls"] / n,
              "patch_tests_per_ray": tot["patch_tests"] / n, "flops_per_ray": flops / n,
