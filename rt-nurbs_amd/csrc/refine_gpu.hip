@@ -93,9 +93,9 @@ __host__ __device__ int knots_to_add_dev(const float4* __restrict__ net, int str
 __global__ void k_refine_count(const RefSurf* __restrict__ S, const float4* __restrict__ nets,
                                const float* __restrict__ knots, float C, int* __restrict__ add) {
   const RefSurf s = S[blockIdx.x];
-  const int spans = s.nn - s.p + 1, cols = s.other + 1;
-  for (int w = threadIdx.x; w < spans * cols; w += blockDim.x) {
-    const int i = s.p + w / cols, c = w % cols;
+  const int64_t spans = s.nn - s.p + 1, cols = s.other + 1;
+  for (int64_t w = threadIdx.x; w < spans * cols; w += blockDim.x) {
+    const int i = s.p + (int)(w / cols), c = (int)(w % cols);
     const int k = knots_to_add_dev(nets + s.net + (int64_t)c * s.stride_c, s.stride_a, knots + s.knots, i, C, s.p);
     if (k > 0) atomicMax(add + s.span + (i - s.p), k);
   }
