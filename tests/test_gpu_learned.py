@@ -107,3 +107,24 @@ def test_learned_packet_forms(rtl, net):
         assert np.array_equal(a[k][m].view(np.uint32), b[k][m].view(np.uint32)), k
     # sharing and the stall exit keep the hits of the unshared form
     assert np.mean(b["hit"] == out["packet0"]["hit"]) > 0.99
+
+
+def test_learned_packet_over_a_batch(rtl, net):
+    """The learned seed in the packet form over a multi-scene batch (two
+    scenes, rays of both mixed within every wave): without seed sharing it
+    equals the wide form over the same batch but for equal-t ties."""
+    from test_gpu_frame import _batch_rays
+    from test_gpu_wide import same_but_ties
+    items = [(rtl.Scene(scenes.config1(k).surfaces).build(), scenes.c1_camera(64, 64)) for k in (2002, 2003)]
+    rays, ids, _ = _batch_rays(rtl, items, spp=2)
+    perm = np.random.default_rng(1).permutation(rays.n)
+    rp = rtl.Rays.from_host([a[perm] for a in rays.get()])
+    ip = rtl.DeviceArray.from_host(ids.get()[perm])
+    batch = rtl.Batch([sc for sc, _ in items])
+    lrn = dict(hull_cull=2, seed_mode=rtl.SEED_LEARNED, seed_net=net, learned_iters=3, fallback_iters=8, seed_share=0,
+               precision=rtl.PREC_MIXED, **CFG)
+    print("case learned batch", flush=True)
+    w, _ = batch.trace(rp, ip, rtl.params(mode=rtl.MODE_WIDE, **lrn))
+    p, tp = batch.trace(rp, ip, rtl.params(mode=rtl.MODE_PACKET, count=1, **lrn))
+    assert tp["overflow"] == 0 and tp["learned_fallbacks"] > 0
+    same_but_ties(p.get(), w.get())
