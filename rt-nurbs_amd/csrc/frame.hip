@@ -72,7 +72,7 @@ int copy_into(DevBuf<T>& dst, size_t at, const DevBuf<T>& src, size_t n) {
 
 // Device control block of a frame.
 struct FrameCtl {
-  unsigned int work1, work2;      // persistent ray counters
+  unsigned int work1[kWorkWords], work2[kWorkWords];  // persistent work queues
   unsigned long long sec_count;   // secondaries written
   unsigned long long totals[12];  // trace totals (overflow at [6])
 };
@@ -80,9 +80,11 @@ struct FrameCtl {
 __global__ void k_frame_reset(FrameCtl* c) {
   const int t = threadIdx.x;
   if (t < 12) c->totals[t] = 0;
-  if (t == 12) c->work1 = 0;
-  if (t == 13) c->work2 = 0;
-  if (t == 14) c->sec_count = 0;
+  if (t == 12) c->sec_count = 0;
+  for (int i = t; i < kWorkWords; i += blockDim.x) {
+    c->work1[i] = 0;
+    c->work2[i] = 0;
+  }
 }
 
 __global__ void k_check_scene_ids(const int* __restrict__ ids, int64_t n, int k, int* __restrict__ bad) {
@@ -192,14 +194,14 @@ extern "C" int rtn_batch_trace(const rtn_batch* b, const float* const rays[6], c
   if (n == 0) return RTN_OK;
   hipStream_t st = (hipStream_t)stream;
   RTN_TRY(check_scene_ids(b, ray_scene, n, st));
-  struct Block { unsigned int work; unsigned int pad[3]; unsigned long long tot[12]; };
+  struct Block { unsigned int work[kWorkWords]; unsigned long long tot[12]; };
   Block* d = nullptr;
   RTN_HIP(hipMallocAsync((void**)&d, sizeof(Block), st));
   Block h{};
   WArgs a;
   int rc = batch_args(b, rays, ray_scene, n, prm, hits, &d->tot[0], a);
   if (rc == RTN_OK) {
-    a.work = &d->work;
+    a.work = d->work;
     RTN_HIP(hipMemcpyAsync(d, &h, sizeof(Block), hipMemcpyHostToDevice, st));
     const bool packet = prm->mode == RTN_MODE_PACKET || prm->mode == RTN_MODE_PERSISTENT_PACKET;
     rc = packet ? launch_packet_kernel(prm->mode == RTN_MODE_PERSISTENT_PACKET, prm->count != 0,
@@ -345,7 +347,7 @@ extern "C" int rtn_frame_create(rtn_frame** out, const rtn_batch* b, const float
   FrameCtl h{};
   const rtn_hits h1 = f->hits1();
   if ((rc = batch_args(b, f->rays, ray_scene, n, &f->fp.trace, &h1, f->ctl.ptr->totals, f->a1))) return fail(rc);
-  f->a1.work = &f->ctl.ptr->work1;
+  f->a1.work = f->ctl.ptr->work1;
   if (f->cap2 > 0) {
     const rtn_hits h2 = f->hits2();
     const float* srays[6];
@@ -353,7 +355,7 @@ extern "C" int rtn_frame_create(rtn_frame** out, const rtn_batch* b, const float
     if ((rc = batch_args(b, srays, f->s_scene.ptr, f->cap2, &f->fp.trace, &h2, f->ctl.ptr->totals, f->a2)))
       return fail(rc);
     f->a2.ray_kind = f->s_kind.ptr;
-    f->a2.work = &f->ctl.ptr->work2;
+    f->a2.work = f->ctl.ptr->work2;
   }
   if (hipMemcpy(f->ctl.ptr, &h, sizeof(h), hipMemcpyHostToDevice) != hipSuccess) return fail(RTN_EHIP);
   *out = f;

@@ -7,9 +7,11 @@
 //                    primary rays only)
 //   Lambert shading  one directional light, spp averaged, replacing the
 //                    reference's debug shaders (testShading.cu:9-55)
+#include <algorithm>
 #include <cmath>
 
 #include "rtn_exact.hpp"
+
 #include "rtn_internal.hpp"
 
 using namespace rtn;
@@ -83,54 +85,58 @@ __device__ __forceinline__ ex::v3<float> facing_normal(const float* dpdu, const 
 // first): shadow ray towards the light and mirror ray, origins offset along
 // the facing normal by eps * (1 + max|S|).  A miss leaves its slots empty
 // (kind -1), so the layout, and the Newton seeds that follow the ray index,
-// are the same on every run.  *count += valid rays (one atomic per wave).
+// are the same on every run.  *count += valid rays.  Grid-stride loop over a
+// grid sized to the device: each lane sums its count and a wave adds it with
+// one atomic at the end (an atomic per wave of rays serialised on the one
+// counter and cost several times the kernel's memory traffic).
 __global__ void k_secondaries(const float* ox, const float* oy, const float* oz, const float* dx, const float* dy,
                               const float* dz, const int* prim, const float* t, const float* dpdu,
                               const float* dpdv, int64_t n, float lx, float ly, float lz, float eps, int want_shadow,
                               int want_mirror, float* sox, float* soy, float* soz, float* sdx, float* sdy, float* sdz,
                               int* parent, int* kind, unsigned long long* count, const int* ray_scene,
                               int* out_scene) {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool hit = r < n && prim[r] >= 0;
   const int per = want_shadow + want_mirror;
-  const unsigned long long bal = __ballot(hit);
-  const int lane = threadIdx.x & 63;
-  if (bal && lane == __ffsll((long long)bal) - 1) atomicAdd(count, (unsigned long long)(__popcll(bal) * per));
-  if (r >= n) return;
-  int64_t slot = (int64_t)per * r;
-  if (!hit) {
-    for (int k = 0; k < per; k++, slot++) {
-      sox[slot] = soy[slot] = soz[slot] = 0.0f;
-      sdx[slot] = sdy[slot] = sdz[slot] = 0.0f;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  unsigned long long valid = 0;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride) {
+    int64_t slot = (int64_t)per * r;
+    if (prim[r] < 0) {
+      for (int k = 0; k < per; k++, slot++) {
+        sox[slot] = soy[slot] = soz[slot] = 0.0f;
+        sdx[slot] = sdy[slot] = sdz[slot] = 0.0f;
+        parent[slot] = (int)r;
+        kind[slot] = -1;
+        if (out_scene) out_scene[slot] = ray_scene[r];
+      }
+      continue;
+    }
+    valid += per;
+    const ex::v3<float> d{dx[r], dy[r], dz[r]};
+    const float th = t[r];
+    const ex::v3<float> S{ox[r] + th * d.x, oy[r] + th * d.y, oz[r] + th * d.z};
+    const ex::v3<float> N = facing_normal(dpdu, dpdv, r, d);
+    const float off = eps * (1.0f + fmaxf(fabsf(S.x), fmaxf(fabsf(S.y), fabsf(S.z))));
+    const ex::v3<float> O{S.x + off * N.x, S.y + off * N.y, S.z + off * N.z};
+    if (want_shadow) {
+      sox[slot] = O.x; soy[slot] = O.y; soz[slot] = O.z;
+      sdx[slot] = lx; sdy[slot] = ly; sdz[slot] = lz;
       parent[slot] = (int)r;
-      kind[slot] = -1;
+      kind[slot] = 0;
+      if (out_scene) out_scene[slot] = ray_scene[r];
+      slot++;
+    }
+    if (want_mirror) {
+      const float dn = 2.0f * (d.x * N.x + d.y * N.y + d.z * N.z);
+      const ex::v3<float> m = unit(ex::v3<float>{d.x - dn * N.x, d.y - dn * N.y, d.z - dn * N.z});
+      sox[slot] = O.x; soy[slot] = O.y; soz[slot] = O.z;
+      sdx[slot] = m.x; sdy[slot] = m.y; sdz[slot] = m.z;
+      parent[slot] = (int)r;
+      kind[slot] = 1;
       if (out_scene) out_scene[slot] = ray_scene[r];
     }
-    return;
   }
-  const ex::v3<float> d{dx[r], dy[r], dz[r]};
-  const float th = t[r];
-  const ex::v3<float> S{ox[r] + th * d.x, oy[r] + th * d.y, oz[r] + th * d.z};
-  const ex::v3<float> N = facing_normal(dpdu, dpdv, r, d);
-  const float off = eps * (1.0f + fmaxf(fabsf(S.x), fmaxf(fabsf(S.y), fabsf(S.z))));
-  const ex::v3<float> O{S.x + off * N.x, S.y + off * N.y, S.z + off * N.z};
-  if (want_shadow) {
-    sox[slot] = O.x; soy[slot] = O.y; soz[slot] = O.z;
-    sdx[slot] = lx; sdy[slot] = ly; sdz[slot] = lz;
-    parent[slot] = (int)r;
-    kind[slot] = 0;
-    if (out_scene) out_scene[slot] = ray_scene[r];
-    slot++;
-  }
-  if (want_mirror) {
-    const float dn = 2.0f * (d.x * N.x + d.y * N.y + d.z * N.z);
-    const ex::v3<float> m = unit(ex::v3<float>{d.x - dn * N.x, d.y - dn * N.y, d.z - dn * N.z});
-    sox[slot] = O.x; soy[slot] = O.y; soz[slot] = O.z;
-    sdx[slot] = m.x; sdy[slot] = m.y; sdz[slot] = m.z;
-    parent[slot] = (int)r;
-    kind[slot] = 1;
-    if (out_scene) out_scene[slot] = ray_scene[r];
-  }
+  for (int sft = 32; sft > 0; sft >>= 1) valid += __shfl_xor(valid, sft);
+  if ((threadIdx.x & 63) == 0 && valid) atomicAdd(count, valid);
 }
 
 // Lambert shading of W*H pixels with spp consecutive rays each:
@@ -166,7 +172,11 @@ void launch_secondaries(const float* const rays[6], const rtn_hits* hits, int64_
                         int want_shadow, int want_mirror, float* const out[6], int* parent, int* kind,
                         unsigned long long* count, const int* ray_scene, int* out_scene, hipStream_t st) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_secondaries, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, rays[0], rays[1], rays[2],
+  int dev = 0, cus = 256;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, (int64_t)cus * 16));
+  hipLaunchKernelGGL(k_secondaries, dim3((unsigned)blocks), dim3(256), 0, st, rays[0], rays[1], rays[2],
                      rays[3], rays[4], rays[5], hits->prim, hits->t, hits->dpdu, hits->dpdv, n, ldir[0], ldir[1],
                      ldir[2], eps, want_shadow, want_mirror, out[0], out[1], out[2], out[3], out[4], out[5], parent,
                      kind, count, ray_scene, out_scene);

@@ -227,6 +227,7 @@ __device__ __forceinline__ void trace_wide_body(const WArgs& A) {
   int cur = -1;           // wide node to expand, or -1
   int cand_first = 0, cand_left = 0;  // primitives of the current leaf still to test
   bool exhausted;         // no more rays to hand out
+  int queue = blockIdx.x % kQueues, queues_left = kQueues;  // persistent: the wave's work queue
   // work totals of the lane: all eleven when counting, otherwise only the
   // stack-overflow count (total 6), so no idle accumulators occupy registers
   constexpr int NACC = COUNT ? 11 : 1;
@@ -244,20 +245,24 @@ __device__ __forceinline__ void trace_wide_body(const WArgs& A) {
       }
     }
     if (PERSISTENT && !exhausted) {
-      // lanes without a ray take consecutive rays from the global counter,
-      // one atomic per wave
-      // a wave takes new rays once `refill_min` of its lanes are idle (or all of them)
+      // lanes without a ray take consecutive positions of the wave's work
+      // queue, one atomic per wave; a wave takes new rays once `refill_min` of
+      // its lanes are idle (or all of them)
       const unsigned long long need = __ballot(!has_ray);
       if (need && (__popcll(need) >= A.refill_min || need == ~0ull)) {
         const int cnt = __popcll(need);
         const int leader = __ffsll((long long)need) - 1;
         unsigned int base = 0;
-        if (lane == leader) base = atomicAdd(A.work, (unsigned)cnt);
+        if (lane == leader) base = atomicAdd(A.work + kQueueStride * queue, (unsigned)cnt);
         base = __shfl(base, leader);
-        if ((int64_t)base + cnt >= A.n_rays) exhausted = true;
+        const int q = queue;
+        if (queue_ray(q, base + (unsigned)cnt - 1u) >= A.n_rays) {  // this queue has run past the last ray
+          queue = (queue + 1) % kQueues;
+          if (--queues_left == 0) exhausted = true;
+        }
         if (!has_ray) {
           const unsigned rank = __popcll(need & ((1ull << lane) - 1));
-          const int64_t r = (int64_t)base + rank;
+          const int64_t r = queue_ray(q, base + rank);
           if (r < A.n_rays) {
             cur = lane_start(L, A, (uint32_t)r);
             if (COUNT) cn = Cnt{};
@@ -671,12 +676,20 @@ __device__ __forceinline__ void packet_body(const WArgs& A) {
   if (!PERSISTENT) {
     packet_chunk<COUNT, LEARNED>(A, (int64_t)blockIdx.x * 64, acc, s_x);
   } else {
-    while (true) {
-      unsigned int base = 0;
-      if (lane == 0) base = atomicAdd(A.work, 64u);
-      base = __shfl(base, 0);
-      if ((int64_t)base >= A.n_rays) break;
-      packet_chunk<COUNT, LEARNED>(A, (int64_t)base, acc, s_x);
+    // the work queues hand out 64-ray chunks (one position per chunk here)
+    const int64_t nchunks = (A.n_rays + 63) / 64;
+    int queue = blockIdx.x % kQueues;
+    for (int queues_left = kQueues; queues_left > 0;) {
+      unsigned int idx = 0;
+      if (lane == 0) idx = atomicAdd(A.work + kQueueStride * queue, 1u);
+      idx = __shfl(idx, 0);
+      const int64_t c = queue + (int64_t)kQueues * idx;
+      if (c >= nchunks) {
+        queue = (queue + 1) % kQueues;
+        queues_left--;
+        continue;
+      }
+      packet_chunk<COUNT, LEARNED>(A, c * 64, acc, s_x);
     }
   }
   if (A.totals) {  // every lane of the wave arrives here together
@@ -700,6 +713,12 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 5)))
 template <bool PERSISTENT, bool COUNT>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) k_trace_packet_learned(const WArgs A) {
   packet_body<PERSISTENT, COUNT, true>(A);
+}
+
+// Zeroes n words (the persistent forms' work queues) with a kernel, so a
+// trace recorded into a graph holds no memset node.
+__global__ void k_zero_words(unsigned int* p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
 }
 
 // Mixed precision, after the trace: every ray whose closest hit was
@@ -878,12 +897,12 @@ int trace_wide_scene(const rtn_scene* s, const float* const rays[6], int64_t n, 
   WArgs A;
   RTN_TRY(fill_wide_args(s, rays, n, prm, hits, d_totals, pairs, A));
   const bool persistent = prm->mode == RTN_MODE_PERSISTENT;
-  // the persistent form's ray counter lives in a small stream-ordered block
+  // the persistent form's work queues live in a small stream-ordered block
   unsigned int* d_work = nullptr;
   if (persistent) {
-    RTN_HIP(hipMallocAsync((void**)&d_work, 4 * sizeof(unsigned int), stream));
-    const unsigned int zero[4] = {0, 0, 0, 0};
-    RTN_HIP(hipMemcpyAsync(d_work, zero, sizeof(zero), hipMemcpyHostToDevice, stream));
+    RTN_HIP(hipMallocAsync((void**)&d_work, kWorkWords * sizeof(unsigned int), stream));
+    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(128), 0, stream, d_work, kWorkWords);
+    RTN_HIP(hipGetLastError());
   }
   A.work = d_work;
   const int rc = launch_wide_kernel(kDefaultShape, persistent, prm->count != 0,
@@ -930,9 +949,9 @@ int trace_packet_scene(const rtn_scene* s, const float* const rays[6], int64_t n
   A.bounds = s->bounds.ptr;
   unsigned int* d_work = nullptr;
   if (persistent) {
-    RTN_HIP(hipMallocAsync((void**)&d_work, 4 * sizeof(unsigned int), stream));
-    const unsigned int zero[4] = {0, 0, 0, 0};
-    RTN_HIP(hipMemcpyAsync(d_work, zero, sizeof(zero), hipMemcpyHostToDevice, stream));
+    RTN_HIP(hipMallocAsync((void**)&d_work, kWorkWords * sizeof(unsigned int), stream));
+    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(128), 0, stream, d_work, kWorkWords);
+    RTN_HIP(hipGetLastError());
   }
   A.work = d_work;
   const int rc = launch_packet_kernel(persistent, prm->count != 0, prm->seed_mode == RTN_SEED_LEARNED, n, stream, A);

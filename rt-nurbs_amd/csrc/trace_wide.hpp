@@ -17,6 +17,20 @@ constexpr int WSTK_DEFAULT = 20;  // stack entries per lane: 20 KB of LDS per 12
 // slot at once instead of waiting for the other waves of its block (+5%).
 constexpr int WB_ONE_RAY = 64;
 
+// Work queues of the persistent forms.  One counter would serialise every
+// wave's atomic on one address (the limit, not the tracing, at tens of
+// thousands of waves); instead kQueues counters, 64 bytes apart, each hand
+// out the 64-ray chunks c with c mod kQueues == its index, in order: position
+// p of queue q is ray ((q + kQueues * (p / 64)) * 64 + p mod 64).  A wave
+// starts on queue blockIdx mod kQueues and moves on when its queue runs past
+// the last ray.  args.work holds kWorkWords zeroed words.
+constexpr int kQueues = 8;
+constexpr int kQueueStride = 16;
+constexpr int kWorkWords = kQueues * kQueueStride;
+__host__ __device__ inline int64_t queue_ray(int q, uint32_t p) {
+  return ((int64_t)q + (int64_t)kQueues * (p >> 6)) * 64 + (p & 63);
+}
+
 // Kernel arguments, passed by value.
 struct WArgs {
   const WideNode* nodes;
@@ -58,7 +72,7 @@ struct WArgs {
   float *t, *u, *v, *dpdu, *dpdv;
   uint32_t* counters;
   unsigned long long* totals;
-  unsigned int* work;     // persistent: next ray to hand out
+  unsigned int* work;     // persistent: the work queues (kWorkWords words)
 };
 
 // Launch shape: lanes per block (64, 128 or 256: 1, 2 or 4 waves), traversal
