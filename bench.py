@@ -423,13 +423,18 @@ def bench_c5(args, L, scenes):
         fp = L.frame_params(**BASE, hull_cull=2, stall_exit=1, precision=L.PREC_MIXED, cond_thr=1e4)
         return L.Frame(batch, rays, ids, fp), rays, ids
 
-    proxy, _, _ = frame_for(4)
-    table = proxy.autotune(reps=2)
-    best = min((e for e in table if e["ms"] > 0), key=lambda e: e["ms"])
-    default = [e for e in table if (e["lanes"], e["stack"], e["blocks_per_cu"], e["refill_min"], e["packet_primary"],
-                                    e["packet_secondary"]) == (128, 20, 8, 64, 0, 0)]
-    log(f"autotune (1/16 of the pixels): best {best}, default {default}")
-    del proxy
+    if args.c5_packet_only:  # no tuning: the default shape, both launches in the packet form (for profiling)
+        table = []
+        best = {"lanes": 128, "stack": 20, "blocks_per_cu": 8, "refill_min": 64, "packet_primary": 1,
+                "packet_secondary": 1}
+    else:
+        proxy, _, _ = frame_for(4)
+        table = proxy.autotune(reps=2)
+        best = min((e for e in table if e["ms"] > 0), key=lambda e: e["ms"])
+        default = [e for e in table if (e["lanes"], e["stack"], e["blocks_per_cu"], e["refill_min"],
+                                        e["packet_primary"], e["packet_secondary"]) == (128, 20, 8, 64, 0, 0)]
+        log(f"autotune (1/16 of the pixels): best {best}, default {default}")
+        del proxy
     frame, rays, ids = frame_for(1)
     frame.set_shape(best["lanes"], best["stack"], best["blocks_per_cu"])
     frame.set_refill(best["refill_min"])
@@ -438,7 +443,7 @@ def bench_c5(args, L, scenes):
     # or in the packet form (one wave per 64 rays): each variant timed as a
     # captured graph, the fastest reported
     variants = {}
-    for pp, ps in ((0, 0), (1, 0), (1, 1)):
+    for pp, ps in ((1, 1),) if args.c5_packet_only else ((0, 0), (1, 0), (1, 1)):
         frame.set_packet(pp, ps)
         frame.capture(stream)
         for _ in range(args.warmup):
@@ -497,6 +502,8 @@ def main():
     ap.add_argument("--train-epochs", type=int, default=16, help="seed network epochs (0: no learned forms)")
     ap.add_argument("--c5-scenes", type=int, default=32)
     ap.add_argument("--c5-scale", type=int, default=1, help="divide config 5 resolutions by this")
+    ap.add_argument("--c5-packet-only", action="store_true",
+                    help="config 5: skip the autotuner and time only the all-packet frame (profiling runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES")
     args = ap.parse_args()
     if args.gpus != 1:
