@@ -352,9 +352,11 @@ int rtn_mlp_train(rtn_mlp* net, const float* feats, const float* targets, int64_
 /* Secondary rays of primary hits (no reference counterpart): per hit, in
  * this order, one shadow ray towards the directional light (direction
  * `light`, normalised) and one mirror ray, both from the hit point offset by
- * eps * (1 + max|S|) along the normal facing the incoming ray.  Primary ray r
- * owns slots per*r .. per*r+per-1 (per = want_shadow + want_mirror), so the
- * layout is deterministic; parent[i] = r, kind[i] = 0 shadow, 1 mirror, -1
+ * eps * (1 + max|S|) along the normal facing the incoming ray.  Slots are
+ * kind-major: primary ray r owns slot j*n + r for the j-th enabled kind
+ * (shadow first; per = want_shadow + want_mirror kinds), so the layout is
+ * deterministic and a wave's 64 consecutive slots hold one kind from 64
+ * neighbouring primaries; parent[i] = r, kind[i] = 0 shadow, 1 mirror, -1
  * empty (a miss).  The trace entry points skip rays whose kind is negative
  * when given the kind array (rtn_frame does).  cap >= per * n.
  * *count_host = valid (non-empty) rays. */
@@ -385,7 +387,7 @@ int rtn_batch_trace(const rtn_batch* batch, const float* const rays_soa[6], cons
 
 /* A frame: primary rays of a batch traced by the persistent kernel, then one
  * shadow ray (any-hit, towards the light) and one mirror ray per hit, in the
- * fixed slots of rtn_rays_secondary, traced by a second persistent launch
+ * fixed (kind-major) slots of rtn_rays_secondary, traced by a second launch
  * (empty slots are skipped).  All buffers are allocated by rtn_frame_create;
  * rtn_frame_run enqueues the whole sequence without allocating or copying
  * from the host, so rtn_frame_capture can record it into a hipGraph that

@@ -2,9 +2,9 @@
 //   silhouette rays  grazing rays aimed at random surface points (mixed-
 //                    precision workload; no reference counterpart)
 //   secondaries      one shadow ray (towards a directional light) and one
-//                    mirror ray per primary hit, in fixed slots per primary
-//                    ray (no reference counterpart; the reference has
-//                    primary rays only)
+//                    mirror ray per primary hit, in fixed kind-major slots
+//                    (no reference counterpart; the reference has primary
+//                    rays only)
 //   Lambert shading  one directional light, spp averaged, replacing the
 //                    reference's debug shaders (testShading.cu:9-55)
 #include <algorithm>
@@ -81,11 +81,13 @@ __device__ __forceinline__ ex::v3<float> facing_normal(const float* dpdu, const 
   return N;
 }
 
-// Secondary rays of primary ray r in slots per*r .. per*r+per-1 (shadow
-// first): shadow ray towards the light and mirror ray, origins offset along
-// the facing normal by eps * (1 + max|S|).  A miss leaves its slots empty
-// (kind -1), so the layout, and the Newton seeds that follow the ray index,
-// are the same on every run.  *count += valid rays.  Grid-stride loop over a
+// Secondary rays of primary ray r in slots j*n + r, j the index of the kind
+// among the enabled ones (shadow first): shadow ray towards the light and
+// mirror ray, origins offset along the facing normal by eps * (1 + max|S|).
+// Kind-major slots keep a wave's 64 secondaries of one kind and from 64
+// neighbouring primaries, so they trace well as a packet.  A miss leaves its
+// slots empty (kind -1), so the layout, and the Newton seeds that follow the
+// ray index, are the same on every run.  *count += valid rays.  Grid-stride loop over a
 // grid sized to the device: each lane sums its count and a wave adds it with
 // one atomic at the end (an atomic per wave of rays serialised on the one
 // counter and cost several times the kernel's memory traffic).
@@ -99,9 +101,9 @@ __global__ void k_secondaries(const float* ox, const float* oy, const float* oz,
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   unsigned long long valid = 0;
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride) {
-    int64_t slot = (int64_t)per * r;
+    int64_t slot = r;
     if (prim[r] < 0) {
-      for (int k = 0; k < per; k++, slot++) {
+      for (int k = 0; k < per; k++, slot += n) {
         sox[slot] = soy[slot] = soz[slot] = 0.0f;
         sdx[slot] = sdy[slot] = sdz[slot] = 0.0f;
         parent[slot] = (int)r;
@@ -123,7 +125,7 @@ __global__ void k_secondaries(const float* ox, const float* oy, const float* oz,
       parent[slot] = (int)r;
       kind[slot] = 0;
       if (out_scene) out_scene[slot] = ray_scene[r];
-      slot++;
+      slot += n;
     }
     if (want_mirror) {
       const float dn = 2.0f * (d.x * N.x + d.y * N.y + d.z * N.z);

@@ -689,8 +689,8 @@ def silhouette_rays(scene, n, seed=0, max_deg=5.0, t0=(1.0, 3.0), rays=None, off
 
 
 def secondary_rays(rays, hits, light, eps=1e-4, shadow=True, mirror=True):
-    """Shadow and mirror rays of the primary hits in fixed slots (kind -1 =
-    empty): (Rays, parent, kind, count of non-empty slots)."""
+    """Shadow and mirror rays of the primary hits in fixed kind-major slots
+    (slot j*n + r; kind -1 = empty): (Rays, parent, kind, count of non-empty slots)."""
     per = int(shadow) + int(mirror)
     cap = max(1, per * rays.n)
     out = Rays(cap, with_idx=False)

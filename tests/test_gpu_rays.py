@@ -73,8 +73,9 @@ def test_secondaries_any_hit_and_shading(rtl):
     nh = int((g["hit"] >= 0).sum())
     assert cnt == 2 * nh
     par, knd = parent.get(), kind.get()
-    assert np.array_equal(par, np.repeat(np.arange(rays.n), 2))  # fixed slots 2r, 2r+1
-    assert np.array_equal(knd >= 0, np.repeat(g["hit"] >= 0, 2))
+    assert np.array_equal(par, np.tile(np.arange(rays.n), 2))  # fixed kind-major slots r, n + r
+    assert np.array_equal(knd >= 0, np.tile(g["hit"] >= 0, 2))
+    assert np.all(knd[:rays.n][g["hit"] >= 0] == 0) and np.all(knd[rays.n:][g["hit"] >= 0] == 1)
     valid = knd >= 0
     par, knd = par[valid], knd[valid]
     assert np.array_equal(par[knd == 0], np.flatnonzero(g["hit"] >= 0))
