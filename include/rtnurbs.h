@@ -254,6 +254,11 @@ typedef struct {
                              reference's Newton carries a disabled divergence exit
                              (bezier_patch.h:231-232)                             */
   const struct rtn_mlp* seed_net; /* RTN_SEED_LEARNED: the seed network          */
+  const int* skip_patch;  /* fast forms: per ray (device), a patch (BVH order, as rtn_hits.prim) the ray
+                             does not test, or -1; NULL = none.  For a secondary ray, the patch it
+                             leaves: a solve on it converges to the ray's own origin, t <= 0, which
+                             is rejected anyway, so the skip changes only the rare ray that a solve
+                             on its own patch would send to a second crossing of that patch */
 } rtn_trace_params;
 
 void rtn_trace_params_default(rtn_trace_params* p);
@@ -401,7 +406,9 @@ typedef struct {
   int packet_primary;     /* 1: trace the primary rays with the packet form (one wave per 64
                              coherent rays) instead of the persistent per-lane form  */
   int packet_secondary;   /* 1: the same for the shadow and mirror rays         */
-  int reserved[2];
+  int skip_origin;        /* 1 (default): a secondary ray does not test the patch it leaves (its
+                             primary ray's hit patch; see rtn_trace_params.skip_patch)  */
+  int reserved;
 } rtn_frame_params;
 typedef struct {
   int64_t n_primary, n_secondary;  /* rays; n_secondary counts non-empty slots   */

@@ -304,6 +304,7 @@ extern "C" void rtn_frame_params_default(rtn_frame_params* fp) {
   fp->eps = 1e-4f;
   fp->shadow = 1;
   fp->mirror = 1;
+  fp->skip_origin = 1;
 }
 
 extern "C" int rtn_frame_create(rtn_frame** out, const rtn_batch* b, const float* const rays[6], const int* ray_scene,
@@ -318,6 +319,9 @@ extern "C" int rtn_frame_create(rtn_frame** out, const rtn_batch* b, const float
   RTN_CHECK((fp->packet_primary == 0 || fp->packet_primary == 1) &&
                 (fp->packet_secondary == 0 || fp->packet_secondary == 1),
             RTN_EINVAL, "rtn_frame_create: packet_primary and packet_secondary are 0 or 1");
+  RTN_CHECK(fp->skip_origin == 0 || fp->skip_origin == 1, RTN_EINVAL, "rtn_frame_create: skip_origin is 0 or 1");
+  RTN_CHECK(!fp->trace.skip_patch, RTN_EINVAL,
+            "rtn_frame_create: trace.skip_patch is not used by frames (skip_origin sets the secondary rays' skip)");
   const float ll = std::sqrt(fp->light[0] * fp->light[0] + fp->light[1] * fp->light[1] + fp->light[2] * fp->light[2]);
   RTN_CHECK(ll > 0.f, RTN_EINVAL, "rtn_frame_create: zero light direction");
   *out = nullptr;
@@ -355,6 +359,10 @@ extern "C" int rtn_frame_create(rtn_frame** out, const rtn_batch* b, const float
     if ((rc = batch_args(b, srays, f->s_scene.ptr, f->cap2, &f->fp.trace, &h2, f->ctl.ptr->totals, f->a2)))
       return fail(rc);
     f->a2.ray_kind = f->s_kind.ptr;
+    if (fp->skip_origin) {  // a secondary ray skips its primary ray's hit patch
+      f->a2.skip_prim = f->prim.ptr;
+      f->a2.skip_parent = f->s_parent.ptr;
+    }
     f->a2.work = f->ctl.ptr->work2;
   }
   if (hipMemcpy(f->ctl.ptr, &h, sizeof(h), hipMemcpyHostToDevice) != hipSuccess) return fail(RTN_EHIP);

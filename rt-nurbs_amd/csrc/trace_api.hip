@@ -52,7 +52,7 @@ extern "C" int rtn_trace(const rtn_scene* s, const float* const rays[6], int64_t
   switch (prm->mode) {
     case RTN_MODE_EXACT:
       RTN_CHECK(prm->seed_mode == RTN_SEED_PLAIN && prm->precision == RTN_PREC_FP32 && !prm->hull_cull &&
-                    !prm->any_hit,
+                    !prm->any_hit && !prm->skip_patch,
                 RTN_EUNSUPPORTED, "rtn_trace: the exact form runs the plain method only");
       rc = trace_exact_scene(s, rays, n_rays, prm, hits, d_tot, st);
       break;

@@ -68,6 +68,13 @@ __device__ __forceinline__ int lane_start(Lane& L, const WArgs& A, uint32_t r) {
   return A.scene_root ? A.scene_root[A.ray_scene[r]] : 0;
 }
 
+// The patch ray r does not test (the patch a secondary ray leaves), or -1;
+// only called when A.skip_prim is set.  Kept in LDS, not in a register: the
+// launches without it (primary rays) keep their register budget.
+__device__ __forceinline__ int skip_patch_of(const WArgs& A, uint32_t r) {
+  return A.skip_prim[A.skip_parent ? A.skip_parent[r] : r];
+}
+
 template <bool COUNT>
 __device__ __forceinline__ void lane_finish(const Lane& L, const WArgs& A, const Cnt& c) {
   // a closest hit that wants the FP64 step is marked -2 - patch until the
@@ -216,6 +223,7 @@ template <int WB, int WSTK, bool PERSISTENT, bool COUNT, bool LEARNED>
 __device__ __forceinline__ void trace_wide_body(const WArgs& A) {
   __shared__ int s_code[WSTK][WB];
   __shared__ float s_te[WSTK][WB];
+  __shared__ int s_skip[WB];  // per lane, skip_patch_of its ray
   // learned form: per-wave feature rows of the seed network (24 BF16 per row)
   __shared__ mlp::bf16x8 s_x[LEARNED ? WB : 1][3];
   const int tid = threadIdx.x;
@@ -241,6 +249,7 @@ __device__ __forceinline__ void trace_wide_body(const WArgs& A) {
       const int64_t r = (int64_t)blockIdx.x * WB + tid;
       if (r < A.n_rays) {
         cur = lane_start(L, A, (uint32_t)r);
+        if (A.skip_prim) s_skip[tid] = skip_patch_of(A, (uint32_t)r);
         has_ray = cur >= 0;
       }
     }
@@ -265,6 +274,7 @@ __device__ __forceinline__ void trace_wide_body(const WArgs& A) {
           const int64_t r = queue_ray(q, base + rank);
           if (r < A.n_rays) {
             cur = lane_start(L, A, (uint32_t)r);
+            if (A.skip_prim) s_skip[tid] = skip_patch_of(A, (uint32_t)r);
             if (COUNT) cn = Cnt{};
             has_ray = cur >= 0;
             off = 0;
@@ -319,6 +329,10 @@ __device__ __forceinline__ void trace_wide_body(const WArgs& A) {
       } else {
         cur = code;
       }
+    }
+    if (A.skip_prim && cand_left > 0 && cand_first == s_skip[tid]) {  // the patch the ray leaves is not tested
+      cand_first++;
+      cand_left--;
     }
     const bool pending = cand_left > 0;
     if (!__ballot(pending)) {
@@ -492,7 +506,7 @@ constexpr int PKT_STACK = 64;  // one entry per lane of the stack register
 // calling wave; work totals accumulate into acc.
 template <bool COUNT, bool LEARNED, int NACC>
 __device__ __forceinline__ void packet_chunk(const WArgs& A, int64_t r0, unsigned long long (&acc)[NACC],
-                                             mlp::bf16x8 (*s_x)[3]) {
+                                             mlp::bf16x8 (*s_x)[3], int* s_skip) {
   const int lane = threadIdx.x & 63;
   Lane L;
   Cnt cn{};
@@ -500,6 +514,7 @@ __device__ __forceinline__ void packet_chunk(const WArgs& A, int64_t r0, unsigne
   {
     const int64_t r = r0 + lane;
     if (r < A.n_rays) root = lane_start(L, A, (uint32_t)r);
+    if (A.skip_prim) s_skip[lane] = root >= 0 ? skip_patch_of(A, (uint32_t)r) : -1;
   }
   bool pending = root >= 0;  // rays whose traversal has not run yet
   const bool had_ray = pending;
@@ -561,7 +576,7 @@ __device__ __forceinline__ void packet_chunk(const WArgs& A, int64_t r0, unsigne
       const int first = leaf & 0xffffff, count = ((leaf >> 24) & 0x7f) + 1;
 #pragma unroll 1
       for (int pi = first; pi < first + count && __ballot(in_leaf); pi++) {
-        bool act = in_leaf;
+        bool act = in_leaf && (!A.skip_prim || pi != s_skip[lane]);  // the patch the ray leaves is not tested
         const int pqv = as_const(A.pq)[pi];
         const cptr<float4> net = as_const(A.cp + as_const(A.cp_off)[pi]);
         if (act && A.hull_cull && (pqv & 0x10000)) {
@@ -670,11 +685,12 @@ __device__ __forceinline__ void packet_chunk(const WArgs& A, int64_t r0, unsigne
 template <bool PERSISTENT, bool COUNT, bool LEARNED>
 __device__ __forceinline__ void packet_body(const WArgs& A) {
   __shared__ mlp::bf16x8 s_x[LEARNED ? 64 : 1][3];
+  __shared__ int s_skip[64];  // per lane, skip_patch_of its ray
   const int lane = threadIdx.x & 63;
   constexpr int NACC = COUNT ? 11 : 1;
   unsigned long long acc[NACC] = {};
   if (!PERSISTENT) {
-    packet_chunk<COUNT, LEARNED>(A, (int64_t)blockIdx.x * 64, acc, s_x);
+    packet_chunk<COUNT, LEARNED>(A, (int64_t)blockIdx.x * 64, acc, s_x, s_skip);
   } else {
     // the work queues hand out 64-ray chunks (one position per chunk here)
     const int64_t nchunks = (A.n_rays + 63) / 64;
@@ -689,7 +705,7 @@ __device__ __forceinline__ void packet_body(const WArgs& A) {
         queues_left--;
         continue;
       }
-      packet_chunk<COUNT, LEARNED>(A, c * 64, acc, s_x);
+      packet_chunk<COUNT, LEARNED>(A, c * 64, acc, s_x, s_skip);
     }
   }
   if (A.totals) {  // every lane of the wave arrives here together
@@ -886,6 +902,8 @@ int fill_wide_args(const rtn_scene* s, const float* const rays[6], int64_t n, co
     A.pair_cap = pairs->cap;
   }
   A.prim = hits->prim; A.t = hits->t; A.u = hits->u; A.v = hits->v; A.dpdu = hits->dpdu; A.dpdv = hits->dpdv;
+  A.skip_prim = prm->skip_patch;
+  A.skip_parent = nullptr;
   A.counters = prm->count ? hits->counters : nullptr;
   A.totals = d_totals;
   return RTN_OK;

@@ -68,6 +68,10 @@ struct WArgs {
   const int* scene_root;
   const int* ray_scene;
   const int* ray_kind;    // per ray, 0 = occlusion (any-hit) ray; null = all as any_hit says
+  // per ray, a patch not tested: skip_prim[skip_parent[r]] (skip_parent null:
+  // skip_prim[r]); skip_prim null = none
+  const int* skip_prim;
+  const int* skip_parent;
   int* prim;
   float *t, *u, *v, *dpdu, *dpdv;
   uint32_t* counters;

@@ -63,7 +63,8 @@ class TraceParams(C.Structure):
                 ("seed_mode", C.c_int), ("precision", C.c_int), ("cond_thr", C.c_float),
                 ("hull_cull", C.c_int), ("count", C.c_int), ("learned_iters", C.c_int),
                 ("any_hit", C.c_int), ("refill_min", C.c_int), ("fallback_iters", C.c_int),
-                ("seed_share", C.c_int), ("stall_exit", C.c_int), ("seed_net", C.c_void_p)]
+                ("seed_share", C.c_int), ("stall_exit", C.c_int), ("seed_net", C.c_void_p),
+                ("skip_patch", C.c_void_p)]
 
 
 class MlpTrainParams(C.Structure):
@@ -79,7 +80,7 @@ class BuildParams(C.Structure):
 class FrameParams(C.Structure):
     _fields_ = [("trace", TraceParams), ("light", C.c_float * 3), ("eps", C.c_float), ("shadow", C.c_int),
                 ("mirror", C.c_int), ("packet_primary", C.c_int), ("packet_secondary", C.c_int),
-                ("reserved", C.c_int * 2)]
+                ("skip_origin", C.c_int), ("reserved", C.c_int)]
 
 
 class FrameStatus(C.Structure):
@@ -450,7 +451,8 @@ class Scene:
 
 
 def params(**kw):
-    """rtn_trace_params with defaults; seed_net takes an Mlp (kept alive by the struct)."""
+    """rtn_trace_params with defaults; seed_net takes an Mlp and skip_patch an int32
+    DeviceArray of one patch per ray (both kept alive by the struct)."""
     p = TraceParams()
     load().rtn_trace_params_default(C.byref(p))
     for k, v in kw.items():
@@ -459,6 +461,9 @@ def params(**kw):
         if k == "seed_net":
             p._net = v
             v = v.h if v is not None else None
+        elif k == "skip_patch":
+            p._skip = v
+            v = v.ptr if v is not None else None
         setattr(p, k, v)
     return p
 
@@ -759,7 +764,7 @@ class Batch:
 
 
 def frame_params(light=(0.3, -0.4, 1.0), eps=1e-4, shadow=1, mirror=1, packet_primary=0, packet_secondary=0,
-                 **trace_kw):
+                 skip_origin=1, **trace_kw):
     fp = FrameParams()
     load().rtn_frame_params_default(C.byref(fp))
     for k, v in trace_kw.items():
@@ -772,6 +777,7 @@ def frame_params(light=(0.3, -0.4, 1.0), eps=1e-4, shadow=1, mirror=1, packet_pr
     fp.light[:] = list(light)
     fp.eps, fp.shadow, fp.mirror = eps, shadow, mirror
     fp.packet_primary, fp.packet_secondary = packet_primary, packet_secondary
+    fp.skip_origin = skip_origin
     return fp
 
 

@@ -117,12 +117,14 @@ def test_frame_graph_replay_and_autotune(rtl):
     hb, _ = batch.trace(rays, ids, rtl.params(mode=rtl.MODE_PERSISTENT, **CFG))
     assert np.array_equal(hb.get()["hit"], ref["primary"]["hit"])
     # shadow rays end at their first hit: same hit/miss as a closest-hit trace
-    # of the same rays (same slots, so the same Newton seeds); empty slots miss
+    # of the same rays (same slots, so the same Newton seeds) that skips the
+    # same patches (each ray's origin patch); empty slots miss
     par, kind = ref["parent"], ref["kind"]
     assert np.all(ref["secondary"]["hit"][kind < 0] == -1)
     sr = rtl.Rays.from_host(ref["secondary_rays"])
     sids = rtl.DeviceArray.from_host(ids.get()[par])
-    hs, _ = batch.trace(sr, sids, rtl.params(mode=rtl.MODE_WIDE, **CFG))
+    origin = rtl.DeviceArray.from_host(np.where(kind >= 0, ref["primary"]["hit"][par], -1).astype(np.int32))
+    hs, _ = batch.trace(sr, sids, rtl.params(mode=rtl.MODE_WIDE, skip_patch=origin, **CFG))
     hs = hs.get()
     sh, mi = kind == 0, kind == 1
     assert np.array_equal(ref["secondary"]["hit"][sh] >= 0, hs["hit"][sh] >= 0)
@@ -179,3 +181,43 @@ def test_frame_graph_replay_and_autotune(rtl):
     else:
         same_but_ties(got["primary"], want["primary"])
         assert np.mean(got["secondary"]["hit"] == ref["secondary"]["hit"]) > 0.999
+
+
+def test_secondaries_skip_their_origin_patch(rtl):
+    """A secondary ray does not test the patch it leaves.  A solve on that patch
+    converges to the ray's own origin (t <= 0, rejected), so skipping it may
+    only change the rays whose unskipped trace hits the origin patch again; all
+    other rays keep their hit bit for bit, and Newton calls drop."""
+    items = _mini_c5(rtl, n=3)
+    rays, ids, counts = _batch_rays(rtl, items, spp=2)
+    batch = rtl.Batch([sc for sc, _ in items])
+    res = {}
+    for skip in (0, 1):
+        fr = rtl.Frame(batch, rays, ids, rtl.frame_params(skip_origin=skip, **CFG))
+        fr.run()
+        res[skip] = fr.results()
+    par, kind = res[0]["parent"], res[0]["kind"]
+    assert np.array_equal(res[1]["secondary_rays"], res[0]["secondary_rays"])
+    origin = np.where(kind >= 0, res[0]["primary"]["hit"][par], -1).astype(np.int32)
+    a, b = res[0]["secondary"], res[1]["secondary"]
+    own = (a["hit"] == origin) & (origin >= 0)
+    print(f"secondary rays {int((kind >= 0).sum())}, unskipped hits on the origin patch {int(own.sum())}", flush=True)
+    assert np.all(b["hit"][origin >= 0] != origin[origin >= 0])
+    same = ~own
+    assert np.array_equal(a["hit"][same], b["hit"][same])
+    m = same & (a["hit"] >= 0) & (kind == 1)
+    assert np.array_equal(a["t"][m].view(np.uint32), b["t"][m].view(np.uint32))
+    assert own.mean() < 0.01
+    # the same skip through rtn_trace's skip_patch, in every fast form, with counters
+    sr = rtl.Rays.from_host(res[0]["secondary_rays"])
+    sids = rtl.DeviceArray.from_host(ids.get()[par])
+    skip = rtl.DeviceArray.from_host(origin)
+    for mode in (rtl.MODE_WIDE, rtl.MODE_PERSISTENT, rtl.MODE_PACKET, rtl.MODE_PERSISTENT_PACKET):
+        h0, t0 = batch.trace(sr, sids, rtl.params(mode=mode, count=1, **CFG))
+        h1, t1 = batch.trace(sr, sids, rtl.params(mode=mode, count=1, skip_patch=skip, **CFG))
+        g0, g1 = h0.get()["hit"], h1.get()["hit"]
+        own = (g0 == origin) & (origin >= 0)
+        assert np.all(g1[origin >= 0] != origin[origin >= 0])
+        assert np.mean(g0[~own] == g1[~own]) > 0.999
+        print(f"mode {mode}: newton calls {t0['newton_calls']} -> {t1['newton_calls']}", flush=True)
+        assert t1["newton_calls"] < t0["newton_calls"]
