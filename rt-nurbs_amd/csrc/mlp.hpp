@@ -249,6 +249,59 @@ __device__ __forceinline__ void forward_seed(const bf16x8* __restrict__ tiles, c
   }
 }
 
+// forward_seed with each layer's weight tiles loaded next to its products
+// instead of ahead: the same products in the same order (bit-identical
+// logits) holding fewer registers over the call.  The packet kernel keeps its
+// whole ray state live across the call; with this form it spills 44-52 bytes
+// per lane instead of 108 and runs 3-4% faster, while the per-lane kernel is
+// 3% faster with the early loads of forward_seed.
+__device__ __forceinline__ void forward_seed_lean(const bf16x8* __restrict__ tiles,
+                                                  const float* __restrict__ bias,
+                                                  const bf16x8 (*s_x)[3], int nblk, int lane, int row, float& lu,
+                                                  float& lv) {
+  const int m = lane & 15, g = lane >> 4;
+#pragma unroll 1
+  for (int nb = 0; nb < nblk; nb++) {
+    int toff = 0;  // opaque offset: each layer's tiles are loaded next to its products
+    asm volatile("" : "+v"(toff));
+    const bf16x8* tp = tiles + toff;
+    bf16x8 x = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (g < 3) x = s_x[16 * nb + m][g];
+    f32x4 d[4];
+#pragma unroll
+    for (int ob = 0; ob < 4; ob++) d[ob] = mfma(tp[(T_L1 + ob) * 64 + lane], x, bias4(bias, BB1 + 16 * ob + 4 * g));
+    bf16x8 h[2];
+#pragma unroll
+    for (int layer = 0; layer < 2; layer++) {
+      const int tl = layer == 0 ? T_L2 : T_L3;
+      const int bb = layer == 0 ? BB2 : BB3;
+#pragma unroll
+      for (int kb = 0; kb < 2; kb++)
+#pragma unroll
+        for (int i = 0; i < 8; i++) h[kb][i] = (__bf16)gelu(d[2 * kb + (i >> 2)][i & 3]);
+      asm volatile("" : "+v"(toff));
+      tp = tiles + toff;
+#pragma unroll
+      for (int ob = 0; ob < 4; ob++)
+        d[ob] = mfma(tp[(tl + 2 * ob + 1) * 64 + lane], h[1],
+                     mfma(tp[(tl + 2 * ob) * 64 + lane], h[0], bias4(bias, bb + 16 * ob + 4 * g)));
+    }
+#pragma unroll
+    for (int kb = 0; kb < 2; kb++)
+#pragma unroll
+      for (int i = 0; i < 8; i++) h[kb][i] = (__bf16)gelu(d[2 * kb + (i >> 2)][i & 3]);
+    asm volatile("" : "+v"(toff));
+    tp = tiles + toff;
+    const f32x4 o =
+        mfma(tp[(T_L4 + 1) * 64 + lane], h[1], mfma(tp[(T_L4 + 0) * 64 + lane], h[0], bias4(bias, BB4 + 4 * g)));
+    const float a = __shfl(o[0], row & 15), bv = __shfl(o[1], row & 15);
+    if ((row >> 4) == nb) {
+      lu = a;
+      lv = bv;
+    }
+  }
+}
+
 // Packs a pair's features as one BF16 row of s_x (32 values, 20..31 zero).
 __device__ __forceinline__ void store_row(bf16x8 (*s_x)[4], int row, const float f[NIN], bool zero) {
   bf16x8 r[4];
@@ -283,13 +336,15 @@ __device__ __forceinline__ void store_features(bf16x8 (*s_x)[3], int row, float 
 // Seeds for the active lanes of a wave from their feature rows: the active
 // pairs' rows were written compacted (rows 0..k-1, see seed_row), so a wave
 // with k active lanes runs ceil(k/16) pair blocks.  Called by all 64 lanes;
-// u, v are written for active lanes only.
+// u, v are written for active lanes only.  LEAN selects forward_seed_lean.
+template <bool LEAN>
 __device__ __forceinline__ void seed_wave_rows(const bf16x8* __restrict__ tiles, const float* __restrict__ bias,
                                                bool active, int row, int n_act, float& u, float& v,
                                                const bf16x8 (*s_x)[3], int lane) {
   wave_lds_sync();
   float lu = 0.0f, lv = 0.0f;
-  forward_seed(tiles, bias, s_x, (n_act + 15) >> 4, lane, row, lu, lv);
+  if (LEAN) forward_seed_lean(tiles, bias, s_x, (n_act + 15) >> 4, lane, row, lu, lv);
+  else forward_seed(tiles, bias, s_x, (n_act + 15) >> 4, lane, row, lu, lv);
   if (active) {
     u = fast_sigmoid(lu);
     v = fast_sigmoid(lv);

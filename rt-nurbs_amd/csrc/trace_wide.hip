@@ -384,7 +384,7 @@ __device__ __forceinline__ void trace_wide_body(const WArgs& A) {
 #pragma unroll
         for (int k = 0; k < 3; k++) rows[row][k] = z;
       }
-      mlp::seed_wave_rows(A.mlp_tiles, A.mlp_bias, own, row, n_act, in.gu, in.gv, rows, lane);
+      mlp::seed_wave_rows<false>(A.mlp_tiles, A.mlp_bias, own, row, n_act, in.gu, in.gv, rows, lane);
       if (A.seed_share) {
         const float lu = __shfl(in.gu, lead), lv = __shfl(in.gv, lead);
         if (pending && !own) {
@@ -624,7 +624,7 @@ __device__ __forceinline__ void packet_chunk(const WArgs& A, int64_t r0, unsigne
 #pragma unroll
             for (int k = 0; k < 3; k++) s_x[row][k] = z;
           }
-          mlp::seed_wave_rows(A.mlp_tiles, A.mlp_bias, own, row, n_act, in.gu, in.gv, s_x, lane);
+          mlp::seed_wave_rows<true>(A.mlp_tiles, A.mlp_bias, own, row, n_act, in.gu, in.gv, s_x, lane);
           if (A.seed_share) {
             const float lu = __shfl(in.gu, lead), lv = __shfl(in.gv, lead);
             if (act && !own) {
@@ -720,7 +720,8 @@ __device__ __forceinline__ void packet_body(const WArgs& A) {
 
 // Five waves per SIMD: the kernel needs 111 registers per lane (four waves);
 // held to 96 it spills 64 bytes and runs 5% faster.  The learned kernel stays
-// at three waves: at four it spills 330 bytes and runs a third slower.
+// at three waves (44-52 bytes spilled): at four it spills 184-196 bytes, and
+// the persistent learned form is no faster.
 template <bool PERSISTENT, bool COUNT>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 5))) k_trace_packet(const WArgs A) {
   packet_body<PERSISTENT, COUNT, false>(A);
