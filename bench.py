@@ -423,19 +423,22 @@ def bench_c5(args, L, scenes):
         fp = L.frame_params(**BASE, hull_cull=2, stall_exit=1, precision=L.PREC_MIXED, cond_thr=1e4)
         return L.Frame(batch, rays, ids, fp), rays, ids
 
+    frame, rays, ids = frame_for(1)
     if args.c5_packet_only:  # no tuning: the default shape, both launches in the packet form (for profiling)
         table = []
         best = {"lanes": 128, "stack": 20, "blocks_per_cu": 8, "refill_min": 64, "packet_primary": 1,
                 "packet_secondary": 1}
     else:
-        proxy, _, _ = frame_for(4)
-        table = proxy.autotune(reps=2)
+        # tuned on the frame itself, not on a lower-resolution proxy: a packet's
+        # coherence depends on the resolution (at 1/16 of the pixels the packet
+        # launches lose, at full resolution they win)
+        t0 = time.perf_counter()
+        table = frame.autotune(reps=2)
         best = min((e for e in table if e["ms"] > 0), key=lambda e: e["ms"])
         default = [e for e in table if (e["lanes"], e["stack"], e["blocks_per_cu"], e["refill_min"],
                                         e["packet_primary"], e["packet_secondary"]) == (128, 20, 8, 64, 0, 0)]
-        log(f"autotune (1/16 of the pixels): best {best}, default {default}")
-        del proxy
-    frame, rays, ids = frame_for(1)
+        log(f"autotune ({len(table)} candidates on the full frame, {time.perf_counter() - t0:.1f} s): best {best}, "
+            f"default {default}")
     frame.set_shape(best["lanes"], best["stack"], best["blocks_per_cu"])
     frame.set_refill(best["refill_min"])
     stream = L.stream_create()
@@ -460,7 +463,7 @@ def bench_c5(args, L, scenes):
         total_rays = st["n_primary"] + st["n_secondary"]
         log(f"c5 frame (packet primary {pp}, secondary {ps}): {st['n_primary']} primary + {st['n_secondary']} "
             f"secondary rays, {variants[(pp, ps)]:.2f} ms, {total_rays / variants[(pp, ps)] / 1e3:.1f} Mrays/s")
-    log(f"autotuner's choice on the proxy: packet primary {best['packet_primary']}, secondary {best['packet_secondary']}")
+    log(f"autotuner's choice: packet primary {best['packet_primary']}, secondary {best['packet_secondary']}")
     (pp, ps), ms = min(variants.items(), key=lambda kv: kv[1])
     names = {0: "persistent per-lane", 1: "packet"}
     wl = {"workload": f"{len(built)} scenes, {n_patches} patches (1k-200k each, degree 2-5), 256^2-2048^2 x 4 spp "
