@@ -5,7 +5,9 @@ A batch only relocates each scene's arrays, so tracing rays of several
 scenes in one launch must reproduce, bit for bit, tracing each scene on its
 own (with Newton's RNG seeded by the same, batch-wide, ray index).  A captured frame must reproduce its eager run on every replay, with
 eager work between replays; a tuned launch shape changes scheduling only, so
-it must reproduce the default shape's results."""
+it must reproduce the default shape's results.  A frame's secondary rays
+skip the patch they leave; the same skip through rtn_trace's skip_patch
+gives the same hits."""
 import numpy as np
 import pytest
 
