@@ -16,11 +16,12 @@ Configurations (--config):
       and one mirror ray per hit: one frame = persistent primary launch +
       persistent secondary launch, hipGraph-captured, launch shape autotuned.
 
-The JSON line's `value` is the headline form's Mrays/s (c2: the fastest form
-running the learned seed and/or the FP64 correction step); every form is
-reported under `forms`.
+The JSON line on stdout is short: `value` is the headline form's Mrays/s
+(c2: the learned seed + mixed precision + persistent form the target names,
+with the same form without the network and the plain method beside it);
+every form's full report goes to the detail file (--detail).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c1|c3|c5]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c1|c3|c5] [--detail PATH]
        python bench.py --calibrate-copy BYTES   (copy of known size, for PMC scaling)
 """
 import argparse
@@ -44,7 +45,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def ray_flops(tot, n_rays, exact):
+def ray_flops(tot, n_rays, exact, box_tot=None):
     """Algorithmic FP32 operations of a trace, priced by a fixed table on the
     work the run performed: 40 per ray (reciprocal direction and ray planes),
     33 per box test (66 per binary interior visit, whose two child boxes are
@@ -52,9 +53,13 @@ def ray_flops(tot, n_rays, exact):
     (corners and bilinear seed), plus each Newton evaluation priced at its
     degree by the reference algorithm's operation count (counted by the
     kernel).  Work a form avoids (rejected patches, fewer iterations) is not
-    priced, and neither is work a form adds beyond the per-ray algorithm (the
-    packet forms are priced by the one-ray form's counts)."""
-    boxes = 66.0 * tot["interior"] + 32.0 * tot["leaves"] if exact else 33.0 * tot["interior"]
+    priced, and neither is work a form adds beyond the per-ray algorithm: a
+    packet lane also tests the child boxes of nodes the wave visits for other
+    lanes, so the packet forms' box tests are priced from `box_tot`, the
+    one-ray form's counts for the same rays (their patch tests and Newton
+    work are their own)."""
+    bt = box_tot if box_tot is not None else tot
+    boxes = 66.0 * bt["interior"] + 32.0 * bt["leaves"] if exact else 33.0 * bt["interior"]
     return 40.0 * n_rays + boxes + 180.0 * tot["patch_tests"] + float(tot["newton_flops"])
 
 
@@ -205,10 +210,9 @@ def load_traffic(form, config):
 
 
 def roofline(tflops, form, config):
-    return {"bound": "mfma", "achieved": round(tflops, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+    return {"bound": "fp32-valu", "achieved": round(tflops, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
             "frac": round(tflops / PEAK_FP32_TFLOPS, 4), "traffic": load_traffic(form, config),
-            "note": "compute roofline of FP32 vector work (Newton, de Casteljau, box tests), priced per ray by a "
-                    "fixed op table over the counted work; peak = MI355X published FP32"}
+            "kernel": form}
 
 
 def run_forms(L, scene, rays, forms, args, ref, stream):
@@ -220,15 +224,15 @@ def run_forms(L, scene, rays, forms, args, ref, stream):
         g = hits_c.get()
         each = {}
         hits, kern_ms, wall_ms = time_form(L, scene, rays, prm, args.steps, args.warmup, stream, each)
-        alg = tot
+        overflow = scene.trace_overflow()  # the last timed launch's stack-overflow count
+        box = tot
         if kw["mode"] in (L.MODE_PACKET, L.MODE_PERSISTENT_PACKET):
-            # a packet lane also tests the child boxes of nodes the wave visits for
-            # other lanes: the algorithmic work of a ray is what the one-ray form
-            # counts for it, so the packet forms are priced by that form's counts
-            _, alg = L.trace(scene, rays, L.params(count=1, **{**kw, "mode": L.MODE_WIDE}))
-        flops = ray_flops(alg, n, kw["mode"] == L.MODE_EXACT)
+            # box tests priced by the one-ray form's counts of the same rays
+            _, box = L.trace(scene, rays, L.params(count=1, **{**kw, "mode": L.MODE_WIDE}))
+        flops = ray_flops(tot, n, kw["mode"] == L.MODE_EXACT, box)
         r = {"mrays_s": n / (kern_ms * 1e-3) / 1e6, "kernel_ms": kern_ms, "wall_ms": wall_ms,
-             "box_tests_per_ray": tot["interior"] / n, "box_tests_per_ray_priced": alg["interior"] / n,
+             "overflow": overflow + tot["overflow"],
+             "box_tests_per_ray": tot["interior"] / n, "box_tests_per_ray_priced": box["interior"] / n,
              "kernel_ms_median": each.get("median"), "kernel_ms_min": each.get("min"),
              "newton_iters_per_ray": tot["newton_evals"] / n, "newton_calls_per_ray": tot["newton_calls"] / n,
              "patch_tests_per_ray": tot["patch_tests"] / n, "flops_per_ray": flops / n,
@@ -322,6 +326,8 @@ def bench_c2_c1(args, L, scenes, learned):
         "persistent_hull2_mixed_stall": dict(mode=P, hull_cull=2, stall_exit=1, **mixed, **BASE),
         "packet_hull2_mixed": dict(mode=K, hull_cull=2, **mixed, **BASE),
         "packet_hull2_mixed_stall": dict(mode=K, hull_cull=2, stall_exit=1, **mixed, **BASE),
+        # the headline form without the network
+        "persistent_packet_hull2_mixed_stall": dict(mode=KP, hull_cull=2, stall_exit=1, **mixed, **BASE),
     }
     if net is not None:
         # a 3-evaluation attempt from the network's seed (two Newton steps: the fewest that
@@ -345,8 +351,15 @@ def bench_c2_c1(args, L, scenes, learned):
         log(f"fp64 reference: {time.perf_counter() - t0:.2f}s, hits {(ref['hit'] >= 0).mean():.3f}")
     results = run_forms(L, scene, rays, forms, args, ref, stream)
     ablation = seed_ablation(L, scene, rays, net) if net is not None else None
-    cand = [k for k in results if "mixed" in k or "learned" in k] or list(results)
-    head = max(cand, key=lambda k: results[k]["mrays_s"])
+    # headline: the learned seed + mixed precision + persistent form the target names;
+    # beside it the same form without the network and the plain method
+    north = "persistent_packet_learned_mixed_stall"
+    twin = "persistent_packet_hull2_mixed_stall"
+    if north in results:
+        head = north
+    else:
+        cand = [k for k in results if "mixed" in k or "learned" in k] or list(results)
+        head = max(cand, key=lambda k: results[k]["mrays_s"])
     H = results[head]
     cpu, cpu_port = None, None
     if not args.no_cpu:
@@ -364,16 +377,44 @@ def bench_c2_c1(args, L, scenes, learned):
         except Exception as e:  # the baseline is a report, never the measurement
             log(f"cpu baseline failed: {e!r}")
     plain = results.get("plain", {}).get("mrays_s")
+    detail = {"forms": results, "seed_network": train, "seed_ablation": ablation, "cpu_baseline": cpu,
+              "cpu_port_baseline": cpu_port if cpu is not cpu_port else None,
+              "speedups_vs_plain": {k: round(v["mrays_s"] / plain, 3) for k, v in results.items()} if plain else None}
+    write_detail(args, detail)
     extra = {"newton_iters_per_ray": round(H["newton_iters_per_ray"], 4),
              "speedup_vs_plain": round(H["mrays_s"] / plain, 3) if plain else None,
-             "speedups_vs_plain": {k: round(v["mrays_s"] / plain, 3) for k, v in results.items()} if plain else None,
-             "forms": results, "seed_network": train, "seed_ablation": ablation,
-             "roofline": roofline(H["tflops"], head, args.config), "cpu_baseline": cpu,
-             "cpu_port_baseline": cpu_port if cpu is not cpu_port else None}
-    extra["kernel_ms_per_launch"] = {"mean": round(H["kernel_ms"], 4), "median": H["kernel_ms_median"],
-                                     "min": H["kernel_ms_min"]}
-    wl = {"workload": workload, "rays": rays.n, "patches": int(st["n_patches"]), "headline_form": head}
+             "summary": {k: brief(results[k]) for k in ("plain", twin, head) if k in results},
+             "roofline": roofline(H["tflops"], head, args.config), "cpu_baseline": short_cpu(cpu),
+             "kernel_ms_per_launch": {"mean": round(H["kernel_ms"], 4), "median": round(H["kernel_ms_median"], 4),
+                                      "min": round(H["kernel_ms_min"], 4)},
+             "detail": args.detail}
+    wl = {"workload": workload, "rays": rays.n, "patches": int(st["n_patches"]), "headline_form": head,
+          "no_network_form": twin if twin in results else None}
     return make_line(args, H["mrays_s"], H["kernel_ms"], "f32", wl, extra)
+
+
+def brief(r):
+    """A form's figures for the stdout line (the full report is in the detail file)."""
+    out = {"mrays_s": round(r["mrays_s"], 1), "iters_per_ray": round(r["newton_iters_per_ray"], 3),
+           "fp64_steps": r["fp64_steps"], "overflow": r["overflow"]}
+    e = r.get("fp64")
+    if e:
+        out.update(mismatch_non_tie=e["patch_mismatch_non_tie"], hit_miss=e["hit_miss_disagree"],
+                   t_le_1e5=round(e.get("t_rel_le_1e-5_frac", float("nan")), 6))
+    return out
+
+
+def short_cpu(c):
+    if c is None:
+        return None
+    return {k: c[k] for k in ("value", "unit", "cores", "kind", "sample")} | {"value": round(c["value"], 4)}
+
+
+def write_detail(args, detail):
+    os.makedirs(os.path.dirname(os.path.abspath(args.detail)), exist_ok=True)
+    with open(args.detail, "w") as f:
+        json.dump(detail, f, indent=1)
+    log(f"detail written to {args.detail}")
 
 
 def bench_c3(args, L, scenes):
@@ -396,10 +437,19 @@ def bench_c3(args, L, scenes):
         results[name]["fp64_silhouette"] = error_stats(sub, rsub)
     head = "wide_hull2_mixed"
     H = results[head]
+    write_detail(args, {"forms": results})
+    summ = {}
+    for k in ("plain", "wide_hull2", head):
+        b = brief(results[k])
+        e = results[k].get("fp64_silhouette", {})
+        b["silhouette_t_le_1e5"] = round(e.get("t_rel_le_1e-5_frac", float("nan")), 6)
+        b["silhouette_mismatch_non_tie"] = e.get("patch_mismatch_non_tie")
+        summ[k] = b
     wl = {"workload": "config 1 scene, 65,536 rays of which 13,107 silhouette rays (grazing 0-5 deg)",
           "rays": rays.n, "headline_form": head}
     return make_line(args, H["mrays_s"], H["kernel_ms"], "f32+f64", wl,
-                     {"forms": results, "roofline": roofline(H["tflops"], head, "c3"), "cpu_baseline": None})
+                     {"summary": summ, "roofline": roofline(H["tflops"], head, "c3"), "cpu_baseline": None,
+                      "detail": args.detail})
 
 
 def bench_c5(args, L, scenes):
@@ -474,9 +524,9 @@ def bench_c5(args, L, scenes):
           "primary_rays": st["n_primary"], "secondary_rays": st["n_secondary"], "patches": n_patches,
           "launch_shape": {"lanes": st["shape_lanes"], "stack": st["shape_stack"],
                            "blocks_per_cu": st["shape_blocks_per_cu"], "refill_min": st["refill_min"]}}
+    write_detail(args, {"autotune": table, "frame_ms": wl["frame_ms"]})
     return make_line(args, total_rays / ms / 1e3, ms, "f32", wl,
-                     {"autotune": table, "roofline": None, "cpu_baseline": None,
-                      "note": "roofline not priced for the frame (no per-ray counters in the captured launches)"})
+                     {"roofline": None, "cpu_baseline": None, "detail": args.detail})
 
 
 def calibrate_copy(nbytes):
@@ -508,7 +558,10 @@ def main():
     ap.add_argument("--c5-packet-only", action="store_true",
                     help="config 5: skip the autotuner and time only the all-packet frame (profiling runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES")
+    ap.add_argument("--detail", default=None, help="per-form report (JSON); default bench_out/<config>_detail.json")
     args = ap.parse_args()
+    if args.detail is None:
+        args.detail = os.path.join("bench_out", f"{args.config}_detail.json")
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(across GPUs the scene is replicated and rays shard by image tile or sample)",

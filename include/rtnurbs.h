@@ -61,6 +61,26 @@ typedef struct {                                         /* SoAQueue::ray_queue,
   float *direction_x, *direction_y, *direction_z;
   int* idx_interact;
 } rtn_ray_queue;
+typedef struct {                                         /* ParametricSurf::nurbsSurf, nurbs.h:17-22 */
+  float *u_knots, *v_knots;                              /* n+p+2 and m+q+2 knots                  */
+  rtn_vec4* control_point;                               /* (n+1)(m+1) homogeneous, u-major        */
+  int n, m;                                              /* indices of the last control point      */
+  int p, q;
+} rtn_nurbs_surf;
+typedef struct { float m[16]; } rtn_mat4;                /* MathTypes::matrix4x4f, column major    */
+typedef struct {                                         /* camera_config, camera.h:13-27 (472 B)  */
+  rtn_mat4 world_from_camera;
+  rtn_mat4 screen_from_camera;
+  rtn_mat4 NDC_from_screen;
+  rtn_mat4 raster_from_NDC;
+  rtn_mat4 raster_from_screen;
+  rtn_mat4 screen_from_raster;
+  rtn_mat4 camera_from_raster;
+  int screen_resolution_x, screen_resolution_y;
+  float screen_window[4];                                /* MathExtra::bounds2f: min x, min y,
+                                                            max x, max y                           */
+} rtn_camera_config;
+enum { RTN_PROJ_ORTHOGRAPHIC = 0, RTN_PROJ_PERSPECTIVE = 1 }; /* enum projection, camera.h:37       */
 
 /* Error message of the calling thread's last failure, and the library
  * version (no reference counterpart: the reference prints errors and goes on,
@@ -74,6 +94,9 @@ int rtn_version(void);
 int rtn_malloc(void** ptr, size_t bytes);
 int rtn_malloc_managed(void** ptr, size_t bytes);
 int rtn_free(void* ptr);
+/* CudaHostAlloc / CudaCpuFree (memory.h:23-28, :60-65): pinned host memory. */
+int rtn_malloc_host(void** ptr, size_t bytes);
+int rtn_free_host(void* ptr);
 /* kind: 0 host->device, 1 device->host, 2 device->device, 3 inferred */
 int rtn_memcpy(void* dst, const void* src, size_t bytes, int kind);
 int rtn_memset(void* dst, int value, size_t bytes);
@@ -95,6 +118,28 @@ int rtn_event_elapsed_ms(void* start, void* stop, float* ms);
 int rtn_camera_perspective(const float eye[3], const float look[3], const float up[3], int W, int H,
                            float fov_deg, float znear, float zfar, float world_from_camera[16],
                            float camera_from_raster[16]);
+
+/* The reference's camera set-up on its full camera_config, host-side and in
+ * its operation order: Transform::lookAt (transform.h:46-60),
+ * CameraOps::init_camera (camera.h:123-141; it calls initCameraTransforms,
+ * :89-119, and leaves screen_from_camera and camera_from_raster zero),
+ * getPerspectiveCamera (:63-85; fov in degrees, radians() applied) and
+ * getOrthographicCamera (:42-59). */
+int rtn_lookat(const float pos[3], const float look[3], const float up[3], float world_from_camera[16]);
+int rtn_camera_init(const float world_from_camera[16], int res_x, int res_y, rtn_camera_config* out);
+int rtn_camera_get_perspective(const rtn_camera_config* in, float fov_deg, float z_near, float z_far,
+                               rtn_camera_config* out);
+int rtn_camera_get_orthographic(const rtn_camera_config* in, float z_near, float z_far, rtn_camera_config* out);
+
+/* calculateRaysWrapper(ray_queue&, projection, camera_config&)
+ * (generateRaysKernel.h:6, .cu:39-67): one ray through the centre of every
+ * pixel of the camera's screen_resolution_x x screen_resolution_y raster,
+ * ray index y*W + x, idx_interact = -1.  RTN_PROJ_PERSPECTIVE:
+ * calculateRay<perspective_projection> (camera.h:162-174); RTN_PROJ_ORTHOGRAPHIC:
+ * calculateRay<orthographic_projection> (camera.h:150-160): origin the pixel's
+ * camera-space point taken to world space, direction the camera's z axis
+ * (world_from_camera.m[8..10]).  Synchronous when stream is NULL. */
+int rtn_rays_generate(const rtn_ray_queue* rays, int projection, const rtn_camera_config* camera, void* stream);
 
 /* calculateRaysWrapper<perspective> (generateRaysKernel.cu:39-67; ray math
  * camera.h:162-174).  Writes W*H*spp rays into the device SoA queue and
@@ -212,9 +257,12 @@ enum {
 };
 enum {
   RTN_PREC_FP32 = 0,      /* FP32 Newton                                         */
-  RTN_PREC_MIXED = 1      /* FP32 Newton; after the trace, one FP64 Newton step on each ray's
-                             closest hit whose cond(J) > cond_thr (fast forms; needs the
-                             t, u and v outputs)                                  */
+  RTN_PREC_MIXED = 1      /* FP32 Newton started from the plain seed evaluated in FP64 (the FP64
+                             reference's seed; the FP32 seed's accept test runs on rounding
+                             noise, see rtn_fast.hpp seed64); after the trace, one FP64 Newton
+                             step on each ray's closest hit whose cond(J) > cond_thr or whose
+                             predicted relative t error (the change one more Newton step would
+                             make) exceeds t_tol.  Fast forms; needs the t, u and v outputs   */
 };
 
 typedef struct {
@@ -259,6 +307,8 @@ typedef struct {
                              leaves: a solve on it converges to the ray's own origin, t <= 0, which
                              is rejected anyway, so the skip changes only the rare ray that a solve
                              on its own patch would send to a second crossing of that patch */
+  float t_tol;            /* RTN_PREC_MIXED: a closest hit whose predicted relative t error exceeds
+                             this also gets the FP64 step (0 = 2.5e-6, a quarter of the 1e-5 bar) */
 } rtn_trace_params;
 
 void rtn_trace_params_default(rtn_trace_params* p);
@@ -291,6 +341,16 @@ typedef struct {
  * range of its ray queue) in the form prm->mode selects. */
 int rtn_trace(const rtn_scene* scene, const float* const rays_soa[6], int64_t n_rays, const rtn_trace_params* prm,
               const rtn_hits* hits, rtn_trace_totals* totals_host, void* stream);
+/* Stack overflow.  Every form counts the rays whose traversal stack
+ * overflowed (their closest hit may be wrong).  rtn_trace returns
+ * RTN_EOVERFLOW when that count is non-zero if it is synchronous (stream NULL)
+ * or totals_host is given; an asynchronous call without totals returns at
+ * launch, and rtn_scene_trace_overflow then gives the last trace's count on
+ * the scene (it synchronises the device).  rtn_trace allocates nothing: its
+ * work queues and totals live in a control block the scene allocates when it
+ * is built, so traces of one scene that overlap in time (on several streams)
+ * share that block and race; trace one scene from one stream at a time. */
+int rtn_scene_trace_overflow(const rtn_scene* scene, unsigned long long* count);
 
 /* Converged (ray, patch, u, v) solutions of the plain method (training pairs
  * of the learned seed; no reference counterpart).  Runs the wide or

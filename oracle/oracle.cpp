@@ -12,6 +12,7 @@
 // Function                      reference                           pinned by
 // ---------------------------   ---------------------------------   -----------------------------
 // lookat / camera_perspective   transform.h:46-60, camera.h:63-141  golden fixtures + closed form
+// camera_config / ortho rays    camera.h:42-160                     golden fixtures (camera.npz)
 // gen_rays                      camera.h:162-174 (host 1/sqrtf,     golden fixtures + pixel-centre
 //                               vector_ops.h:102-106),               closed form
 //                               generateRaysKernel.cu:17-34
@@ -346,6 +347,73 @@ void orc_camera_perspective(int W, int H, float fov_deg, float znear, float zfar
   mat_mul(sc, pt, screen_from_camera);
   mat_inverse(screen_from_camera, inv_sfc);
   mat_mul(inv_sfc, screen_from_raster, camera_from_raster);
+}
+
+// The reference's full camera_config (camera.h:13-27), as 118 floats/ints in
+// its layout: 7 column-major matrices (world_from_camera, screen_from_camera,
+// NDC_from_screen, raster_from_NDC, raster_from_screen, screen_from_raster,
+// camera_from_raster), resolution x, y (int), screen window (4 floats).
+// init_camera + initCameraTransforms (camera.h:89-141) leave screen_from_camera
+// and camera_from_raster zero; proj 1 then applies getPerspectiveCamera
+// (camera.h:63-85), proj 0 getOrthographicCamera (camera.h:42-59).
+void orc_camera_config(const float* wfc, int W, int H, int proj, float fov_deg, float znear, float zfar, void* out) {
+  float c[118];
+  std::memset(c, 0, sizeof(c));
+  float* M[7];
+  for (int k = 0; k < 7; k++) M[k] = c + 16 * k;
+  std::memcpy(M[0], wfc, 64);
+  int res[2] = {W, H};
+  std::memcpy(c + 112, res, 8);
+  float* win = c + 114;
+  float aspect = float(W) / float(H);
+  if (aspect > 1.0f) { win[0] = -aspect; win[1] = -1.0f; win[2] = aspect; win[3] = 1.0f; }
+  else { win[0] = -1.0f; win[1] = -1.0f / aspect; win[2] = 1.0f; win[3] = 1.0f / aspect; }
+  float s[16], t[16];
+  mat_scale(1 / (win[2] - win[0]), 1 / (win[3] - win[1]), 1, s);
+  mat_translate(-win[0], -win[3], 0, t);
+  mat_mul(s, t, M[2]);
+  mat_scale(float(W), float(-H), 1, M[3]);
+  mat_mul(M[3], M[2], M[4]);
+  mat_inverse(M[4], M[5]);
+  float inv[16];
+  if (proj == 1) {
+    const float f = zfar, n = znear;
+    float pt[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, f / (f - n), 1, 0, 0, -f * n / (f - n), 0};
+    const float PI = 3.14159265f;
+    float inv_tan = 1.0f / tanf(((PI / 180) * fov_deg) / 2);
+    float sc[16];
+    mat_scale(inv_tan, inv_tan, 1, sc);
+    mat_mul(sc, pt, M[1]);
+    mat_inverse(M[1], inv);
+    mat_mul(inv, M[5], M[6]);
+  } else if (proj == 0) {
+    float sc[16], tr[16];
+    mat_scale(1, 1, 1 / (zfar - znear), sc);
+    mat_translate(0, 0, -znear, tr);
+    mat_mul(sc, tr, M[1]);
+    mat_inverse(M[1], inv);
+    mat_mul(inv, M[5], M[6]);
+  }
+  std::memcpy(out, c, sizeof(c));
+}
+
+// calculateRay<orthographic_projection> (camera.h:150-160) over a W x H grid,
+// pixel-major: origin = world_from_camera (camera_from_raster (x+.5, y+.5, 0, 1)),
+// direction = the camera's z axis (world_from_camera m[8..10]).
+void orc_gen_rays_ortho(const float* wfc, const float* cfr, int W, int H, float* ox, float* oy, float* oz,
+                        float* dx, float* dy, float* dz) {
+  for (int y = 0; y < H; y++)
+    for (int x = 0; x < W; x++) {
+      long r = (long)y * W + x;
+      float e[4] = {float(x) + 0.5f, float(y) + 0.5f, 0.0f, 1.0f};
+      float c[4], o[4];
+      for (int i = 0; i < 4; i++)
+        c[i] = cfr[0 + i] * e[0] + cfr[4 + i] * e[1] + cfr[8 + i] * e[2] + cfr[12 + i] * e[3];
+      for (int i = 0; i < 4; i++)
+        o[i] = wfc[0 + i] * c[0] + wfc[4 + i] * c[1] + wfc[8 + i] * c[2] + wfc[12 + i] * c[3];
+      ox[r] = o[0]; oy[r] = o[1]; oz[r] = o[2];
+      dx[r] = wfc[8]; dy[r] = wfc[9]; dz[r] = wfc[10];
+    }
 }
 
 // calculateRay<perspective_projection> (camera.h:162-174) over a W x H grid,

@@ -43,6 +43,8 @@ def lib():
         L.orc_lookat.argtypes = [f32p, f32p, f32p, f32p]
         L.orc_camera_perspective.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, f32p]
         L.orc_gen_rays.argtypes = [f32p, f32p, C.c_int, C.c_int, C.c_int, vp, vp] + [f32p] * 6
+        L.orc_camera_config.argtypes = [f32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, vp]
+        L.orc_gen_rays_ortho.argtypes = [f32p, f32p, C.c_int, C.c_int] + [f32p] * 6
         L.orc_segs.argtypes = [f32p, C.c_int, C.c_int, f32p]
         L.orc_segs.restype = C.c_int
         L.orc_count_segments.argtypes = [f32p, C.c_int, C.c_int]
@@ -94,6 +96,9 @@ def ref():
         R.ref_lookat.argtypes = [f32p, f32p, f32p, f32p]
         R.ref_camera.argtypes = [f32p, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, f32p]
         R.ref_gen_rays.argtypes = [f32p, f32p, C.c_int, C.c_int] + [f32p] * 6
+        R.ref_camera_config.argtypes = [f32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, vp]
+        R.ref_camera_config_size.restype = C.c_int
+        R.ref_gen_rays_ortho.argtypes = [f32p, f32p, C.c_int, C.c_int] + [f32p] * 6
         R.ref_get_patch.argtypes = [C.c_int, C.c_int, f32p, C.c_int, C.c_int, f32p, f32p, vp, vp,
                                     C.POINTER(C.c_int), C.POINTER(C.c_int)]
         R.ref_get_patch.restype = C.c_int
@@ -144,6 +149,23 @@ def gen_rays(wfc, cfr, W, H, spp=1, jx=None, jy=None):
     keep = (jx, jy)
     lib().orc_gen_rays(f32(wfc), f32(cfr), W, H, spp, jxp, jyp, *out)
     del keep
+    return out
+
+
+CAMERA_CONFIG_BYTES = 472  # camera_config (camera.h:13-27): 7 matrices, 2 ints, bounds2f
+
+
+def camera_config(wfc, W, H, proj, fov_deg=60.0, znear=0.01, zfar=1000.0):
+    """init_camera then getPerspectiveCamera (proj 1) / getOrthographicCamera
+    (proj 0) / nothing (proj -1): the camera_config bytes as uint8[472]."""
+    out = np.zeros(CAMERA_CONFIG_BYTES, np.uint8)
+    lib().orc_camera_config(f32(wfc), W, H, proj, fov_deg, znear, zfar, out.ctypes.data)
+    return out
+
+
+def gen_rays_ortho(wfc, cfr, W, H):
+    out = [np.zeros(W * H, np.float32) for _ in range(6)]
+    lib().orc_gen_rays_ortho(f32(wfc), f32(cfr), W, H, *out)
     return out
 
 

@@ -37,6 +37,36 @@ void ref_camera(const float* wfc16, int W, int H, float fov_deg, float znear, fl
   std::memcpy(cfr16, cam.camera_from_raster.m, sizeof(cam.camera_from_raster.m));
 }
 
+// The whole camera_config (camera.h:13-27) of init_camera (camera.h:123-141),
+// then getPerspectiveCamera (proj 1, fov in degrees) or getOrthographicCamera
+// (proj 0) (camera.h:42-85), copied out as its bytes; proj < 0 stops after
+// init_camera.
+void ref_camera_config(const float* wfc16, int W, int H, int proj, float fov_deg, float znear, float zfar,
+                       void* out) {
+  MathTypes::matrix4x4f wfc;
+  std::memcpy(wfc.m, wfc16, sizeof(wfc.m));
+  camera_config cam = CameraOps::init_camera(wfc, W, H);
+  if (proj == 1) cam = CameraOps::getPerspectiveCamera(cam, fov_deg, znear, zfar);
+  if (proj == 0) cam = CameraOps::getOrthographicCamera(cam, znear, zfar);
+  std::memcpy(out, &cam, sizeof(cam));
+}
+int ref_camera_config_size() { return (int)sizeof(camera_config); }
+
+// generateRaysKernel.cu:17-34 loop with calculateRay<orthographic_projection>
+void ref_gen_rays_ortho(const float* wfc16, const float* cfr16, int W, int H, float* ox, float* oy, float* oz,
+                        float* dx, float* dy, float* dz) {
+  camera_gpu_payload pay;
+  std::memcpy(pay.world_from_camera.m, wfc16, 64);
+  std::memcpy(pay.camera_from_raster.m, cfr16, 64);
+  for (int y = 0; y < H; y++)
+    for (int x = 0; x < W; x++) {
+      MathExtra::ray r = CameraOps::calculateRay<orthographic_projection>(pay, x, y);
+      long i = (long)y * W + x;
+      ox[i] = r.origin.v[0]; oy[i] = r.origin.v[1]; oz[i] = r.origin.v[2];
+      dx[i] = r.direction.v[0]; dy[i] = r.direction.v[1]; dz[i] = r.direction.v[2];
+    }
+}
+
 // generateRaysKernel.cu:17-34 loop, calling calculateRay<perspective_projection>
 void ref_gen_rays(const float* wfc16, const float* cfr16, int W, int H, float* ox, float* oy, float* oz, float* dx,
                   float* dy, float* dz) {

@@ -46,6 +46,15 @@ extern "C" int rtn_free(void* ptr) {
   if (ptr) RTN_HIP(hipFree(ptr));
   return RTN_OK;
 }
+extern "C" int rtn_malloc_host(void** ptr, size_t bytes) {
+  RTN_CHECK(ptr, RTN_EINVAL, "rtn_malloc_host: null out pointer");
+  RTN_HIP(hipHostMalloc(ptr, bytes ? bytes : 1, hipHostMallocDefault));
+  return RTN_OK;
+}
+extern "C" int rtn_free_host(void* ptr) {
+  if (ptr) RTN_HIP(hipHostFree(ptr));
+  return RTN_OK;
+}
 extern "C" int rtn_memcpy(void* dst, const void* src, size_t bytes, int kind) {
   static const hipMemcpyKind kinds[4] = {hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice,
                                          hipMemcpyDefault};
@@ -152,13 +161,10 @@ void mat_translate(float tx, float ty, float tz, float* m) {
   mat_diag(1.0f, 1.0f, 1.0f, m);
   m[12] = tx; m[13] = ty; m[14] = tz;
 }
-}  // namespace
 
-extern "C" int rtn_camera_perspective(const float eye[3], const float look[3], const float up[3], int W, int H,
-                                      float fov_deg, float znear, float zfar, float wfc[16], float cfr[16]) {
-  RTN_CHECK(eye && look && up && wfc && cfr && W > 0 && H > 0, RTN_EINVAL, "rtn_camera_perspective: bad arguments");
+// Transform::lookAt (transform.h:46-60)
+void lookat(const float* eye, const float* look, const float* up, float* out) {
   using namespace rtn::ex;
-  // lookAt (transform.h:46-60)
   using V = v3<float>;
   V pos{eye[0], eye[1], eye[2]};
   V dir = normalize(sub(V{look[0], look[1], look[2]}, pos));
@@ -166,29 +172,92 @@ extern "C" int rtn_camera_perspective(const float eye[3], const float look[3], c
   V nu = cross(dir, right);
   const float m[16] = {right.x, right.y, right.z, 0.0f, nu.x, nu.y, nu.z, 0.0f,
                        dir.x,   dir.y,   dir.z,   0.0f, pos.x, pos.y, pos.z, 1.0f};
-  memcpy(wfc, m, sizeof(m));
-  // init_camera / initCameraTransforms (camera.h:89-141)
-  float aspect = float(W) / float(H);
-  float wminx, wminy, wmaxx, wmaxy;
-  if (aspect > 1.0f) { wminx = -aspect; wminy = -1.0f; wmaxx = aspect; wmaxy = 1.0f; }
-  else { wminx = -1.0f; wminy = -1.0f / aspect; wmaxx = 1.0f; wmaxy = 1.0f / aspect; }
-  float s[16], t[16], ndc_from_screen[16], raster_from_ndc[16], raster_from_screen[16], screen_from_raster[16];
-  mat_diag(1 / (wmaxx - wminx), 1 / (wmaxy - wminy), 1, s);
-  mat_translate(-wminx, -wmaxy, 0, t);
-  mat_mul(s, t, ndc_from_screen);
-  mat_diag(float(W), float(-H), 1, raster_from_ndc);
-  mat_mul(raster_from_ndc, ndc_from_screen, raster_from_screen);
-  mat_inverse(raster_from_screen, screen_from_raster);
-  // getPerspectiveCamera (camera.h:63-85)
-  const float f = zfar, n = znear;
-  float pt[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, f / (f - n), 1, 0, 0, -f * n / (f - n), 0};
+  memcpy(out, m, sizeof(m));
+}
+
+// init_camera (camera.h:123-141) with initCameraTransforms (:89-119): a zeroed
+// configuration, the screen window from the aspect ratio, then the raster <->
+// screen transforms; screen_from_camera and camera_from_raster stay zero.
+void camera_init(const float* wfc, int W, int H, rtn_camera_config* c) {
+  memset(c, 0, sizeof(*c));
+  memcpy(c->world_from_camera.m, wfc, 64);
+  c->screen_resolution_x = W;
+  c->screen_resolution_y = H;
+  const float aspect = float(W) / float(H);
+  float* win = c->screen_window;  // min x, min y, max x, max y
+  if (aspect > 1.0f) { win[0] = -aspect; win[1] = -1.0f; win[2] = aspect; win[3] = 1.0f; }
+  else { win[0] = -1.0f; win[1] = -1.0f / aspect; win[2] = 1.0f; win[3] = 1.0f / aspect; }
+  float sc[16], tr[16];
+  mat_diag(1 / (win[2] - win[0]), 1 / (win[3] - win[1]), 1, sc);
+  mat_translate(-win[0], -win[3], 0, tr);
+  mat_mul(sc, tr, c->NDC_from_screen.m);
+  mat_diag(float(W), float(-H), 1, c->raster_from_NDC.m);
+  mat_mul(c->raster_from_NDC.m, c->NDC_from_screen.m, c->raster_from_screen.m);
+  mat_inverse(c->raster_from_screen.m, c->screen_from_raster.m);
+}
+
+// getPerspectiveCamera (camera.h:63-85): pbrt's perspective transform scaled
+// by 1/tan(fov/2), fov in degrees through MathUtils::radians.
+void camera_perspective(const rtn_camera_config* in, float fov, float n, float f, rtn_camera_config* out) {
+  const rtn_camera_config c = *in;
+  const float pt[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, f / (f - n), 1, 0, 0, -f * n / (f - n), 0};
   const float PI = 3.14159265f;
-  float inv_tan = 1.0f / tanf(((PI / 180) * fov_deg) / 2);
-  float sc[16], screen_from_camera[16], inv_sfc[16];
+  const float inv_tan = 1.0f / tanf(((PI / 180) * fov) / 2);
+  float sc[16], inv[16];
+  *out = c;
   mat_diag(inv_tan, inv_tan, 1, sc);
-  mat_mul(sc, pt, screen_from_camera);
-  mat_inverse(screen_from_camera, inv_sfc);
-  mat_mul(inv_sfc, screen_from_raster, cfr);
+  mat_mul(sc, pt, out->screen_from_camera.m);
+  mat_inverse(out->screen_from_camera.m, inv);
+  mat_mul(inv, c.screen_from_raster.m, out->camera_from_raster.m);
+}
+
+// getOrthographicCamera (camera.h:42-59): screen_from_camera maps z_near..z_far to 0..1.
+void camera_orthographic(const rtn_camera_config* in, float zn, float zf, rtn_camera_config* out) {
+  const rtn_camera_config c = *in;
+  float sc[16], tr[16], inv[16];
+  *out = c;
+  mat_diag(1, 1, 1 / (zf - zn), sc);
+  mat_translate(0, 0, -zn, tr);
+  mat_mul(sc, tr, out->screen_from_camera.m);
+  mat_inverse(out->screen_from_camera.m, inv);
+  mat_mul(inv, c.screen_from_raster.m, out->camera_from_raster.m);
+}
+}  // namespace
+
+extern "C" int rtn_lookat(const float pos[3], const float look[3], const float up[3], float wfc[16]) {
+  RTN_CHECK(pos && look && up && wfc, RTN_EINVAL, "rtn_lookat: null argument");
+  lookat(pos, look, up, wfc);
+  return RTN_OK;
+}
+
+extern "C" int rtn_camera_init(const float wfc[16], int res_x, int res_y, rtn_camera_config* out) {
+  RTN_CHECK(wfc && out && res_x > 0 && res_y > 0, RTN_EINVAL, "rtn_camera_init: bad arguments");
+  camera_init(wfc, res_x, res_y, out);
+  return RTN_OK;
+}
+
+extern "C" int rtn_camera_get_perspective(const rtn_camera_config* in, float fov_deg, float z_near, float z_far,
+                                          rtn_camera_config* out) {
+  RTN_CHECK(in && out, RTN_EINVAL, "rtn_camera_get_perspective: null argument");
+  camera_perspective(in, fov_deg, z_near, z_far, out);
+  return RTN_OK;
+}
+
+extern "C" int rtn_camera_get_orthographic(const rtn_camera_config* in, float z_near, float z_far,
+                                           rtn_camera_config* out) {
+  RTN_CHECK(in && out, RTN_EINVAL, "rtn_camera_get_orthographic: null argument");
+  camera_orthographic(in, z_near, z_far, out);
+  return RTN_OK;
+}
+
+extern "C" int rtn_camera_perspective(const float eye[3], const float look[3], const float up[3], int W, int H,
+                                      float fov_deg, float znear, float zfar, float wfc[16], float cfr[16]) {
+  RTN_CHECK(eye && look && up && wfc && cfr && W > 0 && H > 0, RTN_EINVAL, "rtn_camera_perspective: bad arguments");
+  lookat(eye, look, up, wfc);
+  rtn_camera_config c;
+  camera_init(wfc, W, H, &c);
+  camera_perspective(&c, fov_deg, znear, zfar, &c);
+  memcpy(cfr, c.camera_from_raster.m, 64);
   return RTN_OK;
 }
 
@@ -254,6 +323,53 @@ extern "C" int rtn_rays_perspective(const float wfc[16], const float cfr[16], in
   const int64_t blocks = (n + 255) / 256;
   hipLaunchKernelGGL(k_rays_perspective, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, cam, W, H, spp,
                      jitter, tiled, jitter_seed, *rays);
+  RTN_HIP(hipGetLastError());
+  if (!stream) RTN_HIP(hipStreamSynchronize(nullptr));
+  return RTN_OK;
+}
+
+// calculateRays<orthographic_projection> (generateRaysKernel.cu:15-35,
+// camera.h:150-160): the pixel centre's camera-space point taken to world
+// space is the origin, the camera's z axis the direction.  Row-major pixels.
+__global__ void __launch_bounds__(256) k_rays_orthographic(CamPayload cam, int W, int H, rtn_ray_queue q) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= (int64_t)W * H) return;
+  const int x = (int)(r % W), y = (int)(r / W);
+  const float e0 = float(x) + 0.5f, e1 = float(y) + 0.5f, e2 = 0.0f, e3 = 1.0f;
+  const float* c = cam.cfr;
+  const float p0 = c[0] * e0 + c[4] * e1 + c[8] * e2 + c[12] * e3;
+  const float p1 = c[1] * e0 + c[5] * e1 + c[9] * e2 + c[13] * e3;
+  const float p2 = c[2] * e0 + c[6] * e1 + c[10] * e2 + c[14] * e3;
+  const float p3 = c[3] * e0 + c[7] * e1 + c[11] * e2 + c[15] * e3;
+  const float* w = cam.wfc;
+  q.origin_x[r] = w[0] * p0 + w[4] * p1 + w[8] * p2 + w[12] * p3;
+  q.origin_y[r] = w[1] * p0 + w[5] * p1 + w[9] * p2 + w[13] * p3;
+  q.origin_z[r] = w[2] * p0 + w[6] * p1 + w[10] * p2 + w[14] * p3;
+  q.direction_x[r] = w[8];
+  q.direction_y[r] = w[9];
+  q.direction_z[r] = w[10];
+  if (q.idx_interact) q.idx_interact[r] = -1;
+}
+
+extern "C" int rtn_rays_generate(const rtn_ray_queue* rays, int projection, const rtn_camera_config* cam,
+                                 void* stream) {
+  RTN_CHECK(rays && cam, RTN_EINVAL, "rtn_rays_generate: null argument");
+  RTN_CHECK(projection == RTN_PROJ_ORTHOGRAPHIC || projection == RTN_PROJ_PERSPECTIVE, RTN_EINVAL,
+            "rtn_rays_generate: unknown projection %d", projection);
+  const int W = cam->screen_resolution_x, H = cam->screen_resolution_y;
+  RTN_CHECK(W > 0 && H > 0, RTN_EINVAL, "rtn_rays_generate: camera resolution %d x %d", W, H);
+  if (projection == RTN_PROJ_PERSPECTIVE)
+    return rtn_rays_perspective(cam->world_from_camera.m, cam->camera_from_raster.m, W, H, 1, 0, 0u, rays, stream);
+  RTN_CHECK(rays->origin_x && rays->origin_y && rays->origin_z && rays->direction_x && rays->direction_y &&
+                rays->direction_z,
+            RTN_EINVAL, "rtn_rays_generate: null ray buffer");
+  const int64_t n = (int64_t)W * H;
+  RTN_CHECK(n < (int64_t)1 << 32, RTN_EINVAL, "rtn_rays_generate: too many rays");
+  CamPayload pay;
+  memcpy(pay.wfc, cam->world_from_camera.m, 64);
+  memcpy(pay.cfr, cam->camera_from_raster.m, 64);
+  hipLaunchKernelGGL(k_rays_orthographic, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, pay,
+                     W, H, *rays);
   RTN_HIP(hipGetLastError());
   if (!stream) RTN_HIP(hipStreamSynchronize(nullptr));
   return RTN_OK;

@@ -93,16 +93,13 @@ __global__ void k_check_scene_ids(const int* __restrict__ ids, int64_t n, int k,
 }
 
 // Every ray's scene index must name a scene of the batch (the kernels index
-// the root table with it).
-int check_scene_ids(const rtn_batch* b, const int* ids, int64_t n, hipStream_t st) {
+// the root table with it).  `bad` is a zeroed device word.
+int check_scene_ids(const rtn_batch* b, const int* ids, int64_t n, int* bad, hipStream_t st) {
   if (n <= 0) return RTN_OK;
-  DevBuf<int> bad;
-  RTN_TRY(bad.alloc(1));
-  RTN_HIP(hipMemsetAsync(bad.ptr, 0, sizeof(int), st));
-  hipLaunchKernelGGL(k_check_scene_ids, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ids, n, b->k, bad.ptr);
+  hipLaunchKernelGGL(k_check_scene_ids, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ids, n, b->k, bad);
   RTN_HIP(hipGetLastError());
   int h = 0;
-  RTN_HIP(hipMemcpyAsync(&h, bad.ptr, sizeof(int), hipMemcpyDeviceToHost, st));
+  RTN_HIP(hipMemcpyAsync(&h, bad, sizeof(int), hipMemcpyDeviceToHost, st));
   RTN_HIP(hipStreamSynchronize(st));
   RTN_CHECK(!h, RTN_EINVAL, "batch: a ray's scene index is outside 0..%d", b->k - 1);
   return RTN_OK;
@@ -140,6 +137,7 @@ extern "C" int rtn_batch_create(rtn_batch** out, rtn_scene* const* scenes, int k
   int rc;
   if ((rc = m.cp.alloc(ncp)) || (rc = m.cp_off.alloc(np)) || (rc = m.pq.alloc(np)) || (rc = m.bounds.alloc(6 * np)) ||
       (rc = m.delta.alloc(np)) || (rc = m.corners.alloc(3 * np)) || (rc = m.obb.alloc(4 * np)) ||
+      (rc = m.seed64.alloc(kSeedTermWords * np)) ||
       (rc = m.wide.alloc(nw)))
     return fail(rc);
   int64_t pb = 0, cb = 0, wb = 0;
@@ -150,7 +148,8 @@ extern "C" int rtn_batch_create(rtn_batch** out, rtn_scene* const* scenes, int k
     if ((rc = copy_into(m.cp, cb, s->cp, s->n_cp)) || (rc = copy_into(m.pq, pb, s->pq, spn)) ||
         (rc = copy_into(m.bounds, 6 * pb, s->bounds, 6 * (size_t)spn)) || (rc = copy_into(m.delta, pb, s->delta, spn)) ||
         (rc = copy_into(m.corners, 3 * pb, s->corners, 3 * (size_t)spn)) ||
-        (rc = copy_into(m.obb, 4 * pb, s->obb, 4 * (size_t)spn)))
+        (rc = copy_into(m.obb, 4 * pb, s->obb, 4 * (size_t)spn)) ||
+        (rc = copy_into(m.seed64, kSeedTermWords * pb, s->seed64, kSeedTermWords * (size_t)spn)))
       return fail(rc);
     hipLaunchKernelGGL(k_add_i32, dim3((spn + 255) / 256), dim3(256), 0, nullptr, s->cp_off.ptr, spn, (int)cb,
                        m.cp_off.ptr + pb);
@@ -164,7 +163,7 @@ extern "C" int rtn_batch_create(rtn_batch** out, rtn_scene* const* scenes, int k
     b->patch_base.push_back(pb);
     m.max_depth = std::max(m.max_depth, s->max_depth);
   }
-  if ((rc = b->roots.upload(b->root_h.data(), k))) return fail(rc);
+  if ((rc = b->roots.upload(b->root_h.data(), k)) || (rc = m.ctl.alloc(1))) return fail(rc);
   if (hipDeviceSynchronize() != hipSuccess) return fail(RTN_EHIP);
   m.n_patches = np;
   m.n_cp = ncp;
@@ -193,16 +192,14 @@ extern "C" int rtn_batch_trace(const rtn_batch* b, const float* const rays[6], c
   RTN_CHECK(!prm->count || hits->counters, RTN_EINVAL, "rtn_batch_trace: count requested without a counter buffer");
   if (n == 0) return RTN_OK;
   hipStream_t st = (hipStream_t)stream;
-  RTN_TRY(check_scene_ids(b, ray_scene, n, st));
-  struct Block { unsigned int work[kWorkWords]; unsigned long long tot[12]; };
-  Block* d = nullptr;
-  RTN_HIP(hipMallocAsync((void**)&d, sizeof(Block), st));
-  Block h{};
+  // the batch's control block (allocated with the batch): work queues, totals, flag
+  TraceCtl* d = b->merged.ctl.ptr;
+  RTN_TRY(ctl_reset(d, st));
+  RTN_TRY(check_scene_ids(b, ray_scene, n, &d->flag, st));
   WArgs a;
-  int rc = batch_args(b, rays, ray_scene, n, prm, hits, &d->tot[0], a);
+  int rc = batch_args(b, rays, ray_scene, n, prm, hits, &d->totals[0], a);
   if (rc == RTN_OK) {
     a.work = d->work;
-    RTN_HIP(hipMemcpyAsync(d, &h, sizeof(Block), hipMemcpyHostToDevice, st));
     const bool packet = prm->mode == RTN_MODE_PACKET || prm->mode == RTN_MODE_PERSISTENT_PACKET;
     rc = packet ? launch_packet_kernel(prm->mode == RTN_MODE_PERSISTENT_PACKET, prm->count != 0,
                                        prm->seed_mode == RTN_SEED_LEARNED, n, st, a)
@@ -211,7 +208,7 @@ extern "C" int rtn_batch_trace(const rtn_batch* b, const float* const rays[6], c
   }
   unsigned long long tot[12] = {0};
   if (rc == RTN_OK) {
-    RTN_HIP(hipMemcpyAsync(tot, d->tot, sizeof(tot), hipMemcpyDeviceToHost, st));
+    RTN_HIP(hipMemcpyAsync(tot, d->totals, sizeof(tot), hipMemcpyDeviceToHost, st));
     RTN_HIP(hipStreamSynchronize(st));
     if (totals) memcpy(totals, tot, sizeof(tot));
     if (tot[6]) {
@@ -219,7 +216,6 @@ extern "C" int rtn_batch_trace(const rtn_batch* b, const float* const rays[6], c
       rc = RTN_EOVERFLOW;
     }
   }
-  (void)hipFreeAsync(d, st);
   return rc;
 }
 
@@ -325,7 +321,8 @@ extern "C" int rtn_frame_create(rtn_frame** out, const rtn_batch* b, const float
   const float ll = std::sqrt(fp->light[0] * fp->light[0] + fp->light[1] * fp->light[1] + fp->light[2] * fp->light[2]);
   RTN_CHECK(ll > 0.f, RTN_EINVAL, "rtn_frame_create: zero light direction");
   *out = nullptr;
-  RTN_TRY(check_scene_ids(b, ray_scene, n, nullptr));
+  RTN_TRY(ctl_reset(b->merged.ctl.ptr, nullptr));
+  RTN_TRY(check_scene_ids(b, ray_scene, n, &b->merged.ctl.ptr->flag, nullptr));
   rtn_frame* f = new rtn_frame();
   f->b = b;
   f->fp = *fp;

@@ -203,17 +203,31 @@ RTN_HD T slab(T lx, T hx, T ly, T hy, T lz, T hz, v3<T> o, v3<T> rd) {
 
 // biliniearRay (bezier_patch.h:273-327), as written (GARP's n = cross(n, pa)
 // is absent in the reference; its accept test therefore runs on rounding
-// noise, and the seeds here follow it exactly).
+// noise, and the seeds here follow it exactly).  Split in two: the terms
+// that depend on the patch alone (its corners' differences), which a scene
+// may compute once per patch, and the per-ray solve; bilinear_seed runs both,
+// and either route performs the same operations in the same order.
+template <class T> struct SeedTerms { v3<T> q00, q10, e00, e11, qn; };
+
 template <class T>
-RTN_HD void bilinear_seed(v3<T> q00, v3<T> q10, v3<T> q11, v3<T> q01, v3<T> orig, v3<T> dir, T& su, T& sv) {
-  v3<T> e10 = sub(q10, q00);
-  v3<T> e11 = sub(q11, q10);
-  v3<T> e00 = sub(q01, q00);
-  v3<T> qn = cross(e10, sub(q01, q11));
-  q00 = sub(q00, orig);
-  q10 = sub(q10, orig);
+RTN_HD SeedTerms<T> seed_terms(v3<T> q00, v3<T> q10, v3<T> q11, v3<T> q01) {
+  SeedTerms<T> s;
+  const v3<T> e10 = sub(q10, q00);
+  s.e11 = sub(q11, q10);
+  s.e00 = sub(q01, q00);
+  s.qn = cross(e10, sub(q01, q11));
+  s.q00 = q00;
+  s.q10 = q10;
+  return s;
+}
+
+template <class T>
+RTN_HD void seed_solve(const SeedTerms<T>& s, v3<T> orig, v3<T> dir, T& su, T& sv) {
+  const v3<T> e00 = s.e00, e11 = s.e11;
+  const v3<T> q00 = sub(s.q00, orig);
+  const v3<T> q10 = sub(s.q10, orig);
   T a = dot(cross(q00, dir), e00);
-  T c = dot(qn, dir);
+  T c = dot(s.qn, dir);
   T b = dot(cross(q10, dir), e11);
   b -= a + c;
   T det = b * b - T(4) * a * c;
@@ -247,6 +261,11 @@ RTN_HD void bilinear_seed(v3<T> q00, v3<T> q10, v3<T> q11, v3<T> q01, v3<T> orig
       best_t = tx;
     }
   }
+}
+
+template <class T>
+RTN_HD void bilinear_seed(v3<T> q00, v3<T> q10, v3<T> q11, v3<T> q01, v3<T> orig, v3<T> dir, T& su, T& sv) {
+  seed_solve(seed_terms(q00, q10, q11, q01), orig, dir, su, sv);
 }
 
 // pcg_hash (math_utils.h:12-17) and randf (math_utils.h:70-74)

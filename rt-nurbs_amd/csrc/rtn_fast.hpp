@@ -1,8 +1,10 @@
-// FP32 arithmetic of the fast trace forms: explicit FMA, tensor-product
-// Bernstein evaluation with the control net held in VGPRs, and the Newton
+// Arithmetic of the fast trace forms: explicit FMA, tensor-product Bernstein
+// evaluation with control-net rows streamed from memory, and the Newton
 // iteration of the plain method (same seed handling, PCG perturbation on a
-// singular Jacobian, clamping) in that arithmetic.  Results agree with the
-// reference-order path to rounding; they are checked against the FP64
+// singular Jacobian, clamping) in that arithmetic; exact rejection tests; and
+// the FP64 pieces of mixed precision (the FP64 seed, the error estimate that
+// selects hits for the FP64 step, and the step itself).  Results agree with
+// the reference-order path to rounding; they are checked against the FP64
 // reference, not bit for bit.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -47,38 +49,6 @@ __device__ __forceinline__ void bernstein(float t, float (&b)[N + 1], float (&db
   }
 }
 
-// Homogeneous point and partials of a degree (P,Q) patch, net row-major
-// net[i*(Q+1)+j], streamed row by row from memory (L1/L2 resident; rays of a
-// wave that test the same patch share the cache lines).
-template <int P, int Q>
-__device__ __forceinline__ void eval_bern(const float4* __restrict__ net, float u, float v, float4& pt, float4& du,
-                                          float4& dv) {
-  float bu[P + 1], dbu[P + 1], bv[Q + 1], dbv[Q + 1];
-  bernstein<P>(u, bu, dbu);
-  bernstein<Q>(v, bv, dbv);
-  pt = make_float4(0.f, 0.f, 0.f, 0.f);
-  du = pt;
-  dv = pt;
-#pragma unroll
-  for (int i = 0; i <= P; i++) {
-    float4 r = make_float4(0.f, 0.f, 0.f, 0.f), rv = r;
-    const float4* row = net + i * (Q + 1);
-    asm volatile("" : "+v"(row));  // the row's loads are issued here, not hoisted with the whole net
-#pragma unroll
-    for (int j = 0; j <= Q; j++) {
-      const float4 c = row[j];
-      r.x = fmaf(bv[j], c.x, r.x); r.y = fmaf(bv[j], c.y, r.y); r.z = fmaf(bv[j], c.z, r.z); r.w = fmaf(bv[j], c.w, r.w);
-      rv.x = fmaf(dbv[j], c.x, rv.x); rv.y = fmaf(dbv[j], c.y, rv.y);
-      rv.z = fmaf(dbv[j], c.z, rv.z); rv.w = fmaf(dbv[j], c.w, rv.w);
-    }
-    pt.x = fmaf(bu[i], r.x, pt.x); pt.y = fmaf(bu[i], r.y, pt.y); pt.z = fmaf(bu[i], r.z, pt.z); pt.w = fmaf(bu[i], r.w, pt.w);
-    du.x = fmaf(dbu[i], r.x, du.x); du.y = fmaf(dbu[i], r.y, du.y);
-    du.z = fmaf(dbu[i], r.z, du.z); du.w = fmaf(dbu[i], r.w, du.w);
-    dv.x = fmaf(bu[i], rv.x, dv.x); dv.y = fmaf(bu[i], rv.y, dv.y);
-    dv.z = fmaf(bu[i], rv.z, dv.z); dv.w = fmaf(bu[i], rv.w, dv.w);
-  }
-}
-
 struct Plane2 { f3 n1, n2; float d1, d2; };
 
 // Ray planes (math_utils.h:33-54) in FMA arithmetic.
@@ -99,6 +69,7 @@ struct Hit {
   float u, v;
   f3 S, Su, Sv;
   float m0, m1, m2, m3;  // Jacobian at the returned iterate
+  float r0, r1;          // residual (ray-plane distances) at the returned iterate
   int evals;
 };
 
@@ -107,74 +78,6 @@ __device__ __forceinline__ void dehom(float4 pt, float4 du, float4 dv, f3& S, f3
   S = {pt.x * iw, pt.y * iw, pt.z * iw};
   Su = {fmaf(-S.x, du.w, du.x) * iw, fmaf(-S.y, du.w, du.y) * iw, fmaf(-S.z, du.w, du.z) * iw};
   Sv = {fmaf(-S.x, dv.w, dv.x) * iw, fmaf(-S.y, dv.w, dv.y) * iw, fmaf(-S.z, dv.w, dv.z) * iw};
-}
-
-// Newton iteration of the plain method (bezier_patch.h:198-266) on a net in
-// registers.  Converged when |r0|+|r1| < thr_res, returning the pre-update
-// iterate; on |det J| < thr_det the iterate is pulled towards the seed by the
-// PCG draws; iterates are clamped to [0,1]^2.
-template <int P, int Q>
-__device__ __forceinline__ Hit newton(const float4* __restrict__ net, const Plane2& rp, float u0, float v0,
-                                      float thr_det, float thr_res, uint32_t seed, int max_iter) {
-  float u = u0, v = v0;
-  Hit h;
-  h.hit = false;
-  h.evals = 0;
-#pragma unroll 1
-  for (int it = 1; it <= max_iter; it++) {
-    float4 pt, du, dv;
-    eval_bern<P, Q>(net, u, v, pt, du, dv);
-    h.evals = it;
-    f3 S, Su, Sv;
-    dehom(pt, du, dv, S, Su, Sv);
-    const float r0 = dot3(rp.n1, S) + rp.d1;
-    const float r1 = dot3(rp.n2, S) + rp.d2;
-    const float m0 = dot3(rp.n1, Su), m1 = dot3(rp.n2, Su), m2 = dot3(rp.n1, Sv), m3 = dot3(rp.n2, Sv);
-    if (fabsf(r0) + fabsf(r1) < thr_res) {
-      h.hit = true;
-      h.u = u; h.v = v; h.S = S; h.Su = Su; h.Sv = Sv;
-      h.m0 = m0; h.m1 = m1; h.m2 = m2; h.m3 = m3;
-      return h;
-    }
-    seed = pcg_hash(seed);
-    const float rand1 = (float)seed * 2.3283064e-10f;
-    seed = pcg_hash(seed);
-    const float rand2 = (float)seed * 2.3283064e-10f;
-    const float det = fmaf(m0, m3, -m2 * m1);
-    if (fabsf(det) < thr_det) {
-      u = fmaf(0.1f, (u0 - u) * rand1, u);
-      v = fmaf(0.1f, (v0 - v) * rand2, v);
-    } else {
-      const float id = 1.0f / det;
-      u = u - fmaf(m3, r0, -m2 * r1) * id;
-      v = v - fmaf(-m1, r0, m0 * r1) * id;
-    }
-    u = fminf(fmaxf(u, 0.0f), 1.0f);
-    v = fminf(fmaxf(v, 0.0f), 1.0f);
-  }
-  return h;
-}
-
-// Exact convex-hull rejection: with positive weights every surface point is a
-// convex combination of the dehomogenised control points, so if all of them
-// lie farther than `margin` on one side of either ray plane, no iterate can
-// reach a residual |r0|+|r1| below margin.  Tested as n.P.xyz + d*P.w against
-// margin*P.w to avoid the divisions.
-template <int P, int Q>
-__device__ __forceinline__ bool hull_culled(const float4* __restrict__ net, const Plane2& rp, float margin) {
-  bool pos1 = true, neg1 = true, pos2 = true, neg2 = true;
-#pragma unroll
-  for (int k = 0; k < (P + 1) * (Q + 1); k++) {
-    const float4 c = net[k];
-    const float s1 = fmaf(rp.n1.x, c.x, fmaf(rp.n1.y, c.y, fmaf(rp.n1.z, c.z, rp.d1 * c.w)));
-    const float s2 = fmaf(rp.n2.x, c.x, fmaf(rp.n2.y, c.y, fmaf(rp.n2.z, c.z, rp.d2 * c.w)));
-    const float mw = margin * c.w;
-    pos1 = pos1 && (s1 > mw);
-    neg1 = neg1 && (s1 < -mw);
-    pos2 = pos2 && (s2 > mw);
-    neg2 = neg2 && (s2 < -mw);
-  }
-  return pos1 || neg1 || pos2 || neg2;
 }
 
 // 2-norm condition number of [m0 m2; m1 m3] from its Frobenius norm and
@@ -192,99 +95,20 @@ __device__ __forceinline__ float cond2(float m0, float m1, float m2, float m3) {
 namespace rtn {
 namespace fast {
 
-// FP64 correction of an ill-conditioned FP32 hit: one Newton step in FP64 from
-// the FP32 (u,v) (evaluation, residual, Jacobian solve, no clamping), then the
-// corrected point and partials.  Returns false if the FP64 Jacobian is singular.
-template <int N>
-__device__ __forceinline__ void bernstein64(double t, double (&b)[N + 1], double (&db)[N + 1]) {
-  const double s = 1.0 - t;
-  double c[N + 1];
-  c[0] = 1.0;
-#pragma unroll
-  for (int k = 1; k < N; k++) {
-    c[k] = t * c[k - 1];
-#pragma unroll
-    for (int i = k - 1; i >= 1; i--) c[i] = fma(s, c[i], t * c[i - 1]);
-    c[0] = s * c[0];
-  }
-#pragma unroll
-  for (int i = 0; i <= N; i++) {
-    const double lo = i > 0 ? c[i - 1] : 0.0;
-    const double hi = i < N ? c[i] : 0.0;
-    db[i] = (double)N * (lo - hi);
-    b[i] = fma(s, hi, t * lo);
-  }
-}
-
 struct D3 { double x, y, z; };
 
-template <int P, int Q>
-__device__ __forceinline__ void eval64(const float4* __restrict__ net, double u, double v, D3& S, D3& Su, D3& Sv) {
-  double bu[P + 1], dbu[P + 1], bv[Q + 1], dbv[Q + 1];
-  bernstein64<P>(u, bu, dbu);
-  bernstein64<Q>(v, bv, dbv);
-  double pt[4] = {0, 0, 0, 0}, du[4] = {0, 0, 0, 0}, dv[4] = {0, 0, 0, 0};
-#pragma unroll
-  for (int i = 0; i <= P; i++) {
-    double r[4] = {0, 0, 0, 0}, rv[4] = {0, 0, 0, 0};
-#pragma unroll
-    for (int j = 0; j <= Q; j++) {
-      const float4 c = net[i * (Q + 1) + j];
-      const double cc[4] = {c.x, c.y, c.z, c.w};
-#pragma unroll
-      for (int k = 0; k < 4; k++) {
-        r[k] = fma(bv[j], cc[k], r[k]);
-        rv[k] = fma(dbv[j], cc[k], rv[k]);
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      pt[k] = fma(bu[i], r[k], pt[k]);
-      du[k] = fma(dbu[i], r[k], du[k]);
-      dv[k] = fma(bu[i], rv[k], dv[k]);
-    }
-  }
-  const double iw = 1.0 / pt[3];
-  S = {pt[0] * iw, pt[1] * iw, pt[2] * iw};
-  Su = {(du[0] - S.x * du[3]) * iw, (du[1] - S.y * du[3]) * iw, (du[2] - S.z * du[3]) * iw};
-  Sv = {(dv[0] - S.x * dv[3]) * iw, (dv[1] - S.y * dv[3]) * iw, (dv[2] - S.z * dv[3]) * iw};
-}
-
-// One FP64 Newton step on the FP64 ray planes from (u,v), then the corrected
-// point: the mixed-precision form's correction for cond(J) above threshold.
-template <int P, int Q>
-__device__ __noinline__ bool correct64(const float4* __restrict__ net, f3 o, f3 d, float& uo, float& vo,
-                                          f3& So, f3& Suo, f3& Svo, double& t_out) {
-  // FP64 ray planes from the FP32 ray
-  const double ox = o.x, oy = o.y, oz = o.z, dx = d.x, dy = d.y, dz = d.z;
-  const double ax = fabs(dx), ay = fabs(dy), az = fabs(dz);
-  const bool xd = (ax > ay) && (ax > az);
-  const double b0 = xd ? dx : dy, b1 = xd ? dy : dz;
-  const double den = 1.0 / sqrt(b0 * b0 + b1 * b1);
-  const D3 n1 = xd ? D3{dy * den, -dx * den, 0.0} : D3{0.0, dz * den, -dy * den};
-  const double il = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
-  const D3 dn{dx * il, dy * il, dz * il};
-  const D3 n2{n1.y * dn.z - n1.z * dn.y, n1.z * dn.x - n1.x * dn.z, n1.x * dn.y - n1.y * dn.x};
-  const double d1 = -(n1.x * ox + n1.y * oy + n1.z * oz), d2 = -(n2.x * ox + n2.y * oy + n2.z * oz);
-  double u = uo, v = vo;
-  D3 S, Su, Sv;
-  eval64<P, Q>(net, u, v, S, Su, Sv);
-  const double r0 = n1.x * S.x + n1.y * S.y + n1.z * S.z + d1;
-  const double r1 = n2.x * S.x + n2.y * S.y + n2.z * S.z + d2;
-  const double m0 = n1.x * Su.x + n1.y * Su.y + n1.z * Su.z, m1 = n2.x * Su.x + n2.y * Su.y + n2.z * Su.z;
-  const double m2 = n1.x * Sv.x + n1.y * Sv.y + n1.z * Sv.z, m3 = n2.x * Sv.x + n2.y * Sv.y + n2.z * Sv.z;
-  const double det = m0 * m3 - m2 * m1;
-  if (det == 0.0) return false;
-  u -= (m3 * r0 - m2 * r1) / det;
-  v -= (-m1 * r0 + m0 * r1) / det;
-  eval64<P, Q>(net, u, v, S, Su, Sv);
-  uo = (float)u;
-  vo = (float)v;
-  So = {(float)S.x, (float)S.y, (float)S.z};
-  Suo = {(float)Su.x, (float)Su.y, (float)Su.z};
-  Svo = {(float)Sv.x, (float)Sv.y, (float)Sv.z};
-  t_out = (S.x - ox) * dx + (S.y - oy) * dy + (S.z - oz) * dz;
-  return true;
+// Distance change along the ray that one more Newton step from an accepted
+// iterate would make: delta = J^-1 r in (u,v), then d . (S_u du + S_v dv).
+// The accepted (pre-update) iterate is off the root by about this much, so
+// |dt| / t predicts the FP32 hit's relative t error; infinite for a singular J.
+__device__ __forceinline__ float predicted_dt(const Hit& h, f3 d) {
+  const float det = fmaf(h.m0, h.m3, -h.m2 * h.m1);
+  if (det == 0.0f) return INFINITY;
+  const float id = 1.0f / det;
+  const float du = fmaf(h.m3, h.r0, -h.m2 * h.r1) * id;
+  const float dv = fmaf(-h.m1, h.r0, h.m0 * h.r1) * id;
+  const f3 dS{fmaf(h.Su.x, du, h.Sv.x * dv), fmaf(h.Su.y, du, h.Sv.y * dv), fmaf(h.Su.z, du, h.Sv.z * dv)};
+  return fabsf(dot3(d, dS));
 }
 
 }  // namespace fast
@@ -387,6 +211,7 @@ __device__ __forceinline__ Hit newton_q(NP net, int P, const Plane2& rp, float u
       h.hit = true;
       h.u = u; h.v = v; h.S = S; h.Su = Su; h.Sv = Sv;
       h.m0 = m0; h.m1 = m1; h.m2 = m2; h.m3 = m3;
+      h.r0 = r0; h.r1 = r1;
       return h;
     }
     // two PCG draws per iteration, as the plain method; the state is only
@@ -419,35 +244,62 @@ __device__ __forceinline__ Hit newton_q(NP net, int P, const Plane2& rp, float u
   return h;
 }
 
-// Bernstein polynomial B_i^n(t) in FP64 from its closed form C(n,i) t^i (1-t)^(n-i),
-// evaluated with scalar loops (no arrays, so the rare FP64 step adds no
-// register arrays to the kernel); zero outside 0 <= i <= n.
-__device__ __forceinline__ double bern64(int n, int i, double t) {
-  if (i < 0 || i > n) return 0.0;
-  double c = 1.0, r = 1.0;
-  for (int k = 1; k <= i; k++) c = c * (double)(n - i + k) / (double)k;
-  for (int k = 0; k < i; k++) r *= t;
-  for (int k = 0; k < n - i; k++) r *= (1.0 - t);
-  return c * r;
+// Bernstein basis of runtime degree n <= MAXD at t and its derivative in FP64
+// (the de Casteljau triangle of bernstein_rt), entries above n zero.
+__device__ __forceinline__ void bernstein64_rt(int n, double t, double (&b)[MAXD + 1], double (&db)[MAXD + 1]) {
+  const double s = 1.0 - t;
+  double c[MAXD + 1];
+#pragma unroll
+  for (int i = 0; i <= MAXD; i++) c[i] = 0.0;
+  c[0] = 1.0;
+#pragma unroll
+  for (int k = 1; k < MAXD; k++) {
+    if (k < n) {
+      c[k] = t * c[k - 1];
+#pragma unroll
+      for (int i = k - 1; i >= 1; i--) c[i] = fma(s, c[i], t * c[i - 1]);
+      c[0] = s * c[0];
+    }
+  }
+  const double dn = (double)n;
+#pragma unroll
+  for (int i = 0; i <= MAXD; i++) {
+    const double lo = i > 0 ? c[i - 1] : 0.0;
+    const double hi = i < n ? c[i] : 0.0;
+    db[i] = i <= n ? dn * (lo - hi) : 0.0;
+    b[i] = i <= n ? fma(s, hi, t * lo) : 0.0;
+  }
 }
 
-// FP64 point and partials, runtime degrees (rows and columns streamed).
+// FP64 point and partials, runtime degrees: the tensor-product Bernstein sum
+// with both bases formed once per evaluation.
 template <class NP = const float4*>
 __device__ __forceinline__ void eval64_rt(NP net, int P, int Q, double u, double v, D3& S,
                                           D3& Su, D3& Sv) {
+  double bu[MAXD + 1], dbu[MAXD + 1], bv[MAXD + 1], dbv[MAXD + 1];
+  bernstein64_rt(P, u, bu, dbu);
+  bernstein64_rt(Q, v, bv, dbv);
   double pt[4] = {0, 0, 0, 0}, du[4] = {0, 0, 0, 0}, dv[4] = {0, 0, 0, 0};
-  for (int i = 0; i <= P; i++) {
-    const double bu = bern64(P, i, u), dbu = P * (bern64(P - 1, i - 1, u) - bern64(P - 1, i, u));
-    for (int j = 0; j <= Q; j++) {
-      const double bv = bern64(Q, j, v), dbv = Q * (bern64(Q - 1, j - 1, v) - bern64(Q - 1, j, v));
+#pragma unroll
+  for (int i = 0; i <= MAXD; i++) {
+    if (i > P) break;
+    double r[4] = {0, 0, 0, 0}, rv[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int j = 0; j <= MAXD; j++) {
+      if (j > Q) break;
       const float4 c = ld4(net, i * (Q + 1) + j);
       const double cc[4] = {c.x, c.y, c.z, c.w};
 #pragma unroll
       for (int k = 0; k < 4; k++) {
-        pt[k] = fma(bu * bv, cc[k], pt[k]);
-        du[k] = fma(dbu * bv, cc[k], du[k]);
-        dv[k] = fma(bu * dbv, cc[k], dv[k]);
+        r[k] = fma(bv[j], cc[k], r[k]);
+        rv[k] = fma(dbv[j], cc[k], rv[k]);
       }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      pt[k] = fma(bu[i], r[k], pt[k]);
+      du[k] = fma(dbu[i], r[k], du[k]);
+      dv[k] = fma(bu[i], rv[k], dv[k]);
     }
   }
   const double iw = 1.0 / pt[3];
@@ -457,10 +309,13 @@ __device__ __forceinline__ void eval64_rt(NP net, int P, int Q, double u, double
 }
 
 // One FP64 Newton step from the FP32 (u,v) on the FP64 ray planes, then the
-// corrected point: the mixed-precision form's step for cond(J) > threshold.
+// corrected point: the mixed-precision form's correction of a flagged hit.
+// The step is kept only when it stays in the patch's domain [0,1]^2 (the
+// plain method clamps its iterates there) and lowers the FP64 residual
+// |r0| + |r1|; otherwise the FP32 hit stands and the call returns false.
 template <class NP = const float4*>
 __device__ __forceinline__ bool correct64_rt(NP net, int P, int Q, f3 o, f3 d, float& uo,
-                                          float& vo, f3& So, f3& Suo, f3& Svo, float& t_out) {
+                                             float& vo, f3& So, f3& Suo, f3& Svo, float& t_out) {
   const double ox = o.x, oy = o.y, oz = o.z, dx = d.x, dy = d.y, dz = d.z;
   const double ax = fabs(dx), ay = fabs(dy), az = fabs(dz);
   const bool xd = (ax > ay) && (ax > az);
@@ -482,7 +337,11 @@ __device__ __forceinline__ bool correct64_rt(NP net, int P, int Q, f3 o, f3 d, f
   if (det == 0.0) return false;
   u -= (m3 * r0 - m2 * r1) / det;
   v -= (-m1 * r0 + m0 * r1) / det;
+  if (!(u >= 0.0 && u <= 1.0 && v >= 0.0 && v <= 1.0)) return false;
   eval64_rt(net, P, Q, u, v, S, Su, Sv);
+  const double s0 = n1.x * S.x + n1.y * S.y + n1.z * S.z + d1;
+  const double s1 = n2.x * S.x + n2.y * S.y + n2.z * S.z + d2;
+  if (!(fabs(s0) + fabs(s1) < fabs(r0) + fabs(r1))) return false;
   uo = (float)u;
   vo = (float)v;
   So = {(float)S.x, (float)S.y, (float)S.z};
@@ -490,6 +349,30 @@ __device__ __forceinline__ bool correct64_rt(NP net, int P, int Q, f3 o, f3 d, f
   Svo = {(float)Sv.x, (float)Sv.y, (float)Sv.z};
   t_out = (float)((S.x - ox) * dx + (S.y - oy) * dy + (S.z - oz) * dz);
   return true;
+}
+
+// The plain bilinear seed (bezier_patch.h:38-44, :273-327) evaluated in FP64,
+// exactly as the FP64 reference trace computes it: the patch's FP64 terms
+// (corners dehomogenised in FP64, their differences; formed once per patch at
+// scene build, ex::seed_terms) and the per-ray solve of the reference
+// (ex::seed_solve, T = double), rounded to FP32.  The seed's accept test runs
+// on rounding noise (its GARP cross product is missing), so an FP32
+// evaluation starts Newton from (0.5, 0.5) or (u_root, ~0) by the sign of
+// FP32 noise, and FP64 by the sign of FP64 noise; mixed precision starts
+// where the FP64 reference starts.
+template <class DP = const double*>
+__device__ __forceinline__ void seed64(DP t, f3 o, f3 d, float& su, float& sv) {
+  ex::SeedTerms<double> s;
+  s.q00 = {t[0], t[1], t[2]};
+  s.q10 = {t[3], t[4], t[5]};
+  s.e00 = {t[6], t[7], t[8]};
+  s.e11 = {t[9], t[10], t[11]};
+  s.qn = {t[12], t[13], t[14]};
+  double gu, gv;
+  ex::seed_solve<double>(s, {(double)o.x, (double)o.y, (double)o.z}, {(double)d.x, (double)d.y, (double)d.z}, gu,
+                         gv);
+  su = (float)gu;
+  sv = (float)gv;
 }
 
 }  // namespace fast

@@ -103,6 +103,21 @@ __host__ __device__ inline void tiled_pixel(int64_t p, int W, int& x, int& y) {
   y = (int)(tile / tiles_x) * 8 + my;
 }
 
+// FP64 patch terms of the bilinear seed per patch (q00, q10, e00, e11, qn, pad).
+constexpr int kSeedTermWords = 16;
+
+// Device control words of one trace: the persistent forms' work queues, the
+// trace totals (stack-overflow count at [6]) and a flag word.  A scene and a
+// batch each own one, allocated when they are built, so a trace allocates
+// nothing; traces that overlap on one scene or batch share it and race.
+constexpr int kCtlWorkWords = 128;
+struct TraceCtl {
+  unsigned int work[kCtlWorkWords];
+  unsigned long long totals[12];
+  int flag;
+  int pad[3];
+};
+
 // Training-pair output of the wide kernels (converged plain-method solves).
 struct PairOut {
   int* ray;
@@ -136,7 +151,9 @@ struct rtn_scene {
   rtn::DevBuf<float> delta;        // patchDelta, BVH order
   rtn::DevBuf<float4> corners;     // dehomogenised corners q00,q10,q11,q01 (xyz) packed 3 float4
   rtn::DevBuf<float4> obb;         // oriented box of the dehomogenised net: (axis k, half extent k) x3, (centre, 0)
+  rtn::DevBuf<double> seed64;      // FP64 patch terms of the bilinear seed (kSeedTermWords per patch)
   rtn::DevBuf<rtn_bvh_node> nodes; // reference layout
   rtn::DevBuf<rtn::WideNode> wide; // 4-wide SoA nodes
+  rtn::DevBuf<rtn::TraceCtl> ctl;  // control words of its traces (allocated at build)
   std::vector<int64_t> deg_hist;   // (RTN_MAX_DEGREE+1)^2
 };

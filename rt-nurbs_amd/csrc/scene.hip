@@ -169,6 +169,24 @@ __global__ void k_bounds(const float4* cp, const int* cp_off, const int* pq, int
   b[0] = mn0; b[1] = mn1; b[2] = mn2; b[3] = mx0; b[4] = mx1; b[5] = mx2;
 }
 
+// The patch terms of the bilinear seed (bezier_patch.h:273-286) in FP64, from
+// the corners dehomogenised in FP64 (bezier_patch.h:32-44), exactly as the
+// FP64 reference trace forms them: mixed precision starts Newton from the
+// FP64 reference's seed and needs only the per-ray part of it per test.
+// Layout per patch: q00, q10, e00, e11, qn (15 doubles) and one pad word.
+__global__ void k_seed_terms(const float4* cp, const int* cp_off, const int* pq, int np, double* out) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= np) return;
+  const int p = pq[g] & 0xff, q = (pq[g] >> 8) & 0xff, row = q + 1;
+  const float4* c = cp + cp_off[g];
+  using namespace ex;
+  const SeedTerms<double> t = seed_terms(dehom(ld4<double>(c[0])), dehom(ld4<double>(c[p * row])),
+                                         dehom(ld4<double>(c[p * row + q])), dehom(ld4<double>(c[q])));
+  const double w[kSeedTermWords] = {t.q00.x, t.q00.y, t.q00.z, t.q10.x, t.q10.y, t.q10.z, t.e00.x, t.e00.y,
+                                    t.e00.z, t.e11.x, t.e11.y, t.e11.z, t.qn.x,  t.qn.y,  t.qn.z,  0.0};
+  for (int k = 0; k < kSeedTermWords; k++) out[(int64_t)kSeedTermWords * g + k] = w[k];
+}
+
 // Seed corners (bezier_patch.h:38-44) and patchDelta (main.cpp:18-43); sets
 // bit 16 of pq when every weight of the net is positive (the condition under
 // which the projected-hull rejection of the fast forms is exact).
@@ -963,6 +981,10 @@ extern "C" int rtn_scene_build_ex(rtn_scene* s, const rtn_build_params* bp) {
   hipLaunchKernelGGL(k_prep, dim3(nblk(np, 256)), dim3(256), 0, nullptr, s->cp.ptr, s->cp_off.ptr, s->pq.ptr, np,
                      s->corners.ptr, s->delta.ptr);
   RTN_HIP(hipGetLastError());
+  RTN_TRY(s->seed64.alloc(kSeedTermWords * (size_t)np));
+  hipLaunchKernelGGL(k_seed_terms, dim3(nblk(np, 256)), dim3(256), 0, nullptr, s->cp.ptr, s->cp_off.ptr, s->pq.ptr,
+                     np, s->seed64.ptr);
+  RTN_HIP(hipGetLastError());
   RTN_TRY(s->obb.alloc(4 * (size_t)np));
   hipLaunchKernelGGL(k_obb, dim3(nblk(np, 256)), dim3(256), 0, nullptr, s->cp.ptr, s->cp_off.ptr, s->pq.ptr, np,
                      s->obb.ptr);
@@ -982,6 +1004,7 @@ extern "C" int rtn_scene_build_ex(rtn_scene* s, const rtn_build_params* bp) {
   s->ms_decomp = std::chrono::duration<double, std::milli>(t1 - t0).count();
   s->ms_bvh = std::chrono::duration<double, std::milli>(t2 - t1).count();
   s->ms_wide = std::chrono::duration<double, std::milli>(t3 - t2).count();
+  if (!s->ctl.ptr) RTN_TRY(s->ctl.alloc(1));
   s->built = true;
   return RTN_OK;
 }

@@ -2,6 +2,7 @@
 #include <cstring>
 
 #include "rtn_internal.hpp"
+#include "trace_wide.hpp"
 
 namespace rtn {
 int trace_exact_scene(const rtn_scene* s, const float* const rays[6], int64_t n, const rtn_trace_params* prm,
@@ -43,18 +44,21 @@ extern "C" int rtn_trace(const rtn_scene* s, const float* const rays[6], int64_t
   RTN_CHECK(!prm->count || hits->counters, RTN_EINVAL, "rtn_trace: count requested without a counter buffer");
   if (n_rays == 0) return RTN_OK;
   hipStream_t st = (hipStream_t)stream;
-  unsigned long long* d_tot = nullptr;
-  if (totals) {
-    RTN_HIP(hipMallocAsync((void**)&d_tot, 12 * sizeof(unsigned long long), st));
-    RTN_HIP(hipMemsetAsync(d_tot, 0, 12 * sizeof(unsigned long long), st));
-  }
+  // the scene's control block: work queues and totals, zeroed by a kernel;
+  // the kernels always add their stack-overflow count to totals[6]
+  TraceCtl* ctl = s->ctl.ptr;
+  RTN_TRY(ctl_reset(ctl, st));
+  unsigned long long* d_tot = ctl->totals;
   int rc;
   switch (prm->mode) {
     case RTN_MODE_EXACT:
       RTN_CHECK(prm->seed_mode == RTN_SEED_PLAIN && prm->precision == RTN_PREC_FP32 && !prm->hull_cull &&
                     !prm->any_hit && !prm->skip_patch,
                 RTN_EUNSUPPORTED, "rtn_trace: the exact form runs the plain method only");
-      rc = trace_exact_scene(s, rays, n_rays, prm, hits, d_tot, st);
+      // the binary traversal pushes at most one entry per level, so with the
+      // 32-entry stack (trees of depth < 32) it cannot overflow and the plain
+      // kernel runs without the totals reduction unless they are asked for
+      rc = trace_exact_scene(s, rays, n_rays, prm, hits, totals || s->max_depth >= 32 ? d_tot : nullptr, st);
       break;
     case RTN_MODE_WIDE:
     case RTN_MODE_PERSISTENT:
@@ -68,20 +72,27 @@ extern "C" int rtn_trace(const rtn_scene* s, const float* const rays[6], int64_t
       set_error("rtn_trace: unknown mode %d", prm->mode);
       rc = RTN_EINVAL;
   }
-  if (totals) {
-    if (rc == RTN_OK) {
-      unsigned long long h[12];
-      RTN_HIP(hipMemcpyAsync(h, d_tot, sizeof(h), hipMemcpyDeviceToHost, st));
-      RTN_HIP(hipStreamSynchronize(st));
-      memcpy(totals, h, sizeof(h));
-      if (h[6]) {
-        set_error("rtn_trace: traversal stack overflowed on %llu rays", h[6]);
-        rc = RTN_EOVERFLOW;
-      }
+  if (rc != RTN_OK) return rc;
+  // totals requested or a synchronous call: read the totals back and report a
+  // stack overflow; an asynchronous call leaves that to rtn_scene_trace_overflow
+  if (totals || !st) {
+    unsigned long long h[12];
+    RTN_HIP(hipMemcpyAsync(h, d_tot, sizeof(h), hipMemcpyDeviceToHost, st));
+    RTN_HIP(hipStreamSynchronize(st));
+    if (totals) memcpy(totals, h, sizeof(h));
+    if (h[6]) {
+      set_error("rtn_trace: traversal stack overflowed on %llu rays", h[6]);
+      rc = RTN_EOVERFLOW;
     }
-    (void)hipFreeAsync(d_tot, st);
   }
   return rc;
+}
+
+extern "C" int rtn_scene_trace_overflow(const rtn_scene* s, unsigned long long* count) {
+  RTN_CHECK(s && s->built && count, RTN_EINVAL, "rtn_scene_trace_overflow: bad arguments");
+  RTN_HIP(hipDeviceSynchronize());
+  RTN_HIP(hipMemcpy(count, &s->ctl.ptr->totals[6], sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  return RTN_OK;
 }
 
 extern "C" int rtn_trace_pairs(const rtn_scene* s, const float* const rays[6], int64_t n_rays,
@@ -108,6 +119,7 @@ extern "C" int rtn_trace_pairs(const rtn_scene* s, const float* const rays[6], i
   rtn_hits h{};
   h.prim = prim.ptr;
   PairOut po{pair_ray, pair_patch, reinterpret_cast<float2*>(pair_uv), cnt.ptr, cap};
+  RTN_TRY(ctl_reset(s->ctl.ptr, st));
   RTN_TRY(trace_wide_scene(s, rays, n_rays, prm, &h, nullptr, st, &po));
   unsigned long long c = 0;
   RTN_HIP(hipMemcpyAsync(&c, cnt.ptr, sizeof(c), hipMemcpyDeviceToHost, st));

@@ -141,7 +141,7 @@ __device__ __forceinline__ void record(Lane& L, const WArgs& A, int pi, float t,
   L.t_max = t;
   L.best = pi;
   L.fix = fix;
-  const uint32_t r = L.r;
+  const int64_t r = L.r;
   if (A.t) A.t[r] = t;
   if (A.u) A.u[r] = u;
   if (A.v) A.v[r] = v;
@@ -163,7 +163,7 @@ struct LeafIn {
   float gu, gv, pu, pv;
   uint32_t seed;
   int max_iter, learned_iters, fallback_iters, mixed, learned, stall;
-  float thr_det, thr_res, cond_thr;
+  float thr_det, thr_res, cond_thr, t_tol;
 };
 
 // Leaf work for one patch with compile-time row length Q+1 and runtime row
@@ -190,10 +190,13 @@ __device__ __forceinline__ LeafOut leaf_q(NP net, int P, const LeafIn& in) {
   }
   if (!h.hit) return o;
   const float t = fmaf(h.S.x - in.o.x, in.d.x, fmaf(h.S.y - in.o.y, in.d.y, (h.S.z - in.o.z) * in.d.z));
-  // ill-conditioned hit (flag 16): if it ends as the ray's closest hit, the
-  // correction pass k_fix64 runs the FP64 step on it after the trace, so the
-  // FP64 code and its registers stay out of the trace kernels
-  if (in.mixed && cond2(h.m0, h.m1, h.m2, h.m3) > in.cond_thr) o.flags |= 16;
+  // FP64 step wanted (flag 16): an ill-conditioned hit (cond(J) above the
+  // threshold), or one whose predicted t error (the change one more Newton
+  // step would make) exceeds t_tol relative.  If it ends as the ray's closest
+  // hit, the correction pass k_fix64 runs the FP64 step on it after the trace,
+  // so the FP64 code and its registers stay out of the trace kernels.
+  if (in.mixed && (cond2(h.m0, h.m1, h.m2, h.m3) > in.cond_thr || predicted_dt(h, in.d) > in.t_tol * t))
+    o.flags |= 16;
   o.flags |= 1;
   o.t = t;
   o.u = h.u;
@@ -283,8 +286,32 @@ __device__ __forceinline__ void trace_wide_body(const WArgs& A) {
         }
       }
     }
-    // ---- traversal: advance every lane that has no pending leaf
-    while (has_ray && cand_left == 0) {
+    // ---- traversal: advance every lane that has no pending leaf.  The
+    // leaf's candidates are checked one at a time before Newton: the patch a
+    // secondary ray leaves is skipped, and with hull_cull each candidate is
+    // first tested against the oriented box, then the projected control points.
+    while (has_ray) {
+      if (cand_left > 0) {
+        bool drop = A.skip_prim && cand_first == s_skip[tid];
+        if (!drop && A.hull_cull) {
+          const int pqv = A.pq[cand_first];
+          if (pqv & 0x10000) {
+            drop = obb_culled(A.obb + 4 * (int64_t)cand_first, L.o, L.d, A.cull_margin);
+            if (!drop) {
+              const int np = ((pqv & 0xff) + 1) * (((pqv >> 8) & 0xff) + 1);
+              drop = hull2d_culled(A.cp + A.cp_off[cand_first], np, L.rp, A.cull_margin, A.hull_cull == 2);
+            }
+            if (COUNT && drop) {
+              cn.ncull++;
+              cn.npt++;
+            }
+          }
+        }
+        if (!drop) break;
+        cand_first++;
+        cand_left--;
+        continue;
+      }
       if (cur >= 0) {
         expand<WB, WSTK, COUNT>(L, cn, A.nodes, cur, s_code, s_te, tid, off);
         cur = -1;
@@ -310,29 +337,9 @@ __device__ __forceinline__ void trace_wide_body(const WArgs& A) {
         cand_first = code & 0xffffff;
         cand_left = ((code >> 24) & 0x7f) + 1;
         if (COUNT) cn.nleaf++;
-        if (A.hull_cull) {  // reject from the oriented box, then the projected control points, before Newton
-          while (cand_left > 0) {
-            const int pqv = A.pq[cand_first];
-            if (!(pqv & 0x10000)) break;
-            if (!obb_culled(A.obb + 4 * (int64_t)cand_first, L.o, L.d, A.cull_margin)) {
-              const int np = ((pqv & 0xff) + 1) * (((pqv >> 8) & 0xff) + 1);
-              if (!hull2d_culled(A.cp + A.cp_off[cand_first], np, L.rp, A.cull_margin, A.hull_cull == 2)) break;
-            }
-            if (COUNT) {
-              cn.ncull++;
-              cn.npt++;
-            }
-            cand_first++;
-            cand_left--;
-          }
-        }
       } else {
         cur = code;
       }
-    }
-    if (A.skip_prim && cand_left > 0 && cand_first == s_skip[tid]) {  // the patch the ray leaves is not tested
-      cand_first++;
-      cand_left--;
     }
     const bool pending = cand_left > 0;
     if (!__ballot(pending)) {
@@ -352,12 +359,15 @@ __device__ __forceinline__ void trace_wide_body(const WArgs& A) {
       p = pqv & 0xff;
       q = (pqv >> 8) & 0xff;
       cls = degree_class(p, q);
-      // the reference's bilinear seed on the dehomogenised corners (bezier_patch.h:38-44, :273-327)
-      c0 = A.corners[3 * pi];
-      c1 = A.corners[3 * pi + 1];
-      c2 = A.corners[3 * pi + 2];
-      ex::bilinear_seed<float>({c0.x, c0.y, c0.z}, {c0.w, c1.x, c1.y}, {c1.z, c1.w, c2.x}, {c2.y, c2.z, c2.w},
-                               {L.o.x, L.o.y, L.o.z}, {L.d.x, L.d.y, L.d.z}, gu, gv);
+      if (A.mixed) {  // the FP64 reference's seed
+        seed64(A.seed64 + (int64_t)kSeedTermWords * pi, L.o, L.d, gu, gv);
+      } else {  // the reference's bilinear seed on the dehomogenised corners (bezier_patch.h:38-44, :273-327)
+        c0 = A.corners[3 * pi];
+        c1 = A.corners[3 * pi + 1];
+        c2 = A.corners[3 * pi + 2];
+        ex::bilinear_seed<float>({c0.x, c0.y, c0.z}, {c0.w, c1.x, c1.y}, {c1.z, c1.w, c2.x}, {c2.y, c2.z, c2.w},
+                                 {L.o.x, L.o.y, L.o.z}, {L.d.x, L.d.y, L.d.z}, gu, gv);
+      }
     }
     LeafIn in;
     in.rp = L.rp;
@@ -403,6 +413,7 @@ __device__ __forceinline__ void trace_wide_body(const WArgs& A) {
     in.thr_det = A.thr_det;
     in.thr_res = A.thr_res;
     in.cond_thr = A.cond_thr;
+    in.t_tol = A.t_tol;
     const float4* net = A.cp + A.cp_off[pi];
     LeafOut res;
     res.flags = 0;
@@ -599,13 +610,17 @@ __device__ __forceinline__ void packet_chunk(const WArgs& A, int64_t r0, unsigne
         res.flags = 0;
         LeafIn in;
         if (act) {
-          const cptr<float4> cn4 = as_const(A.corners + 3 * (int64_t)pi);
-          const float4 c0 = ld4(cn4, 0), c1 = ld4(cn4, 1), c2 = ld4(cn4, 2);
           in.rp = L.rp;
           in.o = L.o;
           in.d = L.d;
-          ex::bilinear_seed<float>({c0.x, c0.y, c0.z}, {c0.w, c1.x, c1.y}, {c1.z, c1.w, c2.x}, {c2.y, c2.z, c2.w},
-                                   {L.o.x, L.o.y, L.o.z}, {L.d.x, L.d.y, L.d.z}, in.gu, in.gv);
+          if (A.mixed) {  // the FP64 reference's seed
+            seed64(as_const(A.seed64 + (int64_t)kSeedTermWords * pi), L.o, L.d, in.gu, in.gv);
+          } else {
+            const cptr<float4> cn4 = as_const(A.corners + 3 * (int64_t)pi);
+            const float4 c0 = ld4(cn4, 0), c1 = ld4(cn4, 1), c2 = ld4(cn4, 2);
+            ex::bilinear_seed<float>({c0.x, c0.y, c0.z}, {c0.w, c1.x, c1.y}, {c1.z, c1.w, c2.x}, {c2.y, c2.z, c2.w},
+                                     {L.o.x, L.o.y, L.o.z}, {L.d.x, L.d.y, L.d.z}, in.gu, in.gv);
+          }
           in.pu = in.gu;
           in.pv = in.gv;
         }
@@ -645,6 +660,7 @@ __device__ __forceinline__ void packet_chunk(const WArgs& A, int64_t r0, unsigne
           in.thr_det = A.thr_det;
           in.thr_res = A.thr_res;
           in.cond_thr = A.cond_thr;
+          in.t_tol = A.t_tol;
           res = leaf_dispatch(degree_class(p, q), net, p, in);  // the patch's degree: a wave-uniform branch
           if (COUNT) {
             cn.nc++;
@@ -732,8 +748,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3)))
   packet_body<PERSISTENT, COUNT, true>(A);
 }
 
-// Zeroes n words (the persistent forms' work queues) with a kernel, so a
-// trace recorded into a graph holds no memset node.
+// Zeroes a trace control block with a kernel, so a trace recorded into a
+// graph holds no memset node.
 __global__ void k_zero_words(unsigned int* p, int n) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
 }
@@ -874,6 +890,7 @@ int fill_wide_args(const rtn_scene* s, const float* const rays[6], int64_t n, co
   A.pq = s->pq.ptr;
   A.corners = s->corners.ptr;
   A.obb = s->obb.ptr;
+  A.seed64 = s->seed64.ptr;
   A.ox = rays[0]; A.oy = rays[1]; A.oz = rays[2]; A.dx = rays[3]; A.dy = rays[4]; A.dz = rays[5];
   A.n_rays = n;
   A.max_iter = prm->max_iter;
@@ -884,6 +901,7 @@ int fill_wide_args(const rtn_scene* s, const float* const rays[6], int64_t n, co
   A.cull_margin = 2.0f * prm->thr_res + 1e-5f;
   A.mixed = prm->precision == RTN_PREC_MIXED;
   A.cond_thr = prm->cond_thr;
+  A.t_tol = prm->t_tol > 0.f ? prm->t_tol : 2.5e-6f;
   A.any_hit = prm->any_hit;
   A.stall_exit = prm->stall_exit;
   A.refill_min = prm->refill_min > 0 ? prm->refill_min : 64;
@@ -916,18 +934,16 @@ int trace_wide_scene(const rtn_scene* s, const float* const rays[6], int64_t n, 
   WArgs A;
   RTN_TRY(fill_wide_args(s, rays, n, prm, hits, d_totals, pairs, A));
   const bool persistent = prm->mode == RTN_MODE_PERSISTENT;
-  // the persistent form's work queues live in a small stream-ordered block
-  unsigned int* d_work = nullptr;
-  if (persistent) {
-    RTN_HIP(hipMallocAsync((void**)&d_work, kWorkWords * sizeof(unsigned int), stream));
-    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(128), 0, stream, d_work, kWorkWords);
-    RTN_HIP(hipGetLastError());
-  }
-  A.work = d_work;
-  const int rc = launch_wide_kernel(kDefaultShape, persistent, prm->count != 0,
-                                    prm->seed_mode == RTN_SEED_LEARNED, n, stream, A);
-  if (d_work) RTN_HIP(hipFreeAsync(d_work, stream));
-  return rc;
+  A.work = s->ctl.ptr->work;  // zeroed by the caller's ctl_reset
+  return launch_wide_kernel(kDefaultShape, persistent, prm->count != 0, prm->seed_mode == RTN_SEED_LEARNED, n,
+                            stream, A);
+}
+
+int ctl_reset(TraceCtl* ctl, hipStream_t stream) {
+  hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(256), 0, stream, reinterpret_cast<unsigned int*>(ctl),
+                     (int)(sizeof(TraceCtl) / sizeof(unsigned int)));
+  RTN_HIP(hipGetLastError());
+  return RTN_OK;
 }
 
 int launch_fix64(int64_t n_max, hipStream_t stream, const WArgs& args) {
@@ -966,16 +982,8 @@ int trace_packet_scene(const rtn_scene* s, const float* const rays[6], int64_t n
   WArgs A;
   RTN_TRY(fill_wide_args(s, rays, n, prm, hits, d_totals, nullptr, A));
   A.bounds = s->bounds.ptr;
-  unsigned int* d_work = nullptr;
-  if (persistent) {
-    RTN_HIP(hipMallocAsync((void**)&d_work, kWorkWords * sizeof(unsigned int), stream));
-    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(128), 0, stream, d_work, kWorkWords);
-    RTN_HIP(hipGetLastError());
-  }
-  A.work = d_work;
-  const int rc = launch_packet_kernel(persistent, prm->count != 0, prm->seed_mode == RTN_SEED_LEARNED, n, stream, A);
-  if (d_work) RTN_HIP(hipFreeAsync(d_work, stream));
-  return rc;
+  A.work = s->ctl.ptr->work;  // zeroed by the caller's ctl_reset
+  return launch_packet_kernel(persistent, prm->count != 0, prm->seed_mode == RTN_SEED_LEARNED, n, stream, A);
 }
 
 }  // namespace rtn

@@ -27,6 +27,7 @@ constexpr int WB_ONE_RAY = 64;
 constexpr int kQueues = 8;
 constexpr int kQueueStride = 16;
 constexpr int kWorkWords = kQueues * kQueueStride;
+static_assert(kWorkWords <= kCtlWorkWords, "the work queues fit the control block");
 __host__ __device__ inline int64_t queue_ray(int q, uint32_t p) {
   return ((int64_t)q + (int64_t)kQueues * (p >> 6)) * 64 + (p & 63);
 }
@@ -39,6 +40,7 @@ struct WArgs {
   const int* pq;
   const float4* corners;  // dehomogenised corners q00,q10,q11,q01 (12 floats per patch)
   const float4* obb;      // oriented box of each net (4 float4 per patch)
+  const double* seed64;   // FP64 patch terms of the bilinear seed (kSeedTermWords per patch)
   const float *ox, *oy, *oz, *dx, *dy, *dz;
   int64_t n_rays;
   int max_iter;
@@ -51,6 +53,7 @@ struct WArgs {
   int stall_exit;         // abandon Newton solves stuck against the patch boundary
   int refill_min;         // persistent: lanes idle before a wave takes new rays
   float cond_thr;
+  float t_tol;            // mixed: predicted relative t error that also selects the FP64 step
   // learned form: BF16 MFMA tiles and biases of the seed network, patch boxes
   const mlp::bf16x8* mlp_tiles;
   const float* mlp_bias;
@@ -104,5 +107,7 @@ int launch_packet_kernel(bool persistent, bool count, bool learned, int64_t n_ma
 // args.mixed is set: the FP64 step of every ray whose closest hit is
 // ill-conditioned.
 int launch_fix64(int64_t n_max, hipStream_t stream, const WArgs& args);
+// Zeroes a trace control block with a kernel (no memset node when captured).
+int ctl_reset(TraceCtl* ctl, hipStream_t stream);
 
 }  // namespace rtn

@@ -40,6 +40,8 @@ EXPORTS = [
     "rtn_batch_patch_base", "rtn_batch_trace", "rtn_frame_params_default", "rtn_frame_create", "rtn_frame_destroy",
     "rtn_frame_run", "rtn_frame_capture", "rtn_frame_status", "rtn_frame_results", "rtn_frame_set_shape",
     "rtn_frame_autotune", "rtn_solve_pairs", "rtn_frame_set_refill", "rtn_frame_set_packet", "rtn_scene_save", "rtn_scene_load",
+    "rtn_scene_trace_overflow", "rtn_malloc_host", "rtn_free_host", "rtn_lookat", "rtn_camera_init",
+    "rtn_camera_get_perspective", "rtn_camera_get_orthographic", "rtn_rays_generate",
 ]
 BVH_EXACT_SAH, BVH_BINNED_SAH = 0, 1
 MLP_NIN, MLP_HIDDEN, MLP_NPARAM = 20, 64, 9794
@@ -64,7 +66,7 @@ class TraceParams(C.Structure):
                 ("hull_cull", C.c_int), ("count", C.c_int), ("learned_iters", C.c_int),
                 ("any_hit", C.c_int), ("refill_min", C.c_int), ("fallback_iters", C.c_int),
                 ("seed_share", C.c_int), ("stall_exit", C.c_int), ("seed_net", C.c_void_p),
-                ("skip_patch", C.c_void_p)]
+                ("skip_patch", C.c_void_p), ("t_tol", C.c_float)]
 
 
 class MlpTrainParams(C.Structure):
@@ -179,6 +181,14 @@ def load():
     L.rtn_trace_params_default.restype = None
     L.rtn_trace.argtypes = [vp, C.POINTER(vp * 6), C.c_int64, C.POINTER(TraceParams), C.POINTER(Hits),
                             C.POINTER(TraceTotals), vp]
+    L.rtn_scene_trace_overflow.argtypes = [vp, C.POINTER(C.c_ulonglong)]
+    L.rtn_lookat.argtypes = [vp, vp, vp, vp]
+    L.rtn_camera_init.argtypes = [vp, i, i, vp]
+    L.rtn_camera_get_perspective.argtypes = [vp, f, f, f, vp]
+    L.rtn_camera_get_orthographic.argtypes = [vp, f, f, vp]
+    L.rtn_rays_generate.argtypes = [C.POINTER(RayQueue), i, vp, vp]
+    L.rtn_malloc_host.argtypes = [C.POINTER(vp), sz]
+    L.rtn_free_host.argtypes = [vp]
     L.rtn_calculate_intersect.argtypes = [C.POINTER(RayQueue), vp, vp, vp, i, i, f, f, vp, vp, vp]
     L.rtn_get_patch.argtypes = [i, i, vp, i, i, vp, vp, C.POINTER(i), C.POINTER(i), vp, vp]
     L.rtn_build_bvh.argtypes = [vp, i, vp, vp, C.POINTER(i)]
@@ -421,6 +431,12 @@ class Scene:
             bp.bvh, bp.bins, bp.max_leaf = bvh, bins, max_leaf
             check(load().rtn_scene_build_ex(self.h, C.byref(bp)))
         return self
+
+    def trace_overflow(self):
+        """Stack-overflow count of the last trace on this scene (synchronises)."""
+        c = C.c_ulonglong(0)
+        check(load().rtn_scene_trace_overflow(self.h, C.byref(c)))
+        return int(c.value)
 
     def stats(self):
         st = SceneStats()

@@ -10,9 +10,10 @@ per-pixel trace (hit patch, t, u, v, dpdu, dpdv, Newton calls) of the
 intersect.cu:33-193 loop.  Inputs are stored beside the outputs so the tests
 need nothing else.  The files are data only (inputs and expected outputs).
 
-Usage: python tests/golden/make_golden.py
+Usage: python tests/golden/make_golden.py [camera,refine34]   (no argument: every fixture)
 """
 import ctypes as C
+import hashlib
 import os
 import subprocess
 import sys
@@ -74,11 +75,86 @@ def ref_scene(R, s, cam, crop=None):
     return out
 
 
+def ref_refine(R, s, C_u, C_v, room=4096):
+    """curvatureRefinementU then V (nurbs.h:290-486) in the reference; the
+    output buffers have `room` spare knots, and the reference's returned last
+    index trims them."""
+    Uo = np.zeros(s.n + room + s.p + 2, np.float32)
+    Po = np.zeros(((s.n + room + 1) * (s.m + 1), 4), np.float32)
+    nn = R.ref_refine_u(O.f32(s.U), s.p, s.n, s.m, C_u, O.f32(s.Pw), Uo.ctypes.data, Po.ctypes.data)
+    Uo, Po = Uo[:nn + s.p + 2].copy(), Po[:(nn + 1) * (s.m + 1)].copy()
+    Vo = np.zeros(s.m + room + s.q + 2, np.float32)
+    Po2 = np.zeros(((nn + 1) * (s.m + room + 1), 4), np.float32)
+    mm = R.ref_refine_v(O.f32(s.V), s.q, s.m, nn, C_v, O.f32(Po), Vo.ctypes.data, Po2.ctypes.data)
+    return nn, mm, Uo, Po, Vo[:mm + s.q + 2].copy(), Po2[:(nn + 1) * (mm + 1)].copy()
+
+
+def make_refine34(R):
+    """Refinement of a bicubic and a degree-4 surface (the reference's degree
+    cap, constants.h:31) at C = 1, 4 and 16."""
+    surfs = {"p3": scenes.random_surface(np.random.default_rng(43), deg_range=(3, 3), cp_range=(5, 6), box=4.0),
+             "p4": scenes.random_surface(np.random.default_rng(44), deg_range=(4, 4), cp_range=(6, 7), box=4.0)}
+    for name, s in surfs.items():
+        d = dict(U=s.U, V=s.V, Pw=s.Pw, nmpq=np.array([s.n, s.m, s.p, s.q], np.int32))
+        for Cc in (1.0, 4.0, 16.0):
+            nn, mm, Uo, Po, Vo, Po2 = ref_refine(R, s, Cc, Cc)
+            k = f"C{int(Cc)}"
+            d.update({f"{k}_nm": np.array([nn, mm], np.int32), f"{k}_U": Uo, f"{k}_V": Vo})
+            if Po2.size <= 65536:  # small nets in full
+                d[f"{k}_Pw"] = Po2
+            else:  # large nets: SHA-256 of the bytes and every 16th control point
+                d[f"{k}_Pw_sha256"] = np.frombuffer(hashlib.sha256(np.ascontiguousarray(Po2).tobytes()).digest(),
+                                                    np.uint8)
+                d[f"{k}_Pw_every16"] = Po2[::16].copy()
+            if Cc == 1.0:
+                d[f"{k}_Pw_mid"] = Po
+            print(f"refine {name} C={Cc:g}: n {s.n} -> {nn}, m {s.m} -> {mm}")
+        np.savez_compressed(os.path.join(HERE, f"refine_{name}.npz"), **d)
+
+
+def make_camera(R):
+    """The reference's camera_config after init_camera, getPerspectiveCamera
+    and getOrthographicCamera, and its perspective and orthographic rays, for
+    a landscape and a portrait raster."""
+    assert R.ref_camera_config_size() == O.CAMERA_CONFIG_BYTES
+    d = {}
+    for tag, (eye, look, up, W, H, fov, zn, zf) in {
+            "land": ((3.0, 3.0, 0.0), (0.0, 0.0, 6.0), (0.0, 0.5, 0.5), 40, 24, 60.0, 0.01, 1000.0),
+            "port": ((0.5, -2.0, 1.5), (0.5, 0.5, 0.0), (0.0, 0.0, 1.0), 18, 30, 45.0, 0.5, 80.0)}.items():
+        wfc = np.zeros(16, np.float32)
+        R.ref_lookat(*(np.asarray(x, np.float32) for x in (eye, look, up)), wfc)
+        cfgs = {}
+        for pname, proj in (("init", -1), ("persp", 1), ("ortho", 0)):
+            b = np.zeros(O.CAMERA_CONFIG_BYTES, np.uint8)
+            R.ref_camera_config(wfc, W, H, proj, fov, zn, zf, b.ctypes.data)
+            cfgs[pname] = b
+            d[f"{tag}_{pname}"] = b
+        rays = {}
+        for pname, fn in (("persp", R.ref_gen_rays), ("ortho", R.ref_gen_rays_ortho)):
+            cfr = cfgs[pname][6 * 64:7 * 64].view(np.float32).copy()
+            r = [np.zeros(W * H, np.float32) for _ in range(6)]
+            fn(wfc, cfr, W, H, *r)
+            d[f"{tag}_{pname}_rays"] = np.stack(r)
+        d[f"{tag}_in"] = np.array([*eye, *look, *up, W, H, fov, zn, zf], np.float32)
+        d[f"{tag}_wfc"] = wfc
+    np.savez_compressed(os.path.join(HERE, "camera.npz"), **d)
+    print("camera: configs and rays written")
+
+
 def main():
     subprocess.run(["bash", os.path.join(ROOT, "oracle", "build_ref.sh")], check=True)
     if not O.ref_available():
         sys.exit("the reference checkout is not present; nothing written")
     R = O.ref()
+    only = sys.argv[1].split(",") if len(sys.argv) > 1 else None
+    if only is not None:  # regenerate only the named groups (camera, refine34)
+        if "camera" in only:
+            make_camera(R)
+        if "refine34" in only:
+            make_refine34(R)
+        return
+    make_camera(R)
+    make_refine34(R)
     cases = {
         "c1_seed0": (scenes.bicubic_grid_surface(0), scenes.c1_camera(256, 256), (96, 96, 64, 64)),
         "c1_seed7": (scenes.bicubic_grid_surface(7), scenes.c1_camera(256, 256), (64, 128, 64, 64)),

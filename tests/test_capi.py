@@ -37,6 +37,29 @@ def test_camera_matches_oracle(oracle, rtl):
         assert_bits_equal(gc, cfr, "camera_from_raster")
 
 
+@pytest.mark.parametrize("tag", ["land", "port"])
+def test_camera_config_matches_reference(rtl, tag):
+    """rtn_lookat, rtn_camera_init, rtn_camera_get_perspective and
+    rtn_camera_get_orthographic (host-side) give the reference's camera_config
+    bytes (tests/golden/camera.npz, from the reference's camera.h)."""
+    L = rtl.load()
+    g = dict(np.load(os.path.join(ROOT, "tests", "golden", "camera.npz")))
+    x = g[f"{tag}_in"]
+    W, H, fov, zn, zf = int(x[9]), int(x[10]), float(x[11]), float(x[12]), float(x[13])
+    f3 = C.c_float * 3
+    wfc = np.zeros(16, np.float32)
+    assert L.rtn_lookat(f3(*x[0:3]), f3(*x[3:6]), f3(*x[6:9]), wfc.ctypes.data) == 0
+    assert_bits_equal(wfc, g[f"{tag}_wfc"], "lookAt")
+    init = np.zeros(472, np.uint8)
+    assert L.rtn_camera_init(wfc.ctypes.data, W, H, init.ctypes.data) == 0
+    assert np.array_equal(init, g[f"{tag}_init"])
+    for name, fn, args in (("persp", L.rtn_camera_get_perspective, (fov, zn, zf)),
+                           ("ortho", L.rtn_camera_get_orthographic, (zn, zf))):
+        out = np.zeros(472, np.uint8)
+        assert fn(init.ctypes.data, *[C.c_float(a) for a in args], out.ctypes.data) == 0
+        assert np.array_equal(out, g[f"{tag}_{name}"]), name
+
+
 @pytest.mark.parametrize("C_", [1.0, 4.0, 16.0])
 def test_refinement_matches_oracle(oracle, rtl, C_):
     for s in (scenes.torus(), scenes.sphere(), scenes.bicubic_grid_surface(2)):
