@@ -754,36 +754,61 @@ __global__ void k_zero_words(unsigned int* p, int n) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
 }
 
-// Mixed precision, after the trace: every ray whose closest hit was
-// ill-conditioned (cond2(J) > cond_thr, marked prim = -2 - patch by the trace)
-// gets one FP64 Newton step from its FP32 (u,v) on FP64 ray planes; t, u, v,
-// dpdu, dpdv are replaced and prim restored.  A pass over prim, so the trace
-// kernels carry no FP64 code.
-__global__ void __launch_bounds__(256) k_fix64(const WArgs A) {
-  unsigned long long done = 0;
-  const int64_t stride = (int64_t)gridDim.x * 256;
-  for (int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x; r < A.n_rays; r += stride) {
-    const int code = A.prim[r];
-    if (code > -2) continue;
-    const int pi = -2 - code;
-    const int pqv = A.pq[pi];
-    const int p = pqv & 0xff, q = (pqv >> 8) & 0xff;
-    const f3 o{A.ox[r], A.oy[r], A.oz[r]}, d{A.dx[r], A.dy[r], A.dz[r]};
-    float u = A.u[r], v = A.v[r], t;
-    f3 S, Su, Sv;
-    if (correct64_rt(A.cp + A.cp_off[pi], p, q, o, d, u, v, S, Su, Sv, t)) {
-      done++;
-      A.t[r] = t;
-      A.u[r] = u;
-      A.v[r] = v;
-      if (A.dpdu) { A.dpdu[3 * r] = Su.x; A.dpdu[3 * r + 1] = Su.y; A.dpdu[3 * r + 2] = Su.z; }
-      if (A.dpdv) { A.dpdv[3 * r] = Sv.x; A.dpdv[3 * r + 1] = Sv.y; A.dpdv[3 * r + 2] = Sv.z; }
-    }
-    A.prim[r] = pi;
+// Mixed precision, after the trace: every ray whose closest hit was flagged
+// (cond(J) above the threshold or predicted t error above t_tol; marked
+// prim = -2 - patch by the trace) gets one FP64 Newton step from its FP32
+// (u,v) on FP64 ray planes; t, u, v, dpdu, dpdv are replaced and prim
+// restored.  A pass over prim, so the trace kernels carry no FP64 code.
+// Flagged rays are rare and scattered: each block compacts the ones of its
+// 256-ray chunks into an LDS list and runs the FP64 steps 256 at a time, so a
+// step costs one lane's work instead of stalling a whole wave per ray.
+constexpr int kFixBlock = 256;
+
+__device__ __forceinline__ bool fix_one(const WArgs& A, int64_t r) {
+  const int pi = -2 - A.prim[r];
+  const int pqv = A.pq[pi];
+  const int p = pqv & 0xff, q = (pqv >> 8) & 0xff;
+  const f3 o{A.ox[r], A.oy[r], A.oz[r]}, d{A.dx[r], A.dy[r], A.dz[r]};
+  float u = A.u[r], v = A.v[r], t;
+  f3 S, Su, Sv;
+  const bool ok = correct64_rt(A.cp + A.cp_off[pi], p, q, o, d, u, v, S, Su, Sv, t);
+  if (ok) {
+    A.t[r] = t;
+    A.u[r] = u;
+    A.v[r] = v;
+    if (A.dpdu) { A.dpdu[3 * r] = Su.x; A.dpdu[3 * r + 1] = Su.y; A.dpdu[3 * r + 2] = Su.z; }
+    if (A.dpdv) { A.dpdv[3 * r] = Sv.x; A.dpdv[3 * r + 1] = Sv.y; A.dpdv[3 * r + 2] = Sv.z; }
   }
+  A.prim[r] = pi;
+  return ok;
+}
+
+__global__ void __launch_bounds__(kFixBlock) k_fix64(const WArgs A) {
+  __shared__ uint32_t s_list[2 * kFixBlock];
+  __shared__ int s_n;
+  const int tid = threadIdx.x;
+  if (tid == 0) s_n = 0;
+  __syncthreads();
+  unsigned long long done = 0;
+  const int64_t stride = (int64_t)gridDim.x * kFixBlock;
+  // every thread of the block runs the same number of chunk iterations
+  for (int64_t base = (int64_t)blockIdx.x * kFixBlock; base < A.n_rays; base += stride) {
+    const int64_t r = base + tid;
+    if (r < A.n_rays && A.prim[r] <= -2) s_list[atomicAdd(&s_n, 1)] = (uint32_t)r;  // at most 2*kFixBlock - 1
+    __syncthreads();
+    const int n = s_n;
+    if (n >= kFixBlock) {  // a full batch: one flagged ray per thread
+      if (fix_one(A, (int64_t)s_list[tid])) done++;
+      __syncthreads();
+      if (tid < n - kFixBlock) s_list[tid] = s_list[kFixBlock + tid];
+      if (tid == 0) s_n = n - kFixBlock;
+    }
+    __syncthreads();
+  }
+  if (tid < s_n && fix_one(A, (int64_t)s_list[tid])) done++;
   if (A.totals) {
     for (int sft = 32; sft > 0; sft >>= 1) done += __shfl_xor(done, sft);
-    if ((threadIdx.x & 63) == 0 && done) atomicAdd(A.totals + 8, done);
+    if ((tid & 63) == 0 && done) atomicAdd(A.totals + 8, done);
   }
 }
 
@@ -951,9 +976,9 @@ int launch_fix64(int64_t n_max, hipStream_t stream, const WArgs& args) {
   int dev = 0, cus = 256;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  const int64_t need = (n_max + 255) / 256;
+  const int64_t need = (n_max + kFixBlock - 1) / kFixBlock;
   const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)cus * 8));
-  hipLaunchKernelGGL(k_fix64, dim3(blocks), dim3(256), 0, stream, args);
+  hipLaunchKernelGGL(k_fix64, dim3(blocks), dim3(kFixBlock), 0, stream, args);
   RTN_HIP(hipGetLastError());
   return RTN_OK;
 }
