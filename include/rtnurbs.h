@@ -528,6 +528,20 @@ int rtn_calculate_intersect(const rtn_ray_queue* rays, const rtn_bvh_node* nodes
                             rtn_surface_interaction* interact, int W, int H, float flat_threshold, float v_fov,
                             const float* delta, unsigned long long* newton_count, unsigned long long* skip_count);
 
+/* rtn_calculate_intersect without any device allocation (no counterpart in the
+ * reference, whose wrapper has no such form): for callers that cannot
+ * allocate device memory (tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, results, synchronous behaviour and errors as
+ * rtn_calculate_intersect; W*H must fit the int idx_interact range.  Its
+ * scratch (the 8-word totals block: Newton and skip counts, stack overflow)
+ * is a file-scope device array of 64 bytes, one copy per GPU kept until the
+ * process ends, whatever the input size; the call zeroes it first.  Calls
+ * that overlap on one GPU (several host threads) share that array and race. */
+int rtn_calculate_intersect_noalloc(const rtn_ray_queue* rays, const rtn_bvh_node* nodes,
+                                    const rtn_bezier_patch* patches, rtn_surface_interaction* interact, int W, int H,
+                                    float flat_threshold, float v_fov, const float* delta,
+                                    unsigned long long* newton_count, unsigned long long* skip_count);
+
 /* getPatch (bezier_patch.h:87-139) with the A5.7 passes run on the GPU.
  * Host in, host out.  First call with cp == NULL to get nbU, nbV. */
 int rtn_get_patch(int n, int p, const float* U, int m, int q, const float* V, const float* Pw, int* nbU, int* nbV,

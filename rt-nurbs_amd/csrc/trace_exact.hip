@@ -11,7 +11,10 @@
 // -ffp-contract=off, so its results equal the host reference bit for bit.
 // T = double is the same traversal in FP64, used as the FP64 reference for
 // error reports (with its own iteration cap and tolerance).
+#include <climits>
 #include <cmath>
+#include <cstdint>
+#include <mutex>
 
 #include "rtn_exact.hpp"
 #include "rtn_internal.hpp"
@@ -317,13 +320,18 @@ extern "C" int rtn_trace_f64(const rtn_scene* s, const float* const rays[6], int
   return RTN_OK;
 }
 
-extern "C" int rtn_calculate_intersect(const rtn_ray_queue* rays, const rtn_bvh_node* nodes,
-                                       const rtn_bezier_patch* patches, rtn_surface_interaction* interact, int W,
-                                       int H, float flat_threshold, float v_fov, const float* delta,
-                                       unsigned long long* newton_count, unsigned long long* skip_count) {
-  RTN_CHECK(rays && nodes && patches && interact && W > 0 && H > 0, RTN_EINVAL,
-            "rtn_calculate_intersect: bad arguments");
-  RTN_CHECK(rays->idx_interact, RTN_EINVAL, "rtn_calculate_intersect: idx_interact is required");
+namespace {
+// The drop-in launch on reference-layout buffers with the caller's totals
+// block tot[8] (zeroed here by a kernel), then the reference's synchronous
+// wrap-up: counters added to, stack overflow reported.
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0ull;
+}
+
+int calculate_intersect(const rtn_ray_queue* rays, const rtn_bvh_node* nodes, const rtn_bezier_patch* patches,
+                        rtn_surface_interaction* interact, int W, int H, float flat_threshold, float v_fov,
+                        const float* delta, unsigned long long* newton_count, unsigned long long* skip_count,
+                        unsigned long long* tot) {
   ExactArgs<float> A{};
   A.nodes = reinterpret_cast<const float4*>(nodes);
   A.ox = rays->origin_x; A.oy = rays->origin_y; A.oz = rays->origin_z;
@@ -341,37 +349,93 @@ extern "C" int rtn_calculate_intersect(const rtn_ray_queue* rays, const rtn_bvh_
   A.seed_base = 0;
   A.inter = interact;
   A.idx_interact = rays->idx_interact;
-  unsigned long long* tot = nullptr;  // totals also carry the stack-overflow flag
-  RTN_HIP(hipMalloc(&tot, 8 * sizeof(unsigned long long)));
-  RTN_HIP(hipMemset(tot, 0, 8 * sizeof(unsigned long long)));
-  A.totals = tot;
+  A.totals = tot;  // the totals also carry the stack-overflow count
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, nullptr, tot, 8);
   DropInPatches P{patches};
   dim3 block(16, 8), grid((W + 15) / 16, (H + 7) / 8);
   hipLaunchKernelGGL((k_trace_exact<32, true, float, DropInPatches>), grid, block, 0, nullptr, A, P);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipDeviceSynchronize();
-  int rc = RTN_OK;
-  if (e != hipSuccess) rc = hip_fail(e, "calculateIntersect launch", __FILE__, __LINE__);
+  if (e != hipSuccess) return hip_fail(e, "calculateIntersect launch", __FILE__, __LINE__);
   unsigned long long h[8];
-  if (rc == RTN_OK && hipMemcpy(h, tot, sizeof(h), hipMemcpyDeviceToHost) == hipSuccess) {
-    // counters are added to, as the reference's atomicAdd does (intersect.cu:189-190)
-    unsigned long long* outs[2] = {newton_count, skip_count};
-    const unsigned long long adds[2] = {h[0], h[2]};
-    for (int k = 0; k < 2; k++) {
-      if (!outs[k]) continue;
-      unsigned long long cur;
-      if (hipMemcpy(&cur, outs[k], 8, hipMemcpyDefault) == hipSuccess) {
-        cur += adds[k];
-        (void)hipMemcpy(outs[k], &cur, 8, hipMemcpyDefault);
-      }
-    }
-    if (h[6]) {
-      set_error("calculateIntersect: traversal stack (32 entries) overflowed on %llu rays", h[6]);
-      rc = RTN_EOVERFLOW;
-    }
+  RTN_HIP(hipMemcpy(h, tot, sizeof(h), hipMemcpyDeviceToHost));
+  // counters are added to, as the reference's atomicAdd does (intersect.cu:189-190)
+  unsigned long long* outs[2] = {newton_count, skip_count};
+  const unsigned long long adds[2] = {h[0], h[2]};
+  for (int k = 0; k < 2; k++) {
+    if (!outs[k]) continue;
+    unsigned long long cur;
+    RTN_HIP(hipMemcpy(&cur, outs[k], 8, hipMemcpyDefault));
+    cur += adds[k];
+    RTN_HIP(hipMemcpy(outs[k], &cur, 8, hipMemcpyDefault));
   }
+  if (h[6]) {
+    set_error("calculateIntersect: traversal stack (32 entries) overflowed on %llu rays", h[6]);
+    return RTN_EOVERFLOW;
+  }
+  return RTN_OK;
+}
+
+int check_dropin_args(const rtn_ray_queue* rays, const rtn_bvh_node* nodes, const rtn_bezier_patch* patches,
+                      rtn_surface_interaction* interact, int W, int H, const char* who) {
+  RTN_CHECK(rays && nodes && patches && interact && W > 0 && H > 0, RTN_EINVAL, "%s: bad arguments", who);
+  RTN_CHECK(rays->idx_interact, RTN_EINVAL, "%s: idx_interact is required", who);
+  return RTN_OK;
+}
+
+// Scratch of rtn_calculate_intersect_noalloc: its totals block.  A file-scope
+// device array, one copy per GPU, held until the process ends; host code uses
+// the device address that hipGetSymbolAddress gives for the current device.
+constexpr int kNoAllocWords = 8;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned long long* g_scratch[kMaxDevices] = {};
+}  // namespace
+
+static __device__ __align__(256) unsigned long long scratch[kNoAllocWords];
+
+namespace {
+// The calling thread's current device's address of `scratch`, looked up once
+// per device.
+int scratch_address(unsigned long long** out) {
+  int dev = 0;
+  RTN_HIP(hipGetDevice(&dev));
+  RTN_CHECK(dev >= 0 && dev < kMaxDevices, RTN_EUNSUPPORTED, "scratch: device %d beyond %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    RTN_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch[dev] = static_cast<unsigned long long*>(p);
+  }
+  *out = g_scratch[dev];
+  return RTN_OK;
+}
+}  // namespace
+
+extern "C" int rtn_calculate_intersect(const rtn_ray_queue* rays, const rtn_bvh_node* nodes,
+                                       const rtn_bezier_patch* patches, rtn_surface_interaction* interact, int W,
+                                       int H, float flat_threshold, float v_fov, const float* delta,
+                                       unsigned long long* newton_count, unsigned long long* skip_count) {
+  RTN_TRY(check_dropin_args(rays, nodes, patches, interact, W, H, "rtn_calculate_intersect"));
+  unsigned long long* tot = nullptr;
+  RTN_HIP(hipMalloc(&tot, 8 * sizeof(unsigned long long)));
+  const int rc = calculate_intersect(rays, nodes, patches, interact, W, H, flat_threshold, v_fov, delta, newton_count,
+                                     skip_count, tot);
   (void)hipFree(tot);
   return rc;
+}
+
+extern "C" int rtn_calculate_intersect_noalloc(const rtn_ray_queue* rays, const rtn_bvh_node* nodes,
+                                               const rtn_bezier_patch* patches, rtn_surface_interaction* interact,
+                                               int W, int H, float flat_threshold, float v_fov, const float* delta,
+                                               unsigned long long* newton_count, unsigned long long* skip_count) {
+  RTN_TRY(check_dropin_args(rays, nodes, patches, interact, W, H, "rtn_calculate_intersect_noalloc"));
+  RTN_CHECK((int64_t)W * H <= INT32_MAX, RTN_EINVAL,
+            "rtn_calculate_intersect_noalloc: %d x %d pixels exceed the int idx_interact range", W, H);
+  unsigned long long* tot = nullptr;
+  RTN_TRY(scratch_address(&tot));
+  return calculate_intersect(rays, nodes, patches, interact, W, H, flat_threshold, v_fov, delta, newton_count,
+                             skip_count, tot);
 }
 
 // ------------------------------------------------------------ pair solves

@@ -42,6 +42,7 @@ EXPORTS = [
     "rtn_frame_autotune", "rtn_solve_pairs", "rtn_frame_set_refill", "rtn_frame_set_packet", "rtn_scene_save", "rtn_scene_load",
     "rtn_scene_trace_overflow", "rtn_malloc_host", "rtn_free_host", "rtn_lookat", "rtn_camera_init",
     "rtn_camera_get_perspective", "rtn_camera_get_orthographic", "rtn_rays_generate",
+    "rtn_calculate_intersect_noalloc",
 ]
 BVH_EXACT_SAH, BVH_BINNED_SAH = 0, 1
 MLP_NIN, MLP_HIDDEN, MLP_NPARAM = 20, 64, 9794
@@ -190,6 +191,7 @@ def load():
     L.rtn_malloc_host.argtypes = [C.POINTER(vp), sz]
     L.rtn_free_host.argtypes = [vp]
     L.rtn_calculate_intersect.argtypes = [C.POINTER(RayQueue), vp, vp, vp, i, i, f, f, vp, vp, vp]
+    L.rtn_calculate_intersect_noalloc.argtypes = [C.POINTER(RayQueue), vp, vp, vp, i, i, f, f, vp, vp, vp]
     L.rtn_get_patch.argtypes = [i, i, vp, i, i, vp, vp, C.POINTER(i), C.POINTER(i), vp, vp]
     L.rtn_build_bvh.argtypes = [vp, i, vp, vp, C.POINTER(i)]
     L.rtn_patch_bounds.argtypes = [vp, i, i, i, vp]
