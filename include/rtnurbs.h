@@ -259,10 +259,12 @@ enum {
   RTN_PREC_FP32 = 0,      /* FP32 Newton                                         */
   RTN_PREC_MIXED = 1      /* FP32 Newton started from the plain seed evaluated in FP64 (the FP64
                              reference's seed; the FP32 seed's accept test runs on rounding
-                             noise, see rtn_fast.hpp seed64); after the trace, one FP64 Newton
-                             step on each ray's closest hit whose cond(J) > cond_thr or whose
-                             predicted relative t error (the change one more Newton step would
-                             make) exceeds t_tol.  Fast forms; needs the t, u and v outputs   */
+                             noise, see rtn_fast.hpp seed64); after the trace, the FP64
+                             correction (up to four FP64 Newton steps, each kept only if it stays
+                             in [0,1]^2 and lowers the residual) on each ray's closest hit whose
+                             cond(J) > cond_thr or whose predicted relative t error (the change
+                             one more Newton step would make) exceeds t_tol.  Fast forms; needs
+                             the t, u and v outputs                                      */
 };
 
 typedef struct {

@@ -308,11 +308,17 @@ __device__ __forceinline__ void eval64_rt(NP net, int P, int Q, double u, double
   Sv = {(dv[0] - S.x * dv[3]) * iw, (dv[1] - S.y * dv[3]) * iw, (dv[2] - S.z * dv[3]) * iw};
 }
 
-// One FP64 Newton step from the FP32 (u,v) on the FP64 ray planes, then the
-// corrected point: the mixed-precision form's correction of a flagged hit.
-// The step is kept only when it stays in the patch's domain [0,1]^2 (the
+// The FP64 correction of a flagged FP32 hit: Newton steps in FP64 from the
+// FP32 (u,v) on FP64 ray planes (at most kFix64Steps; one from the usual FP32
+// accept, a few more from the far accepts of grazing rays, where the t-interval
+// within the residual tolerance is ~ thr / tan(angle)), then the corrected
+// point.  A step is kept only when it stays in the patch's domain [0,1]^2 (the
 // plain method clamps its iterates there) and lowers the FP64 residual
-// |r0| + |r1|; otherwise the FP32 hit stands and the call returns false.
+// |r0| + |r1|; the iteration ends at the first step that does not, or once
+// the residual is at FP64 rounding level.  Returns false (the FP32 hit stands)
+// when not even the first step is kept.
+constexpr int kFix64Steps = 4;
+
 template <class NP = const float4*>
 __device__ __forceinline__ bool correct64_rt(NP net, int P, int Q, f3 o, f3 d, float& uo,
                                              float& vo, f3& So, f3& Suo, f3& Svo, float& t_out) {
@@ -329,19 +335,29 @@ __device__ __forceinline__ bool correct64_rt(NP net, int P, int Q, f3 o, f3 d, f
   double u = uo, v = vo;
   D3 S, Su, Sv;
   eval64_rt(net, P, Q, u, v, S, Su, Sv);
-  const double r0 = n1.x * S.x + n1.y * S.y + n1.z * S.z + d1;
-  const double r1 = n2.x * S.x + n2.y * S.y + n2.z * S.z + d2;
-  const double m0 = n1.x * Su.x + n1.y * Su.y + n1.z * Su.z, m1 = n2.x * Su.x + n2.y * Su.y + n2.z * Su.z;
-  const double m2 = n1.x * Sv.x + n1.y * Sv.y + n1.z * Sv.z, m3 = n2.x * Sv.x + n2.y * Sv.y + n2.z * Sv.z;
-  const double det = m0 * m3 - m2 * m1;
-  if (det == 0.0) return false;
-  u -= (m3 * r0 - m2 * r1) / det;
-  v -= (-m1 * r0 + m0 * r1) / det;
-  if (!(u >= 0.0 && u <= 1.0 && v >= 0.0 && v <= 1.0)) return false;
-  eval64_rt(net, P, Q, u, v, S, Su, Sv);
-  const double s0 = n1.x * S.x + n1.y * S.y + n1.z * S.z + d1;
-  const double s1 = n2.x * S.x + n2.y * S.y + n2.z * S.z + d2;
-  if (!(fabs(s0) + fabs(s1) < fabs(r0) + fabs(r1))) return false;
+  double r0 = n1.x * S.x + n1.y * S.y + n1.z * S.z + d1;
+  double r1 = n2.x * S.x + n2.y * S.y + n2.z * S.z + d2;
+  double err = fabs(r0) + fabs(r1);
+  const double floor_err = 1e-14 * (1.0 + fabs(S.x) + fabs(S.y) + fabs(S.z));
+  bool moved = false;
+  for (int step = 0; step < kFix64Steps && err > floor_err; step++) {
+    const double m0 = n1.x * Su.x + n1.y * Su.y + n1.z * Su.z, m1 = n2.x * Su.x + n2.y * Su.y + n2.z * Su.z;
+    const double m2 = n1.x * Sv.x + n1.y * Sv.y + n1.z * Sv.z, m3 = n2.x * Sv.x + n2.y * Sv.y + n2.z * Sv.z;
+    const double det = m0 * m3 - m2 * m1;
+    if (det == 0.0) break;
+    const double un = u - (m3 * r0 - m2 * r1) / det;
+    const double vn = v - (-m1 * r0 + m0 * r1) / det;
+    if (!(un >= 0.0 && un <= 1.0 && vn >= 0.0 && vn <= 1.0)) break;
+    D3 Sn, Sun, Svn;
+    eval64_rt(net, P, Q, un, vn, Sn, Sun, Svn);
+    const double s0 = n1.x * Sn.x + n1.y * Sn.y + n1.z * Sn.z + d1;
+    const double s1 = n2.x * Sn.x + n2.y * Sn.y + n2.z * Sn.z + d2;
+    const double en = fabs(s0) + fabs(s1);
+    if (!(en < err)) break;
+    u = un; v = vn; S = Sn; Su = Sun; Sv = Svn; r0 = s0; r1 = s1; err = en;
+    moved = true;
+  }
+  if (!moved) return false;
   uo = (float)u;
   vo = (float)v;
   So = {(float)S.x, (float)S.y, (float)S.z};

@@ -756,9 +756,9 @@ __global__ void k_zero_words(unsigned int* p, int n) {
 
 // Mixed precision, after the trace: every ray whose closest hit was flagged
 // (cond(J) above the threshold or predicted t error above t_tol; marked
-// prim = -2 - patch by the trace) gets one FP64 Newton step from its FP32
-// (u,v) on FP64 ray planes; t, u, v, dpdu, dpdv are replaced and prim
-// restored.  A pass over prim, so the trace kernels carry no FP64 code.
+// prim = -2 - patch by the trace) gets the FP64 correction (correct64_rt:
+// guarded FP64 Newton steps from its FP32 (u,v) on FP64 ray planes); t, u, v,
+// dpdu, dpdv are replaced and prim restored.  A pass over prim, so the trace kernels carry no FP64 code.
 // Flagged rays are rare and scattered: each block compacts the ones of its
 // 256-ray chunks into an LDS list and runs the FP64 steps 256 at a time, so a
 // step costs one lane's work instead of stalling a whole wave per ray.

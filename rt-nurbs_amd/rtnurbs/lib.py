@@ -508,6 +508,23 @@ class HitBuffers:
             d["counters"] = self.counters.get()
         return d
 
+    def window(self, wins):
+        """hit, t, u, v of the index windows [(start, count), ...], concatenated."""
+        return {k: grab_windows(a, wins) for k, a in (("hit", self.prim), ("t", self.t), ("u", self.u),
+                                                      ("v", self.v))}
+
+
+def grab_windows(arr, wins):
+    """Host copy of index windows [(start, count), ...] of a 1-D DeviceArray."""
+    it = np.dtype(arr.dtype).itemsize
+    parts = []
+    for a, n in wins:
+        out = np.empty(n, arr.dtype)
+        if n:
+            check(load().rtn_memcpy(out.ctypes.data, arr.ptr + a * it, out.nbytes, 1))
+        parts.append(out)
+    return np.concatenate(parts) if parts else np.empty(0, arr.dtype)
+
 
 def trace(scene, rays, prm, hits=None, totals=True, stream=None):
     """Runs rtn_trace; returns (HitBuffers, totals dict or None)."""
@@ -859,6 +876,31 @@ class Frame:
                    parent=grab(par.value, (n2,), np.int32), kind=grab(knd.value, (n2,), np.int32),
                    secondary_rays=[grab(sr[k], (n2,), np.float32) for k in range(6)])
         return out
+
+    def sample(self, windows_primary, windows_secondary):
+        """Host copies of index windows [(start, count), ...] of the primary and
+        of the secondary outputs (hit, t, u, v; parent and kind for the
+        secondaries), concatenated, without downloading whole frames."""
+        h1, h2 = Hits(), Hits()
+        par, knd = C.c_void_p(), C.c_void_p()
+        sr = (C.c_void_p * 6)()
+        check(load().rtn_frame_results(self.h, C.byref(h1), C.byref(h2), C.byref(par), C.byref(knd), C.byref(sr)))
+
+        def grab(ptr, wins, dt):
+            parts = []
+            for a, n in wins:
+                out = np.empty(n, dt)
+                if n:
+                    check(load().rtn_memcpy(out.ctypes.data, ptr + a * out.itemsize, out.nbytes, 1))
+                parts.append(out)
+            return np.concatenate(parts) if parts else np.empty(0, dt)
+
+        def hits(h, wins):
+            return dict(hit=grab(h.prim, wins, np.int32), t=grab(h.t, wins, np.float32),
+                        u=grab(h.u, wins, np.float32), v=grab(h.v, wins, np.float32))
+        sec = hits(h2, windows_secondary)
+        sec.update(parent=grab(par.value, windows_secondary, np.int32), kind=grab(knd.value, windows_secondary, np.int32))
+        return dict(primary=hits(h1, windows_primary), secondary=sec)
 
     def __del__(self):
         try:

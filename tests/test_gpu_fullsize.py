@@ -5,9 +5,13 @@ jittered spp = 8,294,400 rays), through size-independent properties:
 - the packet form equals the one-ray wide form but for equal-t ties, and the
   persistent packet form equals the packet form bit for bit;
 - exact rejection (oriented box + projected hull) changes nothing;
-- against the FP64 reference, every FP32 form keeps hit t within 1e-5 on at
-  least 99.8% of the same-patch pixels and has at most 0.3% non-tie patch
-  mismatches (the plain method's own level)."""
+- against the FP64 reference, the FP32 forms keep hit t within 1e-5 on at
+  least 99.8% of the same-patch pixels and have at most 0.3% non-tie patch
+  mismatches (the plain method's own level); the mixed-precision forms, which
+  start Newton from the FP64 reference's seed and take the FP64 step where
+  the predicted t error exceeds 2.5e-6, keep t within 1e-5 on at least
+  99.999% and have at most 0.05% non-tie mismatches and 0.01% hit/miss
+  disagreements (3,146, 474 and 0.9999970 measured)."""
 import numpy as np
 import pytest
 
@@ -29,15 +33,22 @@ def c2(rtl):
     return scene, rays, ref
 
 
-def _fp64_check(g, ref, what):
+def _fp64_check(g, ref, what, mixed=False):
     h32, h64 = g["hit"], ref["hit"]
     tie = (h64 >= 0) & np.isfinite(ref["t2"]) & ((ref["t2"] - ref["t"]) <= 1e-5 * np.abs(ref["t"]))
     both = (h32 >= 0) & (h64 >= 0)
     mism = (both & (h32 != h64) & ~tie).sum()
-    assert mism <= 0.003 * len(h32), (what, mism)
+    hm = ((h32 >= 0) != (h64 >= 0)).sum()
     same = both & (h32 == h64)
     rel = np.abs(g["t"][same].astype(np.float64) - ref["t"][same]) / np.abs(ref["t"][same])
-    assert np.mean(rel <= 1e-5) >= 0.998, (what, np.mean(rel <= 1e-5))
+    frac = np.mean(rel <= 1e-5)
+    print(f"{what}: non-tie mismatches {mism}, hit/miss {hm}, t within 1e-5 {frac:.7f}", flush=True)
+    if mixed:
+        assert mism <= 0.0005 * len(h32) and hm <= 0.0001 * len(h32), (what, mism, hm)
+        assert frac >= 0.99999, (what, frac)
+    else:
+        assert mism <= 0.003 * len(h32), (what, mism)
+        assert frac >= 0.998, (what, frac)
 
 
 def test_fullsize_forms(rtl, c2):
@@ -56,7 +67,7 @@ def test_fullsize_forms(rtl, c2):
         h, tot = rtl.trace(rays=rays, scene=scene, prm=rtl.params(count=1, **CFG, **kw))
         assert tot["overflow"] == 0, name
         out[name] = h.get()
-        _fp64_check(out[name], ref, name)
+        _fp64_check(out[name], ref, name, mixed="mixed" in name)
     a, b = out["wide_hull2"], out["wide"]
     assert np.array_equal(a["hit"], b["hit"])
     m = b["hit"] >= 0

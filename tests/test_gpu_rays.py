@@ -114,8 +114,11 @@ def test_secondaries_any_hit_and_shading(rtl):
 
 
 def test_mixed_precision_on_silhouettes(rtl):
-    """Config 3: the FP64 correction step (cond(J) above the threshold) lowers
-    the hit-t error of the corrected rays against the FP64 reference trace."""
+    """Config 3: on the silhouette rays (grazing 0-5 degrees), mixed precision
+    at the specified threshold (cond(J) > 1e4, plus the predicted-t-error
+    rule) takes FP64 steps and brings hit t within 1e-5 of the FP64 reference
+    on at least 99.5% of same-patch pixels (FP32: about 60%), and lowers the
+    error of every ray class it corrects."""
     sc = scenes.config1(0)
     scene = rtl.Scene(sc.surfaces).build()
     rays, mask = scenes.config3_rays(scene, seed=0)
@@ -124,16 +127,19 @@ def test_mixed_precision_on_silhouettes(rtl):
     ref = rtl.trace_f64(scene, rays).get()
     base = dict(mode=rtl.MODE_WIDE, hull_cull=2, **CFG)
     a, _ = rtl.trace(scene, rays, rtl.params(**base))
-    thr = 50.0  # enough corrected rays on 65k rays for a stable mean
-    b, tb = rtl.trace(scene, rays, rtl.params(precision=rtl.PREC_MIXED, cond_thr=thr, count=1, **base))
+    b, tb = rtl.trace(scene, rays, rtl.params(precision=rtl.PREC_MIXED, cond_thr=1e4, count=1, **base))
     a, b = a.get(), b.get()
     assert tb["fp64_steps"] > 0
-    same = (a["hit"] >= 0) & (a["hit"] == ref["hit"]) & (b["hit"] == ref["hit"])
-    changed = same & (a["t"] != b["t"])
-    assert changed.sum() > 0
-    ea = np.abs(a["t"][changed] - ref["t"][changed]) / ref["t"][changed]
-    eb = np.abs(b["t"][changed] - ref["t"][changed]) / ref["t"][changed]
-    assert eb.mean() < ea.mean() and np.median(eb) <= np.median(ea)
+
+    def t_frac(g, sel):
+        same = sel & (g["hit"] >= 0) & (g["hit"] == ref["hit"])
+        e = np.abs(g["t"][same].astype(np.float64) - ref["t"][same]) / ref["t"][same]
+        return np.mean(e <= 1e-5), e
+    fa, _ = t_frac(a, mask)
+    fb, _ = t_frac(b, mask)
+    print(f"silhouette t within 1e-5: FP32 {fa:.4f}, mixed {fb:.4f}, FP64 steps {tb['fp64_steps']}", flush=True)
+    assert fa < 0.8 and fb >= 0.995
+    assert t_frac(b, ~mask)[0] >= t_frac(a, ~mask)[0]
 
 
 def _tiled_source(W, H, spp):
