@@ -515,6 +515,17 @@ def bench_c5(args, L, scenes):
             f"secondary rays, {variants[(pp, ps)]:.2f} ms, {total_rays / variants[(pp, ps)] / 1e3:.1f} Mrays/s")
     log(f"autotuner's choice: packet primary {best['packet_primary']}, secondary {best['packet_secondary']}")
     (pp, ps), ms = min(variants.items(), key=lambda kv: kv[1])
+    # algorithmic work of the reported frame: its launches' Newton and patch-test
+    # counts, box tests priced by the per-lane form's counts of the same rays
+    frame.set_packet(pp, ps)
+    c_prim, c_sec = frame.count()
+    frame.set_packet(0, 0)
+    b_prim, b_sec = frame.count()
+    st = frame.status()
+    flops = (ray_flops(c_prim, st["n_primary"], False, b_prim) + ray_flops(c_sec, st["n_secondary"], False, b_sec))
+    tflops = flops / (ms * 1e-3) / 1e12
+    log(f"c5 frame work: {flops / 1e9:.1f} GFLOP priced, {tflops:.1f} TFLOP/s, newton evals "
+        f"{c_prim['newton_evals'] + c_sec['newton_evals']}, fp64 steps {c_prim['fp64_steps'] + c_sec['fp64_steps']}")
     names = {0: "persistent per-lane", 1: "packet"}
     wl = {"workload": f"{len(built)} scenes, {n_patches} patches (1k-200k each, degree 2-5), 256^2-2048^2 x 4 spp "
                       "+ shadow and mirror ray per hit, one graph-captured frame",
@@ -524,9 +535,13 @@ def bench_c5(args, L, scenes):
           "primary_rays": st["n_primary"], "secondary_rays": st["n_secondary"], "patches": n_patches,
           "launch_shape": {"lanes": st["shape_lanes"], "stack": st["shape_stack"],
                            "blocks_per_cu": st["shape_blocks_per_cu"], "refill_min": st["refill_min"]}}
-    write_detail(args, {"autotune": table, "frame_ms": wl["frame_ms"]})
+    wl["newton_iters_per_ray"] = round((c_prim["newton_evals"] + c_sec["newton_evals"]) / total_rays, 4)
+    rf = roofline(tflops, "frame", "c5")
+    rf["kernel"] = f"frame: primary {names[pp]} + secondary {names[ps]} launches"
+    write_detail(args, {"autotune": table, "frame_ms": wl["frame_ms"], "work_primary": c_prim,
+                        "work_secondary": c_sec, "box_work_per_lane": [b_prim, b_sec]})
     return make_line(args, total_rays / ms / 1e3, ms, "f32", wl,
-                     {"roofline": None, "cpu_baseline": None, "detail": args.detail})
+                     {"roofline": rf, "cpu_baseline": None, "detail": args.detail})
 
 
 def calibrate_copy(nbytes):

@@ -488,6 +488,12 @@ void rtn_frame_destroy(rtn_frame* frame);
 int rtn_frame_run(rtn_frame* frame, void* stream);
 int rtn_frame_capture(rtn_frame* frame, void* stream /* non-default */);
 int rtn_frame_status(const rtn_frame* frame, rtn_frame_status_t* out); /* synchronises the device */
+/* Work totals of the frame's two launches (for pricing their algorithmic
+ * work): runs the frame once, eagerly, with the counting builds of the same
+ * launches (default launch shape: same results), and returns the totals of
+ * the primary and of the secondary launch.  Synchronous; leaves a captured
+ * graph in place. */
+int rtn_frame_count(rtn_frame* frame, rtn_trace_totals* primary, rtn_trace_totals* secondary);
 /* Device pointers of the frame's outputs (any argument may be NULL). */
 int rtn_frame_results(const rtn_frame* frame, rtn_hits* primary, rtn_hits* secondary, const int** parent,
                       const int** kind, const float** secondary_rays_soa6);

@@ -75,11 +75,16 @@ struct FrameCtl {
   unsigned int work1[kWorkWords], work2[kWorkWords];  // persistent work queues
   unsigned long long sec_count;   // secondaries written
   unsigned long long totals[12];  // trace totals (overflow at [6])
+  unsigned long long count1[12], count2[12];  // work totals of a counting run (rtn_frame_count)
 };
 
 __global__ void k_frame_reset(FrameCtl* c) {
   const int t = threadIdx.x;
-  if (t < 12) c->totals[t] = 0;
+  if (t < 12) {
+    c->totals[t] = 0;
+    c->count1[t] = 0;
+    c->count2[t] = 0;
+  }
   if (t == 12) c->sec_count = 0;
   for (int i = t; i < kWorkWords; i += blockDim.x) {
     c->work1[i] = 0;
@@ -247,14 +252,23 @@ namespace {
 
 bool frame_learned(const rtn_frame* f) { return f->fp.trace.seed_mode == RTN_SEED_LEARNED; }
 
-// Enqueues the frame (no allocation, no host copies).
-int frame_enqueue(rtn_frame* f, hipStream_t st) {
+// Enqueues the frame (no allocation, no host copies).  count: the counting
+// builds of the same launches (default shape), their work totals going to
+// count1 / count2 of the control block instead of the overflow word.
+int frame_enqueue(rtn_frame* f, hipStream_t st, bool count = false) {
   FrameCtl* c = f->ctl.ptr;
   hipLaunchKernelGGL(k_frame_reset, dim3(1), dim3(64), 0, st, c);
   RTN_HIP(hipGetLastError());
   const bool learned = frame_learned(f);
-  if (f->fp.packet_primary) RTN_TRY(launch_packet_kernel(false, false, learned, f->n, st, f->a1));
-  else RTN_TRY(launch_wide_kernel(f->shape, true, false, learned, f->n, st, f->a1));
+  WArgs a1 = f->a1, a2 = f->a2;
+  if (count) {
+    a1.totals = c->count1;
+    a2.totals = c->count2;
+    a1.counters = a2.counters = nullptr;
+  }
+  const LaunchShape sh = count ? kDefaultShape : f->shape;
+  if (f->fp.packet_primary) RTN_TRY(launch_packet_kernel(false, count, learned, f->n, st, a1));
+  else RTN_TRY(launch_wide_kernel(sh, true, count, learned, f->n, st, a1));
   if (f->cap2 > 0) {
     const rtn_hits h1 = f->hits1();
     float* outs[6];
@@ -262,8 +276,8 @@ int frame_enqueue(rtn_frame* f, hipStream_t st) {
     launch_secondaries(f->rays, &h1, f->n, f->fp.light, f->fp.eps, f->fp.shadow, f->fp.mirror, outs,
                        f->s_parent.ptr, f->s_kind.ptr, &c->sec_count, f->ray_scene, f->s_scene.ptr, st);
     RTN_HIP(hipGetLastError());
-    if (f->fp.packet_secondary) RTN_TRY(launch_packet_kernel(false, false, learned, f->cap2, st, f->a2));
-    else RTN_TRY(launch_wide_kernel(f->shape, true, false, learned, f->cap2, st, f->a2));
+    if (f->fp.packet_secondary) RTN_TRY(launch_packet_kernel(false, count, learned, f->cap2, st, a2));
+    else RTN_TRY(launch_wide_kernel(sh, true, count, learned, f->cap2, st, a2));
   }
   return RTN_OK;
 }
@@ -410,6 +424,19 @@ extern "C" int rtn_frame_status(const rtn_frame* f, rtn_frame_status_t* out) {
   out->captured = f->exec != nullptr;
   out->refill_min = f->refill;
   out->packet = (f->fp.packet_primary ? 1 : 0) | (f->fp.packet_secondary ? 2 : 0);
+  return RTN_OK;
+}
+
+extern "C" int rtn_frame_count(rtn_frame* f, rtn_trace_totals* primary, rtn_trace_totals* secondary) {
+  RTN_CHECK(f && primary && secondary, RTN_EINVAL, "rtn_frame_count: null argument");
+  RTN_TRY(frame_enqueue(f, nullptr, true));
+  RTN_HIP(hipDeviceSynchronize());
+  FrameCtl h;
+  RTN_HIP(hipMemcpy(&h, f->ctl.ptr, sizeof(h), hipMemcpyDeviceToHost));
+  static_assert(sizeof(rtn_trace_totals) == sizeof(h.count1), "totals layout");
+  memcpy(primary, h.count1, sizeof(h.count1));
+  memcpy(secondary, h.count2, sizeof(h.count2));
+  RTN_CHECK(!h.count1[6] && !h.count2[6], RTN_EOVERFLOW, "rtn_frame_count: traversal stack overflowed");
   return RTN_OK;
 }
 

@@ -42,7 +42,7 @@ EXPORTS = [
     "rtn_frame_autotune", "rtn_solve_pairs", "rtn_frame_set_refill", "rtn_frame_set_packet", "rtn_scene_save", "rtn_scene_load",
     "rtn_scene_trace_overflow", "rtn_malloc_host", "rtn_free_host", "rtn_lookat", "rtn_camera_init",
     "rtn_camera_get_perspective", "rtn_camera_get_orthographic", "rtn_rays_generate",
-    "rtn_calculate_intersect_noalloc",
+    "rtn_calculate_intersect_noalloc", "rtn_frame_count",
 ]
 BVH_EXACT_SAH, BVH_BINNED_SAH = 0, 1
 MLP_NIN, MLP_HIDDEN, MLP_NPARAM = 20, 64, 9794
@@ -171,6 +171,7 @@ def load():
     L.rtn_frame_results.argtypes = [vp, C.POINTER(Hits), C.POINTER(Hits), C.POINTER(vp), C.POINTER(vp),
                                     C.POINTER(vp * 6)]
     L.rtn_frame_set_shape.argtypes = [vp, i, i, i]
+    L.rtn_frame_count.argtypes = [vp, C.POINTER(TraceTotals), C.POINTER(TraceTotals)]
     L.rtn_frame_set_refill.argtypes = [vp, i]
     L.rtn_frame_set_packet.argtypes = [vp, i, i]
     L.rtn_frame_autotune.argtypes = [vp, i, C.POINTER(TuneEntry), i, C.POINTER(i), vp]
@@ -836,6 +837,12 @@ class Frame:
         st = FrameStatus()
         check(load().rtn_frame_status(self.h, C.byref(st)))
         return {k: getattr(st, k) for k, _ in FrameStatus._fields_}
+
+    def count(self):
+        """Work totals (dicts) of the primary and the secondary launch (rtn_frame_count)."""
+        a, b = TraceTotals(), TraceTotals()
+        check(load().rtn_frame_count(self.h, C.byref(a), C.byref(b)))
+        return tuple({k: getattr(t, k) for k, _ in TraceTotals._fields_} for t in (a, b))
 
     def set_shape(self, lanes, stack, blocks_per_cu):
         check(load().rtn_frame_set_shape(self.h, lanes, stack, blocks_per_cu))

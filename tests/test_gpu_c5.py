@@ -9,6 +9,7 @@ spread over every scene, so the host never downloads the frame:
 - the primaries equal a batch trace of the same rays in the same form, bit
   for bit (hit patch and t);
 - a secondary ray never hits the patch it leaves, and empty slots miss;
+- the counting run (rtn_frame_count) gives the same results;
 - graph replays equal the eager run, bit for bit, with eager work between."""
 import numpy as np
 import pytest
@@ -88,6 +89,12 @@ def test_config5_frame_at_full_size(rtl, c5, packet):
     live = kind >= 0
     assert np.all(sec["hit"][live] != prim_of_slot[live])
     assert (sec["hit"][kind == 1] >= 0).any() and (sec["hit"][kind == 0] >= 0).any()
+    # the counting run (work totals for pricing) gives the same results
+    c_prim, c_sec = fr.count()
+    assert c_prim["newton_calls"] > 0 and c_sec["newton_calls"] > 0 and c_prim["overflow"] == 0
+    again = fr.sample(w1, w2)
+    for part in ("primary", "secondary"):
+        assert np.array_equal(again[part]["hit"], eager[part]["hit"]), part
     # graph capture: replays equal the eager run, eager work in between
     stream = rtl.stream_create()
     fr.capture(stream)
