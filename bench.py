@@ -303,13 +303,18 @@ def bench_c2_c1(args, L, scenes, learned):
     rays = L.generate_rays(sc.camera, spp=spp, jitter=jitter, seed=1, tiled=True)
     stream = L.stream_create()
     net, train = None, None
-    if args.train_epochs > 0:
+    if args.load_net:  # parameters saved by an earlier run (profiling passes reuse one training)
+        net = L.Mlp(params=np.load(args.load_net))
+        train = {"loaded": args.load_net}
+    elif args.train_epochs > 0:
         t0 = time.perf_counter()
         net, losses, npairs = learned.build_seed_network(epochs=args.train_epochs)
         train = {"pairs": int(npairs), "epochs": args.train_epochs, "seconds": round(time.perf_counter() - t0, 2),
                  "epoch_loss": [round(float(x), 6) for x in losses]}
         log(f"seed network: {npairs} pairs, {args.train_epochs} epochs, loss {losses[0]:.4f} -> {losses[-1]:.4f}, "
             f"{train['seconds']} s")
+    if net is not None and args.save_net:
+        np.save(args.save_net, net.get_params())
     P, W, K, KP = L.MODE_PERSISTENT, L.MODE_WIDE, L.MODE_PACKET, L.MODE_PERSISTENT_PACKET
     mixed = dict(precision=L.PREC_MIXED, cond_thr=1e4)
     forms = {
@@ -573,6 +578,8 @@ def main():
     ap.add_argument("--c5-packet-only", action="store_true",
                     help="config 5: skip the autotuner and time only the all-packet frame (profiling runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES")
+    ap.add_argument("--save-net", default=None, help="write the trained seed network's parameters (.npy)")
+    ap.add_argument("--load-net", default=None, help="use saved seed-network parameters instead of training")
     ap.add_argument("--detail", default=None, help="per-form report (JSON); default bench_out/<config>_detail.json")
     args = ap.parse_args()
     if args.detail is None:

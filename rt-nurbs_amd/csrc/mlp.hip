@@ -131,8 +131,7 @@ k_train(const bf16x8* __restrict__ tiles, const float* __restrict__ bias, const 
         int64_t end, float inv_b, float* __restrict__ partial, float* __restrict__ loss_part) {
   __shared__ bf16x8 s_x[64][4];
   __shared__ float2 s_logit[64];
-  __shared__ float s_z[3 * NH * 64];
-  __shared__ float s_dh[NH * 64];
+  __shared__ float s_z[3 * NH * 64];  // pre-activations z1 | z2 | z3; z3's slice later holds dh
   __shared__ __bf16 s_dzu[NH * 64];   // [unit][pair]
   __shared__ __bf16 s_dzp[64 * DZP];  // [pair][unit]
   const int lane = threadIdx.x, m = lane & 15, g = lane >> 4;
@@ -180,6 +179,9 @@ k_train(const bf16x8* __restrict__ tiles, const float* __restrict__ bias, const 
     }
   }
   wave_lds_sync();
+  // z3 is consumed: its slice holds dh_prev from here on (69.5 KB of LDS per
+  // block, so two blocks share a compute unit)
+  float* const s_dh = s_z + 2 * NH * 64;
   // layers 3 and 2: dW = dz h_prev^T (k = pairs), dh_prev = W^T dz, dz_prev = dh_prev * gelu'(z_prev)
 #pragma unroll 1
   for (int layer = 2; layer >= 1; layer--) {
@@ -253,19 +255,34 @@ k_train(const bf16x8* __restrict__ tiles, const float* __restrict__ bias, const 
   }
 }
 
-// grad[p] = sum over blocks of partial[b][p] (fixed order); loss[0] += sum of loss parts.
-__global__ void k_reduce(const float* __restrict__ partial, int nblocks, const float* __restrict__ loss_part,
-                         float* __restrict__ grad, float* __restrict__ loss) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+// Gradient reduction in two fixed-order stages (deterministic): group g of
+// kReduceGroup blocks sums its partials into stage[g] (NPARAM values, then the
+// loss parts' sum), then k_reduce sums the groups into grad and adds the loss.
+constexpr int kReduceGroup = 16;
+
+__global__ void k_reduce_groups(const float* __restrict__ partial, int nblocks, const float* __restrict__ loss_part,
+                                float* __restrict__ stage) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x, g = blockIdx.y;
+  const int b0 = g * kReduceGroup, b1 = min(nblocks, b0 + kReduceGroup);
   if (p < NPARAM) {
     float s = 0.f;
-    for (int b = 0; b < nblocks; b++) s += partial[(int64_t)b * NPARAM + p];
-    grad[p] = s;
+    for (int b = b0; b < b1; b++) s += partial[(int64_t)b * NPARAM + p];
+    stage[(int64_t)g * (NPARAM + 1) + p] = s;
   } else if (p == NPARAM) {
     float s = 0.f;
-    for (int b = 0; b < nblocks; b++) s += loss_part[b];
-    loss[0] += s;
+    for (int b = b0; b < b1; b++) s += loss_part[b];
+    stage[(int64_t)g * (NPARAM + 1) + NPARAM] = s;
   }
+}
+
+__global__ void k_reduce(const float* __restrict__ stage, int ngroups, float* __restrict__ grad,
+                         float* __restrict__ loss) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > NPARAM) return;
+  float s = 0.f;
+  for (int g = 0; g < ngroups; g++) s += stage[(int64_t)g * (NPARAM + 1) + p];
+  if (p < NPARAM) grad[p] = s;
+  else loss[0] += s;
 }
 
 __global__ void k_adam(float* __restrict__ w, float* __restrict__ m, float* __restrict__ v,
@@ -287,9 +304,13 @@ int pack(rtn_mlp* net, hipStream_t st) {
   return RTN_OK;
 }
 
+int64_t reduce_groups(int64_t blocks) { return (blocks + kReduceGroup - 1) / kReduceGroup; }
+
+// partial: blocks x NPARAM gradient partials, the blocks' loss parts, then
+// the reduction's group sums (groups x (NPARAM + 1)).
 int ensure_partial(rtn_mlp* net, int64_t blocks) {
   if (blocks <= net->partial_blocks) return RTN_OK;
-  RTN_TRY(net->partial.alloc((size_t)blocks * NPARAM + blocks));
+  RTN_TRY(net->partial.alloc((size_t)blocks * NPARAM + blocks + (size_t)reduce_groups(blocks) * (NPARAM + 1)));
   net->partial_blocks = blocks;
   return RTN_OK;
 }
@@ -304,8 +325,12 @@ int batch_gradient(rtn_mlp* net, const float* feats, const float* targets, const
   hipLaunchKernelGGL(k_train, dim3(blocks), dim3(64), 0, st, net->tiles.ptr, net->bias.ptr, net->w.ptr, feats, targets,
                      perm, base, end, 1.0f / (float)nb, net->partial.ptr, loss_part);
   RTN_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_reduce, dim3((NPARAM + 1 + 255) / 256), dim3(256), 0, st, net->partial.ptr, blocks, loss_part,
-                     net->grad.ptr, net->loss.ptr);
+  float* stage = loss_part + net->partial_blocks;
+  const int groups = (int)reduce_groups(blocks);
+  hipLaunchKernelGGL(k_reduce_groups, dim3((NPARAM + 1 + 255) / 256, groups), dim3(256), 0, st, net->partial.ptr,
+                     blocks, loss_part, stage);
+  hipLaunchKernelGGL(k_reduce, dim3((NPARAM + 1 + 255) / 256), dim3(256), 0, st, stage, groups, net->grad.ptr,
+                     net->loss.ptr);
   RTN_HIP(hipGetLastError());
   return RTN_OK;
 }
@@ -423,8 +448,8 @@ extern "C" void rtn_mlp_train_params_default(rtn_mlp_train_params* tp) {
   if (!tp) return;
   memset(tp, 0, sizeof(*tp));
   tp->epochs = 4;
-  tp->batch = 8192;
-  tp->lr = 2e-3f;
+  tp->batch = 32768;  // 512 single-wave blocks: two per compute unit, one resident round
+  tp->lr = 6e-3f;
   tp->lr_final = 1e-4f;
   tp->beta1 = 0.9f;
   tp->beta2 = 0.999f;

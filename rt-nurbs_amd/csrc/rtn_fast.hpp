@@ -283,12 +283,15 @@ __device__ __forceinline__ void eval64_rt(NP net, int P, int Q, double u, double
 #pragma unroll
   for (int i = 0; i <= MAXD; i++) {
     if (i > P) break;
+    // the row's control points are loaded together (guarded, not a loop of
+    // dependent loads), then accumulated; the basis is zero past the degree
+    float4 row[MAXD + 1];
+#pragma unroll
+    for (int j = 0; j <= MAXD; j++) row[j] = j <= Q ? ld4(net, i * (Q + 1) + j) : make_float4(0.f, 0.f, 0.f, 0.f);
     double r[4] = {0, 0, 0, 0}, rv[4] = {0, 0, 0, 0};
 #pragma unroll
     for (int j = 0; j <= MAXD; j++) {
-      if (j > Q) break;
-      const float4 c = ld4(net, i * (Q + 1) + j);
-      const double cc[4] = {c.x, c.y, c.z, c.w};
+      const double cc[4] = {row[j].x, row[j].y, row[j].z, row[j].w};
 #pragma unroll
       for (int k = 0; k < 4; k++) {
         r[k] = fma(bv[j], cc[k], r[k]);
@@ -315,7 +318,7 @@ __device__ __forceinline__ void eval64_rt(NP net, int P, int Q, double u, double
 // point.  A step is kept only when it stays in the patch's domain [0,1]^2 (the
 // plain method clamps its iterates there) and lowers the FP64 residual
 // |r0| + |r1|; the iteration ends at the first step that does not, or once
-// the residual is at FP64 rounding level.  Returns false (the FP32 hit stands)
+// the residual is below 1e-12 of the point's magnitude.  Returns false (the FP32 hit stands)
 // when not even the first step is kept.
 constexpr int kFix64Steps = 4;
 
@@ -338,7 +341,8 @@ __device__ __forceinline__ bool correct64_rt(NP net, int P, int Q, f3 o, f3 d, f
   double r0 = n1.x * S.x + n1.y * S.y + n1.z * S.z + d1;
   double r1 = n2.x * S.x + n2.y * S.y + n2.z * S.z + d2;
   double err = fabs(r0) + fabs(r1);
-  const double floor_err = 1e-14 * (1.0 + fabs(S.x) + fabs(S.y) + fabs(S.z));
+  // the FP64 reference's own convergence level (thr_res 1e-10), relative to the scene's scale
+  const double floor_err = 1e-12 * (1.0 + fabs(S.x) + fabs(S.y) + fabs(S.z));
   bool moved = false;
   for (int step = 0; step < kFix64Steps && err > floor_err; step++) {
     const double m0 = n1.x * Su.x + n1.y * Su.y + n1.z * Su.z, m1 = n2.x * Su.x + n2.y * Su.y + n2.z * Su.z;

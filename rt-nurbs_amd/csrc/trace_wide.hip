@@ -759,10 +759,14 @@ __global__ void k_zero_words(unsigned int* p, int n) {
 // prim = -2 - patch by the trace) gets the FP64 correction (correct64_rt:
 // guarded FP64 Newton steps from its FP32 (u,v) on FP64 ray planes); t, u, v,
 // dpdu, dpdv are replaced and prim restored.  A pass over prim, so the trace kernels carry no FP64 code.
-// Flagged rays are rare and scattered: each block compacts the ones of its
-// 256-ray chunks into an LDS list and runs the FP64 steps 256 at a time, so a
-// step costs one lane's work instead of stalling a whole wave per ray.
-constexpr int kFixBlock = 256;
+// Flagged rays are rare and scattered: each block takes a range of
+// kFixBlock * kFixPer rays, loads their patch indices with independent loads,
+// compacts the flagged rays into an LDS list and runs their FP64 corrections
+// one per thread.  A correction is a chain of dependent loads and FP64
+// evaluations; with large ranges the grid is about one resident round, so
+// the chains of all flagged rays run side by side instead of a few per block
+// in many rounds.
+constexpr int kFixBlock = 256, kFixPer = 64;  // 16,384 rays per block: about one block per SIMD pair on 8.3M rays
 
 __device__ __forceinline__ bool fix_one(const WArgs& A, int64_t r) {
   const int pi = -2 - A.prim[r];
@@ -784,28 +788,25 @@ __device__ __forceinline__ bool fix_one(const WArgs& A, int64_t r) {
 }
 
 __global__ void __launch_bounds__(kFixBlock) k_fix64(const WArgs A) {
-  __shared__ uint32_t s_list[2 * kFixBlock];
+  __shared__ uint32_t s_list[kFixBlock * kFixPer];  // room for every ray of the block's range
   __shared__ int s_n;
   const int tid = threadIdx.x;
   if (tid == 0) s_n = 0;
   __syncthreads();
-  unsigned long long done = 0;
-  const int64_t stride = (int64_t)gridDim.x * kFixBlock;
-  // every thread of the block runs the same number of chunk iterations
-  for (int64_t base = (int64_t)blockIdx.x * kFixBlock; base < A.n_rays; base += stride) {
-    const int64_t r = base + tid;
-    if (r < A.n_rays && A.prim[r] <= -2) s_list[atomicAdd(&s_n, 1)] = (uint32_t)r;  // at most 2*kFixBlock - 1
-    __syncthreads();
-    const int n = s_n;
-    if (n >= kFixBlock) {  // a full batch: one flagged ray per thread
-      if (fix_one(A, (int64_t)s_list[tid])) done++;
-      __syncthreads();
-      if (tid < n - kFixBlock) s_list[tid] = s_list[kFixBlock + tid];
-      if (tid == 0) s_n = n - kFixBlock;
-    }
-    __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * (kFixBlock * kFixPer);
+  int code[kFixPer];
+#pragma unroll
+  for (int k = 0; k < kFixPer; k++) {
+    const int64_t r = base + (int64_t)k * kFixBlock + tid;
+    code[k] = r < A.n_rays ? A.prim[r] : -1;
   }
-  if (tid < s_n && fix_one(A, (int64_t)s_list[tid])) done++;
+#pragma unroll
+  for (int k = 0; k < kFixPer; k++)
+    if (code[k] <= -2) s_list[atomicAdd(&s_n, 1)] = (uint32_t)(base + (int64_t)k * kFixBlock + tid);
+  __syncthreads();
+  unsigned long long done = 0;
+  for (int i = tid; i < s_n; i += kFixBlock)
+    if (fix_one(A, (int64_t)s_list[i])) done++;
   if (A.totals) {
     for (int sft = 32; sft > 0; sft >>= 1) done += __shfl_xor(done, sft);
     if ((tid & 63) == 0 && done) atomicAdd(A.totals + 8, done);
@@ -973,11 +974,8 @@ int ctl_reset(TraceCtl* ctl, hipStream_t stream) {
 
 int launch_fix64(int64_t n_max, hipStream_t stream, const WArgs& args) {
   if (!args.mixed || n_max <= 0) return RTN_OK;
-  int dev = 0, cus = 256;
-  (void)hipGetDevice(&dev);
-  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  const int64_t need = (n_max + kFixBlock - 1) / kFixBlock;
-  const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)cus * 8));
+  const int64_t per_block = (int64_t)kFixBlock * kFixPer;
+  const unsigned blocks = (unsigned)std::max<int64_t>(1, (n_max + per_block - 1) / per_block);
   hipLaunchKernelGGL(k_fix64, dim3(blocks), dim3(kFixBlock), 0, stream, args);
   RTN_HIP(hipGetLastError());
   return RTN_OK;

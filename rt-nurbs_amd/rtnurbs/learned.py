@@ -89,7 +89,7 @@ class _View:
         self._keep = a
 
 
-def train_seed_network(feats, targets, n, epochs=4, batch=8192, lr=2e-3, lr_final=1e-4, seed=0):
+def train_seed_network(feats, targets, n, epochs=4, batch=32768, lr=6e-3, lr_final=1e-4, seed=0):
     """Trains a fresh network on the first n pairs; returns (Mlp, epoch losses)."""
     net = L.Mlp(seed=seed)
     losses = net.train(_View(feats, n), _View(targets, n), epochs=epochs, batch=batch, lr=lr, lr_final=lr_final,
