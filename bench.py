@@ -63,7 +63,34 @@ def ray_flops(tot, n_rays, exact, box_tot=None):
     return 40.0 * n_rays + boxes + 180.0 * tot["patch_tests"] + float(tot["newton_flops"])
 
 
-def error_stats(g, ref, tie_rel=1e-5):
+def bezier_point64(net, P, Q, u, v):
+    """FP64 point of a rational Bezier patch (homogeneous net, (P+1)(Q+1) x 4,
+    rows along u) at (u, v): the tensor Bernstein sum, dehomogenised."""
+    from math import comb
+    bu = np.array([comb(P, i) * u ** i * (1 - u) ** (P - i) for i in range(P + 1)])
+    bv = np.array([comb(Q, j) * v ** j * (1 - v) ** (Q - j) for j in range(Q + 1)])
+    h = np.einsum("i,j,ijk->k", bu, bv, net.reshape(P + 1, Q + 1, 4).astype(np.float64))
+    return h[:3] / h[3]
+
+
+def hit_distance64(geom, rays_h, idx, prim, u, v):
+    """Distance in FP64 from each ray idx[k] to the surface point of its
+    recorded hit (patch prim[k] at u[k], v[k]); Newton accepts a hit whose
+    two ray-plane residuals sum below thr_res (1e-5 here), so a genuine hit
+    lies within that distance of its ray, up to the FP32 rounding of (u, v)."""
+    cp, off, pq = geom
+    o = np.stack([rays_h[k][idx] for k in range(3)], 1).astype(np.float64)
+    d = np.stack([rays_h[k][idx] for k in range(3, 6)], 1).astype(np.float64)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    out = np.empty(len(idx))
+    for k in range(len(idx)):
+        P, Q = int(pq[prim[k], 0]), int(pq[prim[k], 1])
+        S = bezier_point64(cp[off[prim[k]]:off[prim[k]] + (P + 1) * (Q + 1)], P, Q, float(u[k]), float(v[k]))
+        out[k] = np.linalg.norm(np.cross(S - o[k], d[k]))
+    return out
+
+
+def error_stats(g, ref, tie_rel=1e-5, geom=None, rays_h=None):
     """Per-pixel comparison with the FP64 reference trace.  A pixel is a tie
     when the FP64 trace's second-nearest converged hit on another patch lies
     within tie_rel of its nearest in t.  patch_mismatch_t_separated counts the
@@ -80,6 +107,18 @@ def error_stats(g, ref, tie_rel=1e-5):
            "patch_mismatch_t_separated": int((np.abs(g["t"][mm].astype(np.float64) - t64[mm])
                                               > tie_rel * np.abs(t64[mm])).sum()),
            "hit_miss_disagree": int(((h32 >= 0) != (h64 >= 0)).sum())}
+    # where the form and the reference disagree: a hit the reference does not
+    # have, or a nearer one than the reference's (the form found another root),
+    # against the reference's hits the form lost or replaced by farther ones
+    extra = (h32 >= 0) & (h64 < 0)
+    nearer = mm & (g["t"] < t64)
+    out["disagreements"] = {"extra_hit": int(extra.sum()), "nearer_hit": int(nearer.sum()),
+                            "lost_hit": int(((h32 < 0) & (h64 >= 0)).sum()), "farther_hit": int((mm & ~nearer).sum())}
+    if geom is not None and (extra | nearer).any():
+        idx = np.flatnonzero(extra | nearer)[::max(1, int((extra | nearer).sum()) // 2000)]
+        dist = hit_distance64(geom, rays_h, idx, h32[idx], g["u"][idx], g["v"][idx])
+        out["disagreements"]["extra_or_nearer_checked"] = int(len(idx))
+        out["disagreements"]["extra_or_nearer_on_surface_frac"] = float((dist <= 2e-5).mean())
     if same.any():
         te = np.abs(g["t"][same].astype(np.float64) - t64[same]) / np.abs(t64[same])
         ue = np.maximum(np.abs(g["u"][same] - ref["u"][same]), np.abs(g["v"][same] - ref["v"][same]))
@@ -218,6 +257,11 @@ def roofline(tflops, form, config):
 def run_forms(L, scene, rays, forms, args, ref, stream):
     results = {}
     n = rays.n
+    geom, rays_h = None, None
+    if ref is not None:
+        dl = scene.download()
+        geom = (dl["cp"], dl["off"], dl["pq"])
+        rays_h = [np.asarray(a) for a in rays.get()]
     for name, kw in forms.items():
         prm = L.params(**kw)
         hits_c, tot = L.trace(scene, rays, L.params(count=1, **kw))   # counting run: work and iterations
@@ -238,9 +282,10 @@ def run_forms(L, scene, rays, forms, args, ref, stream):
              "patch_tests_per_ray": tot["patch_tests"] / n, "flops_per_ray": flops / n,
              "hull_culled_per_ray": tot["hull_culled"] / n, "fp64_steps": tot["fp64_steps"],
              "learned_fallbacks_per_ray": tot["learned_fallbacks"] / n,
+             "learned_skips_per_ray": tot["learned_skips"] / n,
              "tflops": flops / (kern_ms * 1e-3) / 1e12}
         if ref is not None:
-            r["fp64"] = error_stats(g, ref)
+            r["fp64"] = error_stats(g, ref, geom=geom, rays_h=rays_h)
         results[name] = r
         log(f"{name}: {r['mrays_s']:.1f} Mrays/s ({kern_ms:.3f} ms kernel, {wall_ms:.3f} ms wall), "
             f"iters/ray {r['newton_iters_per_ray']:.3f}, {r['tflops']:.1f} TFLOP/s"
@@ -256,7 +301,7 @@ def seed_ablation(L, scene, rays, net):
     pr, pp, puv, cnt = L.trace_pairs(scene, rays, prm, 2 * rays.n)
     cnt = min(cnt, 2 * rays.n)
     target = puv.get()[:cnt]
-    luv = net.forward(L.pair_features(scene, rays, pr, pp, cnt))
+    luv = net.forward(L.pair_features(scene, rays, pr, pp, cnt, kind=net.kind))
     seed_p = L.solve_pairs(scene, rays, pr, pp, cnt, L.params(**{**BASE, "max_iter": 1}))["uv"]
     a = L.solve_pairs(scene, rays, pr, pp, cnt, L.params(**BASE))
     b = L.solve_pairs(scene, rays, pr, pp, cnt, L.params(**BASE), seed_uv=luv)
@@ -267,9 +312,24 @@ def seed_ablation(L, scene, rays, net):
            "iters_per_solve_plain": float(a["iters"][both].mean()), "iters_per_solve_learned": float(b["iters"][both].mean()),
            "converged_plain": float(a["converged"].mean()), "converged_learned": float(b["converged"].mean()),
            "converged_learned_or_fallback": float((a["converged"] | b["converged"]).mean())}
-    log(f"seed ablation on {cnt} solved pairs: seed error {out['seed_err_median_plain']:.3f} -> "
+    # the hit output on every solve of the mixed-precision form without the network
+    prm_all = L.params(mode=L.MODE_WIDE, hull_cull=2, precision=L.PREC_MIXED, cond_thr=1e4, stall_exit=1, **BASE)
+    ar, ap, auv, acnt = L.trace_pairs(scene, rays, prm_all, 4 * rays.n, all=True)
+    acnt = min(acnt, 4 * rays.n)
+    _, ph = net.forward(L.pair_features(scene, rays, ar, ap, acnt, kind=net.kind), hit=True)
+    h, ph = auv.get()[:acnt, 0] >= 0, ph.get()
+    order = np.argsort(ph, kind="stable")
+    ranks = np.empty(len(ph))
+    ranks[order] = np.arange(1, len(ph) + 1)
+    npos, nneg = int(h.sum()), int((~h).sum())
+    out["hit_output"] = {"solves": int(acnt), "converged": npos,
+                         "auc": float((ranks[h].sum() - npos * (npos + 1) / 2) / max(1, npos * nneg))}
+    for thr in (0.05, 0.2, 0.5):
+        out["hit_output"][f"below_{thr}"] = {"misses": float((ph[~h] < thr).mean()), "hits": float((ph[h] < thr).mean())}
+    log(f"seed ablation (kind {net.kind}) on {cnt} solved pairs: seed error {out['seed_err_median_plain']:.3f} -> "
         f"{out['seed_err_median_learned']:.3f}, evaluations per solve {out['iters_per_solve_plain']:.3f} -> "
-        f"{out['iters_per_solve_learned']:.3f}")
+        f"{out['iters_per_solve_learned']:.3f}; hit output AUC {out['hit_output']['auc']:.4f}, below 0.5: "
+        f"{out['hit_output']['below_0.5']['misses']:.3f} of misses, {out['hit_output']['below_0.5']['hits']:.4f} of hits")
     return out
 
 
@@ -302,19 +362,37 @@ def bench_c2_c1(args, L, scenes, learned):
     # that a wave's 64 rays cover a compact pixel block; every form gets it
     rays = L.generate_rays(sc.camera, spp=spp, jitter=jitter, seed=1, tiled=True)
     stream = L.stream_create()
-    net, train = None, None
+    nets, train = {}, None
+    kinds = {"projected": L.MLP_FEAT_PROJECTED, "box": L.MLP_FEAT_BOX}
     if args.load_net:  # parameters saved by an earlier run (profiling passes reuse one training)
-        net = L.Mlp(params=np.load(args.load_net))
+        z = np.load(args.load_net)
+        nets = {k: L.Mlp(params=z[k], kind=kinds[k]) for k in kinds if k in z}
         train = {"loaded": args.load_net}
     elif args.train_epochs > 0:
-        t0 = time.perf_counter()
-        net, losses, npairs = learned.build_seed_network(epochs=args.train_epochs)
-        train = {"pairs": int(npairs), "epochs": args.train_epochs, "seconds": round(time.perf_counter() - t0, 2),
-                 "epoch_loss": [round(float(x), 6) for x in losses]}
-        log(f"seed network: {npairs} pairs, {args.train_epochs} epochs, loss {losses[0]:.4f} -> {losses[-1]:.4f}, "
-            f"{train['seconds']} s")
-    if net is not None and args.save_net:
-        np.save(args.save_net, net.get_params())
+        train = {}
+        # the same training scenes for both feature kinds: the pairs are
+        # collected once per kind (the features differ), then trained
+        sl = learned.training_scenes()
+        for name, kind in kinds.items():
+            t0 = time.perf_counter()
+            feats, targets, n, hits = learned.collect_pairs(sl, max_hits=10_000_000, kind=kind)
+            t1 = time.perf_counter()
+            L.synchronize()
+            net, losses = learned.train_seed_network(feats, targets, n, epochs=args.train_epochs, kind=kind)
+            L.synchronize()
+            t2 = time.perf_counter()
+            del feats, targets
+            nets[name] = net
+            train[name] = {"pairs": int(n), "converged_pairs": int(hits), "epochs": args.train_epochs,
+                           "collect_s": round(t1 - t0, 2), "train_s": round(t2 - t1, 3),
+                           "train_mpairs_s": round(n * args.train_epochs / (t2 - t1) / 1e6, 1),
+                           "epoch_loss": [round(float(x), 6) for x in losses]}
+            log(f"seed network ({name} features): {n} pairs ({hits} converged), {args.train_epochs} epochs, "
+                f"loss {losses[0]:.4f} -> {losses[-1]:.4f}, collect {t1 - t0:.2f} s, train {t2 - t1:.2f} s "
+                f"({train[name]['train_mpairs_s']} Mpairs/s)")
+    if nets and args.save_net:
+        np.savez(args.save_net, **{k: v.get_params() for k, v in nets.items()})
+    net = nets.get("projected")
     P, W, K, KP = L.MODE_PERSISTENT, L.MODE_WIDE, L.MODE_PACKET, L.MODE_PERSISTENT_PACKET
     mixed = dict(precision=L.PREC_MIXED, cond_thr=1e4)
     forms = {
@@ -335,17 +413,30 @@ def bench_c2_c1(args, L, scenes, learned):
         "persistent_packet_hull2_mixed_stall": dict(mode=KP, hull_cull=2, stall_exit=1, **mixed, **BASE),
     }
     if net is not None:
-        # a 3-evaluation attempt from the network's seed (two Newton steps: the fewest that
-        # converge from its typical error), then the plain seed's full 8; the four lanes of a
-        # quad testing the same patch share one network evaluation (measured sweep: longer
-        # attempts converge to other roots more often, shared iteration budgets lose hits)
-        lrn = dict(seed_mode=L.SEED_LEARNED, seed_net=net, learned_iters=3, fallback_iters=8, seed_share=1)
+        # a 4-evaluation attempt from the network's seed, then the plain seed's full 8; the
+        # four lanes of a quad testing the same patch share one network evaluation; pairs the
+        # hit output rates below hit_gate (0.7) start from the plain seed directly.  Measured
+        # sweep on this scene: 2 evaluations fall back on a third of the solves, 3 still
+        # cost more evaluations per ray than the form without the network, a 6-evaluation
+        # fallback loses hits; a lower gate sends more misses through the attempt.
+        lrn = dict(seed_mode=L.SEED_LEARNED, seed_net=net, learned_iters=4, fallback_iters=8, seed_share=1,
+                   hit_gate=args.hit_gate)
         forms["wide_learned"] = dict(mode=W, hull_cull=2, **lrn, **BASE)
-        forms["persistent_learned_mixed"] = dict(mode=P, hull_cull=2, **lrn, **mixed, **BASE)
         forms["persistent_learned_mixed_stall"] = dict(mode=P, hull_cull=2, stall_exit=1, **lrn, **mixed, **BASE)
         forms["packet_learned_mixed_stall"] = dict(mode=K, hull_cull=2, stall_exit=1, **lrn, **mixed, **BASE)
         forms["persistent_packet_learned_mixed"] = dict(mode=KP, hull_cull=2, **lrn, **mixed, **BASE)
         forms["persistent_packet_learned_mixed_stall"] = dict(mode=KP, hull_cull=2, stall_exit=1, **lrn, **mixed, **BASE)
+        # the hit output also skips the pairs it rates below hit_skip (no solve at all)
+        forms["persistent_packet_learned_skip_mixed_stall"] = dict(mode=KP, hull_cull=2, stall_exit=1,
+                                                                   **{**lrn, "hit_skip": args.hit_skip}, **mixed,
+                                                                   **BASE)
+        forms["persistent_packet_learned_nogate_mixed_stall"] = dict(mode=KP, hull_cull=2, stall_exit=1,
+                                                                     **{**lrn, "hit_gate": 0.0}, **mixed, **BASE)
+    if "box" in nets:  # the network on the box-frame inputs (ray, four corners, degrees)
+        lrb = dict(seed_mode=L.SEED_LEARNED, seed_net=nets["box"], learned_iters=4, fallback_iters=8, seed_share=1,
+                   hit_gate=args.hit_gate)
+        forms["persistent_packet_learned_box_mixed_stall"] = dict(mode=KP, hull_cull=2, stall_exit=1, **lrb, **mixed,
+                                                                  **BASE)
     if args.forms != "all":
         keep = set(args.forms.split(","))
         forms = {k: v for k, v in forms.items() if k in keep}
@@ -355,7 +446,7 @@ def bench_c2_c1(args, L, scenes, learned):
         ref = L.trace_f64(scene, rays, max_iter=32, thr_det=1e-12, thr_res=1e-10).get()
         log(f"fp64 reference: {time.perf_counter() - t0:.2f}s, hits {(ref['hit'] >= 0).mean():.3f}")
     results = run_forms(L, scene, rays, forms, args, ref, stream)
-    ablation = seed_ablation(L, scene, rays, net) if net is not None else None
+    ablation = {k: seed_ablation(L, scene, rays, v) for k, v in nets.items()} or None
     # headline: the learned seed + mixed precision + persistent form the target names;
     # beside it the same form without the network and the plain method
     north = "persistent_packet_learned_mixed_stall"
@@ -406,6 +497,10 @@ def brief(r):
     if e:
         out.update(mismatch_non_tie=e["patch_mismatch_non_tie"], hit_miss=e["hit_miss_disagree"],
                    t_le_1e5=round(e.get("t_rel_le_1e-5_frac", float("nan")), 6))
+        dg = e.get("disagreements")
+        if dg:  # the reference's hits lost (missing or farther) vs hits it lacks (extra or nearer)
+            out.update(ref_hits_lost=dg["lost_hit"] + dg["farther_hit"], extra_or_nearer=dg["extra_hit"] + dg["nearer_hit"],
+                       extra_or_nearer_on_surface=round(dg.get("extra_or_nearer_on_surface_frac", float("nan")), 4))
     return out
 
 
@@ -578,8 +673,12 @@ def main():
     ap.add_argument("--c5-packet-only", action="store_true",
                     help="config 5: skip the autotuner and time only the all-packet frame (profiling runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES")
-    ap.add_argument("--save-net", default=None, help="write the trained seed network's parameters (.npy)")
+    ap.add_argument("--save-net", default=None, help="write the trained seed networks' parameters (.npz)")
     ap.add_argument("--load-net", default=None, help="use saved seed-network parameters instead of training")
+    ap.add_argument("--hit-gate", type=float, default=0.7, help="learned forms: hit probability below which a "
+                    "pair starts from the plain seed")
+    ap.add_argument("--hit-skip", type=float, default=0.002, help="the skip form: hit probability below which a "
+                    "pair is not solved")
     ap.add_argument("--detail", default=None, help="per-form report (JSON); default bench_out/<config>_detail.json")
     args = ap.parse_args()
     if args.detail is None:

@@ -311,6 +311,12 @@ typedef struct {
                              on its own patch would send to a second crossing of that patch */
   float t_tol;            /* RTN_PREC_MIXED: a closest hit whose predicted relative t error exceeds
                              this also gets the FP64 step (0 = 2.5e-6, a quarter of the 1e-5 bar) */
+  float hit_gate;         /* learned seed: a pair whose predicted hit probability is below this
+                             starts from the plain seed with max_iter iterations (no learned
+                             attempt); 0 = every pair takes the learned attempt */
+  float hit_skip;         /* learned seed: a pair whose predicted hit probability is below this is
+                             not solved at all (counted in learned_skips); 0 = off.  Trades hits
+                             the network misjudges for fewer Newton solves */
 } rtn_trace_params;
 
 void rtn_trace_params_default(rtn_trace_params* p);
@@ -332,7 +338,7 @@ typedef struct {
 /* Totals accumulated over a trace (device atomics; host-readable after sync). */
 typedef struct {
   unsigned long long newton_calls, newton_evals, skips, interior, leaves, patch_tests, overflow, newton_flops;
-  unsigned long long fp64_steps, hull_culled, learned_fallbacks, reserved;
+  unsigned long long fp64_steps, hull_culled, learned_fallbacks, learned_skips;
 } rtn_trace_totals;
 
 /* The trace of calculateIntersectWrapper -> calculateIntersect
@@ -354,14 +360,17 @@ int rtn_trace(const rtn_scene* scene, const float* const rays_soa[6], int64_t n_
  * share that block and race; trace one scene from one stream at a time. */
 int rtn_scene_trace_overflow(const rtn_scene* scene, unsigned long long* count);
 
-/* Converged (ray, patch, u, v) solutions of the plain method (training pairs
- * of the learned seed; no reference counterpart).  Runs the wide or
- * persistent form with plain seeds and appends every converged Newton solve
- * with t > 0 (not only the closest hit) to device arrays of capacity cap;
- * *count_host = number of solves found (may exceed cap; only cap are stored). */
+/* Newton solves of a trace as (ray, patch, u, v) records (training pairs of
+ * the learned seed; no reference counterpart).  Runs the wide or persistent
+ * form with plain seeds (FP32, or the FP64 seed with RTN_PREC_MIXED; stall_exit
+ * as given) and appends to device arrays of capacity cap every converged
+ * Newton solve with t > 0 (not only the closest hit) and, with all = 1, every
+ * other solve the trace runs as (u, v) = (-1, -1): the pairs the trace tests
+ * and whether each converges.  *count_host = number of records (may exceed
+ * cap; only cap are stored). */
 int rtn_trace_pairs(const rtn_scene* scene, const float* const rays_soa[6], int64_t n_rays,
-                    const rtn_trace_params* prm, int* pair_ray, int* pair_patch, float* pair_uv, int64_t cap,
-                    int64_t* count_host, void* stream);
+                    const rtn_trace_params* prm, int all, int* pair_ray, int* pair_patch, float* pair_uv,
+                    int64_t cap, int64_t* count_host, void* stream);
 
 /* The plain method's Newton solve (bezier_patch.h:198-266, reference order)
  * on given (ray, patch) pairs: from the plain bilinear seed (seed_uv NULL)
@@ -374,47 +383,69 @@ int rtn_solve_pairs(const rtn_scene* scene, const float* const rays_soa[6], int6
                     int* converged, float* uv, float* t, void* stream);
 
 /* ------------------------------------------------------------ learned seed
- * No reference counterpart.  A 20 -> 64 -> 64 -> 64 -> 2 perceptron (GELU,
- * tanh form; sigmoid outputs) predicting Newton's starting (u,v) for a
- * (ray, patch) pair from the pair's features: a 3 x 3 sample of the control
- * net projected on the ray's two planes (the ray at the origin), in units of
- * the patch box's half extent, and the degrees.  Inference and training run BF16
- * MFMA with FP32 accumulation; FP32 master parameters, Adam.
- * Parameter layout (RTN_MLP_NPARAM floats): W1[64][20] b1[64] W2[64][64]
- * b2[64] W3[64][64] b3[64] W4[2][64] b4[2], weights row-major (out x in). */
+ * No reference counterpart.  A 20 -> 64 -> 64 -> 64 -> 3 perceptron (GELU,
+ * tanh form; sigmoid outputs) predicting, for a (ray, patch) pair, Newton's
+ * starting (u,v) and the probability that Newton converges on the pair (a
+ * hit).  Inputs, by the network's feature kind:
+ *   RTN_MLP_FEAT_PROJECTED  a 3 x 3 sample of the control net projected on
+ *                           the ray's two planes (the ray at the origin), in
+ *                           units of the patch box's half extent, and the degrees;
+ *   RTN_MLP_FEAT_BOX        the ray (its entry point into the patch box, and its
+ *                           unit direction), the four corner control points
+ *                           and the degrees, in the box's frame (centre,
+ *                           largest half extent).
+ * Inference and training run BF16 MFMA with FP32 accumulation; FP32 master
+ * parameters, Adam.  Parameter layout (RTN_MLP_NPARAM floats): W1[64][20]
+ * b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[3][64] b4[3], weights
+ * row-major (out x in); output rows u, v, hit. */
 #define RTN_MLP_NIN 20
 #define RTN_MLP_HIDDEN 64
-#define RTN_MLP_NPARAM 9794
+#define RTN_MLP_NOUT 3
+#define RTN_MLP_NPARAM 9859
+enum { RTN_MLP_FEAT_PROJECTED = 0, RTN_MLP_FEAT_BOX = 1 };
 typedef struct rtn_mlp rtn_mlp;
 typedef struct {
   int epochs;             /* passes over the pairs, reshuffled each pass          */
   int batch;              /* pairs per Adam step                                  */
   float lr, lr_final;     /* cosine decay from lr to lr_final over all steps      */
   float beta1, beta2, eps;
-  int reserved;
+  float hit_weight;       /* weight of the hit output's cross-entropy in the loss */
   uint64_t seed;          /* shuffle seed                                         */
 } rtn_mlp_train_params;
 
-int rtn_mlp_create(rtn_mlp** net, uint64_t seed); /* Glorot-uniform weights, zero biases */
+/* Glorot-uniform weights, zero biases; kind = RTN_MLP_FEAT_*. */
+int rtn_mlp_create(rtn_mlp** net, uint64_t seed, int kind);
 void rtn_mlp_destroy(rtn_mlp* net);
+int rtn_mlp_kind(const rtn_mlp* net); /* the feature kind, -1 for NULL */
 int rtn_mlp_set_params(rtn_mlp* net, const float* params_host); /* resets Adam state */
 int rtn_mlp_get_params(const rtn_mlp* net, float* params_host);
-/* Features (device float[20*n]) of device pairs (ray index, patch position in BVH order). */
+/* Features of a kind (device float[20*n]) of device pairs (ray index, patch
+ * position in BVH order). */
 int rtn_mlp_pair_features(const rtn_scene* scene, const float* const rays_soa[6], int64_t n_rays,
-                          const int* pair_ray, const int* pair_patch, int64_t n, float* feats, void* stream);
-/* Inference: device feats[20*n] -> device uv[2*n].  valid (device int[n], may be
- * NULL): pairs with valid[i] == 0 are skipped and their uv left untouched; the
- * remaining pairs of each 64-pair group are packed before the matrix products,
- * as in the trace kernels. */
-int rtn_mlp_forward(const rtn_mlp* net, const float* feats, const int* valid, int64_t n, float* uv, void* stream);
-/* Gradient (host float[RTN_MLP_NPARAM]) of mean_n |uv_n - target_n|^2 over device
- * feats[20*n], targets[2*n]; *loss_host = that mean.  Synchronous. */
-int rtn_mlp_gradient(rtn_mlp* net, const float* feats, const float* targets, int64_t n, float* grad_host,
-                     float* loss_host);
+                          const int* pair_ray, const int* pair_patch, int64_t n, int kind, float* feats,
+                          void* stream);
+/* Inference: device feats[20*n] -> device uv[2*n] and, if hit is not NULL,
+ * hit[n] (hit probability).  valid (device int[n], may be NULL): pairs with
+ * valid[i] == 0 are skipped and their outputs left untouched; the remaining
+ * pairs of each 64-pair group are packed before the matrix products, as in
+ * the trace kernels. */
+int rtn_mlp_forward(const rtn_mlp* net, const float* feats, const int* valid, int64_t n, float* uv, float* hit,
+                    void* stream);
+/* Gradient (host float[RTN_MLP_NPARAM]) of the training loss over device
+ * feats[20*n] and targets[3*n] = (u, v, h) per pair, h = 1 for a hit (a
+ * converged solve at (u,v)) and 0 for a miss:
+ *   mean_n [ h_n |uv_n - (u,v)_n|^2 + hit_weight * BCE(hit_n, h_n) ],
+ * BCE(p, h) = -(h log p + (1 - h) log(1 - p)); *loss_host = that mean.
+ * Synchronous. */
+int rtn_mlp_gradient(rtn_mlp* net, const float* feats, const float* targets, int64_t n, float hit_weight,
+                     float* grad_host, float* loss_host);
 /* Defaults: 4 epochs, batch 32768 (one resident round of single-wave blocks at
- * two per compute unit), lr 6e-3 decaying to 1e-4, beta 0.9 / 0.999, eps 1e-8. */
+ * two per compute unit), lr 6e-3 decaying to 1e-4, beta 0.9 / 0.999, eps 1e-8,
+ * hit_weight 0.05. */
 void rtn_mlp_train_params_default(rtn_mlp_train_params* tp);
-/* Adam training on device pairs; epoch_loss_host[epochs] = mean loss per epoch (may be NULL). */
+/* Adam training on device pairs (targets as rtn_mlp_gradient); the pairs are
+ * reshuffled on the device each epoch (a keyed pseudo-random permutation);
+ * epoch_loss_host[epochs] = mean loss per epoch (may be NULL). */
 int rtn_mlp_train(rtn_mlp* net, const float* feats, const float* targets, int64_t n,
                   const rtn_mlp_train_params* tp, float* epoch_loss_host, void* stream);
 

@@ -6,10 +6,14 @@ checked against an independent implementation:
 
   ray_planes(d)       the two unit ray-plane normals (the reference's
                       ray-plane construction, math_utils.h:33-54)
-  features(...)       the 20 pair features (a 3 x 3 sample of the control
-                      net projected on the ray planes, degrees)
-  forward(...)        20 -> 64 -> 64 -> 64 -> 2, GELU (tanh form), sigmoid out
-  loss_and_grad(...)  mean over pairs of |uv - target|^2 and its gradient
+  features(...)       the 20 pair features of kind 0 (a 3 x 3 sample of the
+                      control net projected on the ray planes, degrees)
+  box_features(...)   the 20 pair features of kind 1 (ray entry point and
+                      direction, four corners, degrees, in the patch box's frame)
+  forward(...)        20 -> 64 -> 64 -> 64 -> 3, GELU (tanh form), sigmoid out
+                      (u, v, hit probability)
+  loss_and_grad(...)  mean over pairs of h |uv - target|^2 + lambda BCE(hit, h)
+                      and its gradient
 
 With quantize=True the functions round to BF16 (round to nearest even) at
 exactly the points where the GPU kernels feed the matrix cores: features,
@@ -18,16 +22,16 @@ layers 1..3 that enter products; the arithmetic between those points is
 FP64.  With quantize=False everything is FP64.
 
 Pinning (tests/test_mlp_oracle.py): the FP64 gradient is checked against
-torch.autograd in float64 and against central finite differences; GELU
-against torch's tanh-approximate GELU; features against closed forms
-(axis-aligned rays over flat nets, odd/even middle indices, similarity
-invariance).
+torch.autograd in float64 (torch's own binary cross-entropy with logits) and
+against central finite differences; GELU against torch's tanh-approximate
+GELU; both feature kinds against closed forms (axis-aligned rays over flat
+nets, odd/even middle indices, entry points, similarity invariance).
 """
 from __future__ import annotations
 
 import numpy as np
 
-NIN, NH, NOUT = 20, 64, 2
+NIN, NH, NOUT = 20, 64, 3
 OFF_W1 = 0
 OFF_B1 = OFF_W1 + NH * NIN
 OFF_W2 = OFF_B1 + NH
@@ -125,8 +129,40 @@ def features(o, d, bounds, nets, p, q):
     return out
 
 
+def box_features(o, d, bounds, nets, p, q):
+    """Kind 1: o, d, bounds, nets, p, q as features().  With c the box centre
+    and hs its largest half extent, the ray enters the box at
+    t_e = max(0, max_k min((lo_k - o_k)/d_k, (hi_k - o_k)/d_k)) (NaN terms of a
+    zero direction component dropped; an infinite t_e counts as 0); then
+    (o + t_e d - c)/hs clamped to [-4, 4], d/|d|, (corner - c)/hs for the
+    dehomogenised corners P00, Pp0, P0q, Ppq, and the degrees (p - 3.5)/1.5,
+    (q - 3.5)/1.5."""
+    o, d, bounds = (np.asarray(a, np.float64) for a in (o, d, bounds))
+    p, q = np.asarray(p), np.asarray(q)
+    lo, hi = bounds[:, :3], bounds[:, 3:]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        a, b = (lo - o) / d, (hi - o) / d
+    near = np.fmin(a, b)
+    te = np.fmax(np.fmax(np.fmax(near[:, 0], near[:, 1]), near[:, 2]), 0.0)
+    te = np.where(np.isfinite(te), te, 0.0)
+    c = 0.5 * (lo + hi)
+    hs = np.maximum(0.5 * (hi - lo).max(axis=1), 2e-20)
+    out = np.zeros((len(o), NIN))
+    out[:, 0:3] = np.clip((o + te[:, None] * d - c) / hs[:, None], -4.0, 4.0)
+    out[:, 3:6] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    for i in range(len(o)):
+        pi, qi = int(p[i]), int(q[i])
+        net = np.asarray(nets[i], np.float64).reshape(pi + 1, qi + 1, 4)
+        x = net[..., :3] / net[..., 3:]
+        for k, (r, cc) in enumerate(((0, 0), (pi, 0), (0, qi), (pi, qi))):
+            out[i, 6 + 3 * k:9 + 3 * k] = (x[r, cc] - c[i]) / hs[i]
+        out[i, 18] = (pi - 3.5) / 1.5
+        out[i, 19] = (qi - 3.5) / 1.5
+    return out
+
+
 def forward(w, X, quantize=False):
-    """X: (n, 20) -> (uv (n,2), cache)."""
+    """X: (n, 20) -> (y (n,3): u, v, hit probability; cache)."""
     P = unpack(w)
     q = bf16 if quantize else (lambda a: np.asarray(a, np.float64))
     x = q(X)
@@ -142,15 +178,19 @@ def forward(w, X, quantize=False):
     return y, dict(x=x, W=W, z1=z1, h1=h1, z2=z2, h2=h2, z3=z3, h3=h3, y=y)
 
 
-def loss_and_grad(w, X, T, quantize=False):
-    """Mean over pairs of |y - t|^2 and its gradient (flat, NPARAM)."""
+def loss_and_grad(w, X, T, quantize=False, hit_weight=0.05):
+    """T: (n, 3) targets (u, v, h), h = 1 for a hit.  Mean over pairs of
+    h |y_uv - (u,v)|^2 + hit_weight BCE(y_h, h) and its gradient (flat, NPARAM)."""
     n = X.shape[0]
     y, c = forward(w, X, quantize)
     q = bf16 if quantize else (lambda a: a)
     T = np.asarray(T, np.float64)
-    e = y - T
-    loss = (e * e).sum() / n
-    dz4 = 2.0 * e * y * (1.0 - y) / n
+    h = (T[:, 2] > 0.5).astype(np.float64)
+    e = (y[:, :2] - T[:, :2]) * h[:, None]
+    yh = y[:, 2]
+    bce = -(h * np.log(yh) + (1.0 - h) * np.log(1.0 - yh))
+    loss = ((e * e).sum() + hit_weight * bce.sum()) / n
+    dz4 = np.concatenate([2.0 * e * y[:, :2] * (1.0 - y[:, :2]), (hit_weight * (yh - h))[:, None]], 1) / n
     g = {}
     g["W4"] = dz4.T @ c["h3"]
     g["b4"] = dz4.sum(0)

@@ -1,5 +1,6 @@
-// Learned Newton seed: a 20 -> 64 -> 64 -> 64 -> 2 perceptron (GELU hidden
-// units, sigmoid outputs) mapping a (ray, patch) pair to a starting (u,v).
+// Learned Newton seed: a 20 -> 64 -> 64 -> 64 -> 3 perceptron (GELU hidden
+// units, sigmoid outputs) mapping a (ray, patch) pair to a starting (u,v) and
+// the probability that Newton converges on the pair (a hit).
 //
 // Inference runs per wave on the matrix cores: v_mfma_f32_16x16x32_bf16 with
 // BF16 weights and activations and FP32 accumulation.  The pairs of a wave are
@@ -27,7 +28,14 @@ namespace mlp {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int NIN = 20, NH = 64, NOUT = 2;
+// Inputs: the 20 pair features of one of the feature kinds below.  Outputs:
+// logits of u, v and of the probability that Newton converges (a hit).
+constexpr int NIN = 20, NH = 64, NOUT = 3;
+enum FeatureKind : int {
+  FEAT_PROJECTED = 0,  // the control net projected on the ray planes (pair_features)
+  FEAT_BOX = 1,        // ray, corners and degrees in the patch box's frame (box_features)
+  FEAT_KINDS = 2
+};
 
 // FP32 master parameters, row-major (out x in) weights then bias per layer.
 constexpr int OFF_W1 = 0;
@@ -45,7 +53,8 @@ constexpr int NPARAM = OFF_B4 + NOUT;  // 9794
 // layer (rows 0,1 real, 2 k blocks).  Backward: W2^T and W3^T (natural k order).
 constexpr int T_L1 = 0, T_L2 = 4, T_L3 = 12, T_L4 = 20, T_T2 = 22, T_T3 = 30, NTILE = 38;
 constexpr int NTILE_FWD = 22;
-// FP32 biases as accumulator seeds: b1 | b2 | b3 | b4 padded to 16 rows.
+// FP32 biases as accumulator seeds: b1 | b2 | b3 | b4 padded to 16 rows
+// (rows 0..2 real).
 constexpr int BB1 = 0, BB2 = 64, BB3 = 128, BB4 = 192, NBIAS = 208;
 
 // Unit carried by slot i (0..7) of lane group g within a 32-unit k block.
@@ -71,7 +80,7 @@ __device__ __forceinline__ float gelu_grad(float x) {
 }
 __device__ __forceinline__ float bf16r(float x) { return (float)(__bf16)x; }
 
-// Input features of a (ray, patch) pair.  On the two ray planes the ray is
+// Input features of kind FEAT_PROJECTED (20 values).  On the two ray planes the ray is
 // the origin and the patch a 2D map of (u,v) whose control points are the
 // projected control points: the seed is where that map reaches the origin.
 // The features sample the projected net on a 3 x 3 grid of control indices,
@@ -86,7 +95,7 @@ __device__ __forceinline__ float bf16r(float x) { return (float)(__bf16)x; }
 template <class NP = const float4*, class BP = const float*>
 __device__ __forceinline__ void pair_features(float ox, float oy, float oz, float n1x, float n1y, float n1z,
                                               float n2x, float n2y, float n2z, NP net, int P, int Q, BP bnd,
-                                              float f[NIN]) {
+                                              float* f) {
   const float hs = fmaxf(fmaxf(ld1(bnd, 3) - ld1(bnd, 0), ld1(bnd, 4) - ld1(bnd, 1)),
                          fmaxf(ld1(bnd, 5) - ld1(bnd, 2), 2e-20f)) * 0.5f;
   const float is = __builtin_amdgcn_rcpf(hs);
@@ -116,6 +125,62 @@ __device__ __forceinline__ void pair_features(float ox, float oy, float oz, floa
   f[19] = ((float)Q - 3.5f) * (1.0f / 1.5f);
 }
 
+// Input features of kind FEAT_BOX: the ray origin / direction in the local
+// frame of the patch's box (centre c, largest half extent hs), the degrees and
+// the four corner control points.  The origin is moved along the ray to
+// where it enters the box (t_e = the slab entry, at least 0), so it lies in
+// the box's scale:
+//   f[0..2]   (o + t_e d - c) / hs          f[3..5]  d / |d|
+//   f[6..17]  (corner - c) / hs for the dehomogenised corners P00, Pp0, P0q, Ppq
+//   f[18], f[19]  degrees, (p - 3.5) / 1.5, (q - 3.5) / 1.5
+// Slab terms with a zero direction component are NaN or infinite; fminf /
+// fmaxf drop NaN operands, an infinite entry (a ray parallel to a slab it
+// lies outside) counts as 0, and the entry-point features are clamped to
+// [-4, 4].  Reciprocals are hardware approximations.
+template <class NP = const float4*, class BP = const float*>
+__device__ __forceinline__ void box_features(float ox, float oy, float oz, float dx, float dy, float dz, NP net,
+                                             int P, int Q, BP bnd, float* f) {
+  const float lo[3] = {ld1(bnd, 0), ld1(bnd, 1), ld1(bnd, 2)}, hi[3] = {ld1(bnd, 3), ld1(bnd, 4), ld1(bnd, 5)};
+  const float o[3] = {ox, oy, oz}, d[3] = {dx, dy, dz};
+  float te = 0.0f, hs = 2e-20f, c[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const float id = __builtin_amdgcn_rcpf(d[k]);
+    const float a = (lo[k] - o[k]) * id, b = (hi[k] - o[k]) * id;
+    te = fmaxf(te, fminf(a, b));
+    c[k] = 0.5f * (lo[k] + hi[k]);
+    hs = fmaxf(hs, 0.5f * (hi[k] - lo[k]));
+  }
+  if (!(te < 3.0e38f)) te = 0.0f;
+  const float is = __builtin_amdgcn_rcpf(hs);
+  const float dn = __builtin_amdgcn_rsqf(fmaf(dx, dx, fmaf(dy, dy, dz * dz)));
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    f[k] = fminf(fmaxf((fmaf(te, d[k], o[k]) - c[k]) * is, -4.0f), 4.0f);
+    f[3 + k] = d[k] * dn;
+  }
+  const int at[4] = {0, P * (Q + 1), Q, P * (Q + 1) + Q};  // P00, Pp0, P0q, Ppq
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const float4 x = ld4(net, at[k]);
+    const float iw = __builtin_amdgcn_rcpf(x.w);
+    f[6 + 3 * k] = (x.x * iw - c[0]) * is;
+    f[7 + 3 * k] = (x.y * iw - c[1]) * is;
+    f[8 + 3 * k] = (x.z * iw - c[2]) * is;
+  }
+  f[18] = ((float)P - 3.5f) * (1.0f / 1.5f);
+  f[19] = ((float)Q - 3.5f) * (1.0f / 1.5f);
+}
+
+// The NIN inputs of a pair for a feature kind.
+template <class NP = const float4*, class BP = const float*>
+__device__ __forceinline__ void kind_features(int kind, float ox, float oy, float oz, float dx, float dy, float dz,
+                                              float n1x, float n1y, float n1z, float n2x, float n2y, float n2z,
+                                              NP net, int P, int Q, BP bnd, float f[NIN]) {
+  if (kind == FEAT_BOX) box_features(ox, oy, oz, dx, dy, dz, net, P, Q, bnd, f);
+  else pair_features(ox, oy, oz, n1x, n1y, n1z, n2x, n2y, n2z, net, P, Q, bnd, f);
+}
+
 __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
@@ -133,13 +198,13 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 // Forward pass for nblk blocks of 16 pairs.  s_x[r][c] holds the features of
-// pair row r as BF16 (c = 8-value column group, features 20..31 zero).  The
-// output logits of pair r go to s_logit[r].  KEEP (training) also stores the
+// pair row r as BF16 (c = 8-value column group, inputs 20..31 zero).  The
+// output logits of pair r go to s_logit[r] (x, y, z: u, v, hit).  KEEP (training) also stores the
 // FP32 pre-activations of the three hidden layers, s_z[layer][unit][pair].
 // Must be called by all 64 lanes with nblk wave-uniform.
 template <bool KEEP>
 __device__ __forceinline__ void forward_wave(const bf16x8* __restrict__ tiles, const float* __restrict__ bias,
-                                             const bf16x8 (*s_x)[4], int nblk, int lane, float2* s_logit,
+                                             const bf16x8 (*s_x)[4], int nblk, int lane, float4* s_logit,
                                              float* s_z) {
   const int m = lane & 15, g = lane >> 4;
 #pragma unroll 1
@@ -184,7 +249,7 @@ __device__ __forceinline__ void forward_wave(const bf16x8* __restrict__ tiles, c
     f32x4 o = bias4(bias, BB4 + 4 * g);
     o = mfma(tp[(T_L4 + 0) * 64 + lane], h[0], o);
     o = mfma(tp[(T_L4 + 1) * 64 + lane], h[1], o);
-    if (g == 0) s_logit[16 * nb + m] = make_float2(o[0], o[1]);
+    if (g == 0) s_logit[16 * nb + m] = make_float4(o[0], o[1], o[2], 0.0f);
   }
 }
 
@@ -196,7 +261,7 @@ __device__ __forceinline__ void forward_wave(const bf16x8* __restrict__ tiles, c
 // by all 64 lanes with nblk wave-uniform.
 __device__ __forceinline__ void forward_seed(const bf16x8* __restrict__ tiles, const float* __restrict__ bias,
                                              const bf16x8 (*s_x)[3], int nblk, int lane, int row, float& lu,
-                                             float& lv) {
+                                             float& lv, float& lh) {
   const int m = lane & 15, g = lane >> 4;
   bf16x8 w1[4];
   f32x4 b1[4];
@@ -241,10 +306,11 @@ __device__ __forceinline__ void forward_seed(const bf16x8* __restrict__ tiles, c
       for (int i = 0; i < 8; i++) h[kb][i] = (__bf16)gelu(d[2 * kb + (i >> 2)][i & 3]);
     const f32x4 o = mfma(w41, h[1], mfma(w40, h[0], b4));
     // lane m (< 16) holds the logits of pair 16 nb + m
-    const float a = __shfl(o[0], row & 15), bv = __shfl(o[1], row & 15);
+    const float a = __shfl(o[0], row & 15), bv = __shfl(o[1], row & 15), c = __shfl(o[2], row & 15);
     if ((row >> 4) == nb) {
       lu = a;
       lv = bv;
+      lh = c;
     }
   }
 }
@@ -258,7 +324,7 @@ __device__ __forceinline__ void forward_seed(const bf16x8* __restrict__ tiles, c
 __device__ __forceinline__ void forward_seed_lean(const bf16x8* __restrict__ tiles,
                                                   const float* __restrict__ bias,
                                                   const bf16x8 (*s_x)[3], int nblk, int lane, int row, float& lu,
-                                                  float& lv) {
+                                                  float& lv, float& lh) {
   const int m = lane & 15, g = lane >> 4;
 #pragma unroll 1
   for (int nb = 0; nb < nblk; nb++) {
@@ -294,15 +360,16 @@ __device__ __forceinline__ void forward_seed_lean(const bf16x8* __restrict__ til
     tp = tiles + toff;
     const f32x4 o =
         mfma(tp[(T_L4 + 1) * 64 + lane], h[1], mfma(tp[(T_L4 + 0) * 64 + lane], h[0], bias4(bias, BB4 + 4 * g)));
-    const float a = __shfl(o[0], row & 15), bv = __shfl(o[1], row & 15);
+    const float a = __shfl(o[0], row & 15), bv = __shfl(o[1], row & 15), c = __shfl(o[2], row & 15);
     if ((row >> 4) == nb) {
       lu = a;
       lv = bv;
+      lh = c;
     }
   }
 }
 
-// Packs a pair's features as one BF16 row of s_x (32 values, 20..31 zero).
+// Packs a pair's inputs as one BF16 row of s_x (32 values, 20..31 zero).
 __device__ __forceinline__ void store_row(bf16x8 (*s_x)[4], int row, const float f[NIN], bool zero) {
   bf16x8 r[4];
 #pragma unroll
@@ -316,14 +383,14 @@ __device__ __forceinline__ void store_row(bf16x8 (*s_x)[4], int row, const float
   for (int c = 0; c < 4; c++) s_x[row][c] = r[c];
 }
 
-// Feature row of one pair (pair_features) written into s_x[row] as three
+// Input row of one pair (kind_features) written into s_x[row] as three
 // 8-value BF16 groups (values 20..23 zero).
 template <class NP = const float4*, class BP = const float*>
-__device__ __forceinline__ void store_features(bf16x8 (*s_x)[3], int row, float ox, float oy, float oz, float n1x,
-                                               float n1y, float n1z, float n2x, float n2y, float n2z, NP net,
-                                               int P, int Q, BP bnd) {
+__device__ __forceinline__ void store_features(bf16x8 (*s_x)[3], int row, int kind, float ox, float oy, float oz,
+                                               float dx, float dy, float dz, float n1x, float n1y, float n1z,
+                                               float n2x, float n2y, float n2z, NP net, int P, int Q, BP bnd) {
   float f[NIN];
-  pair_features(ox, oy, oz, n1x, n1y, n1z, n2x, n2y, n2z, net, P, Q, bnd, f);
+  kind_features(kind, ox, oy, oz, dx, dy, dz, n1x, n1y, n1z, n2x, n2y, n2z, net, P, Q, bnd, f);
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     bf16x8 r;
@@ -333,21 +400,23 @@ __device__ __forceinline__ void store_features(bf16x8 (*s_x)[3], int row, float 
   }
 }
 
-// Seeds for the active lanes of a wave from their feature rows: the active
+// Seeds for the active lanes of a wave from their input rows: the active
 // pairs' rows were written compacted (rows 0..k-1, see seed_row), so a wave
 // with k active lanes runs ceil(k/16) pair blocks.  Called by all 64 lanes;
-// u, v are written for active lanes only.  LEAN selects forward_seed_lean.
+// u, v and the hit probability ph are written for active lanes only.  LEAN
+// selects forward_seed_lean.
 template <bool LEAN>
 __device__ __forceinline__ void seed_wave_rows(const bf16x8* __restrict__ tiles, const float* __restrict__ bias,
-                                               bool active, int row, int n_act, float& u, float& v,
+                                               bool active, int row, int n_act, float& u, float& v, float& ph,
                                                const bf16x8 (*s_x)[3], int lane) {
   wave_lds_sync();
-  float lu = 0.0f, lv = 0.0f;
-  if (LEAN) forward_seed_lean(tiles, bias, s_x, (n_act + 15) >> 4, lane, row, lu, lv);
-  else forward_seed(tiles, bias, s_x, (n_act + 15) >> 4, lane, row, lu, lv);
+  float lu = 0.0f, lv = 0.0f, lh = 0.0f;
+  if (LEAN) forward_seed_lean(tiles, bias, s_x, (n_act + 15) >> 4, lane, row, lu, lv, lh);
+  else forward_seed(tiles, bias, s_x, (n_act + 15) >> 4, lane, row, lu, lv, lh);
   if (active) {
     u = fast_sigmoid(lu);
     v = fast_sigmoid(lv);
+    ph = fast_sigmoid(lh);
   }
   wave_lds_sync();  // the next call's row writes come after this call's reads
 }
@@ -361,12 +430,13 @@ __device__ __forceinline__ int seed_row(bool active, int lane, int& n_act) {
   return active ? __popcll(bal & below) : n_act + __popcll(~bal & below);
 }
 
-// Seeds for the active lanes of a wave: active pairs are compacted to the
-// first rows, so a wave with k active lanes runs ceil(k/16) pair blocks.
-// Called by all 64 lanes; u, v are written for active lanes only.
+// Seeds for the active lanes of a wave from their NIN inputs f: active pairs
+// are compacted to the first rows, so a wave with k active lanes runs
+// ceil(k/16) pair blocks.  Called by all 64 lanes; u, v and the hit
+// probability ph are written for active lanes only.
 __device__ __forceinline__ void seed_wave(const bf16x8* __restrict__ tiles, const float* __restrict__ bias,
-                                          bool active, const float f[NIN], float& u, float& v, bf16x8 (*s_x)[4],
-                                          float2* s_logit, int lane) {
+                                          bool active, const float f[NIN], float& u, float& v, float& ph,
+                                          bf16x8 (*s_x)[4], float4* s_logit, int lane) {
   const unsigned long long bal = __ballot(active);
   const int n_act = __popcll(bal);
   const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
@@ -376,9 +446,10 @@ __device__ __forceinline__ void seed_wave(const bf16x8* __restrict__ tiles, cons
   forward_wave<false>(tiles, bias, s_x, (n_act + 15) >> 4, lane, s_logit, nullptr);
   wave_lds_sync();
   if (active) {
-    const float2 lg = s_logit[row];
+    const float4 lg = s_logit[row];
     u = fast_sigmoid(lg.x);
     v = fast_sigmoid(lg.y);
+    ph = fast_sigmoid(lg.z);
   }
   wave_lds_sync();  // the next call's writes come after these reads
 }

@@ -125,6 +125,7 @@ struct PairOut {
   float2* uv;
   unsigned long long* count;
   int64_t cap;
+  int all;  // 1: also the solves that do not converge (or converge at t <= 0), uv = (-1, -1)
 };
 
 }  // namespace rtn

@@ -96,8 +96,8 @@ extern "C" int rtn_scene_trace_overflow(const rtn_scene* s, unsigned long long* 
 }
 
 extern "C" int rtn_trace_pairs(const rtn_scene* s, const float* const rays[6], int64_t n_rays,
-                               const rtn_trace_params* prm, int* pair_ray, int* pair_patch, float* pair_uv,
-                               int64_t cap, int64_t* count_host, void* stream) {
+                               const rtn_trace_params* prm, int all, int* pair_ray, int* pair_patch,
+                               float* pair_uv, int64_t cap, int64_t* count_host, void* stream) {
   RTN_CHECK(s && s->built, RTN_EINVAL, "rtn_trace_pairs: scene not built");
   RTN_CHECK(rays && prm && pair_ray && pair_patch && pair_uv && count_host && cap >= 0, RTN_EINVAL,
             "rtn_trace_pairs: bad arguments");
@@ -105,20 +105,26 @@ extern "C" int rtn_trace_pairs(const rtn_scene* s, const float* const rays[6], i
   RTN_CHECK(n_rays >= 0 && n_rays < ((int64_t)1 << 31), RTN_EINVAL, "rtn_trace_pairs: bad ray count");
   RTN_CHECK(prm->mode == RTN_MODE_WIDE || prm->mode == RTN_MODE_PERSISTENT, RTN_EUNSUPPORTED,
             "rtn_trace_pairs: runs the wide or persistent form");
-  RTN_CHECK(prm->seed_mode == RTN_SEED_PLAIN && prm->precision == RTN_PREC_FP32 && !prm->count, RTN_EINVAL,
-            "rtn_trace_pairs: plain seeds, FP32, no counters");
+  RTN_CHECK(prm->seed_mode == RTN_SEED_PLAIN && !prm->count, RTN_EINVAL, "rtn_trace_pairs: plain seeds, no counters");
   RTN_CHECK(prm->max_iter >= 1 && prm->max_iter <= 64, RTN_EINVAL, "rtn_trace_pairs: max_iter outside 1..64");
   *count_host = 0;
   if (n_rays == 0) return RTN_OK;
   hipStream_t st = (hipStream_t)stream;
   DevBuf<int> prim;
+  DevBuf<float> tuv;  // t, u, v of the closest hits: the FP64 correction pass of the mixed form needs them
   DevBuf<unsigned long long> cnt;
   RTN_TRY(prim.alloc((size_t)n_rays));
   RTN_TRY(cnt.alloc(1));
   RTN_HIP(hipMemsetAsync(cnt.ptr, 0, sizeof(unsigned long long), st));
   rtn_hits h{};
   h.prim = prim.ptr;
-  PairOut po{pair_ray, pair_patch, reinterpret_cast<float2*>(pair_uv), cnt.ptr, cap};
+  if (prm->precision == RTN_PREC_MIXED) {
+    RTN_TRY(tuv.alloc(3 * (size_t)n_rays));
+    h.t = tuv.ptr;
+    h.u = tuv.ptr + n_rays;
+    h.v = tuv.ptr + 2 * n_rays;
+  }
+  PairOut po{pair_ray, pair_patch, reinterpret_cast<float2*>(pair_uv), cnt.ptr, cap, all ? 1 : 0};
   RTN_TRY(ctl_reset(s->ctl.ptr, st));
   RTN_TRY(trace_wide_scene(s, rays, n_rays, prm, &h, nullptr, st, &po));
   unsigned long long c = 0;

@@ -34,7 +34,7 @@ using namespace rtn::fast;
 namespace {
 
 struct Cnt {  // per-ray work counters (counting builds only)
-  uint32_t nc, ne, ncull, nbox, nleaf, npt, ovf, nfl, nfb;  // (FP64 steps are counted by k_fix64)
+  uint32_t nc, ne, ncull, nbox, nleaf, npt, ovf, nfl, nfb, nskip;  // (FP64 steps are counted by k_fix64)
 };
 
 struct Lane {
@@ -222,6 +222,23 @@ __device__ __forceinline__ LeafOut leaf_dispatch(int cls, NP net, int p, const L
   }
 }
 
+// The learned form's use of the network's hit probability ph on a lane's
+// pair (test: the lane solves the pair): below hit_skip the pair is not
+// solved (a skip); below hit_gate it starts from the plain seed with the full
+// iteration count, as without the network.
+__device__ __forceinline__ void gate_pair(const WArgs& A, float ph, bool& test, bool& lrn, LeafIn& in,
+                                          uint32_t& nskip) {
+  if (!test) return;
+  if (ph < A.hit_skip) {
+    test = false;
+    nskip++;
+  } else if (ph < A.hit_gate) {
+    lrn = false;
+    in.gu = in.pu;
+    in.gv = in.pv;
+  }
+}
+
 template <int WB, int WSTK, bool PERSISTENT, bool COUNT, bool LEARNED>
 __device__ __forceinline__ void trace_wide_body(const WArgs& A) {
   __shared__ int s_code[WSTK][WB];
@@ -239,9 +256,9 @@ __device__ __forceinline__ void trace_wide_body(const WArgs& A) {
   int cand_first = 0, cand_left = 0;  // primitives of the current leaf still to test
   bool exhausted;         // no more rays to hand out
   int queue = blockIdx.x % kQueues, queues_left = kQueues;  // persistent: the wave's work queue
-  // work totals of the lane: all eleven when counting, otherwise only the
+  // work totals of the lane: all twelve when counting, otherwise only the
   // stack-overflow count (total 6), so no idle accumulators occupy registers
-  constexpr int NACC = COUNT ? 11 : 1;
+  constexpr int NACC = COUNT ? 12 : 1;
   unsigned long long acc[NACC] = {};
   exhausted = false;
   // One loop shape for both forms (it also keeps the one-ray form's register
@@ -321,7 +338,7 @@ __device__ __forceinline__ void trace_wide_body(const WArgs& A) {
         lane_finish<COUNT>(L, A, cn);
         if constexpr (COUNT) {
           acc[0] += cn.nc; acc[1] += cn.ne; acc[3] += cn.nbox; acc[4] += cn.nleaf; acc[5] += cn.npt;
-          acc[6] += cn.ovf; acc[7] += cn.nfl; acc[9] += cn.ncull; acc[10] += cn.nfb;
+          acc[6] += cn.ovf; acc[7] += cn.nfl; acc[9] += cn.ncull; acc[10] += cn.nfb; acc[11] += cn.nskip;
         } else if (cn.ovf) {
           acc[0] += 1;
           cn.ovf = 0;
@@ -377,6 +394,8 @@ __device__ __forceinline__ void trace_wide_body(const WArgs& A) {
     in.gv = gv;
     in.pu = gu;
     in.pv = gv;
+    bool test = pending;  // the pair is solved (the network's hit probability may skip it)
+    bool lrn = LEARNED;   // the pair starts from the network's seed
     if (LEARNED) {  // network seed for every pending pair of the wave (matrix cores), plain seed kept as fallback
       // with seed sharing, a lane whose quad leader (lane & ~3) tests the same
       // patch takes the leader's seed instead of an evaluation of its own
@@ -387,21 +406,25 @@ __device__ __forceinline__ void trace_wide_body(const WArgs& A) {
       int n_act;
       const int row = mlp::seed_row(own, lane, n_act);
       if (own) {
-        mlp::store_features(rows, row, L.o.x, L.o.y, L.o.z, L.rp.n1.x, L.rp.n1.y, L.rp.n1.z, L.rp.n2.x, L.rp.n2.y,
-                            L.rp.n2.z, A.cp + A.cp_off[pi], p, q, A.bounds + 6 * (int64_t)pi);
+        mlp::store_features(rows, row, A.mlp_kind, L.o.x, L.o.y, L.o.z, L.d.x, L.d.y, L.d.z, L.rp.n1.x, L.rp.n1.y,
+                            L.rp.n1.z, L.rp.n2.x, L.rp.n2.y, L.rp.n2.z, A.cp + A.cp_off[pi], p, q,
+                            A.bounds + 6 * (int64_t)pi);
       } else {
         const mlp::bf16x8 z = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
         for (int k = 0; k < 3; k++) rows[row][k] = z;
       }
-      mlp::seed_wave_rows<false>(A.mlp_tiles, A.mlp_bias, own, row, n_act, in.gu, in.gv, rows, lane);
+      float ph = 1.0f;
+      mlp::seed_wave_rows<false>(A.mlp_tiles, A.mlp_bias, own, row, n_act, in.gu, in.gv, ph, rows, lane);
       if (A.seed_share) {
-        const float lu = __shfl(in.gu, lead), lv = __shfl(in.gv, lead);
+        const float lu = __shfl(in.gu, lead), lv = __shfl(in.gv, lead), lp = __shfl(ph, lead);
         if (pending && !own) {
           in.gu = lu;
           in.gv = lv;
+          ph = lp;
         }
       }
+      gate_pair(A, ph, test, lrn, in, cn.nskip);
     }
     in.seed = L.r + A.seed_base;
     in.max_iter = A.max_iter;
@@ -409,7 +432,7 @@ __device__ __forceinline__ void trace_wide_body(const WArgs& A) {
     in.fallback_iters = A.fallback_iters;
     in.mixed = A.mixed;
     in.stall = A.stall_exit;
-    in.learned = LEARNED;
+    in.learned = lrn;
     in.thr_det = A.thr_det;
     in.thr_res = A.thr_res;
     in.cond_thr = A.cond_thr;
@@ -417,27 +440,28 @@ __device__ __forceinline__ void trace_wide_body(const WArgs& A) {
     const float4* net = A.cp + A.cp_off[pi];
     LeafOut res;
     res.flags = 0;
-    unsigned long long todo = __ballot(pending);
+    unsigned long long todo = __ballot(test);
     while (todo) {
       const int leader = __ffsll((long long)todo) - 1;
       const int c = __shfl(cls, leader);
-      const bool mine = pending && cls == c;
+      const bool mine = test && cls == c;
       if (mine) res = leaf_dispatch(c, net, p, in);
       todo &= ~__ballot(mine);
     }
-    if (pending) {
+    if (test) {
       if (COUNT) {
         cn.nc++;
         cn.ne += res.evals;
         cn.nfl += res.evals * ex::eval_flops(p, q);
         if (res.flags & 8) cn.nfb++;
       }
-      if (A.pair_count && (res.flags & 1) && 0.0f < res.t) {
+      const bool conv = (res.flags & 1) && 0.0f < res.t;
+      if (A.pair_count && (conv || A.pair_all)) {
         const unsigned long long k = atomicAdd(A.pair_count, 1ull);
         if ((int64_t)k < A.pair_cap) {
           A.pair_ray[k] = (int)L.r;
           A.pair_patch[k] = pi;
-          A.pair_uv[k] = make_float2(res.u, res.v);
+          A.pair_uv[k] = conv ? make_float2(res.u, res.v) : make_float2(-1.0f, -1.0f);
         }
       }
       if ((res.flags & 1) && 0.0f < res.t && res.t < L.t_max) {
@@ -624,6 +648,7 @@ __device__ __forceinline__ void packet_chunk(const WArgs& A, int64_t r0, unsigne
           in.pu = in.gu;
           in.pv = in.gv;
         }
+        bool lrn = LEARNED;  // the pair starts from the network's seed
         if (LEARNED) {  // network seeds of the wave's pairs on this patch (matrix cores), plain seed as fallback
           // with seed sharing every lane of an aligned quad whose first lane tests the patch takes that lane's seed
           const int lead = lane & ~3;
@@ -632,21 +657,25 @@ __device__ __forceinline__ void packet_chunk(const WArgs& A, int64_t r0, unsigne
           int n_act;
           const int row = mlp::seed_row(own, lane, n_act);
           if (own) {
-            mlp::store_features(s_x, row, L.o.x, L.o.y, L.o.z, L.rp.n1.x, L.rp.n1.y, L.rp.n1.z, L.rp.n2.x, L.rp.n2.y,
-                                L.rp.n2.z, net, p, q, as_const(A.bounds + 6 * (int64_t)pi));
+            mlp::store_features(s_x, row, A.mlp_kind, L.o.x, L.o.y, L.o.z, L.d.x, L.d.y, L.d.z, L.rp.n1.x,
+                                L.rp.n1.y, L.rp.n1.z, L.rp.n2.x, L.rp.n2.y, L.rp.n2.z, net, p, q,
+                                as_const(A.bounds + 6 * (int64_t)pi));
           } else {
             const mlp::bf16x8 z = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
             for (int k = 0; k < 3; k++) s_x[row][k] = z;
           }
-          mlp::seed_wave_rows<true>(A.mlp_tiles, A.mlp_bias, own, row, n_act, in.gu, in.gv, s_x, lane);
+          float ph = 1.0f;
+          mlp::seed_wave_rows<true>(A.mlp_tiles, A.mlp_bias, own, row, n_act, in.gu, in.gv, ph, s_x, lane);
           if (A.seed_share) {
-            const float lu = __shfl(in.gu, lead), lv = __shfl(in.gv, lead);
+            const float lu = __shfl(in.gu, lead), lv = __shfl(in.gv, lead), lp = __shfl(ph, lead);
             if (act && !own) {
               in.gu = lu;
               in.gv = lv;
+              ph = lp;
             }
           }
+          gate_pair(A, ph, act, lrn, in, cn.nskip);
         }
         if (act) {
           if (COUNT) cn.npt++;
@@ -656,7 +685,7 @@ __device__ __forceinline__ void packet_chunk(const WArgs& A, int64_t r0, unsigne
           in.fallback_iters = A.fallback_iters;
           in.mixed = A.mixed;
           in.stall = A.stall_exit;
-          in.learned = LEARNED;
+          in.learned = lrn;
           in.thr_det = A.thr_det;
           in.thr_res = A.thr_res;
           in.cond_thr = A.cond_thr;
@@ -687,7 +716,7 @@ __device__ __forceinline__ void packet_chunk(const WArgs& A, int64_t r0, unsigne
     lane_finish<COUNT>(L, A, cn);
     if constexpr (COUNT) {
       acc[0] += cn.nc; acc[1] += cn.ne; acc[3] += cn.nbox; acc[4] += cn.nleaf; acc[5] += cn.npt;
-      acc[6] += cn.ovf; acc[7] += cn.nfl; acc[9] += cn.ncull; acc[10] += cn.nfb;
+      acc[6] += cn.ovf; acc[7] += cn.nfl; acc[9] += cn.ncull; acc[10] += cn.nfb; acc[11] += cn.nskip;
     } else if (cn.ovf) {
       acc[0] += 1;
     }
@@ -703,7 +732,7 @@ __device__ __forceinline__ void packet_body(const WArgs& A) {
   __shared__ mlp::bf16x8 s_x[LEARNED ? 64 : 1][3];
   __shared__ int s_skip[64];  // per lane, skip_patch_of its ray
   const int lane = threadIdx.x & 63;
-  constexpr int NACC = COUNT ? 11 : 1;
+  constexpr int NACC = COUNT ? 12 : 1;
   unsigned long long acc[NACC] = {};
   if (!PERSISTENT) {
     packet_chunk<COUNT, LEARNED>(A, (int64_t)blockIdx.x * 64, acc, s_x, s_skip);
@@ -818,6 +847,7 @@ __global__ void __launch_bounds__(kFixBlock) k_fix64(const WArgs A) {
 namespace rtn {
 
 const void* mlp_tiles(const rtn_mlp* net);
+int mlp_kind(const rtn_mlp* net);
 const float* mlp_bias(const rtn_mlp* net);
 
 namespace {
@@ -909,6 +939,8 @@ int fill_wide_args(const rtn_scene* s, const float* const rays[6], int64_t n, co
   RTN_CHECK(prm->precision != RTN_PREC_MIXED || (hits->t && hits->u && hits->v), RTN_EINVAL,
             "rtn_trace: mixed precision needs the t, u and v outputs");
   RTN_CHECK(prm->flat_threshold <= 0.f, RTN_EUNSUPPORTED, "rtn_trace: flat-skip runs in the exact form only");
+  RTN_CHECK(prm->hit_gate >= 0.f && prm->hit_gate <= 1.f && prm->hit_skip >= 0.f && prm->hit_skip <= 1.f, RTN_EINVAL,
+            "rtn_trace: hit_gate and hit_skip lie in [0, 1]");
   A = WArgs{};
   A.nodes = s->wide.ptr;
   A.cp = s->cp.ptr;
@@ -938,6 +970,9 @@ int fill_wide_args(const rtn_scene* s, const float* const rays[6], int64_t n, co
     A.learned_iters = prm->learned_iters > 0 ? prm->learned_iters : prm->max_iter;
     A.fallback_iters = prm->fallback_iters > 0 ? prm->fallback_iters : prm->max_iter;
     A.seed_share = prm->seed_share;
+    A.mlp_kind = mlp_kind(prm->seed_net);
+    A.hit_gate = prm->hit_gate;
+    A.hit_skip = prm->hit_skip;
   }
   if (pairs) {
     A.pair_ray = pairs->ray;
@@ -945,6 +980,7 @@ int fill_wide_args(const rtn_scene* s, const float* const rays[6], int64_t n, co
     A.pair_uv = pairs->uv;
     A.pair_count = pairs->count;
     A.pair_cap = pairs->cap;
+    A.pair_all = pairs->all;
   }
   A.prim = hits->prim; A.t = hits->t; A.u = hits->u; A.v = hits->v; A.dpdu = hits->dpdu; A.dpdv = hits->dpdv;
   A.skip_prim = prm->skip_patch;

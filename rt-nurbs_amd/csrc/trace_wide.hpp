@@ -61,12 +61,17 @@ struct WArgs {
   int learned_iters;
   int fallback_iters;
   int seed_share;
-  // training pairs: every converged solve with t > 0 is appended
+  int mlp_kind;           // the network's feature kind (mlp::FeatureKind)
+  float hit_gate;         // below this hit probability: the plain seed, no learned attempt
+  float hit_skip;         // below this hit probability: the pair is not solved
+  // training pairs: every converged solve with t > 0 is appended (pair_all:
+  // every solve, uv = (-1, -1) for the others)
   int* pair_ray;
   int* pair_patch;
   float2* pair_uv;
   unsigned long long* pair_count;
   int64_t pair_cap;
+  int pair_all;
   // multi-scene batch: root node of each scene and the scene of each ray
   const int* scene_root;
   const int* ray_scene;

@@ -1,15 +1,18 @@
 """Training data and training of the learned Newton seed.
 
 Pairs come from the plain method itself: scenes from the config-1 generator
-(one bicubic 8x8 NURBS, 25 patches) and from the config-2 generator (groups
-of seeded random NURBS of degree 2..5, curvature-refined), each seen from
-random cameras, are traced with plain seeds; every converged Newton solve
-(ray, patch, u, v) with t > 0 is a training pair.  Generator seeds used here
-(>= 1000 for config 1, 20000 + k for config 2) are disjoint from the
-evaluation scenes (config 1 seed 0, config 2 seed 0).
+(one bicubic 8x8 NURBS, 25 patches) and, optionally, from the config-2
+generator (groups of seeded random NURBS of degree 2..5, curvature-refined),
+each seen from random cameras, are traced with the plain seed in the
+mixed-precision form without the network (FP64 seed, stall exit); every
+Newton solve the trace runs is a training pair: a converged solve with t > 0
+gives the target (u, v, 1), any other solve (u, v, 0) (a miss: only the hit
+output learns from it).  Generator seeds used here (>= 1000 for config 1,
+20000 + k for config 2) are disjoint from the evaluation scenes (config 1
+seeds 0 and 2000+, config 2 seed 0).
 
 The network is then trained on the GPU (rtn_mlp_train: BF16 MFMA forward and
-backward kernels, Adam) on the pair features.
+backward kernels, Adam) on the pair features of its feature kind.
 """
 from __future__ import annotations
 
@@ -18,7 +21,8 @@ import numpy as np
 from . import lib as L
 from . import scenes
 
-PAIR_PARAMS = dict(mode=L.MODE_WIDE, max_iter=8, thr_res=1e-5, thr_det=1e-5, hull_cull=2)
+PAIR_PARAMS = dict(mode=L.MODE_WIDE, max_iter=8, thr_res=1e-5, thr_det=1e-5, hull_cull=2, precision=L.PREC_MIXED,
+                   cond_thr=1e4, stall_exit=1)
 
 
 def _orbit_camera(rng, center, radius, W, H):
@@ -56,29 +60,39 @@ def training_scenes(n_c1=48, n_c2=24, res=512, seed=0):
     return out
 
 
-def collect_pairs(scene_list, max_pairs=10_000_000, spp=1, log=None):
-    """Traces each scene with plain seeds and gathers (features, targets) of
-    the converged solves into device arrays; returns (feats, targets, n)."""
-    feats = L.DeviceArray((max_pairs, L.MLP_NIN), np.float32)
-    targets = L.DeviceArray((max_pairs, 2), np.float32)
-    n = 0
+def collect_pairs(scene_list, max_hits=10_000_000, spp=1, kind=L.MLP_FEAT_PROJECTED, log=None):
+    """Traces each scene with the plain seed and gathers (features, targets)
+    of every Newton solve into device arrays until max_hits converged solves
+    are in; returns (feats (n, 20), targets (n, 3) = (u, v, hit), n, hits)."""
+    cap_all = 3 * max_hits
+    feats = L.DeviceArray((cap_all, L.MLP_NIN), np.float32)
+    targets = L.DeviceArray((cap_all, 3), np.float32)
+    n = hits = 0
     prm = L.params(**PAIR_PARAMS)
     for i, (surfs, cam) in enumerate(scene_list):
-        if n >= max_pairs:
+        if hits >= max_hits or n >= cap_all:
             break
         sc = L.Scene(surfs).build()
         rays = L.generate_rays(cam, spp=spp, jitter=spp > 1, seed=i + 1)
-        cap = max(1, 4 * rays.n)
-        pr, pp, puv, cnt = L.trace_pairs(sc, rays, prm, cap)
-        k = min(cnt, cap, max_pairs - n)
+        cap = max(1, 8 * rays.n)
+        pr, pp, puv, cnt = L.trace_pairs(sc, rays, prm, cap, all=True)
+        k = min(cnt, cap, cap_all - n)
         if k > 0:
-            L.pair_features(sc, rays, pr, pp, k, out=feats, out_offset=n)
-            L.check(L.load().rtn_memcpy(targets.ptr + n * 8, puv.ptr, k * 8, 2))
+            uv = puv.get()[:k]
+            h = uv[:, 0] >= 0.0
+            # stop at max_hits converged solves: keep the pairs up to that hit
+            if hits + int(h.sum()) > max_hits:
+                k = int(np.searchsorted(np.cumsum(h), max_hits - hits, side="left")) + 1
+                uv, h = uv[:k], h[:k]
+            t = np.concatenate([np.where(h[:, None], uv, 0.0), h[:, None]], 1).astype(np.float32)
+            L.pair_features(sc, rays, pr, pp, k, out=feats, out_offset=n, kind=kind)
+            L.check(L.load().rtn_memcpy(targets.ptr + n * 12, t.ctypes.data, k * 12, 0))
             n += k
+            hits += int(h.sum())
         if log:
-            log(f"scene {i}: {cnt} pairs (total {n})")
+            log(f"scene {i}: {cnt} solves (total {n}, {hits} hits)")
     L.synchronize()
-    return feats, targets, n
+    return feats, targets, n, hits
 
 
 class _View:
@@ -89,17 +103,19 @@ class _View:
         self._keep = a
 
 
-def train_seed_network(feats, targets, n, epochs=4, batch=32768, lr=6e-3, lr_final=1e-4, seed=0):
+def train_seed_network(feats, targets, n, epochs=4, batch=32768, lr=6e-3, lr_final=1e-4, seed=0,
+                       kind=L.MLP_FEAT_PROJECTED, hit_weight=0.05):
     """Trains a fresh network on the first n pairs; returns (Mlp, epoch losses)."""
-    net = L.Mlp(seed=seed)
+    net = L.Mlp(seed=seed, kind=kind)
     losses = net.train(_View(feats, n), _View(targets, n), epochs=epochs, batch=batch, lr=lr, lr_final=lr_final,
-                       seed=seed + 1)
+                       seed=seed + 1, hit_weight=hit_weight)
     return net, losses
 
 
-def build_seed_network(max_pairs=10_000_000, epochs=4, log=None, **kw):
-    """Training set generation and training in one call."""
+def build_seed_network(max_hits=10_000_000, epochs=4, kind=L.MLP_FEAT_PROJECTED, log=None, **kw):
+    """Training set generation and training in one call; returns
+    (Mlp, epoch losses, pairs, converged pairs)."""
     sl = training_scenes(**kw)
-    feats, targets, n = collect_pairs(sl, max_pairs=max_pairs, log=log)
-    net, losses = train_seed_network(feats, targets, n, epochs=epochs)
-    return net, losses, n
+    feats, targets, n, hits = collect_pairs(sl, max_hits=max_hits, kind=kind, log=log)
+    net, losses = train_seed_network(feats, targets, n, epochs=epochs, kind=kind)
+    return net, losses, n, hits

@@ -34,7 +34,7 @@ EXPORTS = [
     "rtn_scene_download", "rtn_trace_params_default", "rtn_trace", "rtn_calculate_intersect", "rtn_get_patch",
     "rtn_build_bvh", "rtn_patch_bounds", "rtn_refine", "rtn_scene_refine", "rtn_trace_f64", "rtn_event_create", "rtn_event_destroy",
     "rtn_event_record", "rtn_event_elapsed_ms", "rtn_trace_pairs", "rtn_mlp_create", "rtn_mlp_destroy",
-    "rtn_mlp_set_params", "rtn_mlp_get_params", "rtn_mlp_pair_features", "rtn_mlp_forward", "rtn_mlp_gradient",
+    "rtn_mlp_kind", "rtn_mlp_set_params", "rtn_mlp_get_params", "rtn_mlp_pair_features", "rtn_mlp_forward", "rtn_mlp_gradient",
     "rtn_mlp_train_params_default", "rtn_mlp_train", "rtn_rays_silhouette", "rtn_rays_secondary",
     "rtn_shade_lambert", "rtn_build_params_default", "rtn_scene_build_ex", "rtn_batch_create", "rtn_batch_destroy",
     "rtn_batch_patch_base", "rtn_batch_trace", "rtn_frame_params_default", "rtn_frame_create", "rtn_frame_destroy",
@@ -45,7 +45,8 @@ EXPORTS = [
     "rtn_calculate_intersect_noalloc", "rtn_frame_count",
 ]
 BVH_EXACT_SAH, BVH_BINNED_SAH = 0, 1
-MLP_NIN, MLP_HIDDEN, MLP_NPARAM = 20, 64, 9794
+MLP_NIN, MLP_HIDDEN, MLP_NOUT, MLP_NPARAM = 20, 64, 3, 9859
+MLP_FEAT_PROJECTED, MLP_FEAT_BOX = 0, 1
 
 
 class RtnError(RuntimeError):
@@ -67,12 +68,13 @@ class TraceParams(C.Structure):
                 ("hull_cull", C.c_int), ("count", C.c_int), ("learned_iters", C.c_int),
                 ("any_hit", C.c_int), ("refill_min", C.c_int), ("fallback_iters", C.c_int),
                 ("seed_share", C.c_int), ("stall_exit", C.c_int), ("seed_net", C.c_void_p),
-                ("skip_patch", C.c_void_p), ("t_tol", C.c_float)]
+                ("skip_patch", C.c_void_p), ("t_tol", C.c_float), ("hit_gate", C.c_float),
+                ("hit_skip", C.c_float)]
 
 
 class MlpTrainParams(C.Structure):
     _fields_ = [("epochs", C.c_int), ("batch", C.c_int), ("lr", C.c_float), ("lr_final", C.c_float),
-                ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float), ("reserved", C.c_int),
+                ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float), ("hit_weight", C.c_float),
                 ("seed", C.c_uint64)]
 
 
@@ -106,7 +108,7 @@ class Hits(C.Structure):
 class TraceTotals(C.Structure):
     _fields_ = [(k, C.c_ulonglong) for k in ("newton_calls", "newton_evals", "skips", "interior", "leaves",
                                              "patch_tests", "overflow", "newton_flops", "fp64_steps",
-                                             "hull_culled", "learned_fallbacks", "reserved")]
+                                             "hull_culled", "learned_fallbacks", "learned_skips")]
 
 
 class CameraDesc(C.Structure):
@@ -204,16 +206,17 @@ def load():
     L.rtn_event_destroy.argtypes = [vp]
     L.rtn_event_record.argtypes = [vp, vp]
     L.rtn_event_elapsed_ms.argtypes = [vp, vp, C.POINTER(C.c_float)]
-    L.rtn_trace_pairs.argtypes = [vp, C.POINTER(vp * 6), i64, C.POINTER(TraceParams), vp, vp, vp, i64,
+    L.rtn_trace_pairs.argtypes = [vp, C.POINTER(vp * 6), i64, C.POINTER(TraceParams), C.c_int, vp, vp, vp, i64,
                                   C.POINTER(i64), vp]
-    L.rtn_mlp_create.argtypes = [C.POINTER(vp), C.c_uint64]
+    L.rtn_mlp_create.argtypes = [C.POINTER(vp), C.c_uint64, C.c_int]
+    L.rtn_mlp_kind.argtypes = [vp]
     L.rtn_mlp_destroy.argtypes = [vp]
     L.rtn_mlp_destroy.restype = None
     L.rtn_mlp_set_params.argtypes = [vp, vp]
     L.rtn_mlp_get_params.argtypes = [vp, vp]
-    L.rtn_mlp_pair_features.argtypes = [vp, C.POINTER(vp * 6), i64, vp, vp, i64, vp, vp]
-    L.rtn_mlp_forward.argtypes = [vp, vp, vp, i64, vp, vp]
-    L.rtn_mlp_gradient.argtypes = [vp, vp, vp, i64, vp, C.POINTER(C.c_float)]
+    L.rtn_mlp_pair_features.argtypes = [vp, C.POINTER(vp * 6), i64, vp, vp, i64, C.c_int, vp, vp]
+    L.rtn_mlp_forward.argtypes = [vp, vp, vp, i64, vp, vp, vp]
+    L.rtn_mlp_gradient.argtypes = [vp, vp, vp, i64, C.c_float, vp, C.POINTER(C.c_float)]
     L.rtn_mlp_train_params_default.argtypes = [C.POINTER(MlpTrainParams)]
     L.rtn_mlp_train_params_default.restype = None
     L.rtn_mlp_train.argtypes = [vp, vp, vp, i64, C.POINTER(MlpTrainParams), vp, vp]
@@ -634,10 +637,11 @@ def trace_f64(scene, rays, max_iter=32, thr_det=1e-12, thr_res=1e-10, seed_base=
 class Mlp:
     """The seed network (rtn_mlp): BF16 MFMA inference and Adam training on the GPU."""
 
-    def __init__(self, seed=0, params=None):
+    def __init__(self, seed=0, params=None, kind=MLP_FEAT_PROJECTED):
         h = C.c_void_p()
-        check(load().rtn_mlp_create(C.byref(h), seed))
+        check(load().rtn_mlp_create(C.byref(h), seed, kind))
         self.h = h.value
+        self.kind = kind
         if params is not None:
             self.set_params(params)
 
@@ -651,21 +655,28 @@ class Mlp:
         check(load().rtn_mlp_get_params(self.h, w.ctypes.data))
         return w
 
-    def forward(self, feats, valid=None):
-        """feats: DeviceArray (n, 20) float32 -> DeviceArray (n, 2); rows with
-        valid == 0 (device int32 array, optional) are skipped (left zero)."""
+    def forward(self, feats, valid=None, hit=False):
+        """feats: DeviceArray (n, 20) float32 -> DeviceArray (n, 2) of (u,v)
+        (and, with hit=True, a DeviceArray (n,) of hit probabilities); rows
+        with valid == 0 (device int32 array, optional) are skipped (left zero)."""
         n = feats.shape[0]
         uv = DeviceArray((n, 2), np.float32)
         uv.fill_bytes(0)
-        check(load().rtn_mlp_forward(self.h, feats.ptr, valid.ptr if valid is not None else None, n, uv.ptr, None))
+        ph = None
+        if hit:
+            ph = DeviceArray((n,), np.float32)
+            ph.fill_bytes(0)
+        check(load().rtn_mlp_forward(self.h, feats.ptr, valid.ptr if valid is not None else None, n, uv.ptr,
+                                     ph.ptr if hit else None, None))
         synchronize()
-        return uv
+        return (uv, ph) if hit else uv
 
-    def gradient(self, feats, targets):
+    def gradient(self, feats, targets, hit_weight=0.05):
+        """Gradient and loss over device feats (n, 20) and targets (n, 3) = (u, v, hit)."""
         n = feats.shape[0]
         g = np.zeros(MLP_NPARAM, np.float32)
         loss = C.c_float()
-        check(load().rtn_mlp_gradient(self.h, feats.ptr, targets.ptr, n, g.ctypes.data, C.byref(loss)))
+        check(load().rtn_mlp_gradient(self.h, feats.ptr, targets.ptr, n, hit_weight, g.ctypes.data, C.byref(loss)))
         return g, loss.value
 
     def train(self, feats, targets, **kw):
@@ -689,26 +700,30 @@ class Mlp:
             pass
 
 
-def trace_pairs(scene, rays, prm, cap):
-    """Converged plain-method solves: (ray idx, patch, uv) device arrays and the count found."""
+def trace_pairs(scene, rays, prm, cap, all=False):
+    """Plain-seed Newton solves of a trace: (ray idx, patch, uv) device arrays
+    and the count found; converged solves with t > 0 only, or with all=True
+    every solve (uv = (-1, -1) where it does not converge)."""
     pr = DeviceArray((cap,), np.int32)
     pp = DeviceArray((cap,), np.int32)
     puv = DeviceArray((cap, 2), np.float32)
     cnt = C.c_int64()
     arr = rays.ptr_array()
-    check(load().rtn_trace_pairs(scene.h, C.byref(arr), rays.n, C.byref(prm), pr.ptr, pp.ptr, puv.ptr, cap,
+    check(load().rtn_trace_pairs(scene.h, C.byref(arr), rays.n, C.byref(prm), 1 if all else 0, pr.ptr, pp.ptr,
+                                 puv.ptr, cap,
                                  C.byref(cnt), None))
     return pr, pp, puv, cnt.value
 
 
-def pair_features(scene, rays, pair_ray, pair_patch, n, out=None, out_offset=0):
-    """Features (n, 20) of device pairs; written into out[out_offset:] when given."""
+def pair_features(scene, rays, pair_ray, pair_patch, n, out=None, out_offset=0, kind=MLP_FEAT_PROJECTED):
+    """Features (n, 20) of a kind (MLP_FEAT_*) of device pairs; written into
+    out[out_offset:] when given."""
     if out is None:
         out = DeviceArray((n, MLP_NIN), np.float32)
         out_offset = 0
     assert out_offset + n <= out.shape[0]
     arr = rays.ptr_array()
-    check(load().rtn_mlp_pair_features(scene.h, C.byref(arr), rays.n, pair_ray.ptr, pair_patch.ptr, n,
+    check(load().rtn_mlp_pair_features(scene.h, C.byref(arr), rays.n, pair_ray.ptr, pair_patch.ptr, n, kind,
                                        out.ptr + out_offset * MLP_NIN * 4, None))
     return out
 

@@ -1,6 +1,7 @@
 """Learned seed on the GPU against the CPU oracle (oracle/mlp_oracle.py):
-BF16 MFMA inference, the training kernels' gradient, pair features and the
-training pairs produced by the plain method.
+BF16 MFMA inference (seed and hit outputs), the training kernels' gradient
+(squared seed error on hits + cross-entropy of the hit output), both kinds
+of pair features and the training pairs produced by the plain method.
 
 Tolerances: the oracle's quantized mode rounds to BF16 at the same points as
 the kernels, leaving FP32 accumulation order, the hardware exp/reciprocal in
@@ -44,12 +45,15 @@ def test_forward_matches_oracle(rtl):
     valid[:64] = 1
     print("forward n=1000", flush=True)
     Xd = rtl.DeviceArray.from_host(X)
-    uv_all = net.forward(Xd).get()
+    uv_all, ph = net.forward(Xd, hit=True)
+    uv_all, ph = uv_all.get(), ph.get()
     uv_m = net.forward(Xd, rtl.DeviceArray.from_host(valid)).get()
+    assert np.array_equal(net.forward(Xd).get(), uv_all)  # the hit output is optional
     yq, _ = M.forward(w, X.astype(np.float64), quantize=True)
     y, _ = M.forward(w, X.astype(np.float64))
-    assert np.abs(uv_all - yq).max() < 2e-3
-    assert np.abs(uv_all - y).max() < 2e-2
+    got = np.concatenate([uv_all, ph[:, None]], 1)
+    assert np.abs(got - yq).max() < 2e-3
+    assert np.abs(got - y).max() < 2e-2
     m = valid == 1
     assert np.array_equal(uv_m[m], uv_all[m])  # packing does not change a pair's result
     assert np.all(uv_m[~m] == 0)
@@ -60,11 +64,12 @@ def test_gradient_matches_oracle(rtl):
     rng = np.random.default_rng(1)
     n = 300
     X = rng.normal(0, 1, (n, M.NIN)).astype(np.float32)
-    T = rng.uniform(0, 1, (n, 2)).astype(np.float32)
+    T = np.concatenate([rng.uniform(0, 1, (n, 2)), rng.uniform(size=(n, 1)) < 0.6], 1).astype(np.float32)
+    lam = 0.2
     print("gradient n=300", flush=True)
-    g, loss = net.gradient(rtl.DeviceArray.from_host(X), rtl.DeviceArray.from_host(T))
-    lq, gq = M.loss_and_grad(w, X.astype(np.float64), T.astype(np.float64), quantize=True)
-    lf, gf = M.loss_and_grad(w, X.astype(np.float64), T.astype(np.float64))
+    g, loss = net.gradient(rtl.DeviceArray.from_host(X), rtl.DeviceArray.from_host(T), hit_weight=lam)
+    lq, gq = M.loss_and_grad(w, X.astype(np.float64), T.astype(np.float64), quantize=True, hit_weight=lam)
+    lf, gf = M.loss_and_grad(w, X.astype(np.float64), T.astype(np.float64), hit_weight=lam)
     assert abs(loss - lq) < 1e-3 * lq and abs(loss - lf) < 2e-2 * lf
     for name, a, b in TENSORS:
         rq = np.linalg.norm(g[a:b] - gq[a:b]) / np.linalg.norm(gq[a:b])
@@ -101,6 +106,20 @@ def test_pairs_and_features(rtl, name, mk):
     assert 0 < cnt <= 4 * rays.n
     pr, pp, puv = pr.get()[:cnt], pp.get()[:cnt], puv.get()[:cnt]
     assert np.all((puv >= 0) & (puv <= 1))
+    # every solve: the converged ones again, the others with uv = (-1, -1);
+    # with the mixed-precision seed and the stall exit as well
+    for kw in (dict(), dict(precision=rtl.PREC_MIXED, cond_thr=1e4, stall_exit=1)):
+        prm_a = rtl.params(mode=rtl.MODE_WIDE, max_iter=8, thr_res=1e-5, thr_det=1e-5, hull_cull=2, **kw)
+        ar, ap, auv, acnt = rtl.trace_pairs(scene, rays, prm_a, 8 * rays.n, all=True)
+        ar, ap, auv = ar.get()[:acnt], ap.get()[:acnt], auv.get()[:acnt]
+        conv = auv[:, 0] >= 0
+        assert np.all((auv[conv] >= 0) & (auv[conv] <= 1)) and np.all(auv[~conv] == -1.0)
+        assert conv.sum() > 0 and (~conv).sum() > 0
+        _, tot = rtl.trace(scene, rays, rtl.params(count=1, **{k: getattr(prm_a, k) for k in (
+            "mode", "max_iter", "thr_res", "thr_det", "hull_cull", "precision", "cond_thr", "stall_exit")}))
+        assert acnt == tot["newton_calls"]  # one record per Newton solve of the trace
+        if not kw:
+            assert sorted(zip(ar[conv].tolist(), ap[conv].tolist())) == sorted(zip(pr.tolist(), pp.tolist()))
     # every closest hit of the same form is among the converged solves
     hits, _ = rtl.trace(scene, rays, prm)
     h = hits.get()
@@ -117,3 +136,7 @@ def test_pairs_and_features(rtl, name, mk):
     # FP32 projections with hardware reciprocals against FP64: far below the
     # BF16 step (2^-8 relative) the features are rounded to
     np.testing.assert_allclose(f, ref, rtol=1e-4, atol=1e-4)
+    fb = rtl.pair_features(scene, rays, rtl.DeviceArray.from_host(pr), rtl.DeviceArray.from_host(pp), cnt,
+                           kind=rtl.MLP_FEAT_BOX).get()
+    refb = M.box_features(o, d, bounds[pp], [nets[k] for k in pp], dl["pq"][pp, 0], dl["pq"][pp, 1])
+    np.testing.assert_allclose(fb, refb, rtol=1e-4, atol=1e-4)

@@ -1,7 +1,7 @@
 """Pins the CPU oracle of the learned seed (oracle/mlp_oracle.py) against
-independent references: torch's tanh-approximate GELU and autograd in
-float64, central finite differences, torch's BF16 rounding, and closed forms
-of the pair features."""
+independent references: torch's tanh-approximate GELU, binary cross-entropy
+with logits and autograd in float64, central finite differences, torch's
+BF16 rounding, and closed forms of both kinds of pair features."""
 import os
 import sys
 
@@ -25,7 +25,12 @@ def _params(seed, scale=1.0):
     return w
 
 
-def _torch_loss(w, X, T):
+def _targets(rng, n):
+    """(u, v, h) targets, about half of them hits."""
+    return np.concatenate([rng.uniform(0, 1, (n, 2)), (rng.uniform(size=(n, 1)) < 0.5).astype(np.float64)], 1)
+
+
+def _torch_loss(w, X, T, hit_weight=0.05):
     P = M.unpack(w)
     t = {k: torch.tensor(v, dtype=torch.float64, requires_grad=True) for k, v in P.items()}
     x = torch.tensor(X, dtype=torch.float64)
@@ -33,8 +38,12 @@ def _torch_loss(w, X, T):
     h = g(x @ t["W1"].T + t["b1"])
     h = g(h @ t["W2"].T + t["b2"])
     h = g(h @ t["W3"].T + t["b3"])
-    y = torch.sigmoid(h @ t["W4"].T + t["b4"])
-    loss = ((y - torch.tensor(T)) ** 2).sum() / X.shape[0]
+    z = h @ t["W4"].T + t["b4"]
+    T = torch.tensor(T)
+    hit = T[:, 2]
+    se = (((torch.sigmoid(z[:, :2]) - T[:, :2]) ** 2).sum(1) * hit).sum()
+    bce = torch.nn.functional.binary_cross_entropy_with_logits(z[:, 2], hit, reduction="sum")
+    loss = (se + hit_weight * bce) / X.shape[0]
     loss.backward()
     return loss.item(), M.pack({k: t[k].grad.numpy() for k in t})
 
@@ -51,18 +60,23 @@ def test_gelu_matches_torch_tanh_gelu():
 def test_gradient_matches_autograd():
     rng = np.random.default_rng(3)
     X = rng.normal(0, 1, (37, M.NIN))
-    T = rng.uniform(0, 1, (37, 2))
+    T = _targets(rng, 37)
     w = _params(5)
-    l0, g0 = M.loss_and_grad(w, X, T)
-    l1, g1 = _torch_loss(w, X, T)
-    assert abs(l0 - l1) < 1e-14
-    np.testing.assert_allclose(g0, g1, rtol=1e-10, atol=1e-15)
+    for lam in (0.05, 1.0):
+        l0, g0 = M.loss_and_grad(w, X, T, hit_weight=lam)
+        l1, g1 = _torch_loss(w, X, T, hit_weight=lam)
+        assert abs(l0 - l1) < 1e-14
+        np.testing.assert_allclose(g0, g1, rtol=1e-10, atol=1e-15)
+    # misses carry no (u, v) error: their (u, v) targets do not matter
+    T2 = T.copy()
+    T2[T[:, 2] == 0, :2] = 0.123
+    np.testing.assert_array_equal(M.loss_and_grad(w, X, T2)[1], M.loss_and_grad(w, X, T)[1])
 
 
 def test_gradient_matches_finite_differences():
     rng = np.random.default_rng(4)
     X = rng.normal(0, 1, (11, M.NIN))
-    T = rng.uniform(0, 1, (11, 2))
+    T = _targets(rng, 11)
     w = _params(6, scale=2.0)
     _, g = M.loss_and_grad(w, X, T)
     # a few entries of every tensor
@@ -84,7 +98,7 @@ def test_quantized_gradient_close_to_fp64():
     parts in a thousand, not more."""
     rng = np.random.default_rng(8)
     X = rng.normal(0, 1, (200, M.NIN))
-    T = rng.uniform(0, 1, (200, 2))
+    T = _targets(rng, 200)
     w = _params(9)
     _, g = M.loss_and_grad(w, X, T)
     _, gq = M.loss_and_grad(w, X, T, quantize=True)
@@ -161,9 +175,14 @@ def test_features_closed_forms():
 
 def test_forward_known_answers():
     w = np.zeros(M.NPARAM)
-    w[M.OFF_B4:] = [0.0, np.log(3.0)]   # sigmoid(0) = 0.5, sigmoid(log 3) = 0.75
+    w[M.OFF_B4:] = [0.0, np.log(3.0), -np.log(4.0)]   # sigmoid(0) = 0.5, sigmoid(log 3) = 0.75, 0.2
     y, _ = M.forward(w, np.ones((3, M.NIN)))
-    np.testing.assert_allclose(y, [[0.5, 0.75]] * 3)
+    np.testing.assert_allclose(y, [[0.5, 0.75, 0.2]] * 3)
+    # the loss at those outputs: a hit at (0.5, 0.5) costs 0.0625 + lambda (-log 0.2), a miss lambda (-log 0.8)
+    T = np.array([[0.5, 0.5, 1.0], [0.9, 0.1, 0.0], [0.5, 0.75, 1.0]])
+    lam = 0.3
+    want = (0.0625 + lam * -np.log(0.2) + lam * -np.log(0.8) + lam * -np.log(0.2)) / 3
+    assert M.loss_and_grad(w, np.ones((3, M.NIN)), T, hit_weight=lam)[0] == pytest.approx(want, rel=1e-12)
     # one path: x0 -> unit 0 of each layer -> u logit, weight 1 everywhere
     w = np.zeros(M.NPARAM)
     w[M.OFF_W1 + 0] = 1.0
@@ -174,4 +193,53 @@ def test_forward_known_answers():
     x[0, 0] = 1.5
     y, _ = M.forward(w, x)
     assert y[0, 0] == pytest.approx(M.sigmoid(M.gelu(M.gelu(M.gelu(1.5)))))
-    assert y[0, 1] == 0.5
+    assert y[0, 1] == 0.5 and y[0, 2] == 0.5
+
+
+def test_box_features_closed_forms():
+    # a ray along +z through the centre (1, 2) of a flat unit net at z = 3:
+    # it enters the (flat) box at its centre; corners at (+-1, +-1, 0) / 0.5 scale
+    bounds = np.array([[0.5, 1.5, 3.0, 1.5, 2.5, 3.0]])
+    o, d = np.array([[1.0, 2.0, -7.0]]), np.array([[0.0, 0.0, 2.0]])
+    for p, q in ((2, 2), (3, 5), (5, 4)):
+        f = M.box_features(o, d, bounds, [_flat_net(p, q)], [p], [q])[0]
+        np.testing.assert_allclose(f[:3], 0.0, atol=1e-12)
+        np.testing.assert_allclose(f[3:6], [0, 0, 1])
+        np.testing.assert_allclose(f[6:18].reshape(4, 3), [[-1, -1, 0], [1, -1, 0], [-1, 1, 0], [1, 1, 0]])
+        np.testing.assert_allclose(f[18:], [(p - 3.5) / 1.5, (q - 3.5) / 1.5])
+    # the origin moved along the ray (outside the box) leaves the features
+    # unchanged; an origin inside the box is its own entry point
+    rng = np.random.default_rng(7)
+    lo = rng.normal(size=(6, 3))
+    b = np.concatenate([lo, lo + rng.uniform(0.2, 2, (6, 3))], 1)
+    c = 0.5 * (b[:, :3] + b[:, 3:])
+    d = rng.normal(size=(6, 3))
+    o = c - 10.0 * d  # far in front of each box, aimed at its centre
+    p = np.array([2, 3, 3, 4, 5, 2])
+    nets = [np.concatenate([rng.normal(size=((k + 1) * (k + 1), 3)), rng.uniform(0.5, 2, ((k + 1) * (k + 1), 1))], 1)
+            for k in p]
+    f0 = M.box_features(o, d, b, nets, p, p)
+    f1 = M.box_features(o + 3.0 * d, d, b, nets, p, p)
+    np.testing.assert_allclose(f0, f1, atol=1e-12)
+    hs = 0.5 * (b[:, 3:] - b[:, :3]).max(1)
+    entry = f0[:, :3] * hs[:, None] + c  # on the box surface: one coordinate at a face
+    face = np.minimum(np.abs(entry - b[:, :3]), np.abs(entry - b[:, 3:])).min(1)
+    np.testing.assert_allclose(face, 0.0, atol=1e-9)
+    inside = M.box_features(c + 0.1 * d, d, b, nets, p, p)
+    np.testing.assert_allclose(inside[:, :3], 0.1 * d / hs[:, None], atol=1e-12)
+    # the corners are the dehomogenised net corners; weights scale nothing else
+    k0 = nets[0].reshape(3, 3, 4)
+    np.testing.assert_allclose(f0[0, 6:9], (k0[0, 0, :3] / k0[0, 0, 3] - c[0]) / hs[0], atol=1e-12)
+    np.testing.assert_allclose(f0[0, 15:18], (k0[2, 2, :3] / k0[2, 2, 3] - c[0]) / hs[0], atol=1e-12)
+    # similarity invariance (translate + scale), direction scale invariance
+    s, t = 2.5, np.array([-1.0, 4.0, 0.5])
+    nets2 = [np.concatenate([n[:, :3] * s + t * n[:, 3:], n[:, 3:]], 1) for n in nets]  # homogeneous: s x w + t w
+    f2 = M.box_features(o * s + t, 3.0 * d, b * s + np.tile(t, 2), nets2, p, p)
+    np.testing.assert_allclose(f0, f2, atol=1e-10)
+    # a zero direction component: the slab test drops that axis
+    f3 = M.box_features(np.array([[1.0, 2.0, 10.0]]), np.array([[0.0, 0.0, -1.0]]), bounds, [_flat_net(2, 2)], [2], [2])
+    np.testing.assert_allclose(f3[0, :3], 0.0, atol=1e-12)
+    # parallel to a slab it lies outside: an infinite entry counts as 0, and
+    # the entry point is clamped to 4 half extents
+    f4 = M.box_features(np.array([[-9.0, 2.0, 10.0]]), np.array([[0.0, 0.0, -1.0]]), bounds, [_flat_net(2, 2)], [2], [2])
+    np.testing.assert_allclose(f4[0, :3], [-4.0, 0.0, 4.0], atol=1e-12)
