@@ -537,7 +537,6 @@ def bench_c3(args, L, scenes):
         results[name]["fp64_silhouette"] = error_stats(sub, rsub)
     head = "wide_hull2_mixed"
     H = results[head]
-    write_detail(args, {"forms": results})
     summ = {}
     for k in ("plain", "wide_hull2", head):
         b = brief(results[k])
@@ -547,8 +546,18 @@ def bench_c3(args, L, scenes):
         summ[k] = b
     wl = {"workload": "config 1 scene, 65,536 rays of which 13,107 silhouette rays (grazing 0-5 deg)",
           "rays": rays.n, "headline_form": head}
+    cpu = None
+    if not args.no_cpu:  # the reference's own trace code on the same rays (a single-degree scene it can run)
+        try:
+            dl = scene.download()
+            rh = [np.asarray(a) for a in rays.get()]
+            cpu = cpu_reference(dl, rh) or cpu_baseline(dl, rh)
+            log(f"cpu baseline ({cpu['kind']}): {cpu['value']:.3f} Mrays/s on {cpu['cores']} thread(s)")
+        except Exception as e:  # the baseline is a report, never the measurement
+            log(f"cpu baseline failed: {e!r}")
+    write_detail(args, {"forms": results, "cpu_baseline": cpu})
     return make_line(args, H["mrays_s"], H["kernel_ms"], "f32+f64", wl,
-                     {"summary": summ, "roofline": roofline(H["tflops"], head, "c3"), "cpu_baseline": None,
+                     {"summary": summ, "roofline": roofline(H["tflops"], head, "c3"), "cpu_baseline": short_cpu(cpu),
                       "detail": args.detail})
 
 
@@ -638,10 +647,60 @@ def bench_c5(args, L, scenes):
     wl["newton_iters_per_ray"] = round((c_prim["newton_evals"] + c_sec["newton_evals"]) / total_rays, 4)
     rf = roofline(tflops, "frame", "c5")
     rf["kernel"] = f"frame: primary {names[pp]} + secondary {names[ps]} launches"
+    cpu = None
+    if not args.no_cpu:
+        # the oracle's plain method on the host cores over a bounded sample of
+        # the frame's primary rays, spread over all scenes (strided), each scene's
+        # sample traced on its own patches; secondaries are not sampled
+        try:
+            cpu = cpu_frame_sample(L, built, rays, ids, target_s=12.0)
+            log(f"cpu baseline (port): {cpu['value']:.3f} Mrays/s on {cpu['cores']} threads ({cpu['sample']})")
+        except Exception as e:  # the baseline is a report, never the measurement
+            log(f"cpu baseline failed: {e!r}")
     write_detail(args, {"autotune": table, "frame_ms": wl["frame_ms"], "work_primary": c_prim,
-                        "work_secondary": c_sec, "box_work_per_lane": [b_prim, b_sec]})
+                        "work_secondary": c_sec, "box_work_per_lane": [b_prim, b_sec], "cpu_baseline": cpu})
     return make_line(args, total_rays / ms / 1e3, ms, "f32", wl,
-                     {"roofline": rf, "cpu_baseline": None, "detail": args.detail})
+                     {"roofline": rf, "cpu_baseline": short_cpu(cpu), "detail": args.detail})
+
+
+def cpu_frame_sample(L, built, rays, ids, target_s=12.0):
+    """The oracle's plain FP32 method on the host cores over a strided sample of
+    a multi-scene frame's primary rays: each sampled ray is traced on its own
+    scene's patches.  The sample grows until the traces take about target_s."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    O.lib()
+    cores = max(1, min(16, os.cpu_count() or 1))
+    rh = [np.asarray(a) for a in rays.get()]
+    sid = ids.get()
+    geo = {}
+    n_all = len(sid)
+    n = 4096
+    while True:
+        stride = max(1, n_all // n)
+        sel = np.arange(0, n_all, stride)[:n]
+        dt = 0.0
+        for k in np.unique(sid[sel]):
+            if k not in geo:
+                dl = built[int(k)].download()
+                ps = O.PatchSet.__new__(O.PatchSet)
+                ps.p = np.ascontiguousarray(dl["pq"][:, 0], np.int32)
+                ps.q = np.ascontiguousarray(dl["pq"][:, 1], np.int32)
+                ps.off = np.ascontiguousarray(dl["off"], np.int32)
+                ps.cp = np.ascontiguousarray(dl["cp"], np.float32)
+                geo[k] = (dl["nodes"], ps)
+            m = sel[sid[sel] == k]
+            sample = [np.ascontiguousarray(a[m]) for a in rh]
+            t0 = time.perf_counter()
+            O.trace_f32(geo[k][0], geo[k][1], sample, max_iter=8, thr_res=1e-5, thr_det=1e-5, seed_base=0,
+                        threads=cores)
+            dt += time.perf_counter() - t0
+        if dt >= target_s * 0.4 or n >= n_all:
+            break
+        n = min(n_all, int(n * min(8.0, target_s / max(dt, 1e-3))))
+    return {"value": len(sel) / dt / 1e6, "unit": "Mrays/s", "cores": cores, "kind": "port",
+            "sample": f"{len(sel)} of {n_all} primary rays (every {stride}th, all scenes), plain method FP32 "
+                      f"(8 iterations, tol 1e-5), {dt:.1f} s on {cores} threads", "host": host_info()}
 
 
 def calibrate_copy(nbytes):

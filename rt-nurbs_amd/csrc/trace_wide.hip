@@ -788,14 +788,17 @@ __global__ void k_zero_words(unsigned int* p, int n) {
 // prim = -2 - patch by the trace) gets the FP64 correction (correct64_rt:
 // guarded FP64 Newton steps from its FP32 (u,v) on FP64 ray planes); t, u, v,
 // dpdu, dpdv are replaced and prim restored.  A pass over prim, so the trace kernels carry no FP64 code.
-// Flagged rays are rare and scattered: each block takes a range of
-// kFixBlock * kFixPer rays, loads their patch indices with independent loads,
-// compacts the flagged rays into an LDS list and runs their FP64 corrections
-// one per thread.  A correction is a chain of dependent loads and FP64
-// evaluations; with large ranges the grid is about one resident round, so
-// the chains of all flagged rays run side by side instead of a few per block
-// in many rounds.
-constexpr int kFixBlock = 256, kFixPer = 64;  // 16,384 rays per block: about one block per SIMD pair on 8.3M rays
+// Flagged rays are scattered, from a few thousand in 8.3M primary rays to
+// a tenth of a frame's secondary rays.  The grid is one resident round of
+// blocks; each block walks ray ranges of kFixBlock * kFixPer (grid-stride),
+// loads a range's patch indices with independent loads and appends its
+// flagged rays to an LDS list, and runs the list's FP64 corrections one per
+// thread whenever it holds more than a range's worth (and at the end).  A
+// correction is a chain of dependent loads and FP64 evaluations: sparse flags
+// cost one chain per thread however many ranges a block walks, and dense
+// flags spread over all of a block's threads.
+constexpr int kFixBlock = 256, kFixPer = 8;
+constexpr int kFixRange = kFixBlock * kFixPer, kFixList = 2 * kFixRange;
 
 __device__ __forceinline__ bool fix_one(const WArgs& A, int64_t r) {
   const int pi = -2 - A.prim[r];
@@ -816,26 +819,43 @@ __device__ __forceinline__ bool fix_one(const WArgs& A, int64_t r) {
   return ok;
 }
 
+// Runs and empties the block's list (called by every thread of the block).
+__device__ __forceinline__ void fix_list(const WArgs& A, const uint32_t* s_list, int* s_n, unsigned long long& done) {
+  __syncthreads();
+  const int nl = *s_n;
+  for (int i = (int)threadIdx.x; i < nl; i += kFixBlock)
+    if (fix_one(A, (int64_t)s_list[i])) done++;
+  __syncthreads();
+  if (threadIdx.x == 0) *s_n = 0;
+  __syncthreads();
+}
+
 __global__ void __launch_bounds__(kFixBlock) k_fix64(const WArgs A) {
-  __shared__ uint32_t s_list[kFixBlock * kFixPer];  // room for every ray of the block's range
+  __shared__ uint32_t s_list[kFixList];
   __shared__ int s_n;
   const int tid = threadIdx.x;
   if (tid == 0) s_n = 0;
   __syncthreads();
-  const int64_t base = (int64_t)blockIdx.x * (kFixBlock * kFixPer);
-  int code[kFixPer];
-#pragma unroll
-  for (int k = 0; k < kFixPer; k++) {
-    const int64_t r = base + (int64_t)k * kFixBlock + tid;
-    code[k] = r < A.n_rays ? A.prim[r] : -1;
-  }
-#pragma unroll
-  for (int k = 0; k < kFixPer; k++)
-    if (code[k] <= -2) s_list[atomicAdd(&s_n, 1)] = (uint32_t)(base + (int64_t)k * kFixBlock + tid);
-  __syncthreads();
+  const int64_t nranges = (A.n_rays + kFixRange - 1) / kFixRange;
   unsigned long long done = 0;
-  for (int i = tid; i < s_n; i += kFixBlock)
-    if (fix_one(A, (int64_t)s_list[i])) done++;
+  for (int64_t g = blockIdx.x; g < nranges; g += gridDim.x) {  // block-uniform loop
+    const int64_t base = g * kFixRange;
+    int code[kFixPer];
+#pragma unroll
+    for (int k = 0; k < kFixPer; k++) {
+      const int64_t r = base + (int64_t)k * kFixBlock + tid;
+      code[k] = r < A.n_rays ? A.prim[r] : -1;
+    }
+    // s_n <= kFixRange here, so a range's appends stay within kFixList
+#pragma unroll
+    for (int k = 0; k < kFixPer; k++)
+      if (code[k] <= -2) s_list[atomicAdd(&s_n, 1)] = (uint32_t)(base + (int64_t)k * kFixBlock + tid);
+    __syncthreads();
+    const int nl = s_n;
+    __syncthreads();  // every thread has read s_n before any appends to it again
+    if (nl > kFixRange) fix_list(A, s_list, &s_n, done);  // block-uniform
+  }
+  fix_list(A, s_list, &s_n, done);
   if (A.totals) {
     for (int sft = 32; sft > 0; sft >>= 1) done += __shfl_xor(done, sft);
     if ((tid & 63) == 0 && done) atomicAdd(A.totals + 8, done);
@@ -1010,8 +1030,15 @@ int ctl_reset(TraceCtl* ctl, hipStream_t stream) {
 
 int launch_fix64(int64_t n_max, hipStream_t stream, const WArgs& args) {
   if (!args.mixed || n_max <= 0) return RTN_OK;
-  const int64_t per_block = (int64_t)kFixBlock * kFixPer;
-  const unsigned blocks = (unsigned)std::max<int64_t>(1, (n_max + per_block - 1) / per_block);
+  // one resident round (at most one block per range)
+  static const int resident = [] {  // blocks per compute unit x compute units, queried once
+    int dev = 0, cus = 256, occ = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_fix64, kFixBlock, 0) != hipSuccess || occ < 1) occ = 1;
+    return occ * cus;
+  }();
+  const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n_max + kFixRange - 1) / kFixRange, resident));
   hipLaunchKernelGGL(k_fix64, dim3(blocks), dim3(kFixBlock), 0, stream, args);
   RTN_HIP(hipGetLastError());
   return RTN_OK;
