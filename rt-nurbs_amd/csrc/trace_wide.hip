@@ -902,9 +902,10 @@ void launch_shape(bool learned, int bpc, int64_t n_max, hipStream_t stream, cons
 }  // namespace
 
 const LaunchShape kDefaultShape{WB_DEFAULT, WSTK_DEFAULT, 8};
-// persistent packet form: resident single-wave blocks per compute unit (capped
-// by what the kernel's registers keep resident)
-constexpr int kPacketBlocksPerCU = 16;
+// persistent packet form: resident single-wave blocks per compute unit, capped
+// by what the kernel's registers keep resident (20 for the five-wave plain
+// kernel, 12 for the three-wave learned kernel): every SIMD slot holds a wave
+constexpr int kPacketBlocksPerCU = 32;
 const LaunchShape kTunedShapes[] = {{64, 16, 8},  {64, 24, 8},  {64, 32, 8},  {128, 16, 8}, {128, 24, 8},
                                     {128, 32, 8}, {256, 16, 8}, {256, 24, 8}, {256, 32, 8}};
 const int kNumTunedShapes = 9;
