@@ -149,6 +149,32 @@ __device__ __forceinline__ void record(Lane& L, const WArgs& A, int pi, float t,
   if (A.dpdv) { A.dpdv[3 * r] = Sv.x; A.dpdv[3 * r + 1] = Sv.y; A.dpdv[3 * r + 2] = Sv.z; }
 }
 
+// The packet form keeps each lane's closest-hit record (u, v, dpdu, dpdv; t
+// is L.t_max) in LDS, one column per lane, and writes it once when the chunk
+// is done: the 64 lanes of a wave finish together, so the stores of a chunk
+// are contiguous, instead of scattered write-throughs on every improving hit.
+constexpr int kRecWords = 8;
+__device__ __forceinline__ void record_lds(Lane& L, float (*rec)[64], int lane, int pi, float t, float u, float v,
+                                           f3 Su, f3 Sv, bool fix) {
+  L.t_max = t;
+  L.best = pi;
+  L.fix = fix;
+  rec[0][lane] = u;
+  rec[1][lane] = v;
+  rec[2][lane] = Su.x; rec[3][lane] = Su.y; rec[4][lane] = Su.z;
+  rec[5][lane] = Sv.x; rec[6][lane] = Sv.y; rec[7][lane] = Sv.z;
+}
+
+__device__ __forceinline__ void write_record(const Lane& L, const WArgs& A, const float (*rec)[64], int lane) {
+  if (L.best < 0) return;
+  const int64_t r = L.r;
+  if (A.t) A.t[r] = L.t_max;
+  if (A.u) A.u[r] = rec[0][lane];
+  if (A.v) A.v[r] = rec[1][lane];
+  if (A.dpdu) { A.dpdu[3 * r] = rec[2][lane]; A.dpdu[3 * r + 1] = rec[3][lane]; A.dpdu[3 * r + 2] = rec[4][lane]; }
+  if (A.dpdv) { A.dpdv[3 * r] = rec[5][lane]; A.dpdv[3 * r + 1] = rec[6][lane]; A.dpdv[3 * r + 2] = rec[7][lane]; }
+}
+
 // Result of one patch test.
 struct LeafOut {
   float t, u, v;
@@ -541,7 +567,7 @@ constexpr int PKT_STACK = 64;  // one entry per lane of the stack register
 // calling wave; work totals accumulate into acc.
 template <bool COUNT, bool LEARNED, int NACC>
 __device__ __forceinline__ void packet_chunk(const WArgs& A, int64_t r0, unsigned long long (&acc)[NACC],
-                                             mlp::bf16x8 (*s_x)[3], int* s_skip) {
+                                             mlp::bf16x8 (*s_x)[3], int* s_skip, float (*s_rec)[64]) {
   const int lane = threadIdx.x & 63;
   Lane L;
   Cnt cn{};
@@ -698,7 +724,7 @@ __device__ __forceinline__ void packet_chunk(const WArgs& A, int64_t r0, unsigne
             if (res.flags & 8) cn.nfb++;
           }
           if ((res.flags & 1) && 0.0f < res.t && res.t < L.t_max) {
-            record(L, A, pi, res.t, res.u, res.v, res.Su, res.Sv, (res.flags & 16) != 0);
+            record_lds(L, s_rec, lane, pi, res.t, res.u, res.v, res.Su, res.Sv, (res.flags & 16) != 0);
             if (A.any_hit || (A.ray_kind && A.ray_kind[L.r] == 0)) {  // occlusion query: the first hit ends the ray
               active = false;
               in_leaf = false;
@@ -713,6 +739,7 @@ __device__ __forceinline__ void packet_chunk(const WArgs& A, int64_t r0, unsigne
   }
   }
   if (had_ray) {
+    write_record(L, A, s_rec, lane);
     lane_finish<COUNT>(L, A, cn);
     if constexpr (COUNT) {
       acc[0] += cn.nc; acc[1] += cn.ne; acc[3] += cn.nbox; acc[4] += cn.nleaf; acc[5] += cn.npt;
@@ -731,11 +758,12 @@ template <bool PERSISTENT, bool COUNT, bool LEARNED>
 __device__ __forceinline__ void packet_body(const WArgs& A) {
   __shared__ mlp::bf16x8 s_x[LEARNED ? 64 : 1][3];
   __shared__ int s_skip[64];  // per lane, skip_patch_of its ray
+  __shared__ float s_rec[kRecWords][64];  // per lane, its closest hit's record
   const int lane = threadIdx.x & 63;
   constexpr int NACC = COUNT ? 12 : 1;
   unsigned long long acc[NACC] = {};
   if (!PERSISTENT) {
-    packet_chunk<COUNT, LEARNED>(A, (int64_t)blockIdx.x * 64, acc, s_x, s_skip);
+    packet_chunk<COUNT, LEARNED>(A, (int64_t)blockIdx.x * 64, acc, s_x, s_skip, s_rec);
   } else {
     // the work queues hand out 64-ray chunks (one position per chunk here)
     const int64_t nchunks = (A.n_rays + 63) / 64;
@@ -750,7 +778,7 @@ __device__ __forceinline__ void packet_body(const WArgs& A) {
         queues_left--;
         continue;
       }
-      packet_chunk<COUNT, LEARNED>(A, c * 64, acc, s_x, s_skip);
+      packet_chunk<COUNT, LEARNED>(A, c * 64, acc, s_x, s_skip, s_rec);
     }
   }
   if (A.totals) {  // every lane of the wave arrives here together
