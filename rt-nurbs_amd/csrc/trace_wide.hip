@@ -136,8 +136,16 @@ __device__ __forceinline__ void expand(Lane& L, Cnt& cn, const WideNode* __restr
   }
 }
 
-__device__ __forceinline__ void record(Lane& L, const WArgs& A, int pi, float t, float u, float v, f3 Su, f3 Sv,
-                                       bool fix) {
+// The fast forms keep each lane's closest-hit record (u, v, dpdu, dpdv; t is
+// L.t_max) in LDS, one column per lane, and write it once when the ray is
+// done, instead of writing it through on every improving hit (the packet
+// form's 64 lanes finish together, so a chunk's stores are contiguous).  The
+// one-ray and persistent forms with the learned seed write it through
+// (record_global): their LDS already holds the stacks and the network's input
+// rows, and 32 more bytes per lane would cost them resident blocks.
+constexpr int kRecWords = 8;
+__device__ __forceinline__ void record_global(Lane& L, const WArgs& A, int pi, float t, float u, float v, f3 Su,
+                                              f3 Sv, bool fix) {
   L.t_max = t;
   L.best = pi;
   L.fix = fix;
@@ -149,12 +157,8 @@ __device__ __forceinline__ void record(Lane& L, const WArgs& A, int pi, float t,
   if (A.dpdv) { A.dpdv[3 * r] = Sv.x; A.dpdv[3 * r + 1] = Sv.y; A.dpdv[3 * r + 2] = Sv.z; }
 }
 
-// The packet form keeps each lane's closest-hit record (u, v, dpdu, dpdv; t
-// is L.t_max) in LDS, one column per lane, and writes it once when the chunk
-// is done: the 64 lanes of a wave finish together, so the stores of a chunk
-// are contiguous, instead of scattered write-throughs on every improving hit.
-constexpr int kRecWords = 8;
-__device__ __forceinline__ void record_lds(Lane& L, float (*rec)[64], int lane, int pi, float t, float u, float v,
+template <int W>
+__device__ __forceinline__ void record_lds(Lane& L, float (*rec)[W], int lane, int pi, float t, float u, float v,
                                            f3 Su, f3 Sv, bool fix) {
   L.t_max = t;
   L.best = pi;
@@ -165,7 +169,8 @@ __device__ __forceinline__ void record_lds(Lane& L, float (*rec)[64], int lane, 
   rec[5][lane] = Sv.x; rec[6][lane] = Sv.y; rec[7][lane] = Sv.z;
 }
 
-__device__ __forceinline__ void write_record(const Lane& L, const WArgs& A, const float (*rec)[64], int lane) {
+template <int W>
+__device__ __forceinline__ void write_record(const Lane& L, const WArgs& A, const float (*rec)[W], int lane) {
   if (L.best < 0) return;
   const int64_t r = L.r;
   if (A.t) A.t[r] = L.t_max;
@@ -270,6 +275,7 @@ __device__ __forceinline__ void trace_wide_body(const WArgs& A) {
   __shared__ int s_code[WSTK][WB];
   __shared__ float s_te[WSTK][WB];
   __shared__ int s_skip[WB];  // per lane, skip_patch_of its ray
+  __shared__ float s_rec[LEARNED ? 1 : kRecWords][LEARNED ? 1 : WB];  // per lane, its closest hit's record
   // learned form: per-wave feature rows of the seed network (24 BF16 per row)
   __shared__ mlp::bf16x8 s_x[LEARNED ? WB : 1][3];
   const int tid = threadIdx.x;
@@ -361,6 +367,7 @@ __device__ __forceinline__ void trace_wide_body(const WArgs& A) {
         continue;
       }
       if (off == 0) {  // traversal finished: the ray is done
+        if constexpr (!LEARNED) write_record<WB>(L, A, reinterpret_cast<const float(*)[WB]>(&s_rec[0][0]), tid);
         lane_finish<COUNT>(L, A, cn);
         if constexpr (COUNT) {
           acc[0] += cn.nc; acc[1] += cn.ne; acc[3] += cn.nbox; acc[4] += cn.nleaf; acc[5] += cn.npt;
@@ -491,7 +498,9 @@ __device__ __forceinline__ void trace_wide_body(const WArgs& A) {
         }
       }
       if ((res.flags & 1) && 0.0f < res.t && res.t < L.t_max) {
-        record(L, A, pi, res.t, res.u, res.v, res.Su, res.Sv, (res.flags & 16) != 0);
+        if constexpr (LEARNED) record_global(L, A, pi, res.t, res.u, res.v, res.Su, res.Sv, (res.flags & 16) != 0);
+        else record_lds<WB>(L, reinterpret_cast<float(*)[WB]>(&s_rec[0][0]), tid, pi, res.t, res.u, res.v, res.Su,
+                            res.Sv, (res.flags & 16) != 0);
         if (A.any_hit || (A.ray_kind && A.ray_kind[L.r] == 0)) {  // occlusion query: the first hit ends the ray
           off = 0;
           cand_left = 0;
@@ -724,7 +733,7 @@ __device__ __forceinline__ void packet_chunk(const WArgs& A, int64_t r0, unsigne
             if (res.flags & 8) cn.nfb++;
           }
           if ((res.flags & 1) && 0.0f < res.t && res.t < L.t_max) {
-            record_lds(L, s_rec, lane, pi, res.t, res.u, res.v, res.Su, res.Sv, (res.flags & 16) != 0);
+            record_lds<64>(L, s_rec, lane, pi, res.t, res.u, res.v, res.Su, res.Sv, (res.flags & 16) != 0);
             if (A.any_hit || (A.ray_kind && A.ray_kind[L.r] == 0)) {  // occlusion query: the first hit ends the ray
               active = false;
               in_leaf = false;
@@ -739,7 +748,7 @@ __device__ __forceinline__ void packet_chunk(const WArgs& A, int64_t r0, unsigne
   }
   }
   if (had_ray) {
-    write_record(L, A, s_rec, lane);
+    write_record<64>(L, A, s_rec, lane);
     lane_finish<COUNT>(L, A, cn);
     if constexpr (COUNT) {
       acc[0] += cn.nc; acc[1] += cn.ne; acc[3] += cn.nbox; acc[4] += cn.nleaf; acc[5] += cn.npt;
