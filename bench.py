@@ -415,14 +415,20 @@ def bench_c2_c1(args, L, scenes, learned):
     if net is not None:
         # a 4-evaluation attempt from the network's seed, then the plain seed's full 8; the
         # four lanes of a quad testing the same patch share one network evaluation; pairs the
-        # hit output rates below hit_gate (0.7) start from the plain seed directly.  Measured
+        # hit output rates below hit_gate (0.7) start from the plain seed directly; in the
+        # packet forms the network runs only on patch visits of all 64 lanes.  Measured
         # sweep on this scene: 2 evaluations fall back on a third of the solves, 3 still
         # cost more evaluations per ray than the form without the network, a 6-evaluation
-        # fallback loses hits; a lower gate sends more misses through the attempt.
+        # fallback loses hits; a lower gate sends more misses through the attempt; the
+        # network on sparse visits (silhouettes, partial packets) costs time and saves no
+        # evaluations.
         lrn = dict(seed_mode=L.SEED_LEARNED, seed_net=net, learned_iters=4, fallback_iters=8, seed_share=1,
-                   hit_gate=args.hit_gate)
-        forms["wide_learned"] = dict(mode=W, hull_cull=2, **lrn, **BASE)
-        forms["persistent_learned_mixed_stall"] = dict(mode=P, hull_cull=2, stall_exit=1, **lrn, **mixed, **BASE)
+                   hit_gate=args.hit_gate, learned_min_lanes=args.learned_min_lanes)
+        # the one-ray forms run the network for any pending lane (a 64-lane threshold
+        # would rarely be met by their scattered pairs)
+        lrw = {**lrn, "learned_min_lanes": 0}
+        forms["wide_learned"] = dict(mode=W, hull_cull=2, **lrw, **BASE)
+        forms["persistent_learned_mixed_stall"] = dict(mode=P, hull_cull=2, stall_exit=1, **lrw, **mixed, **BASE)
         forms["packet_learned_mixed_stall"] = dict(mode=K, hull_cull=2, stall_exit=1, **lrn, **mixed, **BASE)
         forms["persistent_packet_learned_mixed"] = dict(mode=KP, hull_cull=2, **lrn, **mixed, **BASE)
         forms["persistent_packet_learned_mixed_stall"] = dict(mode=KP, hull_cull=2, stall_exit=1, **lrn, **mixed, **BASE)
@@ -432,9 +438,13 @@ def bench_c2_c1(args, L, scenes, learned):
                                                                    **BASE)
         forms["persistent_packet_learned_nogate_mixed_stall"] = dict(mode=KP, hull_cull=2, stall_exit=1,
                                                                      **{**lrn, "hit_gate": 0.0}, **mixed, **BASE)
+        # the network on every patch visit, however few lanes test the patch
+        forms["persistent_packet_learned_alllanes_mixed_stall"] = dict(mode=KP, hull_cull=2, stall_exit=1,
+                                                                       **{**lrn, "learned_min_lanes": 0}, **mixed,
+                                                                       **BASE)
     if "box" in nets:  # the network on the box-frame inputs (ray, four corners, degrees)
         lrb = dict(seed_mode=L.SEED_LEARNED, seed_net=nets["box"], learned_iters=4, fallback_iters=8, seed_share=1,
-                   hit_gate=args.hit_gate)
+                   hit_gate=args.hit_gate, learned_min_lanes=args.learned_min_lanes)
         forms["persistent_packet_learned_box_mixed_stall"] = dict(mode=KP, hull_cull=2, stall_exit=1, **lrb, **mixed,
                                                                   **BASE)
     if args.forms != "all":
@@ -735,6 +745,8 @@ def main():
     ap.add_argument("--load-net", default=None, help="use saved seed-network parameters instead of training")
     ap.add_argument("--hit-gate", type=float, default=0.7, help="learned forms: hit probability below which a "
                     "pair starts from the plain seed")
+    ap.add_argument("--learned-min-lanes", type=int, default=64, help="packet learned forms: lanes a patch visit "
+                    "needs for the network to run")
     ap.add_argument("--hit-skip", type=float, default=0.002, help="the skip form: hit probability below which a "
                     "pair is not solved")
     ap.add_argument("--detail", default=None, help="per-form report (JSON); default bench_out/<config>_detail.json")

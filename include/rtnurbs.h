@@ -317,6 +317,10 @@ typedef struct {
   float hit_skip;         /* learned seed: a pair whose predicted hit probability is below this is
                              not solved at all (counted in learned_skips); 0 = off.  Trades hits
                              the network misjudges for fewer Newton solves */
+  int learned_min_lanes;  /* learned seed: the network runs for a wave's pairs on a patch (packet
+                             forms) or for its pending pairs (one-ray forms) only when at least
+                             this many lanes have one; otherwise they start from the plain seed
+                             (0 = always) */
 } rtn_trace_params;
 
 void rtn_trace_params_default(rtn_trace_params* p);

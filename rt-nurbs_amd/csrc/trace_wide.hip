@@ -428,8 +428,11 @@ __device__ __forceinline__ void trace_wide_body(const WArgs& A) {
     in.pu = gu;
     in.pv = gv;
     bool test = pending;  // the pair is solved (the network's hit probability may skip it)
-    bool lrn = LEARNED;   // the pair starts from the network's seed
-    if (LEARNED) {  // network seed for every pending pair of the wave (matrix cores), plain seed kept as fallback
+    // the pair starts from the network's seed; with fewer than learned_min_lanes
+    // pending lanes the wave keeps the plain seeds
+    const bool net_pass = LEARNED && __popcll(__ballot(pending)) >= A.learned_min_lanes;
+    bool lrn = net_pass;
+    if (net_pass) {  // network seed for every pending pair of the wave (matrix cores), plain seed kept as fallback
       // with seed sharing, a lane whose quad leader (lane & ~3) tests the same
       // patch takes the leader's seed instead of an evaluation of its own
       const int lead = lane & ~3;
@@ -683,8 +686,11 @@ __device__ __forceinline__ void packet_chunk(const WArgs& A, int64_t r0, unsigne
           in.pu = in.gu;
           in.pv = in.gv;
         }
-        bool lrn = LEARNED;  // the pair starts from the network's seed
-        if (LEARNED) {  // network seeds of the wave's pairs on this patch (matrix cores), plain seed as fallback
+        // the pair starts from the network's seed; with fewer than learned_min_lanes
+        // pairs on the patch the wave keeps the plain seeds
+        const bool net_pass = LEARNED && __popcll(__ballot(act)) >= A.learned_min_lanes;
+        bool lrn = net_pass;
+        if (net_pass) {  // network seeds of the wave's pairs on this patch (matrix cores), plain seed as fallback
           // with seed sharing every lane of an aligned quad whose first lane tests the patch takes that lane's seed
           const int lead = lane & ~3;
           const bool lead_act = __shfl((int)act, lead) != 0;
@@ -999,6 +1005,8 @@ int fill_wide_args(const rtn_scene* s, const float* const rays[6], int64_t n, co
   RTN_CHECK(prm->flat_threshold <= 0.f, RTN_EUNSUPPORTED, "rtn_trace: flat-skip runs in the exact form only");
   RTN_CHECK(prm->hit_gate >= 0.f && prm->hit_gate <= 1.f && prm->hit_skip >= 0.f && prm->hit_skip <= 1.f, RTN_EINVAL,
             "rtn_trace: hit_gate and hit_skip lie in [0, 1]");
+  RTN_CHECK(prm->learned_min_lanes >= 0 && prm->learned_min_lanes <= 64, RTN_EINVAL,
+            "rtn_trace: learned_min_lanes outside 0..64");
   A = WArgs{};
   A.nodes = s->wide.ptr;
   A.cp = s->cp.ptr;
@@ -1031,6 +1039,7 @@ int fill_wide_args(const rtn_scene* s, const float* const rays[6], int64_t n, co
     A.mlp_kind = mlp_kind(prm->seed_net);
     A.hit_gate = prm->hit_gate;
     A.hit_skip = prm->hit_skip;
+    A.learned_min_lanes = prm->learned_min_lanes;
   }
   if (pairs) {
     A.pair_ray = pairs->ray;

@@ -64,6 +64,7 @@ struct WArgs {
   int mlp_kind;           // the network's feature kind (mlp::FeatureKind)
   float hit_gate;         // below this hit probability: the plain seed, no learned attempt
   float hit_skip;         // below this hit probability: the pair is not solved
+  int learned_min_lanes;  // fewer lanes with a pair: no network, plain seeds
   // training pairs: every converged solve with t > 0 is appended (pair_all:
   // every solve, uv = (-1, -1) for the others)
   int* pair_ray;

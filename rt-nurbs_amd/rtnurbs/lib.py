@@ -69,7 +69,7 @@ class TraceParams(C.Structure):
                 ("any_hit", C.c_int), ("refill_min", C.c_int), ("fallback_iters", C.c_int),
                 ("seed_share", C.c_int), ("stall_exit", C.c_int), ("seed_net", C.c_void_p),
                 ("skip_patch", C.c_void_p), ("t_tol", C.c_float), ("hit_gate", C.c_float),
-                ("hit_skip", C.c_float)]
+                ("hit_skip", C.c_float), ("learned_min_lanes", C.c_int)]
 
 
 class MlpTrainParams(C.Structure):

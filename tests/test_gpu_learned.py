@@ -241,7 +241,7 @@ def test_learned_headline_form_on_the_10k_patch_scene(rtl):
                 count=1, **CFG)
     a, ta = rtl.trace(scene, rays, rtl.params(**base))
     b, tb = rtl.trace(scene, rays, rtl.params(**base, seed_mode=rtl.SEED_LEARNED, seed_net=net, learned_iters=4,
-                                              fallback_iters=8, seed_share=1, hit_gate=0.7))
+                                              fallback_iters=8, seed_share=1, hit_gate=0.7, learned_min_lanes=64))
     assert ta["overflow"] == 0 and tb["overflow"] == 0
     ga, gb = a.get(), b.get()
     t64, h64 = ref["t"], ref["hit"]
@@ -265,3 +265,33 @@ def test_learned_headline_form_on_the_10k_patch_scene(rtl):
                                gb["u"][idx], gb["v"][idx])
     print(f"extra or nearer hits on the surface: {np.mean(dist <= 2e-5):.4f} of {len(idx)}", flush=True)
     assert np.mean(dist <= 2e-5) >= 0.99
+
+
+def test_learned_min_lanes(rtl, net):
+    """The network runs for a wave's pairs on a patch only when at least
+    learned_min_lanes lanes have one: a packet of 48 rays with the threshold
+    at 49 never runs it and equals the form without the network bit for bit;
+    at 1 it runs (a one-evaluation learned attempt falls back)."""
+    sc = scenes.config1(2004)
+    scene = rtl.Scene(sc.surfaces).build()
+    full = rtl.generate_rays(sc.camera, spp=4, jitter=True, seed=9, tiled=True)
+    # 48 rays of one 4 x 4-pixel block that sees the surface
+    h0 = rtl.trace(scene, full, rtl.params(mode=rtl.MODE_PACKET, hull_cull=2, **CFG))[0].get()["hit"]
+    blocks = np.flatnonzero((h0.reshape(-1, 64) >= 0).all(1))
+    assert len(blocks) > 0
+    sel = slice(int(blocks[len(blocks) // 2]) * 64, int(blocks[len(blocks) // 2]) * 64 + 48)
+    rays = rtl.Rays.from_host([np.ascontiguousarray(a[sel]) for a in full.get()])
+    base = dict(mode=rtl.MODE_PACKET, hull_cull=2, stall_exit=1, precision=rtl.PREC_MIXED, cond_thr=1e4, count=1,
+                **CFG)
+    lrn = dict(seed_mode=rtl.SEED_LEARNED, seed_net=net, learned_iters=1, fallback_iters=8, seed_share=0)
+    print("case learned_min_lanes", flush=True)
+    a, ta = rtl.trace(scene, rays, rtl.params(**base))
+    b, tb = rtl.trace(scene, rays, rtl.params(**base, **lrn, learned_min_lanes=49))
+    ga, gb = a.get(), b.get()
+    assert np.array_equal(ga["hit"], gb["hit"]) and (ga["hit"] >= 0).all()
+    for k in ("t", "u", "v"):
+        assert np.array_equal(ga[k].view(np.uint32), gb[k].view(np.uint32)), k
+    assert tb["learned_fallbacks"] == 0 and ta["newton_evals"] == tb["newton_evals"]
+    # one evaluation from the network's seed converges on no pair: every pair falls back
+    _, tc = rtl.trace(scene, rays, rtl.params(**base, **lrn, learned_min_lanes=1))
+    assert tc["learned_fallbacks"] > 0
