@@ -577,6 +577,34 @@ constexpr int PKT_STACK = 64;  // one entry per lane of the stack register
 
 // The 64 rays r0 .. r0+63 (those below n_rays) traced as one packet by the
 // calling wave; work totals accumulate into acc.
+// The inputs of a patch test of the packet form: the ray, the plain seed (the
+// FP64 reference's seed with mixed precision) and the method's parameters.
+__device__ __forceinline__ void packet_leaf_in(LeafIn& in, const Lane& L, const WArgs& A, int pi) {
+  in.rp = L.rp;
+  in.o = L.o;
+  in.d = L.d;
+  if (A.mixed) {  // the FP64 reference's seed
+    seed64(as_const(A.seed64 + (int64_t)kSeedTermWords * pi), L.o, L.d, in.gu, in.gv);
+  } else {
+    const cptr<float4> cn4 = as_const(A.corners + 3 * (int64_t)pi);
+    const float4 c0 = ld4(cn4, 0), c1 = ld4(cn4, 1), c2 = ld4(cn4, 2);
+    ex::bilinear_seed<float>({c0.x, c0.y, c0.z}, {c0.w, c1.x, c1.y}, {c1.z, c1.w, c2.x}, {c2.y, c2.z, c2.w},
+                             {L.o.x, L.o.y, L.o.z}, {L.d.x, L.d.y, L.d.z}, in.gu, in.gv);
+  }
+  in.pu = in.gu;
+  in.pv = in.gv;
+  in.seed = L.r + A.seed_base;
+  in.max_iter = A.max_iter;
+  in.learned_iters = A.learned_iters;
+  in.fallback_iters = A.fallback_iters;
+  in.mixed = A.mixed;
+  in.stall = A.stall_exit;
+  in.thr_det = A.thr_det;
+  in.thr_res = A.thr_res;
+  in.cond_thr = A.cond_thr;
+  in.t_tol = A.t_tol;
+}
+
 template <bool COUNT, bool LEARNED, int NACC>
 __device__ __forceinline__ void packet_chunk(const WArgs& A, int64_t r0, unsigned long long (&acc)[NACC],
                                              mlp::bf16x8 (*s_x)[3], int* s_skip, float (*s_rec)[64]) {
@@ -670,23 +698,11 @@ __device__ __forceinline__ void packet_chunk(const WArgs& A, int64_t r0, unsigne
         const int p = pqv & 0xff, q = (pqv >> 8) & 0xff;
         LeafOut res;
         res.flags = 0;
+        // leaf inputs: the learned kernel sets them on every lane (no per-lane merges
+        // of copies that live across the network pass: no spills at three waves),
+        // the plain kernel on the lanes that test the patch (fewer spills at five)
         LeafIn in;
-        if (act) {
-          in.rp = L.rp;
-          in.o = L.o;
-          in.d = L.d;
-          if (A.mixed) {  // the FP64 reference's seed
-            seed64(as_const(A.seed64 + (int64_t)kSeedTermWords * pi), L.o, L.d, in.gu, in.gv);
-          } else {
-            const cptr<float4> cn4 = as_const(A.corners + 3 * (int64_t)pi);
-            const float4 c0 = ld4(cn4, 0), c1 = ld4(cn4, 1), c2 = ld4(cn4, 2);
-            ex::bilinear_seed<float>({c0.x, c0.y, c0.z}, {c0.w, c1.x, c1.y}, {c1.z, c1.w, c2.x}, {c2.y, c2.z, c2.w},
-                                     {L.o.x, L.o.y, L.o.z}, {L.d.x, L.d.y, L.d.z}, in.gu, in.gv);
-          }
-          in.pu = in.gu;
-          in.pv = in.gv;
-        }
-        // the pair starts from the network's seed; with fewer than learned_min_lanes
+        if (LEARNED || act) packet_leaf_in(in, L, A, pi);        // the pair starts from the network's seed; with fewer than learned_min_lanes
         // pairs on the patch the wave keeps the plain seeds
         const bool net_pass = LEARNED && __popcll(__ballot(act)) >= A.learned_min_lanes;
         bool lrn = net_pass;
@@ -720,17 +736,7 @@ __device__ __forceinline__ void packet_chunk(const WArgs& A, int64_t r0, unsigne
         }
         if (act) {
           if (COUNT) cn.npt++;
-          in.seed = L.r + A.seed_base;
-          in.max_iter = A.max_iter;
-          in.learned_iters = A.learned_iters;
-          in.fallback_iters = A.fallback_iters;
-          in.mixed = A.mixed;
-          in.stall = A.stall_exit;
           in.learned = lrn;
-          in.thr_det = A.thr_det;
-          in.thr_res = A.thr_res;
-          in.cond_thr = A.cond_thr;
-          in.t_tol = A.t_tol;
           res = leaf_dispatch(degree_class(p, q), net, p, in);  // the patch's degree: a wave-uniform branch
           if (COUNT) {
             cn.nc++;
@@ -807,9 +813,9 @@ __device__ __forceinline__ void packet_body(const WArgs& A) {
 }
 
 // Five waves per SIMD: the kernel needs 111 registers per lane (four waves);
-// held to 96 it spills 64 bytes and runs 5% faster.  The learned kernel stays
-// at three waves (44-52 bytes spilled): at four it spills 184-196 bytes, and
-// the persistent learned form is no faster.
+// held to 96 it spills 28-60 bytes and runs 5% faster.  The learned kernel
+// stays at three waves, where it needs 157-160 registers and no spills: at
+// four it spills 208-220 bytes and is 1-3% faster.
 template <bool PERSISTENT, bool COUNT>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 5))) k_trace_packet(const WArgs A) {
   packet_body<PERSISTENT, COUNT, false>(A);
