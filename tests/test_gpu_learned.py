@@ -295,3 +295,17 @@ def test_learned_min_lanes(rtl, net):
     # one evaluation from the network's seed converges on no pair: every pair falls back
     _, tc = rtl.trace(scene, rays, rtl.params(**base, **lrn, learned_min_lanes=1))
     assert tc["learned_fallbacks"] > 0
+
+
+def test_learned_parameters_are_checked(rtl, net):
+    """Out-of-range learned-seed parameters are rejected with RTN_EINVAL and a
+    message, before any launch."""
+    sc = scenes.config1(0, 32, 32)
+    scene = rtl.Scene(sc.surfaces).build()
+    rays = rtl.generate_rays(sc.camera)
+    lrn = dict(mode=rtl.MODE_PACKET, seed_mode=rtl.SEED_LEARNED, seed_net=net, **CFG)
+    for bad in (dict(hit_gate=1.5), dict(hit_skip=-0.1), dict(learned_min_lanes=65), dict(learned_min_lanes=-1),
+                dict(seed_share=2)):
+        with pytest.raises(rtl.RtnError) as ei:
+            rtl.trace(scene, rays, rtl.params(**lrn, **bad))
+        assert ei.value.code == -1, bad
