@@ -508,9 +508,9 @@ def brief(r):
         out.update(mismatch_non_tie=e["patch_mismatch_non_tie"], hit_miss=e["hit_miss_disagree"],
                    t_le_1e5=round(e.get("t_rel_le_1e-5_frac", float("nan")), 6))
         dg = e.get("disagreements")
-        if dg:  # the reference's hits lost (missing or farther) vs hits it lacks (extra or nearer)
-            out.update(ref_hits_lost=dg["lost_hit"] + dg["farther_hit"], extra_or_nearer=dg["extra_hit"] + dg["nearer_hit"],
-                       extra_or_nearer_on_surface=round(dg.get("extra_or_nearer_on_surface_frac", float("nan")), 4))
+        if dg:  # the reference's hits lost (missing, or a farther hit) vs hits it lacks (extra, or nearer)
+            out.update(ref_lost=dg["lost_hit"] + dg["farther_hit"], ref_gained=dg["extra_hit"] + dg["nearer_hit"],
+                       gained_on_surface=round(dg.get("extra_or_nearer_on_surface_frac", float("nan")), 4))
     return out
 
 
