@@ -562,6 +562,18 @@ int rtn_frame_autotune(rtn_frame* frame, int reps, rtn_tune_entry* table, int ta
 int rtn_trace_f64(const rtn_scene* scene, const float* const rays_soa[6], int64_t n_rays, int max_iter,
                   double thr_det, double thr_res, uint32_t seed_base, int* prim, double* t, double* u, double* v,
                   double* dpdu, double* dpdv, double* t_second, uint32_t* counters, void* stream);
+/* FP64 closest-root reference (no reference counterpart: the reference runs
+ * one Newton solve per patch from its bilinear seed, intersect.cu:126).
+ * rtn_trace_f64 in which every tested patch also runs Newton from the
+ * centres ((i + 1/2) / grid, (j + 1/2) / grid), i, j < grid, of its domain and
+ * keeps its nearest converged root with t > 0: the pixel's hit is then the
+ * nearest root the searches find, the reference against which a form that
+ * finds roots the plain seed misses is judged.  grid in 0..16 (0 is
+ * rtn_trace_f64); the same outputs. */
+int rtn_trace_f64_roots(const rtn_scene* scene, const float* const rays_soa[6], int64_t n_rays, int max_iter,
+                        double thr_det, double thr_res, int grid, uint32_t seed_base, int* prim, double* t,
+                        double* u, double* v, double* dpdu, double* dpdv, double* t_second, uint32_t* counters,
+                        void* stream);
 
 /* calculateIntersectWrapper (intersect.cu:199-219; declared intersect.h:9-13)
  * on reference-layout buffers: nodes[2N-1], patches[N] whose control_point

@@ -882,6 +882,8 @@ struct TraceParams {
   double thr_det, thr_res, flat_threshold, cam_k;  // FP32 values are held exactly
   uint32_t seed_base;
   double prune_slack;  // FP64 reference only: boxes entered beyond t_max*slack are skipped (1 = reference)
+  int grid;            // closest-root reference only: > 0 adds Newton starts at the centres of a grid x grid
+                       // partition of the domain (no reference counterpart: the reference solves once per patch)
 };
 
 }  // extern "C"
@@ -938,9 +940,22 @@ static void trace_ray(long r, const Node* nodes, const v4<T>* cp, const int* cp_
           nc++;
           NewtonOut<T> nr = newton(rp, P, p, q, gu, gv, thr_det, thr_res, (uint32_t)r + prm.seed_base, prm.max_iter);
           ne += nr.evals;
+          T t = nr.hit ? (nr.S.x - o.x) * d.x + (nr.S.y - o.y) * d.y + (nr.S.z - o.z) * d.z : T(0);
+          // closest-root reference: the patch's hit is its nearest converged
+          // root in front of the origin over the plain seed and every grid start
+          for (int k = 0; k < prm.grid * prm.grid; k++) {
+            const T su = (T(k / prm.grid) + T(0.5)) / T(prm.grid), sv = (T(k % prm.grid) + T(0.5)) / T(prm.grid);
+            NewtonOut<T> g = newton(rp, P, p, q, su, sv, thr_det, thr_res, (uint32_t)r + prm.seed_base, prm.max_iter);
+            ne += g.evals;
+            if (!g.hit) continue;
+            const T tg = (g.S.x - o.x) * d.x + (g.S.y - o.y) * d.y + (g.S.z - o.z) * d.z;
+            if (0 < tg && (!nr.hit || !(0 < t) || tg < t)) {
+              nr = g;
+              t = tg;
+            }
+          }
           if (nr.hit) {
             nh++;
-            T t = (nr.S.x - o.x) * d.x + (nr.S.y - o.y) * d.y + (nr.S.z - o.z) * d.z;
             if (0 < t && t < t_max) {
               t_second = t_max;
               best = pi; bt = t; bu = nr.u; bv = nr.v; bsu = nr.Su; bsv = nr.Sv;
@@ -999,7 +1014,7 @@ void orc_trace_f32(const void* nodes, const float* cp, const int* cp_off, const 
                    uint32_t seed_base, long n_rays, const float* ox, const float* oy, const float* oz, const float* dx,
                    const float* dy, const float* dz, int* hit_prim, float* t, float* u, float* v, float* dpdu,
                    float* dpdv, int64_t* counters, int n_threads) {
-  TraceParams prm{max_iter, thr_det, thr_res, flat_threshold, cam_k, seed_base, 1.0};
+  TraceParams prm{max_iter, thr_det, thr_res, flat_threshold, cam_k, seed_base, 1.0, 0};
   const Node* nd = reinterpret_cast<const Node*>(nodes);
   const v4<float>* P = reinterpret_cast<const v4<float>*>(cp);
 #pragma omp parallel for schedule(dynamic, 256) num_threads(n_threads > 0 ? n_threads : 1)
@@ -1009,20 +1024,33 @@ void orc_trace_f32(const void* nodes, const float* cp, const int* cp_off, const 
 }
 
 // FP64 instantiation of the same traversal (the FP64 oracle): inputs are the
-// FP32 scene and rays widened exactly; thresholds as given.
-void orc_trace_f64(const void* nodes, const float* cp, long n_cp, const int* cp_off, const int* pdeg, const int* qdeg,
-                   int max_iter, double thr_det, double thr_res, double prune_slack, uint32_t seed_base, long n_rays, const float* ox,
-                   const float* oy, const float* oz, const float* dx, const float* dy, const float* dz,
-                   int* hit_prim, double* t, double* u, double* v, double* dpdu, double* dpdv, double* t2,
-                   int64_t* counters, int n_threads) {
+// FP32 scene and rays widened exactly; thresholds as given.  grid > 0 gives
+// the closest-root reference: every tested patch also runs Newton from the
+// centres ((i + 1/2) / grid, (j + 1/2) / grid) of a grid x grid partition of
+// its domain and keeps its nearest converged root with t > 0 (the reference
+// runs one solve per patch from the bilinear seed, intersect.cu:126).
+void orc_trace_f64_roots(const void* nodes, const float* cp, long n_cp, const int* cp_off, const int* pdeg,
+                         const int* qdeg, int max_iter, double thr_det, double thr_res, double prune_slack, int grid,
+                         uint32_t seed_base, long n_rays, const float* ox, const float* oy, const float* oz,
+                         const float* dx, const float* dy, const float* dz, int* hit_prim, double* t, double* u,
+                         double* v, double* dpdu, double* dpdv, double* t2, int64_t* counters, int n_threads) {
   std::vector<v4<double>> P(n_cp);
   for (long i = 0; i < n_cp; i++) P[i] = {cp[4 * i], cp[4 * i + 1], cp[4 * i + 2], cp[4 * i + 3]};
   const Node* nd = reinterpret_cast<const Node*>(nodes);
-  TraceParams prm{max_iter, thr_det, thr_res, 0, 0, seed_base, prune_slack};
+  TraceParams prm{max_iter, thr_det, thr_res, 0, 0, seed_base, prune_slack, grid};
 #pragma omp parallel for schedule(dynamic, 256) num_threads(n_threads > 0 ? n_threads : 1)
   for (long r = 0; r < n_rays; r++)
     trace_ray<double>(r, nd, P.data(), cp_off, pdeg, qdeg, nullptr, prm, ox, oy, oz, dx, dy, dz, hit_prim, t, u, v,
                       dpdu, dpdv, counters, t2);
+}
+
+void orc_trace_f64(const void* nodes, const float* cp, long n_cp, const int* cp_off, const int* pdeg, const int* qdeg,
+                   int max_iter, double thr_det, double thr_res, double prune_slack, uint32_t seed_base, long n_rays,
+                   const float* ox, const float* oy, const float* oz, const float* dx, const float* dy,
+                   const float* dz, int* hit_prim, double* t, double* u, double* v, double* dpdu, double* dpdv,
+                   double* t2, int64_t* counters, int n_threads) {
+  orc_trace_f64_roots(nodes, cp, n_cp, cp_off, pdeg, qdeg, max_iter, thr_det, thr_res, prune_slack, 0, seed_base,
+                      n_rays, ox, oy, oz, dx, dy, dz, hit_prim, t, u, v, dpdu, dpdv, t2, counters, n_threads);
 }
 
 // FP64 Newton polish of one (ray, patch) pair from (u,v): plain Newton with no

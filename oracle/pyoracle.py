@@ -68,6 +68,9 @@ def lib():
         L.orc_trace_f64.argtypes = [vp, f32p, C.c_long, i32p, i32p, i32p, C.c_int, C.c_double, C.c_double,
                                     C.c_double, C.c_uint32, C.c_long] + [f32p] * 6 + \
                                    [i32p, f64p, f64p, f64p, f64p, f64p, f64p, i64p, C.c_int]
+        L.orc_trace_f64_roots.argtypes = [vp, f32p, C.c_long, i32p, i32p, i32p, C.c_int, C.c_double, C.c_double,
+                                          C.c_double, C.c_int, C.c_uint32, C.c_long] + [f32p] * 6 + \
+                                         [i32p, f64p, f64p, f64p, f64p, f64p, f64p, i64p, C.c_int]
         L.orc_polish_f64.argtypes = [f32p, C.c_int, C.c_int, f32p, f32p, C.c_double, C.c_double, f64p]
         L.orc_polish_f64.restype = C.c_int
         L.orc_patch_eval_f32.argtypes = [f32p, C.c_int, C.c_int, C.c_float, C.c_float, f32p]
@@ -257,14 +260,18 @@ def trace_f32(nodes, ps: PatchSet, rays, max_iter=7, thr_det=1e-5, thr_res=1e-4,
 
 
 def trace_f64(nodes, ps: PatchSet, rays, max_iter=7, thr_det=1e-5, thr_res=1e-4, prune_slack=1.0, seed_base=0,
-              threads=os.cpu_count() or 1):
+              threads=os.cpu_count() or 1, grid=0):
+    """FP64 oracle trace; grid > 0 is the closest-root reference (Newton also
+    from the centres of a grid x grid partition of each tested patch, its
+    nearest converged root kept)."""
     n = len(rays[0])
     hit = np.full(n, -1, np.int32)
     t = np.zeros(n); u = np.zeros(n); v = np.zeros(n); t2 = np.zeros(n)
     du = np.zeros((n, 3)); dv = np.zeros((n, 3))
     cnt = np.zeros((n, 8), np.int64)
-    lib().orc_trace_f64(nodes.ctypes.data, ps.cp, len(ps.cp), ps.off, ps.p, ps.q, max_iter, thr_det, thr_res,
-                        prune_slack, seed_base, n, *[f32(a) for a in rays], hit, t, u, v, du, dv, t2, cnt, threads)
+    lib().orc_trace_f64_roots(nodes.ctypes.data, ps.cp, len(ps.cp), ps.off, ps.p, ps.q, max_iter, thr_det,
+                              thr_res, prune_slack, grid, seed_base, n, *[f32(a) for a in rays], hit, t, u, v, du, dv,
+                              t2, cnt, threads)
     return dict(hit=hit, t=t, u=u, v=v, dpdu=du, dpdv=dv, t2=t2, counters=cnt)
 
 

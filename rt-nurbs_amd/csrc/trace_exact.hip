@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdint>
 #include <mutex>
+#include <type_traits>
 
 #include "rtn_exact.hpp"
 #include "rtn_internal.hpp"
@@ -37,6 +38,7 @@ struct ExactArgs {
   int max_iter;
   T thr_det, thr_res;
   T prune_slack;        // boxes entered beyond t_max * slack are skipped (1 = reference)
+  int grid;             // FP64 closest-root reference: > 0 adds grid x grid Newton starts per tested patch
   uint32_t seed_base;
   // scene-form outputs
   int* prim;
@@ -150,11 +152,28 @@ __device__ __forceinline__ void trace_one(const ExactArgs<T>& A, const Patches& 
         }
         if (do_newton) {
           nc++;
-          const NewtonRes<T> nr = newton_dispatch<T>(rp, cp, P, Q, gu, gv, A, seed);
+          NewtonRes<T> nr = newton_dispatch<T>(rp, cp, P, Q, gu, gv, A, seed);
           ne += nr.evals;
           nfl += nr.evals * eval_flops(P, Q);
+          T t = nr.hit ? (nr.S.x - o.x) * d.x + (nr.S.y - o.y) * d.y + (nr.S.z - o.z) * d.z : T(0);
+          if constexpr (std::is_same<T, double>::value) {
+            // closest-root reference: Newton also from the centres of a grid x
+            // grid partition of the domain; the patch's hit is its nearest
+            // converged root in front of the origin over all starts
+            for (int k = 0; k < A.grid * A.grid; k++) {
+              const T su = (T(k / A.grid) + T(0.5)) / T(A.grid), sv = (T(k % A.grid) + T(0.5)) / T(A.grid);
+              const NewtonRes<T> g = newton_dispatch<T>(rp, cp, P, Q, su, sv, A, seed);
+              ne += g.evals;
+              nfl += g.evals * eval_flops(P, Q);
+              if (!g.hit) continue;
+              const T tg = (g.S.x - o.x) * d.x + (g.S.y - o.y) * d.y + (g.S.z - o.z) * d.z;
+              if (0 < tg && (!nr.hit || !(0 < t) || tg < t)) {
+                nr = g;
+                t = tg;
+              }
+            }
+          }
           if (nr.hit) {
-            const T t = (nr.S.x - o.x) * d.x + (nr.S.y - o.y) * d.y + (nr.S.z - o.z) * d.z;
             if (0 < t && t < t_max) {
               t_second = t_max;
               best = pi; bt = t; bu = nr.u; bv = nr.v; bsu = nr.Su; bsv = nr.Sv;
@@ -299,11 +318,14 @@ int trace_exact_scene(const rtn_scene* s, const float* const rays[6], int64_t n,
 // FP64 reference trace: the plain method's traversal and seed in FP64 with
 // its own Newton cap and tolerances; boxes are pruned only beyond
 // t_max * (1 + 4e-5) so the second-nearest hit of near-tie pixels is found.
-extern "C" int rtn_trace_f64(const rtn_scene* s, const float* const rays[6], int64_t n, int max_iter, double thr_det,
-                             double thr_res, uint32_t seed_base, int* prim, double* t, double* u, double* v,
-                             double* dpdu, double* dpdv, double* t_second, uint32_t* counters, void* stream) {
+// grid > 0: the closest-root reference (grid x grid extra starts per patch).
+extern "C" int rtn_trace_f64_roots(const rtn_scene* s, const float* const rays[6], int64_t n, int max_iter,
+                                   double thr_det, double thr_res, int grid, uint32_t seed_base, int* prim, double* t,
+                                   double* u, double* v, double* dpdu, double* dpdv, double* t_second,
+                                   uint32_t* counters, void* stream) {
   RTN_CHECK(s && s->built && rays && prim, RTN_EINVAL, "rtn_trace_f64: bad arguments");
   RTN_CHECK(max_iter >= 1 && max_iter <= 64, RTN_EINVAL, "rtn_trace_f64: max_iter outside 1..64");
+  RTN_CHECK(grid >= 0 && grid <= 16, RTN_EINVAL, "rtn_trace_f64_roots: grid outside 0..16");
   ExactArgs<double> A{};
   A.nodes = reinterpret_cast<const float4*>(s->nodes.ptr);
   A.ox = rays[0]; A.oy = rays[1]; A.oz = rays[2]; A.dx = rays[3]; A.dy = rays[4]; A.dz = rays[5];
@@ -312,12 +334,20 @@ extern "C" int rtn_trace_f64(const rtn_scene* s, const float* const rays[6], int
   A.thr_det = thr_det;
   A.thr_res = thr_res;
   A.prune_slack = 1.0 + 4e-5;
+  A.grid = grid;
   A.seed_base = seed_base;
   A.prim = prim; A.t = t; A.u = u; A.v = v; A.dpdu = dpdu; A.dpdv = dpdv; A.t2 = t_second;
   A.counters = counters;
   RTN_TRY(launch_scene<double>(s, A, n, (hipStream_t)stream));
   if (!stream) RTN_HIP(hipStreamSynchronize(nullptr));
   return RTN_OK;
+}
+
+extern "C" int rtn_trace_f64(const rtn_scene* s, const float* const rays[6], int64_t n, int max_iter, double thr_det,
+                             double thr_res, uint32_t seed_base, int* prim, double* t, double* u, double* v,
+                             double* dpdu, double* dpdv, double* t_second, uint32_t* counters, void* stream) {
+  return rtn_trace_f64_roots(s, rays, n, max_iter, thr_det, thr_res, 0, seed_base, prim, t, u, v, dpdu, dpdv,
+                             t_second, counters, stream);
 }
 
 namespace {

@@ -32,7 +32,7 @@ EXPORTS = [
     "rtn_camera_perspective", "rtn_rays_perspective", "rtn_scene_create", "rtn_scene_destroy",
     "rtn_scene_add_nurbs", "rtn_scene_add_patches", "rtn_scene_build", "rtn_scene_stats_get",
     "rtn_scene_download", "rtn_trace_params_default", "rtn_trace", "rtn_calculate_intersect", "rtn_get_patch",
-    "rtn_build_bvh", "rtn_patch_bounds", "rtn_refine", "rtn_scene_refine", "rtn_trace_f64", "rtn_event_create", "rtn_event_destroy",
+    "rtn_build_bvh", "rtn_patch_bounds", "rtn_refine", "rtn_scene_refine", "rtn_trace_f64", "rtn_trace_f64_roots", "rtn_event_create", "rtn_event_destroy",
     "rtn_event_record", "rtn_event_elapsed_ms", "rtn_trace_pairs", "rtn_mlp_create", "rtn_mlp_destroy",
     "rtn_mlp_kind", "rtn_mlp_set_params", "rtn_mlp_get_params", "rtn_mlp_pair_features", "rtn_mlp_forward", "rtn_mlp_gradient",
     "rtn_mlp_train_params_default", "rtn_mlp_train", "rtn_rays_silhouette", "rtn_rays_secondary",
@@ -202,6 +202,8 @@ def load():
     L.rtn_scene_refine.argtypes = [vp, f, f]
     L.rtn_trace_f64.argtypes = [vp, C.POINTER(vp * 6), C.c_int64, i, C.c_double, C.c_double, C.c_uint32,
                                 vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.rtn_trace_f64_roots.argtypes = [vp, C.POINTER(vp * 6), C.c_int64, i, C.c_double, C.c_double, i, C.c_uint32,
+                                      vp, vp, vp, vp, vp, vp, vp, vp, vp]
     L.rtn_event_create.argtypes = [C.POINTER(vp)]
     L.rtn_event_destroy.argtypes = [vp]
     L.rtn_event_record.argtypes = [vp, vp]
@@ -623,13 +625,17 @@ class HitBuffersF64:
         return d
 
 
-def trace_f64(scene, rays, max_iter=32, thr_det=1e-12, thr_res=1e-10, seed_base=0, counters=False, stream=None):
-    """FP64 reference trace on the device (plain method evaluated in FP64)."""
+def trace_f64(scene, rays, max_iter=32, thr_det=1e-12, thr_res=1e-10, seed_base=0, counters=False, stream=None,
+              grid=0):
+    """FP64 reference trace on the device (plain method evaluated in FP64).
+    grid > 0: the closest-root reference (rtn_trace_f64_roots): each tested
+    patch also runs Newton from grid x grid starts and keeps its nearest
+    converged root."""
     h = HitBuffersF64(rays.n, counters)
     arr = rays.ptr_array()
-    check(load().rtn_trace_f64(scene.h, C.byref(arr), rays.n, max_iter, thr_det, thr_res, seed_base, h.prim.ptr,
-                               h.t.ptr, h.u.ptr, h.v.ptr, h.dpdu.ptr, h.dpdv.ptr, h.t2.ptr,
-                               h.counters.ptr if h.counters else None, stream))
+    check(load().rtn_trace_f64_roots(scene.h, C.byref(arr), rays.n, max_iter, thr_det, thr_res, grid, seed_base,
+                                     h.prim.ptr, h.t.ptr, h.u.ptr, h.v.ptr, h.dpdu.ptr, h.dpdv.ptr, h.t2.ptr,
+                                     h.counters.ptr if h.counters else None, stream))
     return h
 
 

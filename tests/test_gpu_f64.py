@@ -32,14 +32,17 @@ CASES = [
 ]
 
 
+@pytest.mark.parametrize("grid", [0, 4], ids=["plain_seed", "closest_root"])
 @pytest.mark.parametrize("name,mk", CASES, ids=[c[0] for c in CASES])
-def test_trace_f64_matches_oracle(oracle, rtl, name, mk):
+def test_trace_f64_matches_oracle(oracle, rtl, name, mk, grid):
+    """grid 0: rtn_trace_f64; grid 4: the closest-root reference
+    (rtn_trace_f64_roots), 16 extra Newton starts per tested patch."""
     sc = mk()
-    print(f"case {name}", flush=True)
+    print(f"case {name} grid {grid}", flush=True)
     nodes, ps = _oracle_scene(oracle, sc.surfaces)
     scene = rtl.Scene(sc.surfaces).build()
     rays = rtl.generate_rays(sc.camera)
-    kw = dict(max_iter=32, thr_det=1e-12, thr_res=1e-10)
+    kw = dict(max_iter=32, thr_det=1e-12, thr_res=1e-10, grid=grid)
     g = rtl.trace_f64(scene, rays, **kw).get()
     r = oracle.trace_f64(nodes, ps, rays.get(), prune_slack=1.0 + 4e-5, **kw)
     assert np.array_equal(g["hit"], r["hit"])
@@ -65,3 +68,21 @@ def test_fp32_plain_error_vs_fp64(rtl):
     assert same.mean() > 0.3
     te = np.abs(g["t"][same] - r["t"][same]) / r["t"][same]
     assert np.median(te) < 1e-5
+
+
+def test_closest_root_reference_is_never_farther(rtl):
+    """On a mixed-degree scene the closest-root reference hits wherever the
+    plain-seed FP64 reference hits, never farther, and its extra starts find
+    nearer roots on some pixels (a size-independent property: the plain
+    seed's root is one of the candidates each tested patch compares)."""
+    sc = scenes.Scene(c2_small(11, 8), scenes.c2_camera(320, 180))
+    scene = rtl.Scene(sc.surfaces).build()
+    rays = rtl.generate_rays(sc.camera, spp=2, jitter=True, seed=3)
+    a = rtl.trace_f64(scene, rays).get()
+    b = rtl.trace_f64(scene, rays, grid=4).get()
+    ha, hb = a["hit"] >= 0, b["hit"] >= 0
+    assert ha.sum() > 1000
+    assert np.all(hb[ha])
+    assert np.all(b["t"][ha] <= a["t"][ha] * (1 + 1e-12))
+    nearer = ha & (b["t"] < a["t"] * (1 - 1e-5))
+    assert (nearer.sum() + (hb & ~ha).sum()) > 0

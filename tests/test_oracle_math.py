@@ -271,3 +271,56 @@ def test_torus_hits_are_analytic(oracle):
         real = np.sort(roots[np.abs(roots.imag) < 1e-7].real)
         real = real[real > 0]
         assert len(real) and np.min(np.abs(real - r["t"][i])) < 1e-6 * real[0]
+
+
+# --------------------------------------------- closest-root FP64 reference
+def test_closest_root_reference_sphere(oracle):
+    """With grid starts every hit of the rational sphere is the near root of
+    the ray-sphere quadratic, where the plain seed's single solve reaches the
+    far root on some rays (the check a closest-root search that kept any
+    converged root, or the first, would fail)."""
+    cam = scenes.reference_camera(160, 90)
+    P = oracle.c1_pipeline(scenes.Scene([scenes.sphere()], cam))
+    o = np.stack(P["rays"][:3], 1).astype(np.float64)
+    d = np.stack(P["rays"][3:], 1).astype(np.float64)
+    oc = o - np.array([0, 0, 6.0])
+    b = np.einsum("ij,ij->i", oc, d)
+    dd = np.einsum("ij,ij->i", d, d)
+    disc = b * b - (np.einsum("ij,ij->i", oc, oc) - 1.0) * dd
+    near = (-b - np.sqrt(np.maximum(disc, 0))) / dd
+    kw = dict(max_iter=40, thr_res=1e-11, thr_det=1e-14)
+    plain = oracle.trace_f64(P["nodes"], P["ps"], P["rays"], **kw)
+    roots = oracle.trace_f64(P["nodes"], P["ps"], P["rays"], grid=4, **kw)
+    hp, hr = plain["hit"] >= 0, roots["hit"] >= 0
+    assert np.all(hr[hp]) and hr.sum() >= hp.sum() > 100
+    assert np.all(disc[hr] >= 0)
+    assert np.all(np.abs(roots["t"][hr] - near[hr]) < 1e-6 * near[hr])
+    assert np.any(np.abs(plain["t"][hp] - near[hp]) > 1e-3 * near[hp])
+
+
+def test_closest_root_reference_torus(oracle):
+    """Every closest-root hit on the rational torus is the smallest positive
+    real root of the ray-torus quartic; the plain seed misses it on some."""
+    cam = scenes.reference_camera(120, 68)
+    P = oracle.c1_pipeline(scenes.Scene([scenes.torus()], cam))
+    kw = dict(max_iter=40, thr_res=1e-11, thr_det=1e-14)
+    plain = oracle.trace_f64(P["nodes"], P["ps"], P["rays"], **kw)
+    roots = oracle.trace_f64(P["nodes"], P["ps"], P["rays"], grid=4, **kw)
+    R0, r0 = 1.0, 0.4
+    hit = np.nonzero(roots["hit"] >= 0)[0]
+    assert len(hit) > 50
+    plain_far = 0
+    for i in hit:
+        o = np.array([P["rays"][k][i] for k in range(3)], np.float64) - [0, 0, 6.0]
+        d = np.array([P["rays"][k][i] for k in range(3, 6)], np.float64)
+        dd, od, oo = d @ d, o @ d, o @ o
+        k0 = oo + R0 * R0 - r0 * r0
+        coeffs = [dd * dd, 4 * dd * od, 2 * dd * k0 + 4 * od * od - 4 * R0 * R0 * (d[0] ** 2 + d[1] ** 2),
+                  4 * od * k0 - 8 * R0 * R0 * (o[0] * d[0] + o[1] * d[1]), k0 * k0 - 4 * R0 * R0 * (o[0] ** 2 + o[1] ** 2)]
+        roots_q = np.roots(coeffs)
+        real = np.sort(roots_q[np.abs(roots_q.imag) < 1e-7].real)
+        real = real[real > 0]
+        assert len(real) and abs(roots["t"][i] - real[0]) < 1e-6 * real[0]
+        if plain["hit"][i] < 0 or abs(plain["t"][i] - real[0]) > 1e-6 * real[0]:
+            plain_far += 1
+    assert plain_far > 0
