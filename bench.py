@@ -365,7 +365,10 @@ def bench_c2_c1(args, L, scenes, learned):
     rays = L.generate_rays(sc.camera, spp=spp, jitter=jitter, seed=1, tiled=True)
     stream = L.stream_create()
     nets, train = {}, None
-    kinds = {"projected": L.MLP_FEAT_PROJECTED, "box": L.MLP_FEAT_BOX}
+    # the third network: projected inputs, trained on closest-root targets (each
+    # training pair solved from a 4 x 4 grid of starts as well, nearest root kept)
+    kinds = {"projected": L.MLP_FEAT_PROJECTED, "box": L.MLP_FEAT_BOX, "projected_root": L.MLP_FEAT_PROJECTED}
+    root_grids = {"projected_root": 4}
     if args.load_net:  # parameters saved by an earlier run (profiling passes reuse one training)
         z = np.load(args.load_net)
         nets = {k: L.Mlp(params=z[k], kind=kinds[k]) for k in kinds if k in z}
@@ -377,7 +380,8 @@ def bench_c2_c1(args, L, scenes, learned):
         sl = learned.training_scenes()
         for name, kind in kinds.items():
             t0 = time.perf_counter()
-            feats, targets, n, hits = learned.collect_pairs(sl, max_hits=10_000_000, kind=kind)
+            feats, targets, n, hits = learned.collect_pairs(sl, max_hits=10_000_000, kind=kind,
+                                                            root_grid=root_grids.get(name, 0))
             t1 = time.perf_counter()
             L.synchronize()
             net, losses = learned.train_seed_network(feats, targets, n, epochs=args.train_epochs, kind=kind)
@@ -386,6 +390,7 @@ def bench_c2_c1(args, L, scenes, learned):
             del feats, targets
             nets[name] = net
             train[name] = {"pairs": int(n), "converged_pairs": int(hits), "epochs": args.train_epochs,
+                           "closest_root_grid": root_grids.get(name, 0),
                            "collect_s": round(t1 - t0, 2), "train_s": round(t2 - t1, 3),
                            "train_mpairs_s": round(n * args.train_epochs / (t2 - t1) / 1e6, 1),
                            "epoch_loss": [round(float(x), 6) for x in losses]}
@@ -444,6 +449,13 @@ def bench_c2_c1(args, L, scenes, learned):
         forms["persistent_packet_learned_alllanes_mixed_stall"] = dict(mode=KP, hull_cull=2, stall_exit=1,
                                                                        **{**lrn, "learned_min_lanes": 0}, **mixed,
                                                                        **BASE)
+    if "projected_root" in nets:
+        # closest-root targets and no hit gate: every full-packet pair starts from the
+        # network's seed, which reaches roots nearer than the bilinear seed's
+        lrr = dict(seed_mode=L.SEED_LEARNED, seed_net=nets["projected_root"], learned_iters=4, fallback_iters=8,
+                   seed_share=1, hit_gate=0.0, learned_min_lanes=args.learned_min_lanes)
+        forms["persistent_packet_learned_root_nogate_mixed_stall"] = dict(mode=KP, hull_cull=2, stall_exit=1, **lrr,
+                                                                          **mixed, **BASE)
     if "box" in nets:  # the network on the box-frame inputs (ray, four corners, degrees)
         lrb = dict(seed_mode=L.SEED_LEARNED, seed_net=nets["box"], learned_iters=4, fallback_iters=8, seed_share=1,
                    hit_gate=args.hit_gate, learned_min_lanes=args.learned_min_lanes)

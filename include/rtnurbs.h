@@ -385,6 +385,16 @@ int rtn_trace_pairs(const rtn_scene* scene, const float* const rays_soa[6], int6
 int rtn_solve_pairs(const rtn_scene* scene, const float* const rays_soa[6], int64_t n_rays, const int* pair_ray,
                     const int* pair_patch, const float* seed_uv, int64_t n, const rtn_trace_params* prm, int* iters,
                     int* converged, float* uv, float* t, void* stream);
+/* rtn_solve_pairs that also solves each pair from the centres
+ * ((i + 1/2) / grid, (j + 1/2) / grid), i, j < grid, of the domain and
+ * reports the nearest converged root with t > 0 (converged = 1 when any start
+ * reaches one; iters = the evaluations of all starts): the closest-root
+ * training targets of the learned seed.  No reference counterpart (the
+ * reference solves once per pair).  grid in 0..16; 0 is rtn_solve_pairs. */
+int rtn_solve_pairs_roots(const rtn_scene* scene, const float* const rays_soa[6], int64_t n_rays,
+                          const int* pair_ray, const int* pair_patch, const float* seed_uv, int64_t n,
+                          const rtn_trace_params* prm, int grid, int* iters, int* converged, float* uv, float* t,
+                          void* stream);
 
 /* ------------------------------------------------------------ learned seed
  * No reference counterpart.  A 20 -> 64 -> 64 -> 64 -> 3 perceptron (GELU,

@@ -477,7 +477,7 @@ __global__ void k_solve_pairs(const float4* cp, const int* cp_off, const int* pq
                               const float* ox, const float* oy, const float* oz, const float* dx, const float* dy,
                               const float* dz, int64_t n_rays, const int* pair_ray, const int* pair_patch,
                               const float* seed_uv, int64_t n, int max_iter, float thr_det, float thr_res,
-                              uint32_t seed_base, int* iters, int* converged, float* uv_out, float* t_out) {
+                              uint32_t seed_base, int grid, int* iters, int* converged, float* uv_out, float* t_out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int r = pair_ray[i], k = pair_patch[i];
@@ -498,34 +498,60 @@ __global__ void k_solve_pairs(const float4* cp, const int* cp_off, const int* pq
     const float4 c0 = corners[3 * k], c1 = corners[3 * k + 1], c2 = corners[3 * k + 2];
     bilinear_seed<float>({c0.x, c0.y, c0.z}, {c0.w, c1.x, c1.y}, {c1.z, c1.w, c2.x}, {c2.y, c2.z, c2.w}, o, d, u0, v0);
   }
-  const NewtonRes<float> nr = newton<float>(
-      rp, [&](float uu, float vv, v4<float>& du, v4<float>& dv, v4<float>& pt) {
-        patch_eval_rt<RTN_MAX_DEGREE>(net, P, Q, uu, vv, du, dv, pt);
-      },
-      u0, v0, thr_det, thr_res, (uint32_t)r + seed_base, max_iter);
-  iters[i] = nr.evals;
-  converged[i] = nr.hit ? 1 : 0;
-  if (uv_out) {
-    uv_out[2 * i] = nr.hit ? nr.u : u0;
-    uv_out[2 * i + 1] = nr.hit ? nr.v : v0;
+  const auto eval = [&](float uu, float vv, v4<float>& du, v4<float>& dv, v4<float>& pt) {
+    patch_eval_rt<RTN_MAX_DEGREE>(net, P, Q, uu, vv, du, dv, pt);
+  };
+  NewtonRes<float> nr = newton<float>(rp, eval, u0, v0, thr_det, thr_res, (uint32_t)r + seed_base, max_iter);
+  int evals = nr.evals;
+  float t = nr.hit ? (nr.S.x - o.x) * d.x + (nr.S.y - o.y) * d.y + (nr.S.z - o.z) * d.z : -1.0f;
+  bool hit = nr.hit && t > 0.0f;
+  // closest root: the same solve also from the centres of a grid x grid
+  // partition of the domain; the nearest converged root with t > 0 is kept
+  for (int k = 0; k < grid * grid; k++) {
+    const float su = ((float)(k / grid) + 0.5f) / (float)grid, sv = ((float)(k % grid) + 0.5f) / (float)grid;
+    const NewtonRes<float> g = newton<float>(rp, eval, su, sv, thr_det, thr_res, (uint32_t)r + seed_base, max_iter);
+    evals += g.evals;
+    if (!g.hit) continue;
+    const float tg = (g.S.x - o.x) * d.x + (g.S.y - o.y) * d.y + (g.S.z - o.z) * d.z;
+    if (tg > 0.0f && (!hit || tg < t)) {
+      nr = g;
+      t = tg;
+      hit = true;
+    }
   }
-  if (t_out) t_out[i] = nr.hit ? (nr.S.x - o.x) * d.x + (nr.S.y - o.y) * d.y + (nr.S.z - o.z) * d.z : -1.0f;
+  if (grid == 0) hit = nr.hit;  // the single solve reports convergence whatever the sign of t
+  iters[i] = evals;
+  converged[i] = hit ? 1 : 0;
+  if (uv_out) {
+    uv_out[2 * i] = hit ? nr.u : u0;
+    uv_out[2 * i + 1] = hit ? nr.v : v0;
+  }
+  if (t_out) t_out[i] = hit ? t : -1.0f;
 }
 }  // namespace
 
-extern "C" int rtn_solve_pairs(const rtn_scene* s, const float* const rays[6], int64_t n_rays, const int* pair_ray,
-                               const int* pair_patch, const float* seed_uv, int64_t n, const rtn_trace_params* prm,
-                               int* iters, int* converged, float* uv, float* t, void* stream) {
+extern "C" int rtn_solve_pairs_roots(const rtn_scene* s, const float* const rays[6], int64_t n_rays,
+                                     const int* pair_ray, const int* pair_patch, const float* seed_uv, int64_t n,
+                                     const rtn_trace_params* prm, int grid, int* iters, int* converged, float* uv,
+                                     float* t, void* stream) {
   RTN_CHECK(s && s->built && rays && pair_ray && pair_patch && prm && iters && converged && n >= 0, RTN_EINVAL,
             "rtn_solve_pairs: bad arguments");
+  RTN_CHECK(grid >= 0 && grid <= 16, RTN_EINVAL, "rtn_solve_pairs_roots: grid outside 0..16");
   for (int k = 0; k < 6; k++) RTN_CHECK(rays[k], RTN_EINVAL, "rtn_solve_pairs: null ray component %d", k);
   RTN_CHECK(prm->max_iter >= 1 && prm->max_iter <= 64, RTN_EINVAL, "rtn_solve_pairs: max_iter outside 1..64");
   if (n == 0) return RTN_OK;
   hipLaunchKernelGGL(k_solve_pairs, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, (hipStream_t)stream, s->cp.ptr,
                      s->cp_off.ptr, s->pq.ptr, s->corners.ptr, (int)s->n_patches, rays[0], rays[1], rays[2], rays[3],
                      rays[4], rays[5], n_rays, pair_ray, pair_patch, seed_uv, n, prm->max_iter, prm->thr_det,
-                     prm->thr_res, prm->seed_base, iters, converged, uv, t);
+                     prm->thr_res, prm->seed_base, grid, iters, converged, uv, t);
   RTN_HIP(hipGetLastError());
   if (!stream) RTN_HIP(hipStreamSynchronize(nullptr));
   return RTN_OK;
+}
+
+extern "C" int rtn_solve_pairs(const rtn_scene* s, const float* const rays[6], int64_t n_rays, const int* pair_ray,
+                               const int* pair_patch, const float* seed_uv, int64_t n, const rtn_trace_params* prm,
+                               int* iters, int* converged, float* uv, float* t, void* stream) {
+  return rtn_solve_pairs_roots(s, rays, n_rays, pair_ray, pair_patch, seed_uv, n, prm, 0, iters, converged, uv, t,
+                               stream);
 }

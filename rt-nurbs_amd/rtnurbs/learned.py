@@ -23,6 +23,8 @@ from . import scenes
 
 PAIR_PARAMS = dict(mode=L.MODE_WIDE, max_iter=8, thr_res=1e-5, thr_det=1e-5, hull_cull=2, precision=L.PREC_MIXED,
                    cond_thr=1e4, stall_exit=1)
+# the closest-root relabelling's solves (plain Newton, each start)
+ROOT_PARAMS = dict(max_iter=16, thr_res=1e-5, thr_det=1e-5)
 
 
 def _orbit_camera(rng, center, radius, W, H):
@@ -60,10 +62,14 @@ def training_scenes(n_c1=48, n_c2=24, res=512, seed=0):
     return out
 
 
-def collect_pairs(scene_list, max_hits=10_000_000, spp=1, kind=L.MLP_FEAT_PROJECTED, log=None):
+def collect_pairs(scene_list, max_hits=10_000_000, spp=1, kind=L.MLP_FEAT_PROJECTED, log=None, root_grid=0):
     """Traces each scene with the plain seed and gathers (features, targets)
     of every Newton solve into device arrays until max_hits converged solves
-    are in; returns (feats (n, 20), targets (n, 3) = (u, v, hit), n, hits)."""
+    are in; returns (feats (n, 20), targets (n, 3) = (u, v, hit), n, hits).
+    root_grid > 0 relabels every pair with its closest root: the pair is
+    solved again from the plain seed and from root_grid x root_grid starts
+    (16 iterations each) and the nearest converged root with t > 0 becomes the
+    target (hit = 1 when any start converges)."""
     cap_all = 3 * max_hits
     feats = L.DeviceArray((cap_all, L.MLP_NIN), np.float32)
     targets = L.DeviceArray((cap_all, 3), np.float32)
@@ -80,6 +86,10 @@ def collect_pairs(scene_list, max_hits=10_000_000, spp=1, kind=L.MLP_FEAT_PROJEC
         if k > 0:
             uv = puv.get()[:k]
             h = uv[:, 0] >= 0.0
+            if root_grid > 0:
+                sol = L.solve_pairs(sc, rays, pr, pp, k, L.params(**{**ROOT_PARAMS}), grid=root_grid)
+                uv = np.where(sol["converged"][:, None], sol["uv"], -1.0).astype(np.float32)
+                h = sol["converged"]
             # stop at max_hits converged solves: keep the pairs up to that hit
             if hits + int(h.sum()) > max_hits:
                 k = int(np.searchsorted(np.cumsum(h), max_hits - hits, side="left")) + 1

@@ -39,7 +39,7 @@ EXPORTS = [
     "rtn_shade_lambert", "rtn_build_params_default", "rtn_scene_build_ex", "rtn_batch_create", "rtn_batch_destroy",
     "rtn_batch_patch_base", "rtn_batch_trace", "rtn_frame_params_default", "rtn_frame_create", "rtn_frame_destroy",
     "rtn_frame_run", "rtn_frame_capture", "rtn_frame_status", "rtn_frame_results", "rtn_frame_set_shape",
-    "rtn_frame_autotune", "rtn_solve_pairs", "rtn_frame_set_refill", "rtn_frame_set_packet", "rtn_scene_save", "rtn_scene_load",
+    "rtn_frame_autotune", "rtn_solve_pairs", "rtn_solve_pairs_roots", "rtn_frame_set_refill", "rtn_frame_set_packet", "rtn_scene_save", "rtn_scene_load",
     "rtn_scene_trace_overflow", "rtn_malloc_host", "rtn_free_host", "rtn_lookat", "rtn_camera_init",
     "rtn_camera_get_perspective", "rtn_camera_get_orthographic", "rtn_rays_generate",
     "rtn_calculate_intersect_noalloc", "rtn_frame_count",
@@ -179,6 +179,8 @@ def load():
     L.rtn_frame_autotune.argtypes = [vp, i, C.POINTER(TuneEntry), i, C.POINTER(i), vp]
     L.rtn_solve_pairs.argtypes = [vp, C.POINTER(vp * 6), i64, vp, vp, vp, i64, C.POINTER(TraceParams), vp, vp, vp,
                                   vp, vp]
+    L.rtn_solve_pairs_roots.argtypes = [vp, C.POINTER(vp * 6), i64, vp, vp, vp, i64, C.POINTER(TraceParams), i, vp,
+                                        vp, vp, vp, vp]
     L.rtn_scene_stats_get.argtypes = [vp, C.POINTER(SceneStats)]
     L.rtn_scene_download.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
     L.rtn_trace_params_default.argtypes = [C.POINTER(TraceParams)]
@@ -939,16 +941,18 @@ class Frame:
             pass
 
 
-def solve_pairs(scene, rays, pair_ray, pair_patch, n, prm, seed_uv=None):
+def solve_pairs(scene, rays, pair_ray, pair_patch, n, prm, seed_uv=None, grid=0):
     """Plain-method Newton on device pairs from the plain seed or seed_uv:
-    host dict of iters, converged, uv, t."""
+    host dict of iters, converged, uv, t.  grid > 0: also from grid x grid
+    starts, the nearest converged root with t > 0 reported
+    (rtn_solve_pairs_roots)."""
     it = DeviceArray((n,), np.int32)
     cv = DeviceArray((n,), np.int32)
     uv = DeviceArray((n, 2), np.float32)
     t = DeviceArray((n,), np.float32)
     arr = rays.ptr_array()
-    check(load().rtn_solve_pairs(scene.h, C.byref(arr), rays.n, pair_ray.ptr, pair_patch.ptr,
-                                 seed_uv.ptr if seed_uv is not None else None, n, C.byref(prm), it.ptr, cv.ptr, uv.ptr,
-                                 t.ptr, None))
+    check(load().rtn_solve_pairs_roots(scene.h, C.byref(arr), rays.n, pair_ray.ptr, pair_patch.ptr,
+                                       seed_uv.ptr if seed_uv is not None else None, n, C.byref(prm), grid, it.ptr,
+                                       cv.ptr, uv.ptr, t.ptr, None))
     synchronize()
     return dict(iters=it.get(), converged=cv.get().astype(bool), uv=uv.get(), t=t.get())

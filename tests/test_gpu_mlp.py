@@ -140,3 +140,42 @@ def test_pairs_and_features(rtl, name, mk):
                            kind=rtl.MLP_FEAT_BOX).get()
     refb = M.box_features(o, d, bounds[pp], [nets[k] for k in pp], dl["pq"][pp, 0], dl["pq"][pp, 1])
     np.testing.assert_allclose(fb, refb, rtol=1e-4, atol=1e-4)
+
+
+def test_solve_pairs_closest_root_on_the_sphere(rtl):
+    """rtn_solve_pairs_roots (the closest-root training targets): on the
+    rational sphere every converged root is one of the two roots of the
+    ray-sphere quadratic (FP32 tolerance), never farther than the single
+    solve's, found wherever the single solve converges in front of the
+    origin, and on some pairs the near root where the single solve reached
+    the far one (a patch on the front of the sphere seen twice)."""
+    sc = scenes.Scene([scenes.sphere()], scenes.reference_camera(160, 90))
+    scene = rtl.Scene(sc.surfaces).build()
+    rays = rtl.generate_rays(sc.camera)
+    prm = rtl.params(mode=rtl.MODE_WIDE, max_iter=8, thr_res=1e-5, thr_det=1e-5)
+    pr, pp, puv, cnt = rtl.trace_pairs(scene, rays, prm, 8 * rays.n, all=True)
+    assert cnt > 100
+    sp = rtl.params(max_iter=16, thr_res=1e-5, thr_det=1e-5)
+    a = rtl.solve_pairs(scene, rays, pr, pp, cnt, sp)
+    b = rtl.solve_pairs(scene, rays, pr, pp, cnt, sp, grid=4)
+    ha = a["converged"] & (a["t"] > 0)
+    hb = b["converged"]
+    assert np.all(hb[ha]) and np.all(b["t"][hb] > 0)
+    assert np.all(b["t"][ha] <= a["t"][ha] * (1 + 1e-5))
+    r = pr.get()[:cnt]
+    o = np.stack(rays.get()[:3], 1).astype(np.float64)[r] - [0.0, 0.0, 6.0]
+    d = np.stack(rays.get()[3:], 1).astype(np.float64)[r]
+    bq = np.einsum("ij,ij->i", o, d)
+    dd = np.einsum("ij,ij->i", d, d)
+    disc = bq * bq - (np.einsum("ij,ij->i", o, o) - 1.0) * dd
+    near = (-bq - np.sqrt(np.maximum(disc, 0))) / dd
+    far = (-bq + np.sqrt(np.maximum(disc, 0))) / dd
+    assert np.all(disc[hb] >= 0)
+    tb = b["t"].astype(np.float64)
+    is_near, is_far = np.abs(tb - near) <= 2e-5 * near, np.abs(tb - far) <= 2e-5 * far
+    # grazing rays (a chord under 0.1 of the unit sphere) are left out: there
+    # the FP32 acceptance (residual 1e-5) allows a larger error in t
+    firm = hb & (far - near > 0.1)
+    assert firm.sum() > 100 and np.all((is_near | is_far)[firm])
+    ta = a["t"].astype(np.float64)
+    assert np.any((ha & is_near) & (np.abs(ta - far) <= 2e-5 * far))
