@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "rtn_exact.hpp"
 #include "rtn_mem.hpp"
@@ -147,13 +148,70 @@ __device__ __forceinline__ void bernstein_rt(int n, float t, float (&b)[MAXD + 1
   }
 }
 
+// Bernstein basis of a wave-uniform runtime degree n (1..MAXD): a scalar
+// branch to the compile-time basis, entries above n zero.  The same values as
+// bernstein_rt, with no per-entry selects.
+template <int N>
+__device__ __forceinline__ void basis_into(float t, float (&b)[MAXD + 1], float (&db)[MAXD + 1]) {
+  float x[N + 1], y[N + 1];
+  bernstein<N>(t, x, y);
+#pragma unroll
+  for (int i = 0; i <= N; i++) {
+    b[i] = x[i];
+    db[i] = y[i];
+  }
+}
+
+__device__ __forceinline__ void bernstein_uniform(int n, float t, float (&b)[MAXD + 1], float (&db)[MAXD + 1]) {
+#pragma unroll
+  for (int i = 0; i <= MAXD; i++) {
+    b[i] = 0.0f;
+    db[i] = 0.0f;
+  }
+  switch (n) {
+    case 1: basis_into<1>(t, b, db); break;
+    case 2: basis_into<2>(t, b, db); break;
+    case 3: basis_into<3>(t, b, db); break;
+    case 4: basis_into<4>(t, b, db); break;
+    default: basis_into<5>(t, b, db); break;
+  }
+}
+
 // Point and partials of a patch with runtime row count P+1 and compile-time
 // row length Q+1: rows stream from memory; the u-direction weights rotate
-// through registers (weight of row i is always in bu[0]).
+// through registers (weight of row i is always in bu[0]).  A control net
+// read through scalar loads (every lane on the same patch) has a wave-uniform
+// P: its basis takes a scalar branch and its rows are unrolled behind
+// scalar guards, so the next row's loads can issue early.
 template <int Q, class NP = const float4*>
 __device__ __forceinline__ void eval_q(NP net, int P, float u, float v, float4& pt,
                                        float4& du, float4& dv) {
   float bu[MAXD + 1], dbu[MAXD + 1], bv[Q + 1], dbv[Q + 1];
+  if constexpr (std::is_same<NP, cptr<float4>>::value) {
+    bernstein_uniform(P, u, bu, dbu);
+    bernstein<Q>(v, bv, dbv);
+    pt = make_float4(0.f, 0.f, 0.f, 0.f);
+    du = pt;
+    dv = pt;
+#pragma unroll
+    for (int i = 0; i <= MAXD; i++) {
+      if (i > P) break;
+      const NP row = net + i * (Q + 1);
+      float4 r = make_float4(0.f, 0.f, 0.f, 0.f), rv = r;
+#pragma unroll
+      for (int j = 0; j <= Q; j++) {
+        const float4 c = ld4(row, j);
+        r.x = fmaf(bv[j], c.x, r.x); r.y = fmaf(bv[j], c.y, r.y); r.z = fmaf(bv[j], c.z, r.z); r.w = fmaf(bv[j], c.w, r.w);
+        rv.x = fmaf(dbv[j], c.x, rv.x); rv.y = fmaf(dbv[j], c.y, rv.y);
+        rv.z = fmaf(dbv[j], c.z, rv.z); rv.w = fmaf(dbv[j], c.w, rv.w);
+      }
+      const float b = bu[i], db = dbu[i];
+      pt.x = fmaf(b, r.x, pt.x); pt.y = fmaf(b, r.y, pt.y); pt.z = fmaf(b, r.z, pt.z); pt.w = fmaf(b, r.w, pt.w);
+      du.x = fmaf(db, r.x, du.x); du.y = fmaf(db, r.y, du.y); du.z = fmaf(db, r.z, du.z); du.w = fmaf(db, r.w, du.w);
+      dv.x = fmaf(b, rv.x, dv.x); dv.y = fmaf(b, rv.y, dv.y); dv.z = fmaf(b, rv.z, dv.z); dv.w = fmaf(b, rv.w, dv.w);
+    }
+    return;
+  }
   bernstein_rt(P, u, bu, dbu);
   bernstein<Q>(v, bv, dbv);
   pt = make_float4(0.f, 0.f, 0.f, 0.f);
