@@ -326,6 +326,8 @@ extern "C" int rtn_trace_f64_roots(const rtn_scene* s, const float* const rays[6
   RTN_CHECK(s && s->built && rays && prim, RTN_EINVAL, "rtn_trace_f64: bad arguments");
   RTN_CHECK(max_iter >= 1 && max_iter <= 64, RTN_EINVAL, "rtn_trace_f64: max_iter outside 1..64");
   RTN_CHECK(grid >= 0 && grid <= 16, RTN_EINVAL, "rtn_trace_f64_roots: grid outside 0..16");
+  RTN_CHECK(n >= 0 && n < ((int64_t)1 << 40), RTN_EINVAL, "rtn_trace_f64: bad ray count");
+  if (n == 0) return RTN_OK;
   ExactArgs<double> A{};
   A.nodes = reinterpret_cast<const float4*>(s->nodes.ptr);
   A.ox = rays[0]; A.oy = rays[1]; A.oz = rays[2]; A.dx = rays[3]; A.dy = rays[4]; A.dz = rays[5];

@@ -154,3 +154,35 @@ def test_uniform_degree_scenes(oracle, rtl, deg):
     scene, ex = _exact_vs_oracle(oracle, rtl, surfs, rays, f"degree {deg}")
     assert (ex["hit"] >= 0).sum() > 500
     _check_forms(rtl, scene, rays, ex)
+
+
+def test_closest_root_entry_points_check_their_grid(rtl):
+    """rtn_trace_f64_roots and rtn_solve_pairs_roots reject a grid outside
+    0..16 with RTN_EINVAL before any launch; grid 0 equals the single-start
+    entry points bit for bit; zero pairs and zero rays are no-ops."""
+    print("case closest-root entry points: grid checks, grid 0, empty inputs", flush=True)
+    sc = scenes.config1(0, 64, 64)
+    scene = rtl.Scene(sc.surfaces).build()
+    rays = rtl.generate_rays(sc.camera)
+    for g in (-1, 17):
+        with pytest.raises(rtl.RtnError) as e:
+            rtl.trace_f64(scene, rays, grid=g)
+        assert e.value.code == rtl.RTN_EINVAL
+    a = rtl.trace_f64(scene, rays).get()
+    b = rtl.trace_f64(scene, rays, grid=0).get()
+    for k in ("hit", "t", "u", "v", "dpdu", "dpdv", "t2"):
+        assert np.array_equal(a[k], b[k], equal_nan=True), k
+    assert rtl.trace_f64(scene, rtl.Rays(0), grid=4).get()["hit"].shape == (0,)
+    prm = rtl.params(mode=rtl.MODE_WIDE, max_iter=8, thr_res=1e-5, thr_det=1e-5)
+    pr, pp, puv, cnt = rtl.trace_pairs(scene, rays, prm, 4 * rays.n, all=True)
+    assert cnt > 0
+    sp = rtl.params(max_iter=8, thr_res=1e-5, thr_det=1e-5)
+    with pytest.raises(rtl.RtnError) as e:
+        rtl.solve_pairs(scene, rays, pr, pp, cnt, sp, grid=17)
+    assert e.value.code == rtl.RTN_EINVAL
+    x = rtl.solve_pairs(scene, rays, pr, pp, cnt, sp)
+    y = rtl.solve_pairs(scene, rays, pr, pp, cnt, sp, grid=0)
+    for k in ("iters", "converged", "uv", "t"):
+        assert np.array_equal(x[k], y[k]), k
+    z = rtl.solve_pairs(scene, rays, pr, pp, 0, sp, grid=4)
+    assert z["iters"].shape == (0,)
