@@ -130,6 +130,12 @@ def error_stats(g, ref, tie_rel=1e-5, geom=None, rays_h=None):
         for name, e in (("t_rel", te), ("uv_abs", ue), ("normal_abs", ne)):
             out[name] = {"max": float(e.max()), "p99": float(np.quantile(e, 0.99)), "mean": float(e.mean())}
         out["t_rel_le_1e-5_frac"] = float((te <= 1e-5).mean())
+        # the same on the pixels where the form reached the reference's root
+        # ((u, v) within 1e-3): the precision of the hit, apart from root choice
+        root = ue < 1e-3
+        out["same_root_pixels"] = int(root.sum())
+        out["t_rel_le_1e-5_frac_same_root"] = float((te[root] <= 1e-5).mean()) if root.any() else None
+        out["t_rel_max_same_root"] = float(te[root].max()) if root.any() else None
     return out
 
 
@@ -509,6 +515,7 @@ def bench_c2_c1(args, L, scenes, learned):
               "speedups_vs_plain": {k: round(v["mrays_s"] / plain, 3) for k, v in results.items()} if plain else None}
     write_detail(args, detail)
     extra = {"newton_iters_per_ray": round(H["newton_iters_per_ray"], 4),
+             "overflow": max(int(r["overflow"]) for r in results.values()),
              "speedup_vs_plain": round(H["mrays_s"] / plain, 3) if plain else None,
              "summary": {k: brief(results[k]) for k in ("plain", twin, head) if k in results},
              "roofline": roofline(H["tflops"], head, args.config), "cpu_baseline": short_cpu(cpu),
@@ -523,11 +530,12 @@ def bench_c2_c1(args, L, scenes, learned):
 def brief(r):
     """A form's figures for the stdout line (the full report is in the detail file)."""
     out = {"mrays_s": round(r["mrays_s"], 1), "iters_per_ray": round(r["newton_iters_per_ray"], 3),
-           "fp64_steps": r["fp64_steps"], "overflow": r["overflow"]}
+           "fp64_steps": r["fp64_steps"]}
     e = r.get("fp64")
     if e:
         out.update(mismatch_non_tie=e["patch_mismatch_non_tie"], hit_miss=e["hit_miss_disagree"],
-                   t_le_1e5=round(e.get("t_rel_le_1e-5_frac", float("nan")), 6))
+                   t_le_1e5=round(e.get("t_rel_le_1e-5_frac", float("nan")), 6),
+                   t_le_1e5_same_root=round(e.get("t_rel_le_1e-5_frac_same_root") or float("nan"), 7))
         er = r.get("fp64_closest_root")
         if er:  # the same against the FP64 closest-root reference
             out.update(root_mismatch=er["patch_mismatch_non_tie"], root_hit_miss=er["hit_miss_disagree"])

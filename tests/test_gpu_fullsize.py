@@ -78,3 +78,62 @@ def test_fullsize_forms(rtl, c2):
     m = b["hit"] >= 0
     for k in ("t", "u", "v"):
         assert np.array_equal(a[k][m].view(np.uint32), b[k][m].view(np.uint32)), k
+
+
+def _same_root_t(g, ref):
+    """Fraction of the pixels on the reference's patch and root ((u, v) within
+    1e-3) whose hit t is within 1e-5 relative of the reference's."""
+    same = (g["hit"] >= 0) & (g["hit"] == ref["hit"])
+    ue = np.maximum(np.abs(g["u"][same] - ref["u"][same]), np.abs(g["v"][same] - ref["v"][same]))
+    te = np.abs(g["t"][same].astype(np.float64) - ref["t"][same]) / np.abs(ref["t"][same])
+    root = ue < 1e-3
+    return float(np.mean(te[root] <= 1e-5)), int(root.sum())
+
+
+def _non_tie_mismatches(g, ref):
+    tie = (ref["hit"] >= 0) & np.isfinite(ref["t2"]) & ((ref["t2"] - ref["t"]) <= 1e-5 * np.abs(ref["t"]))
+    both = (g["hit"] >= 0) & (ref["hit"] >= 0)
+    return int((both & (g["hit"] != ref["hit"]) & ~tie).sum())
+
+
+def test_fullsize_headline_learned_form(rtl, c2):
+    """The headline form (learned seed with hit gate 0.7 on full 64-lane
+    visits, 4 + 8 evaluations, mixed precision, stall exit, persistent packet
+    chunks) with a network trained as the benchmark trains it (10M converged
+    pairs, 16 epochs): no stack overflow; hit t within 1e-5 of the FP64
+    trace on >= 99.999% of the pixels where it reached the FP64 trace's root;
+    at most 0.3% non-tie patch mismatches and 0.1% hit/miss disagreements
+    with the FP64 trace (the network reaches nearer roots the plain seed
+    misses); against the FP64 closest-root reference fewer non-tie
+    mismatches than the same form without the network, and fewer Newton
+    evaluations per ray (measured: 47-51k against 58k; 3.14-3.16 against
+    3.25)."""
+    from rtnurbs import learned
+    scene, rays, ref = c2
+    print("training the seed network (10M converged pairs, 16 epochs)", flush=True)
+    net, losses, n, hits = learned.build_seed_network(epochs=16)
+    assert hits == 10_000_000 and losses[-1] < losses[0]
+    base = dict(mode=rtl.MODE_PERSISTENT_PACKET, hull_cull=2, stall_exit=1, precision=rtl.PREC_MIXED, cond_thr=1e4,
+                count=1, **CFG)
+    lrn = dict(seed_mode=rtl.SEED_LEARNED, seed_net=net, learned_iters=4, fallback_iters=8, seed_share=1,
+               hit_gate=0.7, learned_min_lanes=64)
+    print("form headline", flush=True)
+    hl, tl = rtl.trace(scene, rays, rtl.params(**base, **lrn))
+    print("form without the network", flush=True)
+    hn, tn = rtl.trace(scene, rays, rtl.params(**base))
+    assert tl["overflow"] == 0 and tn["overflow"] == 0
+    g, gn = hl.get(), hn.get()
+    frac, n_root = _same_root_t(g, ref)
+    mism = _non_tie_mismatches(g, ref)
+    hm = int(((g["hit"] >= 0) != (ref["hit"] >= 0)).sum())
+    print(f"headline: same-root t within 1e-5 {frac:.7f} on {n_root} pixels, non-tie mismatches {mism}, "
+          f"hit/miss {hm}", flush=True)
+    assert n_root > 2_000_000 and frac >= 0.99999
+    assert mism <= 0.003 * rays.n and hm <= 0.001 * rays.n
+    print("closest-root reference", flush=True)
+    roots = rtl.trace_f64(scene, rays, max_iter=32, thr_det=1e-12, thr_res=1e-10, grid=4).get()
+    ml, mn = _non_tie_mismatches(g, roots), _non_tie_mismatches(gn, roots)
+    print(f"against the closest root: headline {ml}, without the network {mn}; evaluations per ray "
+          f"{tl['newton_evals'] / rays.n:.4f} against {tn['newton_evals'] / rays.n:.4f}", flush=True)
+    assert ml < mn
+    assert tl["newton_evals"] < tn["newton_evals"]
