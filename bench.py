@@ -2,19 +2,23 @@
 """Benchmark of the per-pixel BVH traversal + Newton rational-Bezier
 intersection on one MI355X: Mrays/s and mean Newton iterations per ray of
 each trace form, with hit-t / (u,v) / normal error against the FP64
-reference trace, the plain FP32 method (the drop-in) as the baseline.
+reference trace (the plain method in FP64) and patch mismatches against the
+FP64 closest-root reference (every tested patch also solved from a 4 x 4 grid
+of starts), the plain FP32 method (the drop-in) as the baseline.
 
 Configurations (--config):
   c2  (default) 64 seeded random NURBS (degree 2-5) refined to 10,018
       rational Bezier patches, 1920x1080, 4 jittered spp = 8,294,400 rays.
       One step = one trace of the whole batch with scene and rays resident
-      in HBM.  The seed network of the learned forms is trained at start on
-      the GPU (plain-method pairs from training-seed scenes).
+      in HBM.  The seed networks of the learned forms are trained at start on
+      the GPU (plain-method pairs from training-seed scenes; one of them on
+      closest-root targets).
   c1  one bicubic NURBS (25 patches), 256x256, 1 spp.
   c3  config 1 with 20% silhouette rays; FP32 vs the FP64 correction step.
   c5  32 scenes of 1k-200k patches at 256^2..2048^2 x 4 spp with one shadow
-      and one mirror ray per hit: one frame = persistent primary launch +
-      persistent secondary launch, hipGraph-captured, launch shape autotuned.
+      and one mirror ray per hit: one frame = primary launch + secondary
+      launch (persistent per-lane or packet form), hipGraph-captured, launch
+      shape and forms autotuned.
 
 The JSON line on stdout is short: `value` is the headline form's Mrays/s
 (c2: the learned seed + mixed precision + persistent form the target names,
