@@ -523,8 +523,7 @@ def bench_c2_c1(args, L, scenes, learned):
              "speedup_vs_plain": round(H["mrays_s"] / plain, 3) if plain else None,
              "summary": {k: brief(results[k]) for k in ("plain", twin, head) if k in results},
              "roofline": roofline(H["tflops"], head, args.config), "cpu_baseline": short_cpu(cpu),
-             "kernel_ms_per_launch": {"mean": round(H["kernel_ms"], 4), "median": round(H["kernel_ms_median"], 4),
-                                      "min": round(H["kernel_ms_min"], 4)},
+             "kernel_ms_min": round(H["kernel_ms_min"], 4),
              "detail": args.detail}
     wl = {"workload": workload, "rays": rays.n, "patches": int(st["n_patches"]), "headline_form": head,
           "no_network_form": twin if twin in results else None}

@@ -286,7 +286,9 @@ typedef struct {
   int count;              /* 1: fill the per-ray counters                        */
   int learned_iters;      /* learned seed: Newton iterations from the network's seed before the
                              plain-seed fallback (0 = max_iter)                  */
-  int any_hit;            /* fast forms: a ray ends at its first converged hit (shadow rays) */
+  int any_hit;            /* fast forms: a ray ends at its first converged hit (shadow rays); such an
+                             occlusion query (and a frame's shadow ray) takes no FP64 step under
+                             mixed precision: the step cannot change hit or miss   */
   int refill_min;         /* persistent form: a wave takes new rays once this many of its 64 lanes
                              are idle (0 = 64: whole-wave refill, the fastest for coherent
                              primary rays; smaller values suit incoherent rays)     */

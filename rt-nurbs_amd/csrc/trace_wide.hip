@@ -501,10 +501,14 @@ __device__ __forceinline__ void trace_wide_body(const WArgs& A) {
         }
       }
       if ((res.flags & 1) && 0.0f < res.t && res.t < L.t_max) {
-        if constexpr (LEARNED) record_global(L, A, pi, res.t, res.u, res.v, res.Su, res.Sv, (res.flags & 16) != 0);
+        // an occlusion query's answer is hit or miss, which the FP64 step
+        // does not change: its hit is not flagged for the correction pass
+        const bool occl = A.any_hit || (A.ray_kind && A.ray_kind[L.r] == 0);
+        const bool fix = (res.flags & 16) != 0 && !occl;
+        if constexpr (LEARNED) record_global(L, A, pi, res.t, res.u, res.v, res.Su, res.Sv, fix);
         else record_lds<WB>(L, reinterpret_cast<float(*)[WB]>(&s_rec[0][0]), tid, pi, res.t, res.u, res.v, res.Su,
-                            res.Sv, (res.flags & 16) != 0);
-        if (A.any_hit || (A.ray_kind && A.ray_kind[L.r] == 0)) {  // occlusion query: the first hit ends the ray
+                            res.Sv, fix);
+        if (occl) {  // occlusion query: the first hit ends the ray
           off = 0;
           cand_left = 0;
           cur = -1;
@@ -745,8 +749,10 @@ __device__ __forceinline__ void packet_chunk(const WArgs& A, int64_t r0, unsigne
             if (res.flags & 8) cn.nfb++;
           }
           if ((res.flags & 1) && 0.0f < res.t && res.t < L.t_max) {
-            record_lds<64>(L, s_rec, lane, pi, res.t, res.u, res.v, res.Su, res.Sv, (res.flags & 16) != 0);
-            if (A.any_hit || (A.ray_kind && A.ray_kind[L.r] == 0)) {  // occlusion query: the first hit ends the ray
+            // an occlusion query takes no FP64 step (its answer is hit or miss)
+            const bool occl = A.any_hit || (A.ray_kind && A.ray_kind[L.r] == 0);
+            record_lds<64>(L, s_rec, lane, pi, res.t, res.u, res.v, res.Su, res.Sv, (res.flags & 16) != 0 && !occl);
+            if (occl) {  // occlusion query: the first hit ends the ray
               active = false;
               in_leaf = false;
             }
