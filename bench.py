@@ -586,6 +586,7 @@ def bench_c3(args, L, scenes):
         b = brief(results[k])
         e = results[k].get("fp64_silhouette", {})
         b["silhouette_t_le_1e5"] = round(e.get("t_rel_le_1e-5_frac", float("nan")), 6)
+        b["silhouette_t_le_1e5_same_root"] = round(e.get("t_rel_le_1e-5_frac_same_root") or float("nan"), 6)
         b["silhouette_mismatch_non_tie"] = e.get("patch_mismatch_non_tie")
         summ[k] = b
     wl = {"workload": "config 1 scene, 65,536 rays of which 13,107 silhouette rays (grazing 0-5 deg)",
