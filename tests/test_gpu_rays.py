@@ -103,6 +103,13 @@ def test_secondaries_any_hit_and_shading(rtl):
     ap, tap = rtl.trace(scene, shadow, rtl.params(mode=rtl.MODE_PACKET, hull_cull=2, any_hit=1, count=1, **CFG))
     assert np.array_equal(ap.get()["hit"] >= 0, c.get()["hit"] >= 0)
     assert tap["newton_calls"] <= tc["newton_calls"] * 1.05
+    # occlusion queries take no FP64 step under mixed precision (the step cannot
+    # change hit or miss); the same hit/miss as the closest-hit trace
+    for m in (rtl.MODE_WIDE, rtl.MODE_PACKET):
+        am, tam = rtl.trace(scene, shadow, rtl.params(mode=m, hull_cull=2, any_hit=1, count=1,
+                                                      precision=rtl.PREC_MIXED, cond_thr=1e4, **CFG))
+        assert tam["fp64_steps"] == 0
+        assert np.array_equal(am.get()["hit"] >= 0, c.get()["hit"] >= 0)
     # Lambert shading (no shadows) against numpy
     img = rtl.shade_lambert(rays, h, 128, 128, 1, light, albedo=0.8).get().ravel()
     hit = g["hit"] >= 0
