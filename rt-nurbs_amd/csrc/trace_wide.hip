@@ -583,20 +583,25 @@ constexpr int PKT_STACK = 64;  // one entry per lane of the stack register
 // calling wave; work totals accumulate into acc.
 // The inputs of a patch test of the packet form: the ray, the plain seed (the
 // FP64 reference's seed with mixed precision) and the method's parameters.
-__device__ __forceinline__ void packet_leaf_in(LeafIn& in, const Lane& L, const WArgs& A, int pi) {
-  in.rp = L.rp;
-  in.o = L.o;
-  in.d = L.d;
-  if (A.mixed) {  // the FP64 reference's seed
-    seed64(as_const(A.seed64 + (int64_t)kSeedTermWords * pi), L.o, L.d, in.gu, in.gv);
+// The plain seed of the lane's ray on patch pi (the FP64 reference's seed with
+// mixed precision); the learned kernel computes it again for a fallback
+// attempt instead of keeping it live across the network pass.
+__device__ __forceinline__ void packet_plain_seed(const Lane& L, const WArgs& A, int pi, float& su, float& sv) {
+  if (A.mixed) {
+    seed64(as_const(A.seed64 + (int64_t)kSeedTermWords * pi), L.o, L.d, su, sv);
   } else {
     const cptr<float4> cn4 = as_const(A.corners + 3 * (int64_t)pi);
     const float4 c0 = ld4(cn4, 0), c1 = ld4(cn4, 1), c2 = ld4(cn4, 2);
     ex::bilinear_seed<float>({c0.x, c0.y, c0.z}, {c0.w, c1.x, c1.y}, {c1.z, c1.w, c2.x}, {c2.y, c2.z, c2.w},
-                             {L.o.x, L.o.y, L.o.z}, {L.d.x, L.d.y, L.d.z}, in.gu, in.gv);
+                             {L.o.x, L.o.y, L.o.z}, {L.d.x, L.d.y, L.d.z}, su, sv);
   }
-  in.pu = in.gu;
-  in.pv = in.gv;
+}
+
+__device__ __forceinline__ void packet_leaf_in(LeafIn& in, const Lane& L, const WArgs& A, int pi) {
+  in.rp = L.rp;
+  in.o = L.o;
+  in.d = L.d;
+  packet_plain_seed(L, A, pi, in.gu, in.gv);
   in.seed = L.r + A.seed_base;
   in.max_iter = A.max_iter;
   in.learned_iters = A.learned_iters;
@@ -609,8 +614,8 @@ __device__ __forceinline__ void packet_leaf_in(LeafIn& in, const Lane& L, const 
   in.t_tol = A.t_tol;
 }
 
-template <bool COUNT, bool LEARNED, int NACC>
-__device__ __forceinline__ void packet_chunk(const WArgs& A, int64_t r0, unsigned long long (&acc)[NACC],
+template <bool COUNT, bool LEARNED, int NACC, class Acc>
+__device__ __forceinline__ void packet_chunk(const WArgs& A, int64_t r0, Acc (&acc)[NACC],
                                              mlp::bf16x8 (*s_x)[3], int* s_skip, float (*s_rec)[64]) {
   const int lane = threadIdx.x & 63;
   Lane L;
@@ -703,7 +708,7 @@ __device__ __forceinline__ void packet_chunk(const WArgs& A, int64_t r0, unsigne
         LeafOut res;
         res.flags = 0;
         // leaf inputs: the learned kernel sets them on every lane (no per-lane merges
-        // of copies that live across the network pass: no spills at three waves),
+        // of copies that live across the network pass: no spills at four waves),
         // the plain kernel on the lanes that test the patch (fewer spills at five)
         LeafIn in;
         if (LEARNED || act) packet_leaf_in(in, L, A, pi);        // the pair starts from the network's seed; with fewer than learned_min_lanes
@@ -726,35 +731,85 @@ __device__ __forceinline__ void packet_chunk(const WArgs& A, int64_t r0, unsigne
 #pragma unroll
             for (int k = 0; k < 3; k++) s_x[row][k] = z;
           }
-          float ph = 1.0f;
-          mlp::seed_wave_rows<true>(A.mlp_tiles, A.mlp_bias, own, row, n_act, in.gu, in.gv, ph, s_x, lane);
+          float nu = 0.0f, nv = 0.0f, ph = 1.0f;
+          mlp::seed_wave_rows<true>(A.mlp_tiles, A.mlp_bias, own, row, n_act, nu, nv, ph, s_x, lane);
           if (A.seed_share) {
-            const float lu = __shfl(in.gu, lead), lv = __shfl(in.gv, lead), lp = __shfl(ph, lead);
+            const float lu = __shfl(nu, lead), lv = __shfl(nv, lead), lp = __shfl(ph, lead);
             if (act && !own) {
-              in.gu = lu;
-              in.gv = lv;
+              nu = lu;
+              nv = lv;
               ph = lp;
             }
           }
-          gate_pair(A, ph, act, lrn, in, cn.nskip);
-        }
-        if (act) {
-          if (COUNT) cn.npt++;
-          in.learned = lrn;
-          res = leaf_dispatch(degree_class(p, q), net, p, in);  // the patch's degree: a wave-uniform branch
-          if (COUNT) {
-            cn.nc++;
-            cn.ne += res.evals;
-            cn.nfl += res.evals * ex::eval_flops(p, q);
-            if (res.flags & 8) cn.nfb++;
+          // the hit gate (gate_pair's rule): the plain seed stays below hit_gate
+          if (act) {
+            if (ph < A.hit_skip) {
+              act = false;
+              cn.nskip++;
+            } else if (ph < A.hit_gate) {
+              lrn = false;
+            } else {
+              in.gu = nu;
+              in.gv = nv;
+            }
           }
-          if ((res.flags & 1) && 0.0f < res.t && res.t < L.t_max) {
-            // an occlusion query takes no FP64 step (its answer is hit or miss)
-            const bool occl = A.any_hit || (A.ray_kind && A.ray_kind[L.r] == 0);
-            record_lds<64>(L, s_rec, lane, pi, res.t, res.u, res.v, res.Su, res.Sv, (res.flags & 16) != 0 && !occl);
-            if (occl) {  // occlusion query: the first hit ends the ray
-              active = false;
-              in_leaf = false;
+        }
+        if constexpr (!LEARNED) {
+          if (act) {
+            if (COUNT) cn.npt++;
+            in.learned = 0;
+            res = leaf_dispatch(degree_class(p, q), net, p, in);  // the patch's degree: a wave-uniform branch
+            if (COUNT) {
+              cn.nc++;
+              cn.ne += res.evals;
+              cn.nfl += res.evals * ex::eval_flops(p, q);
+            }
+            if ((res.flags & 1) && 0.0f < res.t && res.t < L.t_max) {
+              // an occlusion query takes no FP64 step (its answer is hit or miss)
+              const bool occl = A.any_hit || (A.ray_kind && A.ray_kind[L.r] == 0);
+              record_lds<64>(L, s_rec, lane, pi, res.t, res.u, res.v, res.Su, res.Sv, (res.flags & 16) != 0 && !occl);
+              if (occl) {  // occlusion query: the first hit ends the ray
+                active = false;
+                in_leaf = false;
+              }
+            }
+          }
+        } else {
+          if (COUNT && act) {
+            cn.npt++;
+            cn.nc++;
+          }
+          in.learned = 0;
+          // attempt 0 from the lane's seed; a learned seed whose Newton does not
+          // converge is solved again from the plain seed (attempt 1).  Each
+          // attempt records its own hit, so no result of attempt 0 stays live
+          // (per wave) while other lanes run attempt 1: the kernel keeps four
+          // waves per SIMD
+          bool solve = act;
+          if (lrn) in.max_iter = A.learned_iters;
+#pragma unroll 1
+          for (int attempt = 0; __ballot(solve); attempt++) {
+            if (solve) {
+              res = leaf_dispatch(degree_class(p, q), net, p, in);  // the patch's degree: a wave-uniform branch
+              if (COUNT) {
+                cn.ne += res.evals;
+                cn.nfl += res.evals * ex::eval_flops(p, q);
+              }
+              if ((res.flags & 1) && 0.0f < res.t && res.t < L.t_max) {
+                const bool occl = A.any_hit || (A.ray_kind && A.ray_kind[L.r] == 0);
+                record_lds<64>(L, s_rec, lane, pi, res.t, res.u, res.v, res.Su, res.Sv,
+                               (res.flags & 16) != 0 && !occl);
+                if (occl) {
+                  active = false;
+                  in_leaf = false;
+                }
+              }
+              solve = lrn && attempt == 0 && !(res.flags & 1);
+              if (solve) {
+                if (COUNT) cn.nfb++;
+                packet_plain_seed(L, A, pi, in.gu, in.gv);
+                in.max_iter = A.fallback_iters;
+              }
             }
           }
         }
@@ -788,7 +843,10 @@ __device__ __forceinline__ void packet_body(const WArgs& A) {
   __shared__ float s_rec[kRecWords][64];  // per lane, its closest hit's record
   const int lane = threadIdx.x & 63;
   constexpr int NACC = COUNT ? 12 : 1;
-  unsigned long long acc[NACC] = {};
+  // without counting only the overflowed rays are summed: at most one per lane
+  // and chunk, so 32 bits per lane hold it
+  using Acc = std::conditional_t<COUNT, unsigned long long, uint32_t>;
+  Acc acc[NACC] = {};
   if (!PERSISTENT) {
     packet_chunk<COUNT, LEARNED>(A, (int64_t)blockIdx.x * 64, acc, s_x, s_skip, s_rec);
   } else {
@@ -798,7 +856,7 @@ __device__ __forceinline__ void packet_body(const WArgs& A) {
     for (int queues_left = kQueues; queues_left > 0;) {
       unsigned int idx = 0;
       if (lane == 0) idx = atomicAdd(A.work + kQueueStride * queue, 1u);
-      idx = __shfl(idx, 0);
+      idx = __builtin_amdgcn_readfirstlane(idx);  // wave-uniform: a scalar register
       const int64_t c = queue + (int64_t)kQueues * idx;
       if (c >= nchunks) {
         queue = (queue + 1) % kQueues;
@@ -820,15 +878,16 @@ __device__ __forceinline__ void packet_body(const WArgs& A) {
 
 // Five waves per SIMD: the kernel needs 111 registers per lane (four waves);
 // held to 96 it spills 28-60 bytes and runs 5% faster.  The learned kernel
-// stays at three waves, where it needs 157-160 registers and no spills: at
-// four it spills 208-220 bytes and is 1-3% faster.
+// runs at four waves: with each attempt of a pair recording its own hit it needs
+// 127-128 registers and no spills, 4% (persistent) to 11% (one block per chunk)
+// faster than at three waves.
 template <bool PERSISTENT, bool COUNT>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 5))) k_trace_packet(const WArgs A) {
   packet_body<PERSISTENT, COUNT, false>(A);
 }
 
 template <bool PERSISTENT, bool COUNT>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) k_trace_packet_learned(const WArgs A) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 4))) k_trace_packet_learned(const WArgs A) {
   packet_body<PERSISTENT, COUNT, true>(A);
 }
 
