@@ -10,8 +10,9 @@ checked against an independent implementation:
                       control net projected on the ray planes, degrees)
   box_features(...)   the 20 pair features of kind 1 (ray entry point and
                       direction, four corners, degrees, in the patch box's frame)
-  forward(...)        20 -> 64 -> 64 -> 64 -> 3, GELU (tanh form), sigmoid out
-                      (u, v, hit probability)
+  forward(...)        20 -> 64 -> 64 -> 64 -> 3, GELU (tanh form; with
+                      infer=True its polynomial fit gelu_infer, the GPU's
+                      inference activation), sigmoid out (u, v, hit probability)
   loss_and_grad(...)  mean over pairs of h |uv - target|^2 + lambda BCE(hit, h)
                       and its gradient
 
@@ -24,7 +25,8 @@ FP64.  With quantize=False everything is FP64.
 Pinning (tests/test_mlp_oracle.py): the FP64 gradient is checked against
 torch.autograd in float64 (torch's own binary cross-entropy with logits) and
 against central finite differences; GELU against torch's tanh-approximate
-GELU; both feature kinds against closed forms (axis-aligned rays over flat
+GELU; gelu_infer against the tanh GELU (deviation bound, exact tails beyond
+the clamp, odd/even structure); both feature kinds against closed forms (axis-aligned rays over flat
 nets, odd/even middle indices, entry points, similarity invariance).
 """
 from __future__ import annotations
@@ -72,6 +74,21 @@ def pack(P):
 
 def gelu(x):
     return 0.5 * x * (1.0 + np.tanh(GK0 * (x + GK1 * x ** 3)))
+
+
+# inference activation: x * clip(0.5 + x Q(x^2), 0, 1), Q of degree 4 fitted
+# (minimax) to gelu(x)/x - 0.5 on [0, 3.4] with 3.4 Q(3.4^2) = 0.5
+GELU_INFER_Q = (3.940125455699649e-01, -5.970189424328037e-02, 6.703262281516497e-03, -4.152066351606772e-04,
+                1.0574128852131707e-05)
+
+
+def gelu_infer(x):
+    x = np.asarray(x, np.float64)
+    x2 = x * x
+    t = np.zeros_like(x)
+    for a in GELU_INFER_Q[::-1]:
+        t = t * x2 + a
+    return x * np.clip(0.5 + x * t, 0.0, 1.0)
 
 
 def gelu_grad(x):
@@ -161,18 +178,21 @@ def box_features(o, d, bounds, nets, p, q):
     return out
 
 
-def forward(w, X, quantize=False):
-    """X: (n, 20) -> (y (n,3): u, v, hit probability; cache)."""
+def forward(w, X, quantize=False, infer=False):
+    """X: (n, 20) -> (y (n,3): u, v, hit probability; cache).  infer=True
+    uses the inference activation gelu_infer (the GPU's inference passes),
+    otherwise the tanh GELU of training."""
     P = unpack(w)
     q = bf16 if quantize else (lambda a: np.asarray(a, np.float64))
+    act = gelu_infer if infer else gelu
     x = q(X)
     W = {k: q(P[k]) for k in ("W1", "W2", "W3", "W4")}
     z1 = x @ W["W1"].T + P["b1"]
-    h1 = q(gelu(z1))
+    h1 = q(act(z1))
     z2 = h1 @ W["W2"].T + P["b2"]
-    h2 = q(gelu(z2))
+    h2 = q(act(z2))
     z3 = h2 @ W["W3"].T + P["b3"]
-    h3 = q(gelu(z3))
+    h3 = q(act(z3))
     z4 = h3 @ W["W4"].T + P["b4"]
     y = sigmoid(z4)
     return y, dict(x=x, W=W, z1=z1, h1=h1, z2=z2, h2=h2, z3=z3, h3=h3, y=y)

@@ -1,5 +1,6 @@
 // Learned Newton seed: a 20 -> 64 -> 64 -> 64 -> 3 perceptron (GELU hidden
-// units, sigmoid outputs) mapping a (ray, patch) pair to a starting (u,v) and
+// units: tanh form in training, its polynomial fit gelu_infer in inference;
+// sigmoid outputs) mapping a (ray, patch) pair to a starting (u,v) and
 // the probability that Newton converges on the pair (a hit).
 //
 // Inference runs per wave on the matrix cores: v_mfma_f32_16x16x32_bf16 with
@@ -71,6 +72,21 @@ __device__ __forceinline__ float gelu(float x) {
   constexpr float C = -2.0f * GK0 * LOG2E;  // sigmoid(z) = 1 / (1 + 2^(-z log2 e))
   const float e = __builtin_amdgcn_exp2f(x * C * fmaf(GK1, x * x, 1.0f));
   return x * __builtin_amdgcn_rcpf(1.0f + e);
+}
+// Inference GELU: x * clamp(0.5 + x Q(x^2), 0, 1) with Q of degree 4, the
+// minimax fit of gelu(x)/x - 0.5 on [0, 3.4] held to 3.4 Q(3.4^2) = 0.5, so the
+// clamp takes over at |x| = 3.4 (x above, 0 below).  Within 1.075e-3 of the
+// tanh form on the whole line (the BF16 rounding that follows moves an
+// activation in [0.5, 1) by up to 1.95e-3): full-rate FMAs in place of an
+// exponential and a reciprocal.  The training passes keep the tanh form (its
+// gradient is gelu_grad).
+__device__ __forceinline__ float gelu_infer(float x) {
+  const float x2 = x * x;
+  float t = fmaf(1.0574128852131707e-05f, x2, -4.152066351606772e-04f);
+  t = fmaf(t, x2, 6.703262281516497e-03f);
+  t = fmaf(t, x2, -5.970189424328037e-02f);
+  t = fmaf(t, x2, 3.940125455699649e-01f);
+  return x * __builtin_amdgcn_fmed3f(fmaf(x, t, 0.5f), 0.0f, 1.0f);
 }
 // d gelu / dx = s + 2 x s (1 - s) k0 (1 + 3 k1 x^2), s = sigmoid(2 k0 (x + k1 x^3))
 __device__ __forceinline__ float gelu_grad(float x) {
@@ -200,7 +216,8 @@ __device__ __forceinline__ void wave_lds_sync() {
 // Forward pass for nblk blocks of 16 pairs.  s_x[r][c] holds the features of
 // pair row r as BF16 (c = 8-value column group, inputs 20..31 zero).  The
 // output logits of pair r go to s_logit[r] (x, y, z: u, v, hit).  KEEP (training) also stores the
-// FP32 pre-activations of the three hidden layers, s_z[layer][unit][pair].
+// FP32 pre-activations of the three hidden layers, s_z[layer][unit][pair], and
+// uses the tanh GELU of training; inference uses gelu_infer.
 // Must be called by all 64 lanes with nblk wave-uniform.
 template <bool KEEP>
 __device__ __forceinline__ void forward_wave(const bf16x8* __restrict__ tiles, const float* __restrict__ bias,
@@ -231,7 +248,10 @@ __device__ __forceinline__ void forward_wave(const bf16x8* __restrict__ tiles, c
 #pragma unroll
       for (int kb = 0; kb < 2; kb++)
 #pragma unroll
-        for (int i = 0; i < 8; i++) h[kb][i] = (__bf16)gelu(d[2 * kb + (i >> 2)][i & 3]);
+        for (int i = 0; i < 8; i++) {
+          const float z = d[2 * kb + (i >> 2)][i & 3];
+          h[kb][i] = (__bf16)(KEEP ? gelu(z) : gelu_infer(z));
+        }
       if (layer == 2) break;
       const int tl = layer == 0 ? T_L2 : T_L3;
       const int bb = layer == 0 ? BB2 : BB3;
@@ -294,7 +314,7 @@ __device__ __forceinline__ void forward_seed(const bf16x8* __restrict__ tiles, c
 #pragma unroll
       for (int kb = 0; kb < 2; kb++)
 #pragma unroll
-        for (int i = 0; i < 8; i++) h[kb][i] = (__bf16)gelu(d[2 * kb + (i >> 2)][i & 3]);
+        for (int i = 0; i < 8; i++) h[kb][i] = (__bf16)gelu_infer(d[2 * kb + (i >> 2)][i & 3]);
 #pragma unroll
       for (int ob = 0; ob < 4; ob++) d[ob] = mfma(w[2 * ob + 1], h[1], mfma(w[2 * ob], h[0], b[ob]));
     }
@@ -303,7 +323,7 @@ __device__ __forceinline__ void forward_seed(const bf16x8* __restrict__ tiles, c
 #pragma unroll
     for (int kb = 0; kb < 2; kb++)
 #pragma unroll
-      for (int i = 0; i < 8; i++) h[kb][i] = (__bf16)gelu(d[2 * kb + (i >> 2)][i & 3]);
+      for (int i = 0; i < 8; i++) h[kb][i] = (__bf16)gelu_infer(d[2 * kb + (i >> 2)][i & 3]);
     const f32x4 o = mfma(w41, h[1], mfma(w40, h[0], b4));
     // lane m (< 16) holds the logits of pair 16 nb + m
     const float a = __shfl(o[0], row & 15), bv = __shfl(o[1], row & 15), c = __shfl(o[2], row & 15);
@@ -344,7 +364,7 @@ __device__ __forceinline__ void forward_seed_lean(const bf16x8* __restrict__ til
 #pragma unroll
       for (int kb = 0; kb < 2; kb++)
 #pragma unroll
-        for (int i = 0; i < 8; i++) h[kb][i] = (__bf16)gelu(d[2 * kb + (i >> 2)][i & 3]);
+        for (int i = 0; i < 8; i++) h[kb][i] = (__bf16)gelu_infer(d[2 * kb + (i >> 2)][i & 3]);
       asm volatile("" : "+v"(toff));
       tp = tiles + toff;
 #pragma unroll
@@ -355,7 +375,7 @@ __device__ __forceinline__ void forward_seed_lean(const bf16x8* __restrict__ til
 #pragma unroll
     for (int kb = 0; kb < 2; kb++)
 #pragma unroll
-      for (int i = 0; i < 8; i++) h[kb][i] = (__bf16)gelu(d[2 * kb + (i >> 2)][i & 3]);
+      for (int i = 0; i < 8; i++) h[kb][i] = (__bf16)gelu_infer(d[2 * kb + (i >> 2)][i & 3]);
     asm volatile("" : "+v"(toff));
     tp = tiles + toff;
     const f32x4 o =

@@ -49,7 +49,9 @@ def test_forward_matches_oracle(rtl):
     uv_all, ph = uv_all.get(), ph.get()
     uv_m = net.forward(Xd, rtl.DeviceArray.from_host(valid)).get()
     assert np.array_equal(net.forward(Xd).get(), uv_all)  # the hit output is optional
-    yq, _ = M.forward(w, X.astype(np.float64), quantize=True)
+    # inference uses the polynomial GELU (gelu_infer); the FP64 tanh-GELU
+    # network of training is the looser bound
+    yq, _ = M.forward(w, X.astype(np.float64), quantize=True, infer=True)
     y, _ = M.forward(w, X.astype(np.float64))
     got = np.concatenate([uv_all, ph[:, None]], 1)
     assert np.abs(got - yq).max() < 2e-3

@@ -57,6 +57,24 @@ def test_gelu_matches_torch_tanh_gelu():
     np.testing.assert_allclose(M.gelu_grad(x), xt.grad.numpy(), rtol=1e-12, atol=1e-14)
 
 
+def test_gelu_infer_follows_tanh_gelu():
+    # the inference activation is a polynomial fit of the tanh GELU: within
+    # its stated bound everywhere, the identity gelu(x) - gelu(-x) = x of any
+    # x * sigmoid-like form, and exactly x / 0 beyond the clamp at |x| = 3.4
+    x = np.linspace(-40, 40, 800001)
+    dev = np.abs(M.gelu_infer(x) - M.gelu(x))
+    assert dev.max() < 1.08e-3
+    assert dev.max() > 1.0e-3  # a minimax fit: the bound is reached (equioscillation)
+    inner = np.abs(x) < 3.4
+    np.testing.assert_allclose(M.gelu_infer(x[inner]) - M.gelu_infer(-x[inner]), x[inner], atol=1e-14)
+    big = x >= 3.4
+    assert np.array_equal(M.gelu_infer(x[big]), x[big])
+    assert np.all(M.gelu_infer(-x[big]) == 0)
+    q = M.GELU_INFER_Q
+    assert 3.4 * sum(a * 3.4 ** (2 * i) for i, a in enumerate(q)) == pytest.approx(0.5, abs=1e-12)
+    assert M.gelu_infer(0.0) == 0.0
+
+
 def test_gradient_matches_autograd():
     rng = np.random.default_rng(3)
     X = rng.normal(0, 1, (37, M.NIN))
