@@ -270,6 +270,21 @@ __device__ __forceinline__ void gate_pair(const WArgs& A, float ph, bool& test, 
   }
 }
 
+
+// The kernel's argument block read again from the kernel-argument segment at
+// the point of use: a scalar load through an opaque pointer, which keeps the
+// load there.  The trace kernels hold more values in scalar registers than
+// the 106 a wave has (control-net rows, node boxes, branch masks); arguments
+// kept live across their loops were spilled to vector-register lanes and read
+// back with one VALU instruction per register at every use.  Valid in the
+// kernels whose only argument is their WArgs, passed by value (it starts the
+// segment).
+__device__ __forceinline__ const WArgs& args_at_use() {
+  auto p = (const __attribute__((address_space(4))) WArgs*)__builtin_amdgcn_kernarg_segment_ptr();
+  asm volatile("" : "+s"(p));
+  return *(const WArgs*)p;
+}
+
 template <int WB, int WSTK, bool PERSISTENT, bool COUNT, bool LEARNED>
 __device__ __forceinline__ void trace_wide_body(const WArgs& A) {
   __shared__ int s_code[WSTK][WB];
@@ -297,6 +312,7 @@ __device__ __forceinline__ void trace_wide_body(const WArgs& A) {
   // allocation as small as the persistent one's).
   while (true) {
     if (!PERSISTENT && !exhausted) {  // the lane's one ray, taken on the first pass
+      const WArgs& A = args_at_use();
       exhausted = true;
       const int64_t r = (int64_t)blockIdx.x * WB + tid;
       if (r < A.n_rays) {
@@ -306,6 +322,7 @@ __device__ __forceinline__ void trace_wide_body(const WArgs& A) {
       }
     }
     if (PERSISTENT && !exhausted) {
+      const WArgs& A = args_at_use();
       // lanes without a ray take consecutive positions of the wave's work
       // queue, one atomic per wave; a wave takes new rays once `refill_min` of
       // its lanes are idle (or all of them)
@@ -341,6 +358,7 @@ __device__ __forceinline__ void trace_wide_body(const WArgs& A) {
     // first tested against the oriented box, then the projected control points.
     while (has_ray) {
       if (cand_left > 0) {
+        const WArgs& A = args_at_use();
         bool drop = A.skip_prim && cand_first == s_skip[tid];
         if (!drop && A.hull_cull) {
           const int pqv = A.pq[cand_first];
@@ -362,11 +380,12 @@ __device__ __forceinline__ void trace_wide_body(const WArgs& A) {
         continue;
       }
       if (cur >= 0) {
-        expand<WB, WSTK, COUNT>(L, cn, A.nodes, cur, s_code, s_te, tid, off);
+        expand<WB, WSTK, COUNT>(L, cn, args_at_use().nodes, cur, s_code, s_te, tid, off);
         cur = -1;
         continue;
       }
       if (off == 0) {  // traversal finished: the ray is done
+        const WArgs& A = args_at_use();
         if constexpr (!LEARNED) write_record<WB>(L, A, reinterpret_cast<const float(*)[WB]>(&s_rec[0][0]), tid);
         lane_finish<COUNT>(L, A, cn);
         if constexpr (COUNT) {
@@ -397,6 +416,7 @@ __device__ __forceinline__ void trace_wide_body(const WArgs& A) {
       continue;
     }
     // ---- leaf phase: one primitive per pending lane, Newton by degree class
+    const WArgs& A = args_at_use();
     int pi = 0, p = 0, q = 0, cls = -1;
     float gu = 0.5f, gv = 0.5f;
     float4 c0 = make_float4(0.f, 0.f, 0.f, 0.f), c1 = c0, c2 = c0;
@@ -622,6 +642,7 @@ __device__ __forceinline__ void packet_chunk(const WArgs& A, int64_t r0, Acc (&a
   Cnt cn{};
   int root = -1;
   {
+    const WArgs& A = args_at_use();
     const int64_t r = r0 + lane;
     if (r < A.n_rays) root = lane_start(L, A, (uint32_t)r);
     if (A.skip_prim) s_skip[lane] = root >= 0 ? skip_patch_of(A, (uint32_t)r) : -1;
@@ -641,6 +662,7 @@ __device__ __forceinline__ void packet_chunk(const WArgs& A, int64_t r0, Acc (&a
   int sp = 0;   // wave-uniform depth
   while (__ballot(active)) {
     if (e >= 0) {
+      const WArgs& A = args_at_use();
       const cptr<float> base = as_const(reinterpret_cast<const float*>(A.nodes + e));
       const cptr<int> chp = as_const(reinterpret_cast<const int*>(A.nodes + e) + 24);
       const int cha[4] = {chp[0], chp[1], chp[2], chp[3]};
@@ -686,6 +708,7 @@ __device__ __forceinline__ void packet_chunk(const WArgs& A, int64_t r0, Acc (&a
       const int first = leaf & 0xffffff, count = ((leaf >> 24) & 0x7f) + 1;
 #pragma unroll 1
       for (int pi = first; pi < first + count && __ballot(in_leaf); pi++) {
+        const WArgs& A = args_at_use();
         bool act = in_leaf && (!A.skip_prim || pi != s_skip[lane]);  // the patch the ray leaves is not tested
         const int pqv = as_const(A.pq)[pi];
         const cptr<float4> net = as_const(A.cp + as_const(A.cp_off)[pi]);
@@ -716,6 +739,7 @@ __device__ __forceinline__ void packet_chunk(const WArgs& A, int64_t r0, Acc (&a
         const bool net_pass = LEARNED && __popcll(__ballot(act)) >= A.learned_min_lanes;
         bool lrn = net_pass;
         if (net_pass) {  // network seeds of the wave's pairs on this patch (matrix cores), plain seed as fallback
+          const WArgs& A = args_at_use();
           // with seed sharing every lane of an aligned quad whose first lane tests the patch takes that lane's seed
           const int lead = lane & ~3;
           const bool lead_act = __shfl((int)act, lead) != 0;
@@ -821,6 +845,7 @@ __device__ __forceinline__ void packet_chunk(const WArgs& A, int64_t r0, Acc (&a
   }
   }
   if (had_ray) {
+    const WArgs& A = args_at_use();
     write_record<64>(L, A, s_rec, lane);
     lane_finish<COUNT>(L, A, cn);
     if constexpr (COUNT) {
@@ -851,14 +876,14 @@ __device__ __forceinline__ void packet_body(const WArgs& A) {
     packet_chunk<COUNT, LEARNED>(A, (int64_t)blockIdx.x * 64, acc, s_x, s_skip, s_rec);
   } else {
     // the work queues hand out 64-ray chunks (one position per chunk here)
-    const int64_t nchunks = (A.n_rays + 63) / 64;
     int queue = blockIdx.x % kQueues;
     for (int queues_left = kQueues; queues_left > 0;) {
+      const WArgs& K = args_at_use();
       unsigned int idx = 0;
-      if (lane == 0) idx = atomicAdd(A.work + kQueueStride * queue, 1u);
+      if (lane == 0) idx = atomicAdd(K.work + kQueueStride * queue, 1u);
       idx = __builtin_amdgcn_readfirstlane(idx);  // wave-uniform: a scalar register
       const int64_t c = queue + (int64_t)kQueues * idx;
-      if (c >= nchunks) {
+      if (c >= (K.n_rays + 63) / 64) {
         queue = (queue + 1) % kQueues;
         queues_left--;
         continue;
@@ -934,11 +959,11 @@ __device__ __forceinline__ bool fix_one(const WArgs& A, int64_t r) {
 }
 
 // Runs and empties the block's list (called by every thread of the block).
-__device__ __forceinline__ void fix_list(const WArgs& A, const uint32_t* s_list, int* s_n, unsigned long long& done) {
+__device__ __forceinline__ void fix_list(const uint32_t* s_list, int* s_n, unsigned long long& done) {
   __syncthreads();
   const int nl = *s_n;
   for (int i = (int)threadIdx.x; i < nl; i += kFixBlock)
-    if (fix_one(A, (int64_t)s_list[i])) done++;
+    if (fix_one(args_at_use(), (int64_t)s_list[i])) done++;
   __syncthreads();
   if (threadIdx.x == 0) *s_n = 0;
   __syncthreads();
@@ -967,9 +992,9 @@ __global__ void __launch_bounds__(kFixBlock) k_fix64(const WArgs A) {
     __syncthreads();
     const int nl = s_n;
     __syncthreads();  // every thread has read s_n before any appends to it again
-    if (nl > kFixRange) fix_list(A, s_list, &s_n, done);  // block-uniform
+    if (nl > kFixRange) fix_list(s_list, &s_n, done);  // block-uniform
   }
-  fix_list(A, s_list, &s_n, done);
+  fix_list(s_list, &s_n, done);
   if (A.totals) {
     for (int sft = 32; sft > 0; sft >>= 1) done += __shfl_xor(done, sft);
     if ((tid & 63) == 0 && done) atomicAdd(A.totals + 8, done);
