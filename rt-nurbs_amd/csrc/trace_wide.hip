@@ -585,6 +585,22 @@ k_trace_wide_learned(const WArgs A) {
 __device__ __forceinline__ int rl_int(int x, int l) { return __builtin_amdgcn_readlane(x, l); }
 __device__ __forceinline__ float rl_float(float x, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l)); }
 
+// Component k (a compile-time index after unrolling) of a float4.
+__device__ __forceinline__ float f4c(const float4& v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; }
+
+// The slab test of the lane's ray against the box [lx,hx] x [ly,hy] x [lz,hz]
+// (child_box's arithmetic, on values already in registers).
+__device__ __forceinline__ bool slab_box(float lx, float hx, float ly, float hy, float lz, float hz, const Lane& L,
+                                         float& tmin) {
+  const float ox = L.o.x * L.rd.x, oy = L.o.y * L.rd.y, oz = L.o.z * L.rd.z;
+  const float tx0 = fmaf(lx, L.rd.x, -ox), tx1 = fmaf(hx, L.rd.x, -ox);
+  const float ty0 = fmaf(ly, L.rd.y, -oy), ty1 = fmaf(hy, L.rd.y, -oy);
+  const float tz0 = fmaf(lz, L.rd.z, -oz), tz1 = fmaf(hz, L.rd.z, -oz);
+  tmin = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+  const float tmax = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1)) * 1.0000005f;
+  return tmin <= tmax && tmin < L.t_max;
+}
+
 // Entry and exit distances of the lane's ray through child slot k of a node
 // (the one-ray form's slab test, expand()).
 __device__ __forceinline__ bool child_box(cptr<float> f, int k, const Lane& L, float& tmin) {
@@ -663,33 +679,41 @@ __device__ __forceinline__ void packet_chunk(const WArgs& A, int64_t r0, Acc (&a
   while (__ballot(active)) {
     if (e >= 0) {
       const WArgs& A = args_at_use();
-      const cptr<float> base = as_const(reinterpret_cast<const float*>(A.nodes + e));
+      // the whole node in one batch of scalar loads (no load behind a per-child
+      // branch), then the four slab tests on every lane, masked afterwards
+      const cptr<float4> nd = as_const(reinterpret_cast<const float4*>(A.nodes + e));
+      const float4 bx[6] = {ld4(nd, 0), ld4(nd, 1), ld4(nd, 2), ld4(nd, 3), ld4(nd, 4), ld4(nd, 5)};
       const cptr<int> chp = as_const(reinterpret_cast<const int*>(A.nodes + e) + 24);
       const int cha[4] = {chp[0], chp[1], chp[2], chp[3]};
-      float key[4];
+      // entry distances as their bit patterns: every key is +0 .. +inf, where the
+      // unsigned order of the bits is the order of the values, so the sorting
+      // network runs on the scalar unit
+      uint32_t key[4];
       int code[4];
 #pragma unroll
       for (int k = 0; k < 4; k++) {
         float tmin;
-        const bool hit = active && cha[k] != 0x7fffffff && child_box(base, k, L, tmin);
+        const bool in_box = slab_box(f4c(bx[0], k), f4c(bx[1], k), f4c(bx[2], k), f4c(bx[3], k), f4c(bx[4], k),
+                                     f4c(bx[5], k), L, tmin);
+        const bool hit = active & (cha[k] != 0x7fffffff) & in_box;
         const unsigned long long m = __ballot(hit);
-        key[k] = INFINITY;
-        if (m) key[k] = rl_float(tmin, __ffsll((long long)m) - 1);
+        key[k] = 0x7f800000u;
+        if (m) key[k] = (uint32_t)rl_int(__float_as_int(tmin), __ffsll((long long)m) - 1) & 0x7fffffffu;
         // leaves are pushed as their parent and slot, re-tested on the pop
         code[k] = cha[k] >= 0 ? cha[k] : (int)(0x80000000u | ((uint32_t)e << 2) | (uint32_t)k);
       }
       if (COUNT && active) cn.nbox += 4;
-#define RTN_CSWAP(a, b)                                   \
-  if (key[a] < key[b]) {                                  \
-    float tt = key[a]; key[a] = key[b]; key[b] = tt;      \
-    int cc = code[a]; code[a] = code[b]; code[b] = cc;    \
+#define RTN_CSWAP(a, b)                                      \
+  if (key[a] < key[b]) {                                     \
+    uint32_t tt = key[a]; key[a] = key[b]; key[b] = tt;      \
+    int cc = code[a]; code[a] = code[b]; code[b] = cc;       \
   }
       RTN_CSWAP(0, 1) RTN_CSWAP(2, 3) RTN_CSWAP(0, 2) RTN_CSWAP(1, 3) RTN_CSWAP(1, 2)
 #undef RTN_CSWAP
       // descending keys: push far to near so the nearest is on top
 #pragma unroll
       for (int k = 0; k < 4; k++) {
-        if (key[k] < INFINITY) {
+        if (key[k] < 0x7f800000u) {
           if (sp < PKT_STACK) {
             stk = lane == sp ? code[k] : stk;
             sp++;
@@ -703,7 +727,7 @@ __device__ __forceinline__ void packet_chunk(const WArgs& A, int64_t r0, Acc (&a
       const cptr<float> base = as_const(reinterpret_cast<const float*>(A.nodes + parent));
       const int leaf = as_const(reinterpret_cast<const int*>(A.nodes + parent) + 24)[slot];
       float tmin;
-      bool in_leaf = active && child_box(base, slot, L, tmin);
+      bool in_leaf = active & child_box(base, slot, L, tmin);  // no branch around the box's loads
       if (COUNT && in_leaf) cn.nleaf++;
       const int first = leaf & 0xffffff, count = ((leaf >> 24) & 0x7f) + 1;
 #pragma unroll 1
