@@ -588,17 +588,17 @@ __device__ __forceinline__ float rl_float(float x, int l) { return __int_as_floa
 // Component k (a compile-time index after unrolling) of a float4.
 __device__ __forceinline__ float f4c(const float4& v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; }
 
-// The slab test of the lane's ray against the box [lx,hx] x [ly,hy] x [lz,hz]
-// (child_box's arithmetic, on values already in registers).
-__device__ __forceinline__ bool slab_box(float lx, float hx, float ly, float hy, float lz, float hz, const Lane& L,
-                                         float& tmin) {
+// The slab interval of the lane's ray through the box [lx,hx] x [ly,hy] x
+// [lz,hz] (child_box's arithmetic, on values already in registers); the ray
+// enters the box when tmin <= tmax and tmin < L.t_max.
+__device__ __forceinline__ void slab_box(float lx, float hx, float ly, float hy, float lz, float hz, const Lane& L,
+                                         float& tmin, float& tmax) {
   const float ox = L.o.x * L.rd.x, oy = L.o.y * L.rd.y, oz = L.o.z * L.rd.z;
   const float tx0 = fmaf(lx, L.rd.x, -ox), tx1 = fmaf(hx, L.rd.x, -ox);
   const float ty0 = fmaf(ly, L.rd.y, -oy), ty1 = fmaf(hy, L.rd.y, -oy);
   const float tz0 = fmaf(lz, L.rd.z, -oz), tz1 = fmaf(hz, L.rd.z, -oz);
   tmin = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
-  const float tmax = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1)) * 1.0000005f;
-  return tmin <= tmax && tmin < L.t_max;
+  tmax = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1)) * 1.0000005f;
 }
 
 // Entry and exit distances of the lane's ray through child slot k of a node
@@ -690,13 +690,17 @@ __device__ __forceinline__ void packet_chunk(const WArgs& A, int64_t r0, Acc (&a
       // network runs on the scalar unit
       uint32_t key[4];
       int code[4];
+      const unsigned long long act_mask = __ballot(active);
 #pragma unroll
       for (int k = 0; k < 4; k++) {
-        float tmin;
-        const bool in_box = slab_box(f4c(bx[0], k), f4c(bx[1], k), f4c(bx[2], k), f4c(bx[3], k), f4c(bx[4], k),
-                                     f4c(bx[5], k), L, tmin);
-        const bool hit = active & (cha[k] != 0x7fffffff) & in_box;
-        const unsigned long long m = __ballot(hit);
+        float tmin, tmax;
+        slab_box(f4c(bx[0], k), f4c(bx[1], k), f4c(bx[2], k), f4c(bx[3], k), f4c(bx[4], k), f4c(bx[5], k), L, tmin,
+                 tmax);
+        // the lanes that enter the child: a ballot per compare (each a compare's own
+        // lane mask) and scalar ANDs, no per-lane boolean to materialise
+        const unsigned long long slot_mask = cha[k] != 0x7fffffff ? ~0ull : 0ull;  // no branch: the compiler
+        // would split the slab arithmetic around one
+        const unsigned long long m = __ballot(tmin <= tmax) & __ballot(tmin < L.t_max) & act_mask & slot_mask;
         key[k] = 0x7f800000u;
         if (m) key[k] = (uint32_t)rl_int(__float_as_int(tmin), __ffsll((long long)m) - 1) & 0x7fffffffu;
         // leaves are pushed as their parent and slot, re-tested on the pop
